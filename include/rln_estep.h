@@ -1,0 +1,260 @@
+/*
+ * rln_estep.h -- C ABI of the MI355X-native RELION expectation step.
+ *
+ * One shared library (librln_estep.so) exports three objects and their calls:
+ *
+ *   rln_projector      device copy of one class's padded Fourier reference
+ *                      (replaces AccProjector, src/acc/acc_projector.h:17-104)
+ *   rln_backprojector  device accumulators of one class's padded Fourier data
+ *                      and weight volumes (replaces AccBackprojector,
+ *                      src/acc/acc_backprojector.h:24-99)
+ *   rln_estep          the per-device E-step engine: coarse pass, weights and
+ *                      significance, fine pass, weighted sums and
+ *                      backprojection for a batch of prepared particles
+ *                      (replaces MlOptimiserAccGPU::doThreadExpectationSomeParticles,
+ *                      src/acc/hip/hip_ml_optimiser.hip.cpp:250-298, which drives
+ *                      accDoExpectationOneParticle, src/acc/acc_ml_optimiser_impl.h:3673-3962)
+ *
+ * Conventions (all follow the reference CPU path, src/ml_optimiser.cpp):
+ *   - 2-D images are FFTW half-complex arrays of size s x (s/2+1), row i holds
+ *     frequency y = i for i <= s/2 and y = i - s otherwise; complex values are
+ *     interleaved (re, im).
+ *   - A padded reference volume has dims zdim x ydim x xdim with
+ *     zdim = ydim = pad_size, xdim = pad_size/2 + 1,
+ *     pad_size = 2*(ROUND(padding_factor*r_max)+1)+1, r_max = current_size/2,
+ *     logical x in [0, xdim), y,z in [-(pad_size/2), pad_size/2]
+ *     (src/projector.cpp:59-89).
+ *   - Offsets in rln_particle_meta are in pixels, as METADATA_XOFF/YOFF of
+ *     exp_metadata (src/ml_optimiser.cpp:5880-5886, 10429).
+ *
+ * Every call returns RLN_OK (0) or a negative status; rln_last_error()
+ * returns a thread-local description of the last failure.
+ */
+#ifndef RLN_ESTEP_H
+#define RLN_ESTEP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RLN_OK = 0,
+    RLN_ERR_ARG = -1,            /* invalid argument or unsupported option */
+    RLN_ERR_HIP = -2,            /* HIP runtime failure */
+    RLN_ERR_STATE = -3,          /* call out of order (e.g. model not set) */
+    RLN_ERR_FILTERED_ZERO = -4,  /* ERRFILTEREDZERO: no positive weight (src/error.h:260-280) */
+    RLN_ERR_NO_SIGNIFICANT = -5, /* ERRNOSIGNIFS: no significant coarse sample */
+    RLN_ERR_SUM_WEIGHT_ZERO = -6,/* ERRSUMWEIGHTZERO: zero or NaN sum of weights */
+    RLN_ERR_MEMORY = -7          /* device allocation failed */
+};
+
+/* Precision of the squared-difference (diff2) kernels. */
+enum {
+    RLN_PREC_PLAIN_FP32 = 0      /* direct sum of |CTF*s*P - X_t|^2 in FP32 (fallback path) */
+};
+
+/* Options of one E-step run; names follow MlOptimiser / MlModel / HealpixSampling members. */
+typedef struct rln_estep_config {
+    /* MlModel geometry */
+    int    ori_size;              /* mymodel.ori_size (square particle box, pixels) */
+    int    nr_classes;            /* mymodel.nr_classes (K) */
+    double pixel_size;            /* Angstrom per pixel (model and images) */
+    double padding_factor;        /* mymodel.padding_factor */
+    int    current_size;          /* image_current_size: fine pass / weighted sums; r_max = current_size/2 */
+    int    coarse_size;           /* image_coarse_size: coarse pass (src/ml_optimiser.cpp:5763-5779) */
+    /* HealpixSampling */
+    int    healpix_order;         /* coarse directions = 12*4^order (C1) */
+    int    adaptive_oversampling; /* fine pass oversampling order; only 1 is supported */
+    double psi_step;              /* degrees; <= 0 selects 360/(6*2^order) (src/healpix_sampling.cpp:73-74) */
+    double offset_range;          /* Angstrom */
+    double offset_step;           /* Angstrom */
+    /* orientational priors */
+    int    local_search;          /* 0: NOPRIOR (pdf_direction); 1: Gaussian priors around each particle's angles */
+    int    precision;             /* RLN_PREC_* */
+    double sigma_rot, sigma_tilt, sigma_psi; /* degrees, local search (sqrt(mymodel.sigma2_rot) ...) */
+    double sigma_cutoff;          /* 3 (src/healpix_sampling.h:236-242) */
+    /* MlOptimiser */
+    double adaptive_fraction;     /* --adaptive_fraction, 0.999 */
+    long long maximum_significants; /* --maxsig; <= 0: no cap */
+    double sigma2_fudge;          /* sigma2_fudge, 1 */
+    double sigma2_offset;         /* mymodel.sigma2_offset, Angstrom^2 */
+    int    do_ctf_correction;     /* do_ctf_correction && refs_are_ctf_corrected */
+    int    do_scale_correction;   /* do_scale_correction */
+} rln_estep_config;
+
+/* Per-particle scalars that the reference reads from exp_metadata / mymodel. */
+typedef struct rln_particle_meta {
+    double old_offset_x, old_offset_y;     /* pixels: rounded offsets already applied to the image */
+    double prior_offset_x, prior_offset_y; /* pixels: exp_prior */
+    double prior_rot, prior_tilt, prior_psi; /* degrees: centre of local searches */
+    double scale;                          /* mymodel.scale_correction[group_id] */
+    double highres_Xi2;                    /* sum |X|^2 beyond current_size (src/ml_optimiser.cpp:6386-6404) */
+} rln_particle_meta;
+
+/* Per-particle outputs (exp_metadata columns and the E-step scalars). */
+typedef struct rln_particle_result {
+    double rot, tilt, psi;        /* METADATA_ROT/TILT/PSI of the max-weight pose */
+    double xoff, yoff;            /* METADATA_XOFF/YOFF (pixels) */
+    double pmax;                  /* METADATA_PMAX */
+    double nr_significant;        /* METADATA_NR_SIGN (coarse pass) */
+    double dLL;                   /* METADATA_DLL */
+    double sum_weight;            /* exp_sum_weight after the fine pass */
+    double min_diff2;             /* exp_min_diff2 after the fine pass */
+    double significant_weight;    /* exp_significant_weight after the fine pass */
+    double coarse_sum_weight;     /* exp_sum_weight after the coarse pass */
+    double coarse_min_diff2;      /* exp_min_diff2 after the coarse pass */
+    double coarse_significant_weight;
+    double wsum_norm_correction;  /* exp_wsum_norm_correction (without the tail beyond current_size) */
+    double signal_product;        /* exp_wsum_scale_correction_XA / scale */
+    double reference_power;       /* exp_wsum_scale_correction_AA / scale^2 */
+    long long best_ihidden_over;  /* fine-pass index of the max-weight pose (src/healpix_sampling.cpp:1711-1722) */
+    int    iclass;                /* METADATA_CLASS (1-based) */
+    int    nr_fine_evaluated;     /* fine-pass poses evaluated */
+} rln_particle_result;
+
+/* Weighted sums accumulated (+=) over all particles of a run (MlWsumModel members). */
+typedef struct rln_wsum {
+    double *sigma2_noise;         /* [ori_size/2+1] wsum_model.sigma2_noise (shells < current_size/2+1) */
+    double *pdf_class;            /* [K] */
+    double *pdf_direction;        /* [K][12*4^order] */
+    double sigma2_offset;
+    double sumw_group;
+    double signal_product;        /* wsum_signal_product (summed over particles) */
+    double reference_power;       /* wsum_reference_power */
+    double LL;                    /* sum of dLL */
+    double ave_Pmax;              /* sum of Pmax */
+} rln_wsum;
+
+/* ---------------------------------------------------------------------------------------------
+ * Projector: AccProjector (src/acc/acc_projector.h:80-97, impl src/acc/acc_projector_impl.h:5-392)
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rln_projector rln_projector;
+
+/* AccProjector::AccProjector() -- binds to a HIP device. */
+int rln_projector_create(int device, rln_projector **out);
+/* AccProjector::setMdlDim(xdim, ydim, zdim, inity, initz, maxr, padding_factor) (acc_projector.h:80-86);
+ * *changed = 1 when the dimensions changed (the reference returns false when unchanged). */
+int rln_projector_set_mdl_dim(rln_projector *p, int xdim, int ydim, int zdim, int inity, int initz,
+                              int maxr, double padding_factor, int *changed);
+/* AccProjector::initMdl(XFLOAT *real, XFLOAT *imag) (acc_projector.h:88-95): host split float arrays. */
+int rln_projector_init_mdl(rln_projector *p, const float *real, const float *imag);
+/* AccProjector::initMdl(Complex *data) (acc_projector.h:88-95): host interleaved double complex. */
+int rln_projector_init_mdl_complex(rln_projector *p, const double *data);
+/* AccProjector::clear() (acc_projector.h:97) + destructor. */
+int rln_projector_destroy(rln_projector *p);
+
+/* ---------------------------------------------------------------------------------------------
+ * Backprojector: AccBackprojector (src/acc/acc_backprojector.h:60-94, impl acc_backprojector_impl.h:13-190)
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rln_backprojector rln_backprojector;
+
+int rln_backprojector_create(int device, rln_backprojector **out);
+/* AccBackprojector::setMdlDim(...) (acc_backprojector.h:60-66): *bytes = device bytes held. */
+int rln_backprojector_set_mdl_dim(rln_backprojector *b, int xdim, int ydim, int zdim, int inity, int initz,
+                                  int maxr, double padding_factor, size_t *bytes);
+/* AccBackprojector::initMdl() (acc_backprojector.h:68): zero data and weight. */
+int rln_backprojector_init_mdl(rln_backprojector *b);
+/* AccBackprojector::getMdlData(real, imag, weight) (acc_backprojector_impl.h:109-138):
+ * synchronise and copy to host float arrays of xdim*ydim*zdim elements. */
+int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real, float *imag, float *weight);
+int rln_backprojector_destroy(rln_backprojector *b);
+
+/* ---------------------------------------------------------------------------------------------
+ * E-step engine (MlDeviceBundle + MlOptimiserAccGPU, src/acc/hip/hip_ml_optimiser.hip.cpp:70-298)
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rln_estep rln_estep;
+
+/* MlDeviceBundle::setupFixedSizedObjects/setupTunableSizedObjects (hip_ml_optimiser.hip.cpp:88-226). */
+int rln_estep_create(int device, const rln_estep_config *cfg, rln_estep **out);
+int rln_estep_destroy(rln_estep *e);
+
+/* Binds the per-iteration model: one projector and one backprojector per class (borrowed, not owned),
+ * pdf_class[K], pdf_direction[K][12*4^order] (NOPRIOR mode), sigma2_noise[ori_size/2+1] and, if
+ * do_scale_correction, data_vs_prior[K][ori_size/2+1]. Host arrays are copied. */
+int rln_estep_set_model(rln_estep *e, rln_projector *const *projectors, rln_backprojector *const *backprojectors,
+                        const double *pdf_class, const double *pdf_direction, const double *sigma2_noise,
+                        const double *data_vs_prior);
+
+/* Runs the E-step of n particles (accDoExpectationOneParticle for each, batched):
+ *   fimg, fimg_nomask: [n][current_size][current_size/2+1] interleaved complex float (masked / unmasked image)
+ *   fctf:              [n][current_size][current_size/2+1] float
+ *   inputs_on_device:  1 if the three image arrays are device pointers on the engine's device, 0 if host
+ *   meta:              [n] host
+ * Outputs: results[n] (host), wsum (host arrays, accumulated), and backprojection into the bound
+ * backprojectors (device, accumulated). */
+int rln_estep_run(rln_estep *e, int n, const float *fimg, const float *fimg_nomask, const float *fctf,
+                  int inputs_on_device, const rln_particle_meta *meta,
+                  rln_particle_result *results, rln_wsum *wsum);
+
+/* Stage hooks for parity tests. After rln_estep_run with keep_intermediates set, these copy the
+ * last run's per-particle intermediates to host (particle index ip within that run):
+ *   coarse: diff2 per coarse pose in ihidden order (K * N_dir_b * N_psi_b * N_tr), and weights;
+ *   fine:   n_fine (ihidden_over, diff2, weight) triplets of the evaluated fine poses. */
+int rln_estep_keep_intermediates(rln_estep *e, int keep);
+int rln_estep_coarse_size(rln_estep *e, int ip, long long *n_coarse, int *n_dir, int *n_psi, int *n_trans);
+int rln_estep_get_coarse(rln_estep *e, int ip, float *diff2, double *weight, unsigned char *significant);
+int rln_estep_fine_size(rln_estep *e, int ip, long long *n_fine);
+int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *diff2, double *weight);
+
+/* Wall time of the device work of the last run, in milliseconds, per stage (for profiling). */
+int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
+
+/* ---------------------------------------------------------------------------------------------
+ * Sampling helpers (HealpixSampling, src/healpix_sampling.cpp), host-side.
+ * ------------------------------------------------------------------------------------------- */
+/* image_coarse_size rule of src/ml_optimiser.cpp:5757-5779 (adaptive_oversampling > 0). */
+int rln_coarse_image_size(int ori_size, double pixel_size, int healpix_order, double particle_diameter,
+                          int current_size);
+/* Number of directions (C1), psi angles and translations of the coarse grid. */
+int rln_sampling_counts(const rln_estep_config *cfg, int *n_dir, int *n_psi, int *n_trans);
+/* Coarse grid: rot/tilt per direction (degrees), psi per psi index, translations (Angstrom). */
+int rln_sampling_grid(const rln_estep_config *cfg, double *rot, double *tilt, double *psi,
+                      double *trans_x, double *trans_y);
+/* selectOrientationsWithNonZeroPriorProbability (src/healpix_sampling.cpp:695-1076) for one particle:
+ * fills up to max_dir / max_psi entries; returns counts. */
+int rln_sampling_local(const rln_estep_config *cfg, double prior_rot, double prior_tilt, double prior_psi,
+                       int *dir_index, double *dir_prior, int max_dir, int *n_dir,
+                       int *psi_index, double *psi_prior, int max_psi, int *n_psi);
+
+/* ---------------------------------------------------------------------------------------------
+ * Synthetic data (seeded Gaussian-blob maps and particles; the reference's simulator is
+ * src/apps/project.cpp:66-106). Host-side, OpenMP.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rln_synth_spec {
+    unsigned long long seed;
+    int    ori_size;
+    double pixel_size;
+    int    nr_blobs;
+    double blob_radius;           /* centres uniform within this radius (pixels) */
+    double sigma_min, sigma_max;  /* blob sigma range (pixels) */
+    double amp_min, amp_max;
+    double snr_min, snr_max;      /* per-particle SNR, log-uniform in [min, max] (min == max: fixed) */
+    double defocus_min, defocus_max; /* Angstrom */
+    double max_astigmatism;       /* Angstrom */
+    double voltage, cs, q0, bfactor;
+    double offset_max;            /* true offsets uniform in [-max, max] pixels */
+    double prior_sigma_deg;       /* prior angles = true pose perturbed by this rotation sigma */
+} rln_synth_spec;
+
+/* Padded Fourier reference of the blob map for class `iclass` (blob set drawn from seed+iclass;
+ * share_fraction of the blobs are drawn from a class-independent stream). Output: interleaved
+ * double complex, zdim*ydim*xdim as in the projector layout for (current_size, padding_factor). */
+int rln_synth_reference(const rln_synth_spec *spec, int iclass, double share_fraction, int current_size,
+                        double padding_factor, double *out);
+/* Particles [first, first+n): images at current_size (interleaved float complex), CTFs, metadata,
+ * true poses (rot, tilt, psi, xoff, yoff, class) and the noise variance per Fourier component. */
+int rln_synth_particles(const rln_synth_spec *spec, int nr_classes, double share_fraction, int current_size,
+                        long long first, int n, float *fimg, float *fctf, rln_particle_meta *meta,
+                        double *true_pose, double *noise_var);
+/* CTF value of the reference model (src/ctf.h:184-240) at frequency (x, y) in 1/Angstrom. */
+double rln_ctf_value(double defU, double defV, double defAngle, double voltage, double cs, double q0,
+                     double bfactor, double scale, double phase_shift, double x, double y);
+
+const char *rln_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RLN_ESTEP_H */

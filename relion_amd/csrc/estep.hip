@@ -1,0 +1,1192 @@
+// librln_estep: projector / backprojector objects and the batched E-step engine behind the C ABI of
+// include/rln_estep.h. Host C++ driving the HIP kernels of kernels.h on one device and one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "rln_common.h"
+#include "sampling.h"
+
+namespace rln {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+const char *last_error() { return g_last_error.c_str(); }
+
+}  // namespace rln
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t _e = (expr);                                                                           \
+        if (_e != hipSuccess) {                                                                           \
+            ::rln::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+            return RLN_ERR_HIP;                                                                           \
+        }                                                                                                 \
+    } while (0)
+
+#define RLN_TRY(expr)             \
+    do {                          \
+        int _rc = (expr);         \
+        if (_rc != RLN_OK) return _rc; \
+    } while (0)
+
+namespace rln {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    int ensure(size_t need) {
+        if (need <= bytes) return RLN_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        size_t want = std::max<size_t>(need, 256);
+        if (hipMalloc(&ptr, want) != hipSuccess) {
+            ptr = nullptr;
+            set_error("device allocation of %zu bytes failed", want);
+            return RLN_ERR_MEMORY;
+        }
+        bytes = want;
+        return RLN_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T *as() const {
+        return reinterpret_cast<T *>(ptr);
+    }
+};
+
+struct MdlDims {
+    int xdim = 0, ydim = 0, zdim = 0, inity = 0, initz = 0, maxr = 0;
+    double pad = 0.;
+    bool operator==(const MdlDims &o) const {
+        return xdim == o.xdim && ydim == o.ydim && zdim == o.zdim && inity == o.inity && initz == o.initz &&
+               maxr == o.maxr && pad == o.pad;
+    }
+    size_t voxels() const { return (size_t)xdim * ydim * zdim; }
+};
+
+int check_dims(const MdlDims &d) {
+    if (d.xdim <= 1 || d.ydim <= 1 || d.zdim <= 1 || d.maxr <= 0 || !(d.pad >= 1.))
+        RLN_FAIL(RLN_ERR_ARG, "invalid model dimensions %d x %d x %d (maxr %d, pad %g)", d.xdim, d.ydim, d.zdim,
+                 d.maxr, d.pad);
+    return RLN_OK;
+}
+
+}  // namespace rln
+
+struct rln_projector {
+    int device = 0;
+    rln::MdlDims dims;
+    rln::DevBuf data;  // float2 [z][y][x]
+    unsigned long long version = 0;
+};
+
+struct rln_backprojector {
+    int device = 0;
+    rln::MdlDims dims;
+    rln::DevBuf re, im, w;
+};
+
+namespace rln {
+
+// Stage timers of one run (HIP events on the engine's stream).
+enum Stage { kStPrep, kStCoarse, kStCoarseW, kStJobs, kStFine, kStFineW, kStWsum, kStCount };
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rln_estep_config cfg{};
+    PoseGrid grid;
+    int NT = 0, n_or = 0;
+    int sc = 0, sf = 0;
+    rlnk::ImgGeom igc{}, igf{};
+    rlnk::VolGeom vg{};
+    int nshell = 0;
+    // model
+    int K = 0;
+    std::vector<rln_projector *> proj;
+    std::vector<rln_backprojector *> bproj;
+    std::vector<unsigned long long> proj_version;
+    std::vector<double> pdf_class, pdf_direction, sigma2_noise, data_vs_prior;
+    bool model_set = false;
+    bool coarse_valid = false;
+    // constant tables
+    DevBuf d_frames_c, d_frames_f, d_proj_c;
+    DevBuf d_q_c, d_q_f, d_minv_w, d_mresol, d_xa_shell;
+    DevBuf d_ex_c, d_ey_c, d_ex_f, d_ey_f, d_tr_ix, d_tr_iy, d_over_x, d_over_y;
+    DevBuf d_vols, d_bp_ptrs;
+    std::vector<double> over_x, over_y;  // [NT*4] pixels
+    std::vector<int> mresol_f;
+    double logsigma2 = 0.;
+    // per-chunk buffers
+    DevBuf d_fimg, d_fimgn, d_fctf;
+    DevBuf d_meta_f, d_meta_d, d_meta_ll;
+    DevBuf d_or_off, d_or_cnt, d_or_proj, d_or_prior, d_or_frame, d_or_class, d_or_dir, d_tr_prior;
+    DevBuf d_ctfq, d_yq, d_tiles;
+    DevBuf d_diff2, d_weight, d_mask, d_stats_c, d_stats_f;
+    DevBuf d_npairs, d_njobs, d_nchunks, d_pair_off, d_job_off, d_chunk_off, d_pairs, d_jobs, d_chunk_job,
+        d_chunk_first;
+    DevBuf d_fdiff2, d_fweight, d_fine_n, d_nent, d_foff;
+    DevBuf d_shell_acc, d_part_acc, d_pdfc_acc, d_pdfd_acc;
+    // intermediates kept for tests
+    bool keep = false;
+    struct Kept {
+        long long n_coarse = 0;
+        int n_dir = 0, n_psi = 0;
+        std::vector<float> cdiff2;
+        std::vector<double> cweight;
+        std::vector<unsigned char> cmask;
+        std::vector<long long> f_index;
+        std::vector<float> fdiff2;
+        std::vector<double> fweight;
+    };
+    std::vector<Kept> kept;
+    // timings
+    hipEvent_t ev[kStCount + 1] = {};
+    double stage_ms[kStCount] = {};
+    bool have_timing = false;
+    std::mutex mtx;
+};
+
+namespace {
+
+// Mresol of an s x (s/2+1) image (src/ml_optimiser.cpp:5782-5812).
+std::vector<int> mresol_of(int s) {
+    const int xs = s / 2 + 1;
+    std::vector<int> m((size_t)s * xs, -1);
+    for (int i = 0; i < s; ++i) {
+        const int ip = (i < xs) ? i : i - s;
+        for (int j = 0; j < xs; ++j) {
+            const int ires = ROUND(sqrt((double)(ip * ip + j * j)));
+            if (ires < s / 2 + 1 && !(j == 0 && ip < 0)) m[(size_t)i * xs + j] = ires;
+        }
+    }
+    return m;
+}
+
+rlnk::ImgGeom img_geom(int s, int r_max, double pad) {
+    rlnk::ImgGeom g;
+    g.s = s;
+    g.xs = s / 2 + 1;
+    g.P = s * g.xs;
+    const int rmo = g.xs - 1;
+    const long long rmr = (long long)(r_max * pad);
+    const double pad2 = pad * pad;
+    int r2 = rmo * rmo;
+    while (r2 > 0 && pad2 * (double)r2 > (double)(rmr * rmr)) --r2;
+    g.proj_r2_max = r2;
+    const long long rr = ROUND(r_max * pad);
+    int b2 = (int)std::min<long long>(rr * rr, 4LL * s * s);
+    while (b2 > 0 && pad2 * (double)b2 > (double)(rr * rr)) --b2;
+    g.bp_r2_max = b2;
+    return g;
+}
+
+rlnk::Frame frame_of(double rot, double tilt, double psi, double pad) {
+    double A[9];
+    euler_angles2matrix(rot, tilt, psi, A);
+    rlnk::Frame f;
+    f.e0x = (float)(pad * A[0]);
+    f.e0y = (float)(pad * A[1]);
+    f.e0z = (float)(pad * A[2]);
+    f.e1x = (float)(pad * A[3]);
+    f.e1y = (float)(pad * A[4]);
+    f.e1z = (float)(pad * A[5]);
+    return f;
+}
+
+// exp(-2 pi i k shift / N) for k on the axis of an image (x: 0..xs-1; y rows: FFTW signed order).
+void phase_rows(const std::vector<double> &shifts, int s, bool yaxis, double N, std::vector<float2> &out) {
+    const int xs = s / 2 + 1;
+    const int len = yaxis ? s : xs;
+    out.resize(shifts.size() * (size_t)len);
+    for (size_t j = 0; j < shifts.size(); ++j)
+        for (int i = 0; i < len; ++i) {
+            const int k = yaxis ? ((i < xs) ? i : i - s) : i;
+            // reduce the phase in turns before the trigonometry
+            double turns = -(double)k * shifts[j] / N;
+            turns -= std::nearbyint(turns);
+            const double a = 2. * kPi * turns;
+            out[j * len + i] = make_float2((float)cos(a), (float)sin(a));
+        }
+}
+
+template <class T>
+int upload(DevBuf &buf, const std::vector<T> &v, hipStream_t st) {
+    RLN_TRY(buf.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) HIP_TRY(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return RLN_OK;
+}
+
+int launch_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) RLN_FAIL(RLN_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return RLN_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Engine setup
+// ------------------------------------------------------------------------------------------------
+static int engine_init(Engine &e, const rln_estep_config &cfg) {
+    e.cfg = cfg;
+    if (cfg.nr_classes <= 0 || cfg.ori_size <= 0 || (cfg.ori_size & 1) || cfg.current_size <= 0 ||
+        (cfg.current_size & 1) || cfg.coarse_size <= 0 || (cfg.coarse_size & 1) || cfg.coarse_size > cfg.current_size ||
+        cfg.current_size > cfg.ori_size || !(cfg.padding_factor >= 1.))
+        RLN_FAIL(RLN_ERR_ARG, "invalid image sizes (ori %d, current %d, coarse %d) or padding %g", cfg.ori_size,
+                 cfg.current_size, cfg.coarse_size, cfg.padding_factor);
+    if (cfg.adaptive_oversampling != 1) RLN_FAIL(RLN_ERR_ARG, "only adaptive_oversampling = 1 is supported");
+    if (cfg.precision != RLN_PREC_PLAIN_FP32) RLN_FAIL(RLN_ERR_ARG, "unsupported precision mode %d", cfg.precision);
+    if (!(cfg.sigma2_fudge > 0.) || !(cfg.adaptive_fraction > 0. && cfg.adaptive_fraction <= 1.))
+        RLN_FAIL(RLN_ERR_ARG, "invalid sigma2_fudge or adaptive_fraction");
+    if (cfg.ori_size / 2 + 1 > 512) RLN_FAIL(RLN_ERR_ARG, "ori_size %d above the supported 1022", cfg.ori_size);
+    if (!e.grid.build(cfg)) RLN_FAIL(RLN_ERR_ARG, "invalid sampling options");
+    e.NT = e.grid.n_trans;
+    if (4 * e.NT > rlnk::kMaxSigT)
+        RLN_FAIL(RLN_ERR_ARG, "%d translations: at most %d are supported", e.NT, rlnk::kMaxSigT / 4);
+    e.n_or = e.grid.n_dir * e.grid.n_psi;
+    e.sc = cfg.coarse_size;
+    e.sf = cfg.current_size;
+    const int r_max = std::min(cfg.current_size / 2, cfg.ori_size / 2);
+    e.igc = img_geom(e.sc, r_max, cfg.padding_factor);
+    e.igf = img_geom(e.sf, r_max, cfg.padding_factor);
+    e.nshell = cfg.ori_size / 2 + 1;
+    HIP_TRY(hipSetDevice(e.device));
+    HIP_TRY(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    for (int i = 0; i <= kStCount; ++i) HIP_TRY(hipEventCreate(&e.ev[i]));
+    // orientation frames of the coarse grid and of its order-1 oversampling
+    std::vector<rlnk::Frame> fc(e.n_or), ff((size_t)e.n_or * 8);
+    double r8[8], t8[8], p8[8];
+    for (int d = 0; d < e.grid.n_dir; ++d)
+        for (int p = 0; p < e.grid.n_psi; ++p) {
+            const int o = d * e.grid.n_psi + p;
+            fc[o] = frame_of(e.grid.rot[d], e.grid.tilt[d], e.grid.psi[p], cfg.padding_factor);
+            e.grid.oversampled_orientations(d, p, r8, t8, p8);
+            for (int r = 0; r < 8; ++r) ff[(size_t)o * 8 + r] = frame_of(r8[r], t8[r], p8[r], cfg.padding_factor);
+        }
+    RLN_TRY(upload(e.d_frames_c, fc, e.stream));
+    RLN_TRY(upload(e.d_frames_f, ff, e.stream));
+    // translations: coarse phase rows per translation, fine rows per distinct sub-shift
+    std::vector<double> cx(e.NT), cy(e.NT);
+    for (int t = 0; t < e.NT; ++t) {
+        cx[t] = e.grid.tx[t] / cfg.pixel_size;
+        cy[t] = e.grid.ty[t] / cfg.pixel_size;
+    }
+    std::vector<float2> rows;
+    phase_rows(cx, e.sc, false, cfg.ori_size, rows);
+    RLN_TRY(upload(e.d_ex_c, rows, e.stream));
+    phase_rows(cy, e.sc, true, cfg.ori_size, rows);
+    RLN_TRY(upload(e.d_ey_c, rows, e.stream));
+    e.over_x.resize((size_t)e.NT * 4);
+    e.over_y.resize((size_t)e.NT * 4);
+    for (int t = 0; t < e.NT; ++t) e.grid.oversampled_translations(t, &e.over_x[t * 4], &e.over_y[t * 4]);
+    std::map<double, int> ux, uy;
+    for (double v : e.over_x) ux.emplace(v, 0);
+    for (double v : e.over_y) uy.emplace(v, 0);
+    std::vector<double> vx, vy;
+    for (auto &kv : ux) {
+        kv.second = (int)vx.size();
+        vx.push_back(kv.first);
+    }
+    for (auto &kv : uy) {
+        kv.second = (int)vy.size();
+        vy.push_back(kv.first);
+    }
+    std::vector<int> tix(e.NT * 4), tiy(e.NT * 4);
+    for (int i = 0; i < e.NT * 4; ++i) {
+        tix[i] = ux[e.over_x[i]];
+        tiy[i] = uy[e.over_y[i]];
+    }
+    phase_rows(vx, e.sf, false, cfg.ori_size, rows);
+    RLN_TRY(upload(e.d_ex_f, rows, e.stream));
+    phase_rows(vy, e.sf, true, cfg.ori_size, rows);
+    RLN_TRY(upload(e.d_ey_f, rows, e.stream));
+    RLN_TRY(upload(e.d_tr_ix, tix, e.stream));
+    RLN_TRY(upload(e.d_tr_iy, tiy, e.stream));
+    RLN_TRY(upload(e.d_over_x, e.over_x, e.stream));
+    RLN_TRY(upload(e.d_over_y, e.over_y, e.stream));
+    e.mresol_f = mresol_of(e.sf);
+    RLN_TRY(upload(e.d_mresol, e.mresol_f, e.stream));
+    HIP_TRY(hipStreamSynchronize(e.stream));
+    return RLN_OK;
+}
+
+// Per-model tables: Minvsigma2 at both sizes, volume pointers, backprojector pointers.
+static int engine_model_tables(Engine &e) {
+    const rln_estep_config &cfg = e.cfg;
+    auto q_of = [&](int s, std::vector<float> &q, std::vector<float> *minv_w) {
+        const std::vector<int> m = mresol_of(s);
+        q.assign(m.size(), 0.f);
+        if (minv_w) minv_w->assign(m.size(), 0.f);
+        for (size_t n = 0; n < m.size(); ++n) {
+            const int ires = m[n];
+            double minv = 0.;
+            if (ires > 0 && ires < e.nshell) minv = 1. / (cfg.sigma2_fudge * e.sigma2_noise[ires]);
+            q[n] = (float)sqrt(0.5 * minv);
+            if (minv_w) (*minv_w)[n] = (float)minv;
+        }
+        if (minv_w) (*minv_w)[0] = (float)(1. / (cfg.sigma2_fudge * e.sigma2_noise[0]));  // origin restored
+    };
+    std::vector<float> qc, qf, mw;
+    q_of(e.sc, qc, nullptr);
+    q_of(e.sf, qf, &mw);
+    RLN_TRY(upload(e.d_q_c, qc, e.stream));
+    RLN_TRY(upload(e.d_q_f, qf, e.stream));
+    RLN_TRY(upload(e.d_minv_w, mw, e.stream));
+    // logsigma2 of the dLL (src/ml_optimiser.cpp:9024-9033)
+    e.logsigma2 = 0.;
+    for (int ires : e.mresol_f)
+        if (ires > 0 && ires < e.nshell) e.logsigma2 += log(2. * kPi * e.sigma2_noise[ires]);
+    std::vector<const float2 *> vp(e.K);
+    std::vector<float *> bp(3 * e.K);
+    for (int k = 0; k < e.K; ++k) {
+        vp[k] = e.proj[k]->data.as<const float2>();
+        bp[k] = e.bproj[k]->re.as<float>();
+        bp[e.K + k] = e.bproj[k]->im.as<float>();
+        bp[2 * e.K + k] = e.bproj[k]->w.as<float>();
+    }
+    RLN_TRY(upload(e.d_vols, vp, e.stream));
+    RLN_TRY(upload(e.d_bp_ptrs, bp, e.stream));
+    if (cfg.do_scale_correction && !e.data_vs_prior.empty()) {
+        std::vector<unsigned char> xa((size_t)e.K * e.nshell);
+        for (size_t i = 0; i < xa.size(); ++i) xa[i] = e.data_vs_prior[i] > 3. ? 1 : 0;
+        RLN_TRY(upload(e.d_xa_shell, xa, e.stream));
+    }
+    RLN_TRY(e.d_pdfd_acc.ensure(sizeof(double) * (size_t)e.K * e.grid.n_dir));
+    return RLN_OK;
+}
+
+static int engine_coarse_projections(Engine &e) {
+    bool stale = !e.coarse_valid;
+    for (int k = 0; k < e.K; ++k)
+        if (e.proj_version[k] != e.proj[k]->version) stale = true;
+    if (!stale) return RLN_OK;
+    const size_t n = (size_t)e.K * e.n_or * e.igc.P;
+    RLN_TRY(e.d_proj_c.ensure(n * sizeof(float2)));
+    dim3 grid(e.K * e.n_or, (e.igc.P + rlnk::kBlock - 1) / rlnk::kBlock);
+    hipLaunchKernelGGL(rlnk::k_project_grid, grid, dim3(rlnk::kBlock), 0, e.stream, e.d_vols.as<const float2 *const>(),
+                       e.vg, e.d_frames_c.as<const rlnk::Frame>(), e.n_or, e.igc, e.d_proj_c.as<float2>());
+    RLN_TRY(launch_check("k_project_grid"));
+    for (int k = 0; k < e.K; ++k) e.proj_version[k] = e.proj[k]->version;
+    e.coarse_valid = true;
+    return RLN_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// One chunk of particles
+// ------------------------------------------------------------------------------------------------
+struct ChunkHost {
+    int B = 0;
+    std::vector<int> or_off, or_cnt, or_proj, or_frame, or_class, or_dir;
+    std::vector<int> nd, np;  // per particle: directions, psi
+    std::vector<int> dir_idx, psi_idx;  // flattened, offsets in dir_off/psi_off
+    std::vector<int> dir_off, psi_off;
+    std::vector<double> or_prior, tr_prior;
+    std::vector<long long> d2_off, n_entries;
+    std::vector<float> xi2half, scale, scale_w;
+    std::vector<double> old_off, prior_off;
+    std::vector<rlnk::CoarseTile> tiles;
+    long long total_entries = 0;
+};
+
+static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, ChunkHost &h) {
+    const rln_estep_config &cfg = e.cfg;
+    const int K = e.K, NT = e.NT;
+    h = ChunkHost();
+    h.B = B;
+    h.or_off.resize(B);
+    h.or_cnt.resize(B);
+    h.nd.resize(B);
+    h.np.resize(B);
+    h.dir_off.resize(B);
+    h.psi_off.resize(B);
+    h.tr_prior.resize((size_t)B * NT);
+    h.d2_off.resize(B);
+    h.n_entries.resize(B);
+    h.xi2half.resize(B);
+    h.scale.resize(B);
+    h.scale_w.resize(B);
+    h.old_off.resize(2 * B);
+    h.prior_off.resize(2 * B);
+    const double px = cfg.pixel_size;
+    const double s2 = cfg.sigma2_offset;
+    const double offset_norm = 2. * kPi * sqrt(s2) * sqrt(s2);
+    std::vector<int> di, pi;
+    std::vector<double> dp, pp;
+    long long d2 = 0;
+    for (int b = 0; b < B; ++b) {
+        const rln_particle_meta &m = meta[b];
+        if (cfg.local_search) {
+            e.grid.local_priors(cfg, m.prior_rot, m.prior_tilt, m.prior_psi, di, dp, pi, pp);
+        } else {
+            di.resize(e.grid.n_dir);
+            pi.resize(e.grid.n_psi);
+            for (int i = 0; i < e.grid.n_dir; ++i) di[i] = i;
+            for (int i = 0; i < e.grid.n_psi; ++i) pi[i] = i;
+        }
+        const int ND = (int)di.size(), NP = (int)pi.size();
+        h.nd[b] = ND;
+        h.np[b] = NP;
+        h.dir_off[b] = (int)h.dir_idx.size();
+        h.psi_off[b] = (int)h.psi_idx.size();
+        h.dir_idx.insert(h.dir_idx.end(), di.begin(), di.end());
+        h.psi_idx.insert(h.psi_idx.end(), pi.begin(), pi.end());
+        // orientation priors, normalised by their mean over classes, directions and psi (src/ml_optimiser.cpp:7790-7803)
+        const int base = (int)h.or_proj.size();
+        h.or_off[b] = base;
+        h.or_cnt[b] = K * ND * NP;
+        double mean = 0.;
+        for (int k = 0; k < K; ++k)
+            for (int id = 0; id < ND; ++id)
+                for (int ip = 0; ip < NP; ++ip) {
+                    const double po = cfg.local_search ? dp[id] * pp[ip]
+                                                       : e.pdf_direction[(size_t)k * e.grid.n_dir + di[id]];
+                    mean += po;
+                    const int og = di[id] * e.grid.n_psi + pi[ip];
+                    const bool on = po > 0. && e.pdf_class[k] > 0.;
+                    h.or_proj.push_back(on ? k * e.n_or + og : -1);
+                    h.or_prior.push_back(po);
+                    h.or_frame.push_back(og * 8);
+                    h.or_class.push_back(k);
+                    h.or_dir.push_back(di[id]);
+                }
+        mean /= (double)((long long)K * ND * NP);
+        if (mean != 0.)
+            for (int i = 0; i < K * ND * NP; ++i) h.or_prior[base + i] /= mean;
+        // offset priors (src/ml_optimiser.cpp:7805-7872): offsets in pixels plus translations in Angstrom
+        double tmean = 0.;
+        for (int t = 0; t < NT; ++t) {
+            const double ox = m.old_offset_x + e.grid.tx[t], oy = m.old_offset_y + e.grid.ty[t];
+            double td = (ox - m.prior_offset_x) * (ox - m.prior_offset_x) / (-2. * s2);
+            td += (oy - m.prior_offset_y) * (oy - m.prior_offset_y) / (-2. * s2);
+            td *= px * px;
+            const double pf = (s2 < 0.0001) ? ((td > 0.) ? 0. : 1.) : exp(td) / offset_norm;
+            h.tr_prior[(size_t)b * NT + t] = pf;
+            tmean += pf;
+        }
+        tmean = tmean * K / (double)(K * NT);  // mean over classes x translations (same per class)
+        if (tmean > 0.)
+            for (int t = 0; t < NT; ++t) h.tr_prior[(size_t)b * NT + t] /= tmean;
+        h.d2_off[b] = d2;
+        h.n_entries[b] = (long long)h.or_cnt[b] * NT;
+        d2 += h.n_entries[b];
+        for (int o = 0; o < h.or_cnt[b]; o += rlnk::kCoTO) h.tiles.push_back(rlnk::CoarseTile{b, o});
+        h.xi2half[b] = (float)(m.highres_Xi2 / 2.);
+        const double sc = cfg.do_scale_correction ? m.scale : 1.;
+        h.scale[b] = (float)sc;
+        h.scale_w[b] = (float)std::max(0.001, sc);
+        h.old_off[2 * b] = m.old_offset_x;
+        h.old_off[2 * b + 1] = m.old_offset_y;
+        h.prior_off[2 * b] = m.prior_offset_x;
+        h.prior_off[2 * b + 1] = m.prior_offset_y;
+    }
+    h.total_entries = d2;
+}
+
+static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, const float *fctf, int on_device,
+                     const rln_particle_meta *meta, rln_particle_result *results, rln_wsum *wsum) {
+    const rln_estep_config &cfg = e.cfg;
+    const int K = e.K, NT = e.NT;
+    hipStream_t st = e.stream;
+    ChunkHost h;
+    build_chunk_host(e, B, meta, h);
+    for (int b = 0; b < B; ++b)
+        if (meta[b].scale > 10000. && cfg.do_scale_correction)
+            RLN_FAIL(RLN_ERR_ARG, "rlnMicrographScaleCorrection is very high (particle %d)", b);
+    const size_t Pf = e.igf.P, Pc = e.igc.P;
+    HIP_TRY(hipEventRecord(e.ev[kStPrep], st));
+    // inputs
+    const float2 *dX, *dXn;
+    const float *dC;
+    if (on_device) {
+        dX = reinterpret_cast<const float2 *>(fimg);
+        dXn = reinterpret_cast<const float2 *>(fimgn);
+        dC = fctf;
+    } else {
+        RLN_TRY(e.d_fimg.ensure(sizeof(float2) * Pf * B));
+        RLN_TRY(e.d_fctf.ensure(sizeof(float) * Pf * B));
+        HIP_TRY(hipMemcpyAsync(e.d_fimg.ptr, fimg, sizeof(float2) * Pf * B, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(e.d_fctf.ptr, fctf, sizeof(float) * Pf * B, hipMemcpyHostToDevice, st));
+        if (fimgn != fimg) {
+            RLN_TRY(e.d_fimgn.ensure(sizeof(float2) * Pf * B));
+            HIP_TRY(hipMemcpyAsync(e.d_fimgn.ptr, fimgn, sizeof(float2) * Pf * B, hipMemcpyHostToDevice, st));
+            dXn = e.d_fimgn.as<const float2>();
+        } else {
+            dXn = e.d_fimg.as<const float2>();
+        }
+        dX = e.d_fimg.as<const float2>();
+        dC = e.d_fctf.as<const float>();
+    }
+    RLN_TRY(upload(e.d_or_off, h.or_off, st));
+    RLN_TRY(upload(e.d_or_cnt, h.or_cnt, st));
+    RLN_TRY(upload(e.d_or_proj, h.or_proj, st));
+    RLN_TRY(upload(e.d_or_prior, h.or_prior, st));
+    RLN_TRY(upload(e.d_or_frame, h.or_frame, st));
+    RLN_TRY(upload(e.d_or_class, h.or_class, st));
+    RLN_TRY(upload(e.d_or_dir, h.or_dir, st));
+    RLN_TRY(upload(e.d_tr_prior, h.tr_prior, st));
+    RLN_TRY(upload(e.d_tiles, h.tiles, st));
+    std::vector<long long> d2off = h.d2_off;
+    RLN_TRY(upload(e.d_meta_ll, d2off, st));
+    DevBuf &d_d2off = e.d_meta_ll;
+    DevBuf &d_nent_tmp = e.d_nent;  // entries per particle
+    RLN_TRY(upload(d_nent_tmp, h.n_entries, st));
+    std::vector<float> mf(3 * (size_t)B);
+    for (int b = 0; b < B; ++b) {
+        mf[b] = h.xi2half[b];
+        mf[B + b] = h.scale[b];
+        mf[2 * B + b] = h.scale_w[b];
+    }
+    RLN_TRY(upload(e.d_meta_f, mf, st));
+    std::vector<double> md(4 * (size_t)B);
+    for (int b = 0; b < 2 * B; ++b) {
+        md[b] = h.old_off[b];
+        md[2 * B + b] = h.prior_off[b];
+    }
+    RLN_TRY(upload(e.d_meta_d, md, st));
+    const float *d_xi2 = e.d_meta_f.as<const float>();
+    const float *d_scale = d_xi2 + B;
+    const float *d_scale_w = d_xi2 + 2 * B;
+    const double *d_old = e.d_meta_d.as<const double>();
+    const double *d_prior = d_old + 2 * B;
+
+    // coarse: scaled images, shifted images
+    RLN_TRY(e.d_ctfq.ensure(sizeof(float) * Pc * B));
+    RLN_TRY(e.d_yq.ensure(sizeof(float2) * Pc * NT * B));
+    hipLaunchKernelGGL(rlnk::k_prep_coarse, dim3((Pc + rlnk::kBlock - 1) / rlnk::kBlock, NT, B), dim3(rlnk::kBlock), 0,
+                       st, dX, dC, d_scale, e.sf, (int)Pf, e.igc, e.d_q_c.as<const float>(),
+                       e.d_ex_c.as<const float2>(), e.d_ey_c.as<const float2>(), NT, cfg.do_ctf_correction,
+                       e.d_ctfq.as<float>(), e.d_yq.as<float2>());
+    RLN_TRY(launch_check("k_prep_coarse"));
+    HIP_TRY(hipEventRecord(e.ev[kStCoarse], st));
+    RLN_TRY(e.d_diff2.ensure(sizeof(float) * std::max<long long>(1, h.total_entries)));
+    RLN_TRY(e.d_weight.ensure(sizeof(double) * std::max<long long>(1, h.total_entries)));
+    RLN_TRY(e.d_mask.ensure(std::max<long long>(1, h.total_entries)));
+    if (!h.tiles.empty()) {
+        hipLaunchKernelGGL(rlnk::k_coarse_diff2, dim3((unsigned)h.tiles.size(), (NT + rlnk::kCoTT - 1) / rlnk::kCoTT),
+                           dim3(rlnk::kBlock), 0, st, e.d_tiles.as<const rlnk::CoarseTile>(), e.d_or_off.as<const int>(),
+                           e.d_or_cnt.as<const int>(), e.d_or_proj.as<const int>(), e.d_proj_c.as<const float2>(),
+                           e.d_ctfq.as<const float>(), e.d_yq.as<const float2>(), d_xi2, d_d2off.as<const long long>(),
+                           NT, (int)Pc, e.d_diff2.as<float>());
+        RLN_TRY(launch_check("k_coarse_diff2"));
+    }
+    HIP_TRY(hipEventRecord(e.ev[kStCoarseW], st));
+    RLN_TRY(e.d_stats_c.ensure(sizeof(rlnk::PassStats) * B));
+    RLN_TRY(e.d_stats_f.ensure(sizeof(rlnk::PassStats) * B));
+    rlnk::WeightArgs wa{};
+    wa.diff2 = e.d_diff2.as<const float>();
+    wa.d2_off = d_d2off.as<const long long>();
+    wa.n_entries = d_nent_tmp.as<const long long>();
+    wa.or_prior = e.d_or_prior.as<const double>();
+    wa.or_off = e.d_or_off.as<const int>();
+    wa.tr_prior = e.d_tr_prior.as<const double>();
+    wa.NT = NT;
+    wa.fine = 0;
+    wa.adaptive_fraction = cfg.adaptive_fraction;
+    wa.maxsig = cfg.maximum_significants;
+    wa.weight = e.d_weight.as<double>();
+    wa.mask = e.d_mask.as<unsigned char>();
+    wa.stats = e.d_stats_c.as<rlnk::PassStats>();
+    hipLaunchKernelGGL(rlnk::k_weights, dim3(B), dim3(rlnk::kWBlock), 0, st, wa);
+    RLN_TRY(launch_check("k_weights(coarse)"));
+    HIP_TRY(hipEventRecord(e.ev[kStJobs], st));
+    // fine-pass job lists
+    RLN_TRY(e.d_npairs.ensure(sizeof(int) * 3 * B));
+    int *d_np = e.d_npairs.as<int>();
+    int *d_nj = d_np + B;
+    hipLaunchKernelGGL(rlnk::k_count_jobs, dim3(B), dim3(rlnk::kWBlock), 0, st, e.d_mask.as<const unsigned char>(),
+                       d_d2off.as<const long long>(), e.d_or_cnt.as<const int>(), NT, d_np, d_nj);
+    RLN_TRY(launch_check("k_count_jobs"));
+    std::vector<int> counts(2 * B);
+    std::vector<rlnk::PassStats> stc(B);
+    HIP_TRY(hipMemcpyAsync(counts.data(), d_np, sizeof(int) * 2 * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(stc.data(), e.d_stats_c.ptr, sizeof(rlnk::PassStats) * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int b = 0; b < B; ++b)
+        if (stc[b].status != 0) {
+            int code = stc[b].status;
+            RLN_FAIL(code, "coarse pass of particle %d: %s", b,
+                     code == RLN_ERR_SUM_WEIGHT_ZERO ? "zero or NaN sum of weights" : "no significant samples");
+        }
+    std::vector<long long> pair_off(B);
+    std::vector<int> job_off(B);
+    long long n_pairs = 0;
+    int n_jobs = 0;
+    for (int b = 0; b < B; ++b) {
+        pair_off[b] = n_pairs;
+        job_off[b] = n_jobs;
+        n_pairs += counts[b];
+        n_jobs += counts[B + b];
+    }
+    RLN_TRY(upload(e.d_pair_off, pair_off, st));
+    RLN_TRY(upload(e.d_job_off, job_off, st));
+    RLN_TRY(e.d_pairs.ensure(sizeof(int) * std::max<long long>(1, n_pairs)));
+    RLN_TRY(e.d_jobs.ensure(sizeof(rlnk::FineJob) * std::max(1, n_jobs)));
+    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(B), dim3(rlnk::kWBlock), 0, st, e.d_mask.as<const unsigned char>(),
+                       d_d2off.as<const long long>(), e.d_or_cnt.as<const int>(), NT, e.d_pair_off.as<const long long>(),
+                       e.d_job_off.as<const int>(), e.d_pairs.as<int>(), e.d_jobs.as<rlnk::FineJob>());
+    RLN_TRY(launch_check("k_write_jobs"));
+    std::vector<rlnk::FineJob> jobs(n_jobs);
+    if (n_jobs) HIP_TRY(hipMemcpyAsync(jobs.data(), e.d_jobs.ptr, sizeof(rlnk::FineJob) * n_jobs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int> cjob, cfirst;
+    for (int j = 0; j < n_jobs; ++j)
+        for (int f = 0; f < jobs[j].s_count; f += rlnk::kFineTC) {
+            cjob.push_back(j);
+            cfirst.push_back(f);
+        }
+    RLN_TRY(upload(e.d_chunk_job, cjob, st));
+    RLN_TRY(upload(e.d_chunk_first, cfirst, st));
+    HIP_TRY(hipEventRecord(e.ev[kStFine], st));
+    // fine diff2
+    RLN_TRY(e.d_fdiff2.ensure(sizeof(float) * 32 * std::max<long long>(1, n_pairs)));
+    RLN_TRY(e.d_fweight.ensure(sizeof(double) * 32 * std::max<long long>(1, n_pairs)));
+    rlnk::FineArgs fa{};
+    fa.jobs = e.d_jobs.as<const rlnk::FineJob>();
+    fa.chunk_job = e.d_chunk_job.as<const int>();
+    fa.chunk_first = e.d_chunk_first.as<const int>();
+    fa.pairs = e.d_pairs.as<const int>();
+    fa.pair_off = e.d_pair_off.as<const long long>();
+    fa.or_off = e.d_or_off.as<const int>();
+    fa.or_frame = e.d_or_frame.as<const int>();
+    fa.or_class = e.d_or_class.as<const int>();
+    fa.vols = e.d_vols.as<const float2 *const>();
+    fa.vg = e.vg;
+    fa.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
+    fa.ig = e.igf;
+    fa.fimg = dX;
+    fa.fctf = dC;
+    fa.scale = d_scale;
+    fa.q = e.d_q_f.as<const float>();
+    fa.ex = e.d_ex_f.as<const float2>();
+    fa.ey = e.d_ey_f.as<const float2>();
+    fa.tr_ix = e.d_tr_ix.as<const int>();
+    fa.tr_iy = e.d_tr_iy.as<const int>();
+    fa.xi2half = d_xi2;
+    fa.NT = NT;
+    fa.do_ctf = cfg.do_ctf_correction;
+    fa.diff2 = e.d_fdiff2.as<float>();
+    if (!cjob.empty()) {
+        hipLaunchKernelGGL(rlnk::k_fine_diff2, dim3((unsigned)cjob.size()), dim3(rlnk::kBlock), 0, st, fa);
+        RLN_TRY(launch_check("k_fine_diff2"));
+    }
+    HIP_TRY(hipEventRecord(e.ev[kStFineW], st));
+    // fine weights
+    std::vector<long long> fent(B), foff(B);
+    for (int b = 0; b < B; ++b) {
+        fent[b] = 32LL * counts[b];
+        foff[b] = 32LL * pair_off[b];
+    }
+    RLN_TRY(upload(e.d_fine_n, fent, st));
+    DevBuf &d_foff = e.d_foff;
+    RLN_TRY(upload(d_foff, foff, st));
+    rlnk::WeightArgs wf = wa;
+    wf.diff2 = e.d_fdiff2.as<const float>();
+    wf.d2_off = d_foff.as<const long long>();
+    wf.n_entries = e.d_fine_n.as<const long long>();
+    wf.pairs = e.d_pairs.as<const int>();
+    wf.pair_off = e.d_pair_off.as<const long long>();
+    wf.fine = 1;
+    wf.maxsig = 0;
+    wf.weight = e.d_fweight.as<double>();
+    wf.mask = nullptr;
+    wf.stats = e.d_stats_f.as<rlnk::PassStats>();
+    hipLaunchKernelGGL(rlnk::k_weights, dim3(B), dim3(rlnk::kWBlock), 0, st, wf);
+    RLN_TRY(launch_check("k_weights(fine)"));
+    HIP_TRY(hipEventRecord(e.ev[kStWsum], st));
+    // weighted sums and backprojection
+    RLN_TRY(e.d_shell_acc.ensure(sizeof(double) * e.nshell * B));
+    RLN_TRY(e.d_part_acc.ensure(sizeof(double) * 8 * B));
+    RLN_TRY(e.d_pdfc_acc.ensure(sizeof(double) * K * B));
+    HIP_TRY(hipMemsetAsync(e.d_shell_acc.ptr, 0, sizeof(double) * e.nshell * B, st));
+    HIP_TRY(hipMemsetAsync(e.d_part_acc.ptr, 0, sizeof(double) * 8 * B, st));
+    HIP_TRY(hipMemsetAsync(e.d_pdfc_acc.ptr, 0, sizeof(double) * K * B, st));
+    rlnk::WsumArgs ws{};
+    ws.jobs = e.d_jobs.as<const rlnk::FineJob>();
+    ws.pairs = e.d_pairs.as<const int>();
+    ws.pair_off = e.d_pair_off.as<const long long>();
+    ws.or_off = e.d_or_off.as<const int>();
+    ws.or_frame = e.d_or_frame.as<const int>();
+    ws.or_class = e.d_or_class.as<const int>();
+    ws.or_dir = e.d_or_dir.as<const int>();
+    ws.vols = e.d_vols.as<const float2 *const>();
+    ws.vg = e.vg;
+    ws.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
+    ws.ig = e.igf;
+    ws.fimg = dX;
+    ws.fimg_nomask = dXn;
+    ws.fctf = dC;
+    ws.scale_w = d_scale_w;
+    ws.minv_w = e.d_minv_w.as<const float>();
+    ws.mresol = e.d_mresol.as<const int>();
+    ws.xa_shell = (cfg.do_scale_correction && e.d_xa_shell.ptr) ? e.d_xa_shell.as<const unsigned char>() : nullptr;
+    ws.ex = e.d_ex_f.as<const float2>();
+    ws.ey = e.d_ey_f.as<const float2>();
+    ws.tr_ix = e.d_tr_ix.as<const int>();
+    ws.tr_iy = e.d_tr_iy.as<const int>();
+    ws.over_x = e.d_over_x.as<const double>();
+    ws.over_y = e.d_over_y.as<const double>();
+    ws.weight = e.d_fweight.as<const double>();
+    ws.stats = e.d_stats_f.as<const rlnk::PassStats>();
+    ws.old_off = d_old;
+    ws.prior_off = d_prior;
+    ws.pixel_size = cfg.pixel_size;
+    ws.NT = NT;
+    ws.nshell = e.nshell;
+    ws.n_dir = e.grid.n_dir;
+    ws.do_ctf = cfg.do_ctf_correction;
+    ws.do_scale = cfg.do_scale_correction;
+    float *const *bpp = e.d_bp_ptrs.as<float *const>();
+    ws.bp_re = bpp;
+    ws.bp_im = bpp + K;
+    ws.bp_w = bpp + 2 * K;
+    ws.shell_acc = e.d_shell_acc.as<double>();
+    ws.part_acc = e.d_part_acc.as<double>();
+    ws.pdf_class = e.d_pdfc_acc.as<double>();
+    ws.pdf_direction = e.d_pdfd_acc.as<double>();
+    ws.K = K;
+    if (n_jobs) {
+        hipLaunchKernelGGL(rlnk::k_wsum_bp, dim3((unsigned)n_jobs * 8), dim3(rlnk::kBlock), 0, st, ws);
+        RLN_TRY(launch_check("k_wsum_bp"));
+    }
+    HIP_TRY(hipEventRecord(e.ev[kStCount], st));
+    // results
+    std::vector<rlnk::PassStats> stf(B);
+    std::vector<double> shells((size_t)e.nshell * B), pacc(8 * (size_t)B), pdfc((size_t)K * B);
+    HIP_TRY(hipMemcpyAsync(stf.data(), e.d_stats_f.ptr, sizeof(rlnk::PassStats) * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(shells.data(), e.d_shell_acc.ptr, sizeof(double) * shells.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(pacc.data(), e.d_part_acc.ptr, sizeof(double) * pacc.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(pdfc.data(), e.d_pdfc_acc.ptr, sizeof(double) * pdfc.size(), hipMemcpyDeviceToHost, st));
+    // best fine pose per particle: the pair's coarse index and the job's orientation entry
+    std::vector<int> all_pairs(std::max<long long>(1, n_pairs));
+    if (n_pairs)
+        HIP_TRY(hipMemcpyAsync(all_pairs.data(), e.d_pairs.ptr, sizeof(int) * n_pairs, hipMemcpyDeviceToHost, st));
+    if (e.keep) {
+        e.kept.assign(B, Engine::Kept());
+        for (int b = 0; b < B; ++b) {
+            Engine::Kept &kp = e.kept[b];
+            kp.n_coarse = h.n_entries[b];
+            kp.n_dir = h.nd[b];
+            kp.n_psi = h.np[b];
+            kp.cdiff2.resize(kp.n_coarse);
+            kp.cweight.resize(kp.n_coarse);
+            kp.cmask.resize(kp.n_coarse);
+            HIP_TRY(hipMemcpyAsync(kp.cdiff2.data(), e.d_diff2.as<float>() + h.d2_off[b], sizeof(float) * kp.n_coarse,
+                                   hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(kp.cweight.data(), e.d_weight.as<double>() + h.d2_off[b],
+                                   sizeof(double) * kp.n_coarse, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(kp.cmask.data(), e.d_mask.as<unsigned char>() + h.d2_off[b], kp.n_coarse,
+                                   hipMemcpyDeviceToHost, st));
+            const long long nf = 32LL * counts[b];
+            std::vector<int> pr(counts[b]);
+            kp.fdiff2.resize(nf);
+            kp.fweight.resize(nf);
+            kp.f_index.resize(nf);
+            if (counts[b]) {
+                HIP_TRY(hipMemcpyAsync(pr.data(), e.d_pairs.as<int>() + pair_off[b], sizeof(int) * counts[b],
+                                       hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(kp.fdiff2.data(), e.d_fdiff2.as<float>() + foff[b], sizeof(float) * nf,
+                                       hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(kp.fweight.data(), e.d_fweight.as<double>() + foff[b], sizeof(double) * nf,
+                                       hipMemcpyDeviceToHost, st));
+            }
+            HIP_TRY(hipStreamSynchronize(st));
+            for (long long f = 0; f < nf; ++f) kp.f_index[f] = (long long)pr[f >> 5] * 32 + (f & 31);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < kStCount; ++i) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e.ev[i], e.ev[i + 1]);
+        e.stage_ms[i] += ms;
+    }
+    for (int b = 0; b < B; ++b) {
+        const rlnk::PassStats &sf = stf[b];
+        if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
+        rln_particle_result &r = results[b];
+        std::memset(&r, 0, sizeof(r));
+        r.coarse_min_diff2 = stc[b].min_diff2;
+        r.coarse_sum_weight = stc[b].sum_weight;
+        r.coarse_significant_weight = stc[b].threshold;
+        r.nr_significant = (double)stc[b].nr_sign;
+        r.min_diff2 = sf.min_diff2;
+        r.sum_weight = sf.sum_weight;
+        r.significant_weight = sf.threshold;
+        r.nr_fine_evaluated = 32 * counts[b];
+        r.pmax = sf.max_weight / sf.sum_weight;
+        // decode the best pose
+        if (sf.best_index < 0) RLN_FAIL(RLN_ERR_FILTERED_ZERO, "no positive fine weight for particle %d", b);
+        const int c = all_pairs[pair_off[b] + (sf.best_index >> 5)];
+        const int o = c / NT, t = c % NT;
+        const int rr = (int)((sf.best_index >> 2) & 7), u = (int)(sf.best_index & 3);
+        const int ND = h.nd[b], NP = h.np[b];
+        const int k = o / (ND * NP), id = (o / NP) % ND, ip = o % NP;
+        const int gd = h.dir_idx[h.dir_off[b] + id], gp = h.psi_idx[h.psi_off[b] + ip];
+        double r8[8], t8[8], p8[8];
+        e.grid.oversampled_orientations(gd, gp, r8, t8, p8);
+        r.rot = r8[rr];
+        r.tilt = t8[rr];
+        r.psi = p8[rr];
+        r.xoff = meta[b].old_offset_x + e.over_x[t * 4 + u];
+        r.yoff = meta[b].old_offset_y + e.over_y[t * 4 + u];
+        r.iclass = k + 1;
+        r.best_ihidden_over = ((long long)o * NT + t) * 32 + rr * 4 + u;
+        r.wsum_norm_correction = pacc[8 * b + 2];
+        double xa = pacc[8 * b + 3], aa = pacc[8 * b + 4];
+        if (cfg.do_scale_correction) {
+            xa /= meta[b].scale;
+            aa /= meta[b].scale * meta[b].scale;
+        }
+        r.signal_product = xa;
+        r.reference_power = aa;
+        r.dLL = log(sf.sum_weight) - sf.min_diff2 - e.logsigma2;
+        if (wsum) {
+            if (wsum->sigma2_noise)
+                for (int s = 0; s < e.nshell; ++s) wsum->sigma2_noise[s] += shells[(size_t)b * e.nshell + s];
+            if (wsum->pdf_class)
+                for (int kk = 0; kk < K; ++kk) wsum->pdf_class[kk] += pdfc[(size_t)b * K + kk];
+            wsum->sumw_group += pacc[8 * b + 0];
+            wsum->sigma2_offset += pacc[8 * b + 1];
+            wsum->signal_product += xa;
+            wsum->reference_power += aa;
+            wsum->LL += r.dLL;
+            wsum->ave_Pmax += r.pmax;
+        }
+    }
+    return RLN_OK;
+}
+
+}  // namespace rln
+
+struct rln_estep : rln::Engine {};
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------------
+extern "C" const char *rln_last_error(void) { return rln::last_error(); }
+
+extern "C" int rln_projector_create(int device, rln_projector **out) {
+    if (!out) RLN_FAIL(RLN_ERR_ARG, "rln_projector_create: null output");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) RLN_FAIL(RLN_ERR_ARG, "rln_projector_create: no HIP device %d", device);
+    rln_projector *p = new rln_projector();
+    p->device = device;
+    *out = p;
+    return RLN_OK;
+}
+
+extern "C" int rln_projector_set_mdl_dim(rln_projector *p, int xdim, int ydim, int zdim, int inity, int initz,
+                                         int maxr, double padding_factor, int *changed) {
+    if (!p) RLN_FAIL(RLN_ERR_ARG, "null projector");
+    rln::MdlDims d{xdim, ydim, zdim, inity, initz, maxr, padding_factor};
+    RLN_TRY(rln::check_dims(d));
+    const bool same = d == p->dims && p->data.ptr;
+    if (changed) *changed = same ? 0 : 1;
+    if (same) return RLN_OK;
+    HIP_TRY(hipSetDevice(p->device));
+    p->data.release();
+    p->dims = d;
+    RLN_TRY(p->data.ensure(sizeof(float2) * d.voxels()));
+    p->version++;
+    return RLN_OK;
+}
+
+extern "C" int rln_projector_init_mdl(rln_projector *p, const float *real, const float *imag) {
+    if (!p || !p->data.ptr || !real || !imag) RLN_FAIL(RLN_ERR_STATE, "projector not dimensioned or null data");
+    const size_t n = p->dims.voxels();
+    std::vector<float2> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = make_float2(real[i], imag[i]);
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipMemcpy(p->data.ptr, tmp.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    p->version++;
+    return RLN_OK;
+}
+
+extern "C" int rln_projector_init_mdl_complex(rln_projector *p, const double *data) {
+    if (!p || !p->data.ptr || !data) RLN_FAIL(RLN_ERR_STATE, "projector not dimensioned or null data");
+    const size_t n = p->dims.voxels();
+    std::vector<float2> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = make_float2((float)data[2 * i], (float)data[2 * i + 1]);
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipMemcpy(p->data.ptr, tmp.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    p->version++;
+    return RLN_OK;
+}
+
+extern "C" int rln_projector_destroy(rln_projector *p) {
+    if (!p) return RLN_OK;
+    (void)hipSetDevice(p->device);
+    p->data.release();
+    delete p;
+    return RLN_OK;
+}
+
+extern "C" int rln_backprojector_create(int device, rln_backprojector **out) {
+    if (!out) RLN_FAIL(RLN_ERR_ARG, "rln_backprojector_create: null output");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) RLN_FAIL(RLN_ERR_ARG, "rln_backprojector_create: no HIP device %d", device);
+    rln_backprojector *b = new rln_backprojector();
+    b->device = device;
+    *out = b;
+    return RLN_OK;
+}
+
+extern "C" int rln_backprojector_set_mdl_dim(rln_backprojector *b, int xdim, int ydim, int zdim, int inity, int initz,
+                                             int maxr, double padding_factor, size_t *bytes) {
+    if (!b) RLN_FAIL(RLN_ERR_ARG, "null backprojector");
+    rln::MdlDims d{xdim, ydim, zdim, inity, initz, maxr, padding_factor};
+    RLN_TRY(rln::check_dims(d));
+    HIP_TRY(hipSetDevice(b->device));
+    if (!(d == b->dims) || !b->re.ptr) {
+        b->re.release();
+        b->im.release();
+        b->w.release();
+        b->dims = d;
+        RLN_TRY(b->re.ensure(sizeof(float) * d.voxels()));
+        RLN_TRY(b->im.ensure(sizeof(float) * d.voxels()));
+        RLN_TRY(b->w.ensure(sizeof(float) * d.voxels()));
+    }
+    if (bytes) *bytes = 3 * sizeof(float) * d.voxels();
+    return RLN_OK;
+}
+
+extern "C" int rln_backprojector_init_mdl(rln_backprojector *b) {
+    if (!b || !b->re.ptr) RLN_FAIL(RLN_ERR_STATE, "backprojector not dimensioned");
+    HIP_TRY(hipSetDevice(b->device));
+    const size_t n = b->dims.voxels() * sizeof(float);
+    HIP_TRY(hipMemset(b->re.ptr, 0, n));
+    HIP_TRY(hipMemset(b->im.ptr, 0, n));
+    HIP_TRY(hipMemset(b->w.ptr, 0, n));
+    return RLN_OK;
+}
+
+extern "C" int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real, float *imag, float *weight) {
+    if (!b || !b->re.ptr) RLN_FAIL(RLN_ERR_STATE, "backprojector not dimensioned");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipDeviceSynchronize());
+    const size_t n = b->dims.voxels() * sizeof(float);
+    if (real) HIP_TRY(hipMemcpy(real, b->re.ptr, n, hipMemcpyDeviceToHost));
+    if (imag) HIP_TRY(hipMemcpy(imag, b->im.ptr, n, hipMemcpyDeviceToHost));
+    if (weight) HIP_TRY(hipMemcpy(weight, b->w.ptr, n, hipMemcpyDeviceToHost));
+    return RLN_OK;
+}
+
+extern "C" int rln_backprojector_destroy(rln_backprojector *b) {
+    if (!b) return RLN_OK;
+    (void)hipSetDevice(b->device);
+    b->re.release();
+    b->im.release();
+    b->w.release();
+    delete b;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_create(int device, const rln_estep_config *cfg, rln_estep **out) {
+    if (!cfg || !out) RLN_FAIL(RLN_ERR_ARG, "rln_estep_create: null argument");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) RLN_FAIL(RLN_ERR_ARG, "rln_estep_create: no HIP device %d", device);
+    rln_estep *e = new rln_estep();
+    e->device = device;
+    const int rc = rln::engine_init(*e, *cfg);
+    if (rc != RLN_OK) {
+        std::string msg = rln::last_error();
+        rln_estep_destroy(e);
+        rln::set_error("%s", msg.c_str());
+        return rc;
+    }
+    *out = e;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_destroy(rln_estep *e) {
+    if (!e) return RLN_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    rln::DevBuf *bufs[] = {&e->d_frames_c, &e->d_frames_f, &e->d_proj_c, &e->d_q_c, &e->d_q_f, &e->d_minv_w,
+                           &e->d_mresol, &e->d_xa_shell, &e->d_ex_c, &e->d_ey_c, &e->d_ex_f, &e->d_ey_f,
+                           &e->d_tr_ix, &e->d_tr_iy, &e->d_over_x, &e->d_over_y, &e->d_vols, &e->d_bp_ptrs,
+                           &e->d_fimg, &e->d_fimgn, &e->d_fctf, &e->d_meta_f, &e->d_meta_d, &e->d_meta_ll,
+                           &e->d_or_off, &e->d_or_cnt, &e->d_or_proj, &e->d_or_prior, &e->d_or_frame,
+                           &e->d_or_class, &e->d_or_dir, &e->d_tr_prior, &e->d_ctfq, &e->d_yq, &e->d_tiles,
+                           &e->d_diff2, &e->d_weight, &e->d_mask, &e->d_stats_c, &e->d_stats_f, &e->d_npairs,
+                           &e->d_njobs, &e->d_nchunks, &e->d_pair_off, &e->d_job_off, &e->d_chunk_off, &e->d_pairs,
+                           &e->d_jobs, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fweight,
+                           &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc};
+    for (rln::DevBuf *b : bufs) b->release();
+    for (int i = 0; i <= rln::kStCount; ++i)
+        if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_set_model(rln_estep *e, rln_projector *const *projectors,
+                                   rln_backprojector *const *backprojectors, const double *pdf_class,
+                                   const double *pdf_direction, const double *sigma2_noise,
+                                   const double *data_vs_prior) {
+    if (!e || !projectors || !backprojectors || !pdf_class || !sigma2_noise)
+        RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_model: null argument");
+    if (!e->cfg.local_search && !pdf_direction) RLN_FAIL(RLN_ERR_ARG, "pdf_direction required for global searches");
+    std::lock_guard<std::mutex> lk(e->mtx);
+    const int K = e->cfg.nr_classes;
+    const int r_max = std::min(e->cfg.current_size / 2, e->cfg.ori_size / 2);
+    const int pad_size = 2 * (rln::ROUND(e->cfg.padding_factor * r_max) + 1) + 1;
+    for (int k = 0; k < K; ++k) {
+        const rln::MdlDims &d = projectors[k]->dims;
+        if (!projectors[k]->data.ptr || d.xdim != pad_size / 2 + 1 || d.ydim != pad_size || d.zdim != pad_size ||
+            d.inity != -(pad_size / 2) || d.initz != -(pad_size / 2))
+            RLN_FAIL(RLN_ERR_ARG, "projector %d dims do not match current_size %d, padding %g", k, e->cfg.current_size,
+                     e->cfg.padding_factor);
+        const rln::MdlDims &bd = backprojectors[k]->dims;
+        if (!backprojectors[k]->re.ptr || !(bd == d))
+            RLN_FAIL(RLN_ERR_ARG, "backprojector %d dims do not match the projector", k);
+        if (projectors[k]->device != e->device || backprojectors[k]->device != e->device)
+            RLN_FAIL(RLN_ERR_ARG, "projector/backprojector %d on another device", k);
+    }
+    e->K = K;
+    e->proj.assign(projectors, projectors + K);
+    e->bproj.assign(backprojectors, backprojectors + K);
+    e->proj_version.assign(K, ~0ULL);
+    e->pdf_class.assign(pdf_class, pdf_class + K);
+    const int nd = e->grid.n_dir;
+    if (pdf_direction)
+        e->pdf_direction.assign(pdf_direction, pdf_direction + (size_t)K * nd);
+    else
+        e->pdf_direction.assign((size_t)K * nd, 1.);
+    e->sigma2_noise.assign(sigma2_noise, sigma2_noise + e->nshell);
+    if (data_vs_prior)
+        e->data_vs_prior.assign(data_vs_prior, data_vs_prior + (size_t)K * e->nshell);
+    else
+        e->data_vs_prior.clear();
+    const rln::MdlDims &d = projectors[0]->dims;
+    e->vg = rlnk::VolGeom{d.xdim, d.ydim, d.zdim, d.inity, d.initz};
+    HIP_TRY(hipSetDevice(e->device));
+    RLN_TRY(rln::engine_model_tables(*e));
+    e->coarse_valid = false;
+    e->model_set = true;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float *fimg_nomask, const float *fctf,
+                             int inputs_on_device, const rln_particle_meta *meta, rln_particle_result *results,
+                             rln_wsum *wsum) {
+    if (!e || n < 0 || (n > 0 && (!fimg || !fctf || !meta || !results)))
+        RLN_FAIL(RLN_ERR_ARG, "rln_estep_run: null argument");
+    std::lock_guard<std::mutex> lk(e->mtx);
+    if (!e->model_set) RLN_FAIL(RLN_ERR_STATE, "rln_estep_run: model not set");
+    if (!fimg_nomask) fimg_nomask = fimg;
+    HIP_TRY(hipSetDevice(e->device));
+    for (int i = 0; i < rln::kStCount; ++i) e->stage_ms[i] = 0.;
+    HIP_TRY(hipMemsetAsync(e->d_pdfd_acc.ptr, 0, sizeof(double) * (size_t)e->K * e->grid.n_dir, e->stream));
+    RLN_TRY(rln::engine_coarse_projections(*e));
+    // chunk size from a device-memory budget for the coarse intermediates
+    const size_t Pc = e->igc.P, Pf = e->igf.P;
+    long long max_entries = 1;
+    if (e->cfg.local_search) {
+        max_entries = (long long)e->K * e->n_or * e->NT / 8;  // local searches: a fraction of the grid
+    } else {
+        max_entries = (long long)e->K * e->n_or * e->NT;
+    }
+    const double per_particle = 13.0 * (double)max_entries + 8.0 * Pc * e->NT + 20.0 * Pf + 64.0 * 1024;
+    const double budget = 24.0e9;
+    int chunk = (int)std::max(1.0, std::min(4096.0, budget / per_particle));
+    if (e->keep) chunk = std::max(chunk, n);
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+        const int B = std::min(chunk, n - i0);
+        const size_t off_c = (size_t)i0 * Pf;
+        RLN_TRY(rln::run_chunk(*e, B, fimg + 2 * off_c, fimg_nomask + 2 * off_c, fctf + off_c, inputs_on_device,
+                               meta + i0, results + i0, wsum));
+    }
+    e->have_timing = true;
+    if (wsum && wsum->pdf_direction) {
+        std::vector<double> pd((size_t)e->K * e->grid.n_dir);
+        HIP_TRY(hipMemcpyAsync(pd.data(), e->d_pdfd_acc.ptr, sizeof(double) * pd.size(), hipMemcpyDeviceToHost,
+                               e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        for (size_t i = 0; i < pd.size(); ++i) wsum->pdf_direction[i] += pd[i];
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_keep_intermediates(rln_estep *e, int keep) {
+    if (!e) RLN_FAIL(RLN_ERR_ARG, "null engine");
+    e->keep = keep != 0;
+    if (!e->keep) e->kept.clear();
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_coarse_size(rln_estep *e, int ip, long long *n_coarse, int *n_dir, int *n_psi,
+                                     int *n_trans) {
+    if (!e || ip < 0 || ip >= (int)e->kept.size()) RLN_FAIL(RLN_ERR_STATE, "no kept intermediates for particle %d", ip);
+    const rln::Engine::Kept &k = e->kept[ip];
+    if (n_coarse) *n_coarse = k.n_coarse;
+    if (n_dir) *n_dir = k.n_dir;
+    if (n_psi) *n_psi = k.n_psi;
+    if (n_trans) *n_trans = e->NT;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_get_coarse(rln_estep *e, int ip, float *diff2, double *weight, unsigned char *significant) {
+    if (!e || ip < 0 || ip >= (int)e->kept.size()) RLN_FAIL(RLN_ERR_STATE, "no kept intermediates for particle %d", ip);
+    const rln::Engine::Kept &k = e->kept[ip];
+    if (diff2) std::copy(k.cdiff2.begin(), k.cdiff2.end(), diff2);
+    if (weight) std::copy(k.cweight.begin(), k.cweight.end(), weight);
+    if (significant) std::copy(k.cmask.begin(), k.cmask.end(), significant);
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_fine_size(rln_estep *e, int ip, long long *n_fine) {
+    if (!e || ip < 0 || ip >= (int)e->kept.size()) RLN_FAIL(RLN_ERR_STATE, "no kept intermediates for particle %d", ip);
+    *n_fine = (long long)e->kept[ip].fdiff2.size();
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *diff2, double *weight) {
+    if (!e || ip < 0 || ip >= (int)e->kept.size()) RLN_FAIL(RLN_ERR_STATE, "no kept intermediates for particle %d", ip);
+    const rln::Engine::Kept &k = e->kept[ip];
+    if (ihidden_over) std::copy(k.f_index.begin(), k.f_index.end(), ihidden_over);
+    if (diff2) std::copy(k.fdiff2.begin(), k.fdiff2.end(), diff2);
+    if (weight) std::copy(k.fweight.begin(), k.fweight.end(), weight);
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out) {
+    if (!e || !ms) RLN_FAIL(RLN_ERR_ARG, "null argument");
+    const int n = std::min<int>(n_max, rln::kStCount);
+    for (int i = 0; i < n; ++i) ms[i] = e->stage_ms[i];
+    if (n_out) *n_out = n;
+    return RLN_OK;
+}

@@ -1,0 +1,989 @@
+// Device kernels of the plain-FP32 E-step for CDNA4 (gfx950). Included by estep.hip only.
+//
+// Image geometry: an s x (s/2+1) half-complex image, pixel p = i*(s/2+1) + x, frequency
+// (x, y) with y = i for i <= s/2 else i - s (FOR_ALL_ELEMENTS_IN_FFTW_TRANSFORM, src/fftw.h:99-102).
+// Volume geometry: projector layout of src/projector.cpp:59-89, interleaved float2 [z][y][x].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rlnk {
+
+constexpr int kBlock = 256;
+
+// Geometry of one image size, precomputed on the host in double (exact integer tests).
+struct ImgGeom {
+    int s, xs, P;
+    int proj_r2_max;  // pixels projected: x^2+y^2 <= proj_r2_max (Projector::project circle and r_max test)
+    int bp_r2_max;    // pixels backprojected: x^2+y^2 <= bp_r2_max (backproject2Dto3D ellipse/r_max test)
+};
+
+struct VolGeom {
+    int xdim, ydim, zdim, starty, startz;
+};
+
+// Per-orientation projection frame: 3-D coordinates of pixel (x, y) are x*e0 + y*e1 (padding included),
+// e0 = pad * row 0 of A, e1 = pad * row 1 of A (Ainv = A^T in Projector::project, src/projector.cpp:638-640).
+struct Frame {
+    float e0x, e0y, e0z, e1x, e1y, e1z;
+};
+
+__device__ __forceinline__ void pixel_xy(int p, int s, int xs, int &x, int &y) {
+    const int i = p / xs;
+    x = p - i * xs;
+    y = (i < xs) ? i : i - s;
+}
+
+// Trilinear gather of the padded reference at (xp, yp, zp), Hermitian flip for xp < 0
+// (Projector::project TRILINEAR branch, src/projector.cpp:676-744). Out-of-box samples give 0.
+__device__ __forceinline__ float2 sample_volume(const float2 *__restrict__ vol, const VolGeom &g, float xp, float yp,
+                                                float zp) {
+    const bool neg = xp < 0.f;
+    if (neg) {
+        xp = -xp;
+        yp = -yp;
+        zp = -zp;
+    }
+    const float fx0 = floorf(xp), fy0 = floorf(yp), fz0 = floorf(zp);
+    const float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
+    const int x0 = (int)fx0;
+    const int y0 = (int)fy0 - g.starty;
+    const int z0 = (int)fz0 - g.startz;
+    if (x0 < 0 || x0 + 1 >= g.xdim || y0 < 0 || y0 + 1 >= g.ydim || z0 < 0 || z0 + 1 >= g.zdim)
+        return make_float2(0.f, 0.f);
+    const size_t row = (size_t)g.xdim;
+    const size_t slab = row * (size_t)g.ydim;
+    const float2 *b = vol + (size_t)z0 * slab + (size_t)y0 * row + x0;
+    const float2 d000 = b[0], d001 = b[1];
+    const float2 d010 = b[row], d011 = b[row + 1];
+    const float2 d100 = b[slab], d101 = b[slab + 1];
+    const float2 d110 = b[slab + row], d111 = b[slab + row + 1];
+    float2 r;
+    {
+        const float dx00 = d000.x + (d001.x - d000.x) * fx, dx01 = d100.x + (d101.x - d100.x) * fx;
+        const float dx10 = d010.x + (d011.x - d010.x) * fx, dx11 = d110.x + (d111.x - d110.x) * fx;
+        const float dxy0 = dx00 + (dx10 - dx00) * fy, dxy1 = dx01 + (dx11 - dx01) * fy;
+        r.x = dxy0 + (dxy1 - dxy0) * fz;
+    }
+    {
+        const float dx00 = d000.y + (d001.y - d000.y) * fx, dx01 = d100.y + (d101.y - d100.y) * fx;
+        const float dx10 = d010.y + (d011.y - d010.y) * fx, dx11 = d110.y + (d111.y - d110.y) * fx;
+        const float dxy0 = dx00 + (dx10 - dx00) * fy, dxy1 = dx01 + (dx11 - dx01) * fy;
+        r.y = dxy0 + (dxy1 - dxy0) * fz;
+    }
+    if (neg) r.y = -r.y;
+    return r;
+}
+
+__device__ __forceinline__ float2 project_pixel(const float2 *__restrict__ vol, const VolGeom &g, const Frame &f,
+                                                int x, int y) {
+    const float fxp = (float)x, fyp = (float)y;
+    return sample_volume(vol, g, f.e0x * fxp + f.e1x * fyp, f.e0y * fxp + f.e1y * fyp, f.e0z * fxp + f.e1z * fyp);
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// --------------------------------------------------------------------------------------------------
+// Coarse pass
+// --------------------------------------------------------------------------------------------------
+
+// Projections of every coarse orientation of every class at the coarse size:
+// proj[(k*n_or + o)*P + p] (class-major). One thread per pixel, grid (K*n_or, ceil(P/256)).
+__global__ void __launch_bounds__(kBlock) k_project_grid(const float2 *const *__restrict__ vols, VolGeom vg,
+                                                         const Frame *__restrict__ frames, int n_or, ImgGeom ig,
+                                                         float2 *__restrict__ proj) {
+    const int p = blockIdx.y * kBlock + threadIdx.x;
+    const int ko = blockIdx.x;
+    if (p >= ig.P) return;
+    const int k = ko / n_or, o = ko - k * n_or;
+    int x, y;
+    pixel_xy(p, ig.s, ig.xs, x, y);
+    float2 v = make_float2(0.f, 0.f);
+    if (x * x + y * y <= ig.proj_r2_max) v = project_pixel(vols[k], vg, frames[o], x, y);
+    proj[(size_t)ko * ig.P + p] = v;
+}
+
+// Windowed, sqrt(Minvsigma2/2)-scaled images of the coarse pass for B particles:
+//   ctfq[b][p] = q_p * ctf_b(p) * scale_b  and  yq[b][t][p] = q_p * X_b(p) * exp(-2 pi i (x tx + y ty)/N),
+// with the phase factored as ex[t][x] * ey[t][i]. X and ctf are read from the fine-size input (window of
+// src/fftw.h:809-857). grid (ceil(P/256), NT, B).
+__global__ void __launch_bounds__(kBlock) k_prep_coarse(const float2 *__restrict__ fimg, const float *__restrict__ fctf,
+                                                        const float *__restrict__ scale, int sf, int Pf, ImgGeom ig,
+                                                        const float *__restrict__ q, const float2 *__restrict__ ex,
+                                                        const float2 *__restrict__ ey, int NT, int do_ctf,
+                                                        float *__restrict__ ctfq, float2 *__restrict__ yq) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int t = blockIdx.y, b = blockIdx.z;
+    if (p >= ig.P) return;
+    const int i = p / ig.xs, x = p - i * ig.xs;
+    const int y = (i < ig.xs) ? i : i - ig.s;
+    const int isrc = (y < 0) ? y + sf : y;
+    const size_t src = (size_t)b * Pf + (size_t)isrc * (sf / 2 + 1) + x;
+    const float qp = q[p];
+    const float2 X = fimg[src];
+    const float2 ph = cmul(ex[(size_t)t * ig.xs + x], ey[(size_t)t * ig.s + i]);
+    const float2 v = cmul(make_float2(X.x * qp, X.y * qp), ph);
+    yq[((size_t)b * NT + t) * ig.P + p] = v;
+    if (t == 0) ctfq[(size_t)b * ig.P + p] = qp * (do_ctf ? fctf[src] : 1.f) * scale[b];
+}
+
+// Tile of the coarse squared differences: TO orientations x TT translations of one particle.
+constexpr int kCoTO = 64, kCoTT = 64, kCoCP = 32;
+
+struct CoarseTile {
+    int b;        // particle within the batch
+    int o_begin;  // first orientation (within the particle's list)
+};
+
+// diff2[b][o][t] = highres_Xi2/2 + sum_p |ctfq_b(p) * P_o(p) - yq_b,t(p)|^2 (getAllSquaredDifferences,
+// src/ml_optimiser.cpp:7430-7442). Orientations whose prior or class prior is zero are marked -999 (never
+// computed, as in the reference's exp_Mweight initialisation, :7155). grid (n_tiles, ceil(NT/TT)).
+__global__ void __launch_bounds__(kBlock) k_coarse_diff2(const CoarseTile *__restrict__ tiles,
+                                                         const int *__restrict__ or_off, const int *__restrict__ or_cnt,
+                                                         const int *__restrict__ or_proj, const float2 *__restrict__ proj,
+                                                         const float *__restrict__ ctfq, const float2 *__restrict__ yq,
+                                                         const float *__restrict__ xi2half, const long long *__restrict__ d2_off,
+                                                         int NT, int P, float *__restrict__ diff2) {
+    __shared__ float2 As[kCoCP][kCoTO];
+    __shared__ float2 Ys[kCoCP][kCoTT];
+    __shared__ int og_s[kCoTO];
+    const CoarseTile tile = tiles[blockIdx.x];
+    const int b = tile.b;
+    const int n_o = or_cnt[b];
+    const int t_begin = blockIdx.y * kCoTT;
+    const int tid = threadIdx.x;
+    const int to = tid >> 4, tt = tid & 15;
+    if (tid < kCoTO) {
+        const int o = tile.o_begin + tid;
+        og_s[tid] = (o < n_o) ? or_proj[or_off[b] + o] : -1;
+    }
+    __syncthreads();
+    const float *cq = ctfq + (size_t)b * P;
+    const float2 *yb = yq + (size_t)b * NT * P;
+    float tot[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tot[i][j] = 0.f;
+    for (int p0 = 0; p0 < P; p0 += kCoCP) {
+        // stage c*P for TO orientations (coalesced along p)
+        for (int idx = tid; idx < kCoCP * kCoTO; idx += kBlock) {
+            const int oo = idx / kCoCP, pp = idx - oo * kCoCP;
+            const int p = p0 + pp;
+            float2 v = make_float2(0.f, 0.f);
+            const int og = og_s[oo];
+            if (og >= 0 && p < P) {
+                const float2 pr = proj[(size_t)og * P + p];
+                const float c = cq[p];
+                v = make_float2(c * pr.x, c * pr.y);
+            }
+            As[pp][oo] = v;
+        }
+        for (int idx = tid; idx < kCoCP * kCoTT; idx += kBlock) {
+            const int tq = idx / kCoCP, pp = idx - tq * kCoCP;
+            const int p = p0 + pp, t = t_begin + tq;
+            Ys[pp][tq] = (t < NT && p < P) ? yb[(size_t)t * P + p] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        float acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+#pragma unroll 4
+        for (int pp = 0; pp < kCoCP; ++pp) {
+            float2 a[4], y[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[pp][to + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) y[j] = Ys[pp][tt + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float dr = a[i].x - y[j].x, di = a[i].y - y[j].y;
+                    acc[i][j] = fmaf(dr, dr, acc[i][j]);
+                    acc[i][j] = fmaf(di, di, acc[i][j]);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) tot[i][j] += acc[i][j];
+        __syncthreads();
+    }
+    const float base = xi2half[b];
+    float *out = diff2 + d2_off[b];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int oo = to + 16 * i;
+        const int o = tile.o_begin + oo;
+        if (o >= n_o) continue;
+        const bool skip = og_s[oo] < 0 || or_proj[or_off[b] + o] < 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int t = t_begin + tt + 16 * j;
+            if (t < NT) out[(size_t)o * NT + t] = skip ? -999.f : base + tot[i][j];
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
+// Weights and significance (convertAllSquaredDifferencesToWeights, src/ml_optimiser.cpp:7704-8239)
+// --------------------------------------------------------------------------------------------------
+
+struct PassStats {
+    double min_diff2;
+    double sum_weight;
+    double threshold;      // exp_significant_weight
+    double max_weight;
+    long long nr_sign;     // my_nr_significant_coarse_samples
+    long long n_above;     // entries with weight >= threshold
+    long long best_key;    // tie-break key of the max weight (wsum loop order)
+    long long best_index;  // entry index of the max weight
+    int status;            // 0 ok, RLN_ERR_* otherwise
+    int pad_;
+};
+
+constexpr int kWBlock = 1024;
+
+__device__ __forceinline__ double block_sum_d(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.;
+    if (threadIdx.x < 64) {
+        t = (threadIdx.x < (blockDim.x >> 6)) ? red[threadIdx.x] : 0.;
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if (threadIdx.x == 0) red[0] = t;
+    }
+    __syncthreads();
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ float block_min_f(float v, float *red) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float t = (threadIdx.x < (blockDim.x >> 6)) ? red[threadIdx.x] : 3.0e38f;
+        for (int o = 32; o > 0; o >>= 1) t = fminf(t, __shfl_xor(t, o));
+        if (threadIdx.x == 0) red[0] = t;
+    }
+    __syncthreads();
+    const float r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ long long block_sum_ll(long long v, long long *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        long long t = (threadIdx.x < (blockDim.x >> 6)) ? red[threadIdx.x] : 0;
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if (threadIdx.x == 0) red[0] = t;
+    }
+    __syncthreads();
+    const long long r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Mode of k_weights: coarse entries are e = o*NT + t; fine entries are e = s*32 + r*4 + u where pair s
+// holds the coarse index (o*NT + t) in pairs[].
+struct WeightArgs {
+    const float *diff2;          // per particle at d2_off[b]
+    const long long *d2_off;
+    const long long *n_entries;  // per particle
+    const double *or_prior;      // normalised pdf_orientation per orientation entry (or_off based)
+    const int *or_off;
+    const double *tr_prior;      // normalised pdf_offset [b][NT]
+    const int *pairs;            // fine mode: coarse index per pair, at pair_off[b]
+    const long long *pair_off;
+    int NT;
+    int fine;                    // 0 coarse, 1 fine
+    double adaptive_fraction;
+    long long maxsig;            // <= 0: none (coarse pass only)
+    double *weight;              // out, same indexing as diff2
+    unsigned char *mask;         // out (coarse mode), weight >= threshold
+    PassStats *stats;            // out [b]
+};
+
+__device__ __forceinline__ double entry_prior(const WeightArgs &a, int b, long long e) {
+    long long c;
+    if (a.fine) {
+        c = a.pairs[a.pair_off[b] + (e >> 5)];
+    } else {
+        c = e;
+    }
+    const long long o = c / a.NT, t = c - o * a.NT;
+    return a.or_prior[a.or_off[b] + o] * a.tr_prior[(size_t)b * a.NT + t];
+}
+
+// Key that orders entries like storeWeightedSums visits them (class, direction, psi, iover_rot, itrans,
+// iover_trans; src/ml_optimiser.cpp:8329-8470), used to break ties of the max weight as the reference does.
+__device__ __forceinline__ long long wsum_order_key(const WeightArgs &a, int b, long long e) {
+    if (!a.fine) return e;
+    const long long s = e >> 5;
+    const int r = (int)((e >> 2) & 7), u = (int)(e & 3);
+    const long long c = a.pairs[a.pair_off[b] + s];
+    const long long o = c / a.NT, t = c - o * a.NT;
+    return ((o * 8 + r) * a.NT + t) * 4 + u;
+}
+
+// One 1024-thread block per particle: min diff2, weights in FP64, their sum, the significance threshold by
+// radix selection over the weights' bit patterns (positive doubles order like their uint64 patterns), the
+// coarse significance mask and the max weight. The threshold is the weight at which the descending
+// cumulative sum first exceeds adaptive_fraction * sum (src/ml_optimiser.cpp:8145-8168); maxsig caps the
+// count as in :8147-8157.
+__global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
+    __shared__ double red_d[16];
+    __shared__ float red_f[16];
+    __shared__ long long red_l[16];
+    __shared__ unsigned int hcnt[256];
+    __shared__ double hsum[256];
+    __shared__ unsigned long long s_prefix, s_mask;
+    __shared__ double s_above_sum;
+    __shared__ long long s_above_cnt;
+    __shared__ int s_done;
+    const int b = blockIdx.x;
+    const long long n = a.n_entries[b];
+    const float *d2 = a.diff2 + a.d2_off[b];
+    double *w = a.weight + a.d2_off[b];
+    const int tid = threadIdx.x;
+
+    float mn = 3.0e38f;
+    for (long long e = tid; e < n; e += kWBlock) {
+        const float v = d2[e];
+        if (v >= 0.f) mn = fminf(mn, v);
+    }
+    mn = block_min_f(mn, red_f);
+    const double mind = (double)mn;
+    double sum = 0.;
+    for (long long e = tid; e < n; e += kWBlock) {
+        const float v = d2[e];
+        double wt = 0.;
+        if (v >= 0.f) {
+            const double diff = (double)v - mind;
+            wt = entry_prior(a, b, e);
+            wt = (diff > 700.) ? 0. : wt * exp(-diff);
+        }
+        w[e] = wt;
+        sum += wt;
+    }
+    sum = block_sum_d(sum, red_d);
+    PassStats st;
+    st.min_diff2 = mind;
+    st.sum_weight = sum;
+    st.threshold = 0.;
+    st.max_weight = 0.;
+    st.nr_sign = 0;
+    st.n_above = 0;
+    st.best_key = -1;
+    st.best_index = -1;
+    st.status = 0;
+    st.pad_ = 0;
+    if (!(sum > 0.)) {  // zero or NaN (src/ml_optimiser.cpp:8054-8104)
+        if (tid == 0) {
+            st.status = -6;
+            a.stats[b] = st;
+        }
+        return;
+    }
+
+    // radix selection of the crossing element: 8 digits of 8 bits from the most significant
+    const double target = a.adaptive_fraction * sum;
+    if (tid == 0) {
+        s_prefix = 0ULL;
+        s_mask = 0ULL;
+        s_above_sum = 0.;
+        s_above_cnt = 0;
+        s_done = 0;
+    }
+    __syncthreads();
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 256; i += kWBlock) {
+            hcnt[i] = 0;
+            hsum[i] = 0.;
+        }
+        __syncthreads();
+        const unsigned long long pre = s_prefix, msk = s_mask;
+        for (long long e = tid; e < n; e += kWBlock) {
+            const double v = w[e];
+            if (!(v > 0.)) continue;
+            const unsigned long long key = (unsigned long long)__double_as_longlong(v);
+            if ((key & msk) != pre) continue;
+            const int bin = (int)((key >> shift) & 255ULL);
+            atomicAdd(&hcnt[bin], 1u);
+            atomicAdd(&hsum[bin], v);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double acc = s_above_sum;
+            long long cnt = s_above_cnt;
+            int bin = -1;
+            for (int k = 255; k >= 0; --k) {
+                if (hcnt[k] == 0) continue;
+                if (acc + hsum[k] > target) {
+                    bin = k;
+                    break;
+                }
+                acc += hsum[k];
+                cnt += hcnt[k];
+            }
+            if (bin < 0) {  // rounding: the total never exceeds the target; take the smallest present bin
+                for (int k = 0; k < 256; ++k)
+                    if (hcnt[k]) {
+                        bin = k;
+                        break;
+                    }
+                // undo its accumulation
+                acc -= hsum[bin];
+                cnt -= hcnt[bin];
+            }
+            s_above_sum = acc;
+            s_above_cnt = cnt;
+            s_prefix |= ((unsigned long long)bin << shift);
+            s_mask |= (255ULL << shift);
+        }
+        __syncthreads();
+    }
+    double thr = __longlong_as_double((long long)s_prefix);
+    // elements equal to thr needed to cross the target
+    long long nr_sign = 0;
+    {
+        const double rem = target - s_above_sum;
+        long long m = (long long)floor(rem / thr) + 1;
+        if (m < 1) m = 1;
+        nr_sign = s_above_cnt + m;
+    }
+    if (a.maxsig > 0 && !a.fine && nr_sign > a.maxsig) {
+        // rank selection of the maxsig-th largest weight
+        if (tid == 0) {
+            s_prefix = 0ULL;
+            s_mask = 0ULL;
+            s_above_cnt = 0;
+        }
+        __syncthreads();
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            for (int i = tid; i < 256; i += kWBlock) hcnt[i] = 0;
+            __syncthreads();
+            const unsigned long long pre = s_prefix, msk = s_mask;
+            for (long long e = tid; e < n; e += kWBlock) {
+                const double v = w[e];
+                if (!(v > 0.)) continue;
+                const unsigned long long key = (unsigned long long)__double_as_longlong(v);
+                if ((key & msk) != pre) continue;
+                atomicAdd(&hcnt[(int)((key >> shift) & 255ULL)], 1u);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                long long cnt = s_above_cnt;
+                int bin = 0;
+                for (int k = 255; k >= 0; --k) {
+                    if (cnt + (long long)hcnt[k] >= a.maxsig) {
+                        bin = k;
+                        break;
+                    }
+                    cnt += hcnt[k];
+                }
+                s_above_cnt = cnt;
+                s_prefix |= ((unsigned long long)bin << shift);
+                s_mask |= (255ULL << shift);
+            }
+            __syncthreads();
+        }
+        thr = __longlong_as_double((long long)s_prefix);
+        nr_sign = a.maxsig;
+    }
+    // mask, count above threshold, max weight with the reference's loop-order tie break
+    long long above = 0;
+    double bw = -1.;
+    long long bkey = 0x7fffffffffffffffLL, bidx = -1;
+    for (long long e = tid; e < n; e += kWBlock) {
+        const double v = w[e];
+        const bool sig = v >= thr && v > 0.;
+        if (!a.fine) a.mask[a.d2_off[b] + e] = sig ? 1 : 0;
+        above += sig ? 1 : 0;
+        if (v > 0.) {
+            const long long key = wsum_order_key(a, b, e);
+            if (v > bw || (v == bw && key < bkey)) {
+                bw = v;
+                bkey = key;
+                bidx = e;
+            }
+        }
+    }
+    above = block_sum_ll(above, red_l);
+    // arg-max across the block
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ow = __shfl_xor(bw, o);
+        const long long ok = __shfl_xor(bkey, o), oi = __shfl_xor(bidx, o);
+        if (ow > bw || (ow == bw && ok < bkey)) {
+            bw = ow;
+            bkey = ok;
+            bidx = oi;
+        }
+    }
+    __shared__ double s_bw[16];
+    __shared__ long long s_bk[16], s_bi[16];
+    if ((tid & 63) == 0) {
+        s_bw[tid >> 6] = bw;
+        s_bk[tid >> 6] = bkey;
+        s_bi[tid >> 6] = bidx;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 1; i < (kWBlock >> 6); ++i)
+            if (s_bw[i] > bw || (s_bw[i] == bw && s_bk[i] < bkey)) {
+                bw = s_bw[i];
+                bkey = s_bk[i];
+                bidx = s_bi[i];
+            }
+        st.threshold = thr;
+        st.nr_sign = nr_sign;
+        st.n_above = above;
+        st.max_weight = bw;
+        st.best_key = bkey;
+        st.best_index = bidx;
+        if (!a.fine && nr_sign == 0) st.status = -5;
+        a.stats[b] = st;
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
+// Fine-pass job lists from the coarse mask (the reference builds them on the host:
+// makeJobsForDiff2Fine, src/acc/acc_helper_functions_impl.h:26-102).
+// A job is one coarse orientation with >= 1 significant translation; its pairs are consecutive in
+// ihidden order. pairs[] holds the particle-local coarse index o*NT + t of every significant pair.
+// --------------------------------------------------------------------------------------------------
+
+struct FineJob {
+    int b;          // particle
+    int o;          // orientation entry within the particle's list
+    int s_begin;    // first pair (particle-local)
+    int s_count;    // number of significant translations
+};
+
+__device__ __forceinline__ int block_exclusive_scan_i(int v, int *tmp, int &total) {
+    // wave-level inclusive scan
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+            const int t = tmp[i];
+            tmp[i] = acc;
+            acc += t;
+        }
+        tmp[blockDim.x >> 6] = acc;
+    }
+    __syncthreads();
+    const int r = x - v + tmp[w];
+    total = tmp[blockDim.x >> 6];
+    __syncthreads();
+    return r;
+}
+
+// Counts per particle: significant pairs and jobs. grid (B), block 1024.
+__global__ void __launch_bounds__(kWBlock) k_count_jobs(const unsigned char *__restrict__ mask,
+                                                        const long long *__restrict__ d2_off,
+                                                        const int *__restrict__ or_cnt, int NT,
+                                                        int *__restrict__ n_pairs, int *__restrict__ n_jobs) {
+    __shared__ long long red[16];
+    const int b = blockIdx.x;
+    const unsigned char *m = mask + d2_off[b];
+    long long np = 0, nj = 0;
+    for (int o = threadIdx.x; o < or_cnt[b]; o += kWBlock) {
+        int c = 0;
+        for (int t = 0; t < NT; ++t) c += m[(size_t)o * NT + t];
+        np += c;
+        nj += (c > 0);
+    }
+    np = block_sum_ll(np, red);
+    nj = block_sum_ll(nj, red);
+    if (threadIdx.x == 0) {
+        n_pairs[b] = (int)np;
+        n_jobs[b] = (int)nj;
+    }
+}
+
+// Writes pairs and jobs at the offsets computed from k_count_jobs. grid (B), block 1024.
+__global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__restrict__ mask,
+                                                        const long long *__restrict__ d2_off,
+                                                        const int *__restrict__ or_cnt, int NT,
+                                                        const long long *__restrict__ pair_off,
+                                                        const int *__restrict__ job_off, int *__restrict__ pairs,
+                                                        FineJob *__restrict__ jobs) {
+    __shared__ int tmp[24];
+    const int b = blockIdx.x;
+    const unsigned char *m = mask + d2_off[b];
+    int pair_base = 0, job_base = 0;
+    const int n_o = or_cnt[b];
+    for (int o0 = 0; o0 < n_o; o0 += kWBlock) {
+        const int o = o0 + threadIdx.x;
+        int c = 0;
+        if (o < n_o)
+            for (int t = 0; t < NT; ++t) c += m[(size_t)o * NT + t];
+        int tot_p, tot_j;
+        const int ps = block_exclusive_scan_i(c, tmp, tot_p);
+        const int js = block_exclusive_scan_i(c > 0 ? 1 : 0, tmp, tot_j);
+        if (o < n_o && c > 0) {
+            int s = pair_base + ps;
+            FineJob j;
+            j.b = b;
+            j.o = o;
+            j.s_begin = s;
+            j.s_count = c;
+            jobs[job_off[b] + job_base + js] = j;
+            for (int t = 0; t < NT; ++t)
+                if (m[(size_t)o * NT + t]) pairs[pair_off[b] + (s++)] = o * NT + t;
+        }
+        pair_base += tot_p;
+        job_base += tot_j;
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
+// Fine pass: one block per (job, chunk of kFineTC coarse translations); each thread owns pixels, projects the
+// 8 oversampled orientations once per pixel and accumulates the 8 x 4*kFineTC squared differences.
+// --------------------------------------------------------------------------------------------------
+constexpr int kFineTC = 2;                 // coarse translations per block
+constexpr int kFineNF = 4 * kFineTC;       // fine translations per block
+
+struct FineArgs {
+    const FineJob *jobs;
+    const int *chunk_job;          // job of each block
+    const int *chunk_first;        // first job-local pair of each block
+    const int *pairs;
+    const long long *pair_off;
+    const int *or_off;
+    const int *or_frame;           // fine frame base (coarse orientation * 8) per orientation entry
+    const int *or_class;
+    const float2 *const *vols;
+    VolGeom vg;
+    const Frame *fine_frames;      // [n_dir * n_psi * 8]
+    ImgGeom ig;
+    const float2 *fimg;            // [b][Pf]
+    const float *fctf;
+    const float *scale;
+    const float *q;                // sqrt(Minvsigma2/2) of the fine size
+    const float2 *ex;              // [n_xs][xs] phase factors of the fine x shifts
+    const float2 *ey;              // [n_ys][s]
+    const int *tr_ix;              // [NT*4] x-shift index of fine translation t*4+u
+    const int *tr_iy;
+    const float *xi2half;
+    int NT;
+    int do_ctf;
+    float *diff2;                  // out [pair_off[b]*32 + s*32 + r*4 + u]
+};
+
+__global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
+    __shared__ float red[kBlock / 64][8 * kFineNF];
+    __shared__ int s_tix[kFineNF], s_tiy[kFineNF];
+    __shared__ int s_t[kFineTC];
+    const int job_id = a.chunk_job[blockIdx.x];
+    const int first = a.chunk_first[blockIdx.x];
+    const FineJob job = a.jobs[job_id];
+    const int b = job.b;
+    const int nc = min(kFineTC, job.s_count - first);
+    const int tid = threadIdx.x;
+    if (tid < kFineTC) {
+        const int s = job.s_begin + first + tid;
+        s_t[tid] = (tid < nc) ? a.pairs[a.pair_off[b] + s] % a.NT : 0;
+    }
+    __syncthreads();
+    if (tid < kFineNF) {
+        const int c = tid >> 2, u = tid & 3;
+        const int t = s_t[c];
+        s_tix[tid] = a.tr_ix[t * 4 + u];
+        s_tiy[tid] = a.tr_iy[t * 4 + u];
+    }
+    __syncthreads();
+    const int oe = a.or_off[b] + job.o;
+    const int k = a.or_class[oe];
+    const Frame *fr = a.fine_frames + a.or_frame[oe];
+    const float2 *vol = a.vols[k];
+    const float2 *X = a.fimg + (size_t)b * a.ig.P;
+    const float *C = a.fctf + (size_t)b * a.ig.P;
+    const float sc = a.scale[b];
+    float acc[8][kFineNF];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int j = 0; j < kFineNF; ++j) acc[r][j] = 0.f;
+    for (int p = tid; p < a.ig.P; p += kBlock) {
+        const float qp = a.q[p];
+        if (qp == 0.f) continue;  // Minvsigma2 == 0: no contribution
+        int x, y;
+        pixel_xy(p, a.ig.s, a.ig.xs, x, y);
+        const int i = p / a.ig.xs;
+        const float c = qp * (a.do_ctf ? C[p] : 1.f) * sc;
+        float2 A[8];
+        const bool inside = x * x + y * y <= a.ig.proj_r2_max;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            float2 v = make_float2(0.f, 0.f);
+            if (inside) v = project_pixel(vol, a.vg, fr[r], x, y);
+            A[r] = make_float2(c * v.x, c * v.y);
+        }
+        const float2 Xp = X[p];
+        const float2 Xq = make_float2(qp * Xp.x, qp * Xp.y);
+#pragma unroll
+        for (int j = 0; j < kFineNF; ++j) {
+            const float2 ph = cmul(a.ex[(size_t)s_tix[j] * a.ig.xs + x], a.ey[(size_t)s_tiy[j] * a.ig.s + i]);
+            const float2 Y = cmul(Xq, ph);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const float dr = A[r].x - Y.x, di = A[r].y - Y.y;
+                acc[r][j] = fmaf(dr, dr, acc[r][j]);
+                acc[r][j] = fmaf(di, di, acc[r][j]);
+            }
+        }
+    }
+    // block reduction: wave shuffles, then across the 4 waves
+    const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int j = 0; j < kFineNF; ++j) {
+            float v = acc[r][j];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0) red[w][r * kFineNF + j] = v;
+        }
+    __syncthreads();
+    if (tid < 8 * kFineNF) {
+        const int r = tid / kFineNF, j = tid - r * kFineNF;
+        const int c = j >> 2, u = j & 3;
+        if (c < nc) {
+            float v = 0.f;
+            for (int ww = 0; ww < kBlock / 64; ++ww) v += red[ww][tid];
+            const long long s = job.s_begin + first + c;
+            a.diff2[(a.pair_off[b] + s) * 32 + r * 4 + u] = a.xi2half[b] + v;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
+// Weighted sums and backprojection (storeWeightedSums, src/ml_optimiser.cpp:8241-9156, with
+// BackProjector::backproject2Dto3D, src/backprojector.cpp:55-357). One block per (job, iover_rot).
+// --------------------------------------------------------------------------------------------------
+constexpr int kMaxSigT = 1024;  // fine translations of one orientation (4 per coarse translation); host checks 4*NT
+
+struct WsumArgs {
+    const FineJob *jobs;
+    const int *pairs;
+    const long long *pair_off;
+    const int *or_off;
+    const int *or_frame;
+    const int *or_class;
+    const int *or_dir;              // global direction index per orientation entry
+    const float2 *const *vols;
+    VolGeom vg;
+    const Frame *fine_frames;
+    ImgGeom ig;
+    const float2 *fimg;
+    const float2 *fimg_nomask;
+    const float *fctf;
+    const float *scale_w;           // max(scale, 0.001) (src/ml_optimiser.cpp:8402-8416)
+    const float *minv_w;            // Minvsigma2 with the origin restored
+    const int *mresol;              // resolution shell per pixel, -1 excluded
+    const unsigned char *xa_shell;  // [K][nshell] data_vs_prior > 3 (scale correction) or null
+    const float2 *ex;
+    const float2 *ey;
+    const int *tr_ix;
+    const int *tr_iy;
+    const double *over_x;           // [NT*4] fine shifts in pixels
+    const double *over_y;
+    const double *weight;           // fine weights [pair_off*32 + ...]
+    const PassStats *stats;         // fine-pass stats per particle
+    const double *old_off;          // [b][2]
+    const double *prior_off;        // [b][2]
+    double pixel_size;
+    int NT;
+    int nshell;
+    int n_dir;
+    int do_ctf;
+    int do_scale;
+    float *const *bp_re;            // [K] backprojector arrays
+    float *const *bp_im;
+    float *const *bp_w;
+    double *shell_acc;              // [b][nshell] wsum sigma2_noise per particle
+    double *part_acc;               // [b][8]: sumw, sigma2_offset, norm_corr, XA, AA
+    double *pdf_class;              // [b][K]
+    double *pdf_direction;          // [K][n_dir] (all particles)
+    int K;
+};
+
+__global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
+    __shared__ int s_ix[kMaxSigT], s_iy[kMaxSigT];
+    __shared__ float s_w[kMaxSigT];
+    __shared__ int s_n;
+    __shared__ float shell[512];
+    __shared__ double s_red[16];
+    const int job_id = blockIdx.x >> 3;
+    const int r = blockIdx.x & 7;
+    const FineJob job = a.jobs[job_id];
+    const int b = job.b;
+    const int tid = threadIdx.x;
+    const double thr = a.stats[b].threshold;
+    const double sumw = a.stats[b].sum_weight;
+    if (tid == 0) s_n = 0;
+    for (int i = tid; i < a.nshell; i += kBlock) shell[i] = 0.f;
+    __syncthreads();
+    // significant fine translations of this orientation; scalar sums in FP64 by thread 0 in order
+    const long long base = (a.pair_off[b] + job.s_begin) * 32;
+    if (tid == 0) {
+        double sw = 0., so = 0.;
+        int cnt = 0;
+        const double px2 = a.pixel_size * a.pixel_size;
+        for (int sidx = 0; sidx < job.s_count; ++sidx) {
+            const int t = a.pairs[a.pair_off[b] + job.s_begin + sidx] % a.NT;
+            for (int u = 0; u < 4; ++u) {
+                const double wt = a.weight[base + (long long)sidx * 32 + r * 4 + u];
+                if (!(wt >= thr)) continue;
+                const double wn = wt / sumw;
+                if (cnt < kMaxSigT) {
+                    s_ix[cnt] = a.tr_ix[t * 4 + u];
+                    s_iy[cnt] = a.tr_iy[t * 4 + u];
+                    s_w[cnt] = (float)wn;
+                }
+                ++cnt;
+                sw += wn;
+                const double dx = a.prior_off[2 * b] - a.old_off[2 * b] - a.over_x[t * 4 + u];
+                const double dy = a.prior_off[2 * b + 1] - a.old_off[2 * b + 1] - a.over_y[t * 4 + u];
+                so += wn * px2 * (dx * dx + dy * dy);
+            }
+        }
+        s_n = cnt;
+        if (cnt > 0) {
+            const int oe = a.or_off[b] + job.o;
+            const int k = a.or_class[oe];
+            atomicAdd(&a.part_acc[8 * b + 0], sw);
+            atomicAdd(&a.part_acc[8 * b + 1], so);
+            atomicAdd(&a.pdf_class[(size_t)b * a.K + k], sw);
+            atomicAdd(&a.pdf_direction[(size_t)k * a.n_dir + a.or_dir[oe]], sw);
+        }
+    }
+    __syncthreads();
+    const int nsig = s_n;
+    if (nsig == 0) return;
+    const int oe = a.or_off[b] + job.o;
+    const int k = a.or_class[oe];
+    const Frame fr = a.fine_frames[a.or_frame[oe] + r];
+    const float2 *vol = a.vols[k];
+    const float2 *X = a.fimg + (size_t)b * a.ig.P;
+    const float2 *Xn = a.fimg_nomask + (size_t)b * a.ig.P;
+    const float *C = a.fctf + (size_t)b * a.ig.P;
+    const float scw = a.do_scale ? a.scale_w[b] : 1.f;
+    float *bre = a.bp_re[k], *bim = a.bp_im[k], *bw = a.bp_w[k];
+    const unsigned char *xash = a.xa_shell ? a.xa_shell + (size_t)k * a.nshell : nullptr;
+    double xa = 0., aa = 0., nc = 0.;
+    const int sh = a.ig.xs;
+    for (int p = tid; p < a.ig.P; p += kBlock) {
+        int x, y;
+        pixel_xy(p, a.ig.s, a.ig.xs, x, y);
+        const int i = p / a.ig.xs;
+        const float ctf = (a.do_ctf ? C[p] : 1.f) * scw;
+        float2 ref = make_float2(0.f, 0.f);
+        if (x * x + y * y <= a.ig.proj_r2_max) {
+            const float2 v = project_pixel(vol, a.vg, fr, x, y);
+            ref = make_float2(ctf * v.x, ctf * v.y);
+        }
+        const int ires = a.mresol[p];
+        const float minv = a.minv_w[p];
+        const bool do_xa = a.do_scale && xash && ires > -1 && xash[ires];
+        float wd2 = 0.f, fre = 0.f, fim = 0.f, fw = 0.f, pxa = 0.f, paa = 0.f;
+        const float2 Xp = X[p], Xnp = Xn[p];
+        for (int j = 0; j < nsig; ++j) {
+            const float wn = s_w[j];
+            const float2 ph = cmul(a.ex[(size_t)s_ix[j] * a.ig.xs + x], a.ey[(size_t)s_iy[j] * a.ig.s + i]);
+            if (ires > -1) {
+                const float2 Y = cmul(Xp, ph);
+                const float dr = ref.x - Y.x, di = ref.y - Y.y;
+                wd2 = fmaf(wn, dr * dr + di * di, wd2);
+                if (do_xa) {
+                    pxa = fmaf(wn, ref.x * Y.x + ref.y * Y.y, pxa);
+                    paa = fmaf(wn, ref.x * ref.x + ref.y * ref.y, paa);
+                }
+            }
+            const float2 Z = cmul(Xnp, ph);
+            const float wx = wn * ctf * minv;
+            fre = fmaf(Z.x, wx, fre);
+            fim = fmaf(Z.y, wx, fim);
+            fw = fmaf(wx, ctf, fw);
+        }
+        if (ires > -1) {
+            atomicAdd(&shell[ires], wd2);
+            nc += wd2;
+            xa += pxa;
+            aa += paa;
+        }
+        // backprojection of (fre, fim) with weight fw (backproject2Dto3D, src/backprojector.cpp:124-296)
+        if (fw > 0.f && x * x + y * y <= a.ig.bp_r2_max && !(i >= sh && x == 0)) {
+            const float fxp = (float)x, fyp = (float)y;
+            float xp = fr.e0x * fxp + fr.e1x * fyp;
+            float yp = fr.e0y * fxp + fr.e1y * fyp;
+            float zp = fr.e0z * fxp + fr.e1z * fyp;
+            float vim = fim;
+            if (xp < 0.f) {
+                xp = -xp;
+                yp = -yp;
+                zp = -zp;
+                vim = -vim;
+            }
+            const float fx0 = floorf(xp), fy0 = floorf(yp), fz0 = floorf(zp);
+            const float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
+            const int x0 = (int)fx0, y0 = (int)fy0 - a.vg.starty, z0 = (int)fz0 - a.vg.startz;
+            if (!(x0 < 0 || x0 + 1 >= a.vg.xdim || y0 < 0 || y0 + 1 >= a.vg.ydim || z0 < 0 ||
+                  z0 + 1 >= a.vg.zdim)) {
+                const float mfx = 1.f - fx, mfy = 1.f - fy, mfz = 1.f - fz;
+                const size_t row = (size_t)a.vg.xdim, slab = row * (size_t)a.vg.ydim;
+                const size_t v0 = (size_t)z0 * slab + (size_t)y0 * row + x0;
+                const size_t off[8] = {0, 1, row, row + 1, slab, slab + 1, slab + row, slab + row + 1};
+                const float dd[8] = {mfz * mfy * mfx, mfz * mfy * fx, mfz * fy * mfx, mfz * fy * fx,
+                                     fz * mfy * mfx,  fz * mfy * fx,  fz * fy * mfx,  fz * fy * fx};
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    atomicAdd(bre + v0 + off[c], dd[c] * fre);
+                    atomicAdd(bim + v0 + off[c], dd[c] * vim);
+                    atomicAdd(bw + v0 + off[c], dd[c] * fw);
+                }
+            }
+        }
+    }
+    nc = block_sum_d(nc, s_red);
+    xa = block_sum_d(xa, s_red);
+    aa = block_sum_d(aa, s_red);
+    __syncthreads();
+    double *sa = a.shell_acc + (size_t)b * a.nshell;
+    for (int s = tid; s < a.nshell; s += kBlock)
+        if (shell[s] != 0.f) atomicAdd(&sa[s], (double)shell[s]);
+    if (tid == 0) {
+        atomicAdd(&a.part_acc[8 * b + 2], nc);
+        atomicAdd(&a.part_acc[8 * b + 3], xa);
+        atomicAdd(&a.part_acc[8 * b + 4], aa);
+    }
+}
+
+}  // namespace rlnk
