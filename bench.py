@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""E-step throughput on one MI355X: particles/s of the expectation step at box 256, HEALPix order 3.
+
+Workload (stated choices marked *): one 256^3 map of 150 Gaussian blobs at 1.0 A/pixel, 10,000 particles
+(256x256, SNR 0.03, defocus U(0.8, 2.5) um, 300 kV, Cs 2.7 mm, Q0 0.1, offsets U(+-3 px)), HEALPix order 3
+local search (*sigma_ang 7.5 deg = twice the fine angular step, cut at 3 sigma; *priors = true pose + 3 deg
+Gaussian noise), offsets 4 A range, 0.5 A step, oversampling 1, padding 2, adaptive_fraction 0.999,
+*particle diameter 200 A (coarse image 48 px), *current_size = box. Data are synthetic (seeded).
+
+A step is one E-step over all particles: projections of the coarse grid, coarse pass, weights and
+significance, fine pass, weighted sums and backprojection; particle images are resident in HBM before the
+timed region. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=10000)
+    ap.add_argument("--box", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=16, help="particles timed on the CPU oracle (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg, args):
+    if not args.quiet:
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs the E-step on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        return 2
+    import relion_amd as ra
+    from relion_amd import hipmem
+
+    if hipmem.device_count() < 1:
+        print(json.dumps({"error": "no HIP device visible"}))
+        return 2
+    hipmem.set_device(0)
+    box, px, order = args.box, 256.0 / args.box if args.box != 256 else 1.0, 3
+    diameter = 200.0 * box / 256.0
+    coarse = ra.coarse_image_size(box, px, order, diameter)
+    cfg = ra.make_config(box, px, order, offset_range=4.0, offset_step=0.5, coarse_size=coarse, local_search=True,
+                         sigma_ang=7.5)
+    spec = ra.synth_spec(args.seed, box, px, 150, snr=0.03, defocus=(8000.0, 25000.0), offset_max=3.0,
+                         prior_sigma_deg=3.0)
+    t0 = time.time()
+    vol = ra.synth_reference(spec, 0, box)
+    n = args.particles
+    fimg, fctf, meta, pose, nvar = ra.synth_particles(spec, n, box)
+    log(f"generated reference + {n} particles in {time.time() - t0:.1f} s (coarse {coarse}px)", args)
+    sigma2 = np.full(box // 2 + 1, float(nvar.mean()))
+    d_img = hipmem.DeviceArray(fimg)
+    d_ctf = hipmem.DeviceArray(fctf)
+    del fimg, fctf
+    dims = ra.volume_dims(box)
+    proj = ra.AccProjector(0)
+    proj.setMdlDim(*dims, 2.0)
+    proj.initMdl(vol)
+    bp = ra.AccBackprojector(0)
+    bp.setMdlDim(*dims, 2.0)
+    es = ra.ExpectationStep(cfg, 0)
+
+    def step():
+        bp.initMdl()
+        es.set_model([proj], [bp], [1.0], sigma2)   # per-iteration model: invalidates the grid projections
+        return es.run(d_img.address, d_ctf.address, meta, on_device=True)
+
+    for w in range(args.warmup):
+        t = time.time()
+        step()
+        log(f"warmup {w}: {time.time() - t:.2f} s", args)
+    hipmem.synchronize()
+    t_start = time.time()
+    for s in range(args.steps):
+        res, wsum = step()
+    hipmem.synchronize()
+    elapsed = time.time() - t_start
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = n * args.steps / elapsed
+    timings = es.timings()
+    counters = es.counters()
+    log(f"timed {args.steps} steps: {elapsed:.2f} s; stages {json.dumps({k: round(v, 1) for k, v in timings.items()})}",
+        args)
+    log(f"counters {json.dumps(counters)}", args)
+
+    # accuracy against the generating poses (orientation error of the max-likelihood pose)
+    def rotm(a, b, c):
+        a, b, c = np.radians([a, b, c])
+        ca, sa, cb, sb, cg, sg = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+        return np.array([[cg * cb * ca - sg * sa, cg * cb * sa + sg * ca, -cg * sb],
+                         [-sg * cb * ca - cg * sa, -sg * cb * sa + cg * ca, sg * sb],
+                         [sb * ca, sb * sa, cb]])
+    errs = []
+    for i in range(0, n, max(1, n // 500)):
+        A = rotm(*pose[i, :3])
+        B = rotm(res[i]["rot"], res[i]["tilt"], res[i]["psi"])
+        errs.append(np.degrees(np.arccos(np.clip((np.trace(A @ B.T) - 1) / 2, -1, 1))))
+    errs = np.array(errs)
+
+    # roofline of the dominant kernel: the fine-pass squared differences
+    stage = max(("fine_diff2", "coarse_diff2", "wsum_backproject"), key=lambda k: timings[k])
+    if stage == "fine_diff2":
+        work = 4.0 * counters["fine_poses"] * counters["fine_pixels"]          # FLOP, cross-term count
+    elif stage == "coarse_diff2":
+        work = 4.0 * counters["coarse_poses"] * counters["coarse_pixels"]
+    else:
+        work = 8.0 * counters["wsum_pairs"] * counters["fine_pixels"]          # FLOP of the weighted sums
+    achieved = work / (timings[stage] * 1e-3) / 1e12
+    roofline = {"kernel": stage, "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "stage_ms": round(timings[stage], 2)}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        import oracle
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        m = min(args.cpu_sample, n)
+        sub_img, sub_ctf, sub_meta, _, _ = ra.synth_particles(spec, m, box)
+        t = time.time()
+        oracle.estep(cfg, [vol], [1.0], sigma2, sub_meta, sub_img, sub_ctf, threads=threads)
+        dt = time.time() - t
+        cpu = {"value": round(m / dt, 4), "unit": "particles/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} particles of the same workload, FP64 oracle, OpenMP over particles"}
+        log(f"cpu oracle: {m} particles in {dt:.1f} s on {threads} threads", args)
+
+    out = {
+        "metric": "E-step particles/s, box 256 refine: plain FP32 vs proposal+split-BF16",
+        "value": round(value, 3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"refine box{box} order{order} local-search {n} particles", "path": "plain-fp32",
+                   "global_batch": n, "coarse_size": coarse, "sigma_ang_deg": 7.5, "particle_diameter_A": diameter,
+                   "parallelism": "1 GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "stages_ms": {k: round(v, 2) for k, v in timings.items()}, "counters": counters,
+        "accuracy": {"median_angular_error_deg": round(float(np.median(errs)), 3),
+                     "frac_within_3deg": round(float((errs < 3.0).mean()), 4)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

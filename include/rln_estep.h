@@ -198,8 +198,13 @@ int rln_estep_get_coarse(rln_estep *e, int ip, float *diff2, double *weight, uns
 int rln_estep_fine_size(rln_estep *e, int ip, long long *n_fine);
 int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *diff2, double *weight);
 
-/* Wall time of the device work of the last run, in milliseconds, per stage (for profiling). */
+/* Device time of the last run per stage, in milliseconds, from HIP events on the engine's stream:
+ * prep, coarse_diff2, coarse_weights, jobs, fine_diff2, fine_weights, wsum_backproject. */
 int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
+/* Work counters of the last run: particles, coarse poses, coarse pixels per image, fine poses, fine pixels
+ * per image, fine diff2 blocks, backprojected fine orientations, significant (orientation, translation)
+ * pairs summed in the weighted sums, coarse grid projections computed, coarse tiles, fine jobs. */
+int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out);
 
 /* ---------------------------------------------------------------------------------------------
  * Sampling helpers (HealpixSampling, src/healpix_sampling.cpp), host-side.

@@ -123,6 +123,7 @@ _SIGNATURES = {
     "rln_estep_fine_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong)]),
     "rln_estep_get_fine": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _fp, _dp]),
     "rln_estep_last_timings": (C.c_int, [_vp, _dp, C.c_int, _ip]),
+    "rln_estep_last_counters": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.c_int, _ip]),
     "rln_coarse_image_size": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int]),
     "rln_sampling_counts": (C.c_int, [C.POINTER(EstepConfig), _ip, _ip, _ip]),
     "rln_sampling_grid": (C.c_int, [C.POINTER(EstepConfig), _dp, _dp, _dp, _dp, _dp]),
@@ -139,7 +140,7 @@ STAGES = ("prep", "coarse_diff2", "coarse_weights", "jobs", "fine_diff2", "fine_
 
 def _load():
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m relion_amd._build` "
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python relion_amd/_build.py` "
                           "(or __graft_entry__.build()); there is no CPU fallback")
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
@@ -467,6 +468,15 @@ class ExpectationStep:
         n = C.c_int()
         _check(lib.rln_estep_last_timings(self._h, _ptr(ms, C.c_double), len(STAGES), C.byref(n)))
         return dict(zip(STAGES, ms[:n.value].tolist()))
+
+    COUNTERS = ("particles", "coarse_poses", "coarse_pixels", "fine_poses", "fine_pixels", "fine_blocks",
+                "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs")
+
+    def counters(self):
+        out = np.zeros(16, np.int64)
+        n = C.c_int()
+        _check(lib.rln_estep_last_counters(self._h, _ptr(out, C.c_longlong), 16, C.byref(n)))
+        return dict(zip(self.COUNTERS, out.tolist()))
 
     def close(self):
         if self._h:

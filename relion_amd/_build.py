@@ -1,6 +1,6 @@
 """Build step of librln_estep.so: hipcc for gfx950 (device kernels + engine), g++ for the OpenMP host modules.
 
-Run as ``python -m relion_amd._build`` or through ``__graft_entry__.build()``. Outputs go to
+Run as ``python relion_amd/_build.py`` (by path, so the package and a stale library are not imported) or through ``__graft_entry__.build()``. Outputs go to
 ``relion_amd/librln_estep.so`` (objects under ``relion_amd/_build/``).
 """
 import os

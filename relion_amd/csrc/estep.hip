@@ -172,6 +172,7 @@ struct Engine {
     // timings
     hipEvent_t ev[kStCount + 1] = {};
     double stage_ms[kStCount] = {};
+    long long counters[16] = {};
     bool have_timing = false;
     std::mutex mtx;
 };
@@ -398,6 +399,7 @@ static int engine_coarse_projections(Engine &e) {
     RLN_TRY(launch_check("k_project_grid"));
     for (int k = 0; k < e.K; ++k) e.proj_version[k] = e.proj[k]->version;
     e.coarse_valid = true;
+    e.counters[8] += (long long)e.K * e.n_or;
     return RLN_OK;
 }
 
@@ -829,6 +831,18 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         (void)hipEventElapsedTime(&ms, e.ev[i], e.ev[i + 1]);
         e.stage_ms[i] += ms;
     }
+    e.counters[0] += B;
+    e.counters[1] += h.total_entries;
+    e.counters[2] = (long long)Pc;
+    e.counters[3] += 32 * n_pairs;
+    e.counters[4] = (long long)Pf;
+    e.counters[5] += (long long)cjob.size();
+    for (int b = 0; b < B; ++b) {
+        e.counters[6] += (long long)pacc[8 * b + 5];
+        e.counters[7] += (long long)pacc[8 * b + 6];
+    }
+    e.counters[9] += (long long)h.tiles.size();
+    e.counters[10] += n_jobs;
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
         if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
@@ -1109,6 +1123,7 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     if (!fimg_nomask) fimg_nomask = fimg;
     HIP_TRY(hipSetDevice(e->device));
     for (int i = 0; i < rln::kStCount; ++i) e->stage_ms[i] = 0.;
+    for (long long &c : e->counters) c = 0;
     HIP_TRY(hipMemsetAsync(e->d_pdfd_acc.ptr, 0, sizeof(double) * (size_t)e->K * e->grid.n_dir, e->stream));
     RLN_TRY(rln::engine_coarse_projections(*e));
     // chunk size from a device-memory budget for the coarse intermediates
@@ -1180,6 +1195,14 @@ extern "C" int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over,
     if (ihidden_over) std::copy(k.f_index.begin(), k.f_index.end(), ihidden_over);
     if (diff2) std::copy(k.fdiff2.begin(), k.fdiff2.end(), diff2);
     if (weight) std::copy(k.fweight.begin(), k.fweight.end(), weight);
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out) {
+    if (!e || !out) RLN_FAIL(RLN_ERR_ARG, "null argument");
+    const int n = std::min(n_max, 16);
+    for (int i = 0; i < n; ++i) out[i] = e->counters[i];
+    if (n_out) *n_out = n;
     return RLN_OK;
 }
 

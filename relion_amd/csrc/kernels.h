@@ -829,7 +829,7 @@ struct WsumArgs {
     float *const *bp_im;
     float *const *bp_w;
     double *shell_acc;              // [b][nshell] wsum sigma2_noise per particle
-    double *part_acc;               // [b][8]: sumw, sigma2_offset, norm_corr, XA, AA
+    double *part_acc;               // [b][8]: sumw, sigma2_offset, norm_corr, XA, AA, blocks, translations
     double *pdf_class;              // [b][K]
     double *pdf_direction;          // [K][n_dir] (all particles)
     int K;
@@ -881,6 +881,8 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
             const int k = a.or_class[oe];
             atomicAdd(&a.part_acc[8 * b + 0], sw);
             atomicAdd(&a.part_acc[8 * b + 1], so);
+            atomicAdd(&a.part_acc[8 * b + 5], 1.0);
+            atomicAdd(&a.part_acc[8 * b + 6], (double)cnt);
             atomicAdd(&a.pdf_class[(size_t)b * a.K + k], sw);
             atomicAdd(&a.pdf_direction[(size_t)k * a.n_dir + a.or_dir[oe]], sw);
         }
