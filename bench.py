@@ -33,7 +33,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=10000)
     ap.add_argument("--box", type=int, default=256)
-    ap.add_argument("--cpu-sample", type=int, default=16, help="particles timed on the CPU oracle (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU oracle budget: batches of one particle per thread until this much time (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--quiet", action="store_true")
@@ -133,16 +134,20 @@ def main():
                 "stage_ms": round(timings[stage], 2)}
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_seconds > 0:
         import oracle
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        m = min(args.cpu_sample, n)
-        sub_img, sub_ctf, sub_meta, _, _ = ra.synth_particles(spec, m, box)
-        t = time.time()
-        oracle.estep(cfg, [vol], [1.0], sigma2, sub_meta, sub_img, sub_ctf, threads=threads)
-        dt = time.time() - t
+        m, dt = 0, 0.0
+        while dt < args.cpu_seconds and m < n:
+            k = min(threads, n - m)
+            sub_img, sub_ctf, sub_meta, _, _ = ra.synth_particles(spec, k, box, first=m)
+            t = time.time()
+            oracle.estep(cfg, [vol], [1.0], sigma2, sub_meta, sub_img, sub_ctf, threads=threads)
+            dt += time.time() - t
+            m += k
         cpu = {"value": round(m / dt, 4), "unit": "particles/s", "cores": threads, "kind": "port",
-               "sample": f"first {m} particles of the same workload, FP64 oracle, OpenMP over particles"}
+               "sample": f"first {m} particles of the same workload ({dt:.1f} s), FP64 oracle, one particle per "
+                         "OpenMP thread"}
         log(f"cpu oracle: {m} particles in {dt:.1f} s on {threads} threads", args)
 
     out = {

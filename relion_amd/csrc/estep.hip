@@ -105,15 +105,34 @@ int check_dims(const MdlDims &d) {
 struct rln_projector {
     int device = 0;
     rln::MdlDims dims;
-    rln::DevBuf data;  // float2 [z][y][x]
+    rln::DevBuf data;  // corner-packed volume: 4 float4 per trilinear base voxel
     unsigned long long version = 0;
+    size_t bases() const { return (size_t)(dims.xdim - 1) * (dims.ydim - 1) * (dims.zdim - 1); }
 };
 
 struct rln_backprojector {
     int device = 0;
     rln::MdlDims dims;
-    rln::DevBuf re, im, w;
+    rln::DevBuf slots;  // corner-slot accumulators: kSlot floats per trilinear base voxel
+    size_t bases() const { return (size_t)(dims.xdim - 1) * (dims.ydim - 1) * (dims.zdim - 1); }
 };
+
+namespace rln {
+// Uploads a plain [z][y][x] float2 volume and packs it into the projector's corner layout.
+static int pack_into_projector(rln_projector *p, const float2 *host_plain) {
+    const rln::MdlDims &d = p->dims;
+    DevBuf plain;
+    RLN_TRY(plain.ensure(sizeof(float2) * d.voxels()));
+    HIP_TRY(hipMemcpy(plain.ptr, host_plain, sizeof(float2) * d.voxels(), hipMemcpyHostToDevice));
+    const rlnk::VolGeom g{d.xdim, d.ydim, d.zdim, d.inity, d.initz};
+    const size_t n = p->bases();
+    hipLaunchKernelGGL(rlnk::k_pack_volume, dim3((unsigned)((n + rlnk::kBlock - 1) / rlnk::kBlock)),
+                       dim3(rlnk::kBlock), 0, 0, plain.as<const float2>(), g, p->data.as<float4>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return RLN_OK;
+}
+}  // namespace rln
 
 namespace rln {
 
@@ -367,13 +386,11 @@ static int engine_model_tables(Engine &e) {
     e.logsigma2 = 0.;
     for (int ires : e.mresol_f)
         if (ires > 0 && ires < e.nshell) e.logsigma2 += log(2. * kPi * e.sigma2_noise[ires]);
-    std::vector<const float2 *> vp(e.K);
-    std::vector<float *> bp(3 * e.K);
+    std::vector<const float4 *> vp(e.K);
+    std::vector<float *> bp(e.K);
     for (int k = 0; k < e.K; ++k) {
-        vp[k] = e.proj[k]->data.as<const float2>();
-        bp[k] = e.bproj[k]->re.as<float>();
-        bp[e.K + k] = e.bproj[k]->im.as<float>();
-        bp[2 * e.K + k] = e.bproj[k]->w.as<float>();
+        vp[k] = e.proj[k]->data.as<const float4>();
+        bp[k] = e.bproj[k]->slots.as<float>();
     }
     RLN_TRY(upload(e.d_vols, vp, e.stream));
     RLN_TRY(upload(e.d_bp_ptrs, bp, e.stream));
@@ -394,7 +411,7 @@ static int engine_coarse_projections(Engine &e) {
     const size_t n = (size_t)e.K * e.n_or * e.igc.P;
     RLN_TRY(e.d_proj_c.ensure(n * sizeof(float2)));
     dim3 grid(e.K * e.n_or, (e.igc.P + rlnk::kBlock - 1) / rlnk::kBlock);
-    hipLaunchKernelGGL(rlnk::k_project_grid, grid, dim3(rlnk::kBlock), 0, e.stream, e.d_vols.as<const float2 *const>(),
+    hipLaunchKernelGGL(rlnk::k_project_grid, grid, dim3(rlnk::kBlock), 0, e.stream, e.d_vols.as<const float4 *const>(),
                        e.vg, e.d_frames_c.as<const rlnk::Frame>(), e.n_or, e.igc, e.d_proj_c.as<float2>());
     RLN_TRY(launch_check("k_project_grid"));
     for (int k = 0; k < e.K; ++k) e.proj_version[k] = e.proj[k]->version;
@@ -442,19 +459,19 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     const double px = cfg.pixel_size;
     const double s2 = cfg.sigma2_offset;
     const double offset_norm = 2. * kPi * sqrt(s2) * sqrt(s2);
-    std::vector<int> di, pi;
-    std::vector<double> dp, pp;
+    std::vector<LocalSet> sets;
+    if (cfg.local_search) local_priors_batch(e.grid, cfg, meta, B, sets);
+    std::vector<int> all_d(e.grid.n_dir), all_p(e.grid.n_psi);
+    for (int i = 0; i < e.grid.n_dir; ++i) all_d[i] = i;
+    for (int i = 0; i < e.grid.n_psi; ++i) all_p[i] = i;
+    const std::vector<double> none;
     long long d2 = 0;
     for (int b = 0; b < B; ++b) {
         const rln_particle_meta &m = meta[b];
-        if (cfg.local_search) {
-            e.grid.local_priors(cfg, m.prior_rot, m.prior_tilt, m.prior_psi, di, dp, pi, pp);
-        } else {
-            di.resize(e.grid.n_dir);
-            pi.resize(e.grid.n_psi);
-            for (int i = 0; i < e.grid.n_dir; ++i) di[i] = i;
-            for (int i = 0; i < e.grid.n_psi; ++i) pi[i] = i;
-        }
+        const std::vector<int> &di = cfg.local_search ? sets[b].dir_idx : all_d;
+        const std::vector<int> &pi = cfg.local_search ? sets[b].psi_idx : all_p;
+        const std::vector<double> &dp = cfg.local_search ? sets[b].dir_prior : none;
+        const std::vector<double> &pp = cfg.local_search ? sets[b].psi_prior : none;
         const int ND = (int)di.size(), NP = (int)pi.size();
         h.nd[b] = ND;
         h.np[b] = NP;
@@ -660,12 +677,21 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     std::vector<rlnk::FineJob> jobs(n_jobs);
     if (n_jobs) HIP_TRY(hipMemcpyAsync(jobs.data(), e.d_jobs.ptr, sizeof(rlnk::FineJob) * n_jobs, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    // blocks of the fine pass: jobs with <= 2 significant translations use the 2-translation variant, the
+    // others blocks of 4 (the 8 projections of a pixel are recomputed once per block)
     std::vector<int> cjob, cfirst;
     for (int j = 0; j < n_jobs; ++j)
-        for (int f = 0; f < jobs[j].s_count; f += rlnk::kFineTC) {
+        if (jobs[j].s_count <= 2) {
             cjob.push_back(j);
-            cfirst.push_back(f);
+            cfirst.push_back(0);
         }
+    const size_t n_small = cjob.size();
+    for (int j = 0; j < n_jobs; ++j)
+        if (jobs[j].s_count > 2)
+            for (int f = 0; f < jobs[j].s_count; f += 4) {
+                cjob.push_back(j);
+                cfirst.push_back(f);
+            }
     RLN_TRY(upload(e.d_chunk_job, cjob, st));
     RLN_TRY(upload(e.d_chunk_first, cfirst, st));
     HIP_TRY(hipEventRecord(e.ev[kStFine], st));
@@ -681,7 +707,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     fa.or_off = e.d_or_off.as<const int>();
     fa.or_frame = e.d_or_frame.as<const int>();
     fa.or_class = e.d_or_class.as<const int>();
-    fa.vols = e.d_vols.as<const float2 *const>();
+    fa.vols = e.d_vols.as<const float4 *const>();
     fa.vg = e.vg;
     fa.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
     fa.ig = e.igf;
@@ -697,9 +723,17 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     fa.NT = NT;
     fa.do_ctf = cfg.do_ctf_correction;
     fa.diff2 = e.d_fdiff2.as<float>();
-    if (!cjob.empty()) {
-        hipLaunchKernelGGL(rlnk::k_fine_diff2, dim3((unsigned)cjob.size()), dim3(rlnk::kBlock), 0, st, fa);
-        RLN_TRY(launch_check("k_fine_diff2"));
+    if (n_small) {
+        hipLaunchKernelGGL(rlnk::k_fine_diff2<2>, dim3((unsigned)n_small), dim3(rlnk::kBlock), 0, st, fa);
+        RLN_TRY(launch_check("k_fine_diff2<2>"));
+    }
+    if (cjob.size() > n_small) {
+        rlnk::FineArgs fb = fa;
+        fb.chunk_job += n_small;
+        fb.chunk_first += n_small;
+        hipLaunchKernelGGL(rlnk::k_fine_diff2<4>, dim3((unsigned)(cjob.size() - n_small)), dim3(rlnk::kBlock), 0, st,
+                           fb);
+        RLN_TRY(launch_check("k_fine_diff2<4>"));
     }
     HIP_TRY(hipEventRecord(e.ev[kStFineW], st));
     // fine weights
@@ -724,7 +758,6 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     wf.stats = e.d_stats_f.as<rlnk::PassStats>();
     hipLaunchKernelGGL(rlnk::k_weights, dim3(B), dim3(rlnk::kWBlock), 0, st, wf);
     RLN_TRY(launch_check("k_weights(fine)"));
-    HIP_TRY(hipEventRecord(e.ev[kStWsum], st));
     // weighted sums and backprojection
     RLN_TRY(e.d_shell_acc.ensure(sizeof(double) * e.nshell * B));
     RLN_TRY(e.d_part_acc.ensure(sizeof(double) * 8 * B));
@@ -732,6 +765,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     HIP_TRY(hipMemsetAsync(e.d_shell_acc.ptr, 0, sizeof(double) * e.nshell * B, st));
     HIP_TRY(hipMemsetAsync(e.d_part_acc.ptr, 0, sizeof(double) * 8 * B, st));
     HIP_TRY(hipMemsetAsync(e.d_pdfc_acc.ptr, 0, sizeof(double) * K * B, st));
+    HIP_TRY(hipEventRecord(e.ev[kStWsum], st));
     rlnk::WsumArgs ws{};
     ws.jobs = e.d_jobs.as<const rlnk::FineJob>();
     ws.pairs = e.d_pairs.as<const int>();
@@ -740,7 +774,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     ws.or_frame = e.d_or_frame.as<const int>();
     ws.or_class = e.d_or_class.as<const int>();
     ws.or_dir = e.d_or_dir.as<const int>();
-    ws.vols = e.d_vols.as<const float2 *const>();
+    ws.vols = e.d_vols.as<const float4 *const>();
     ws.vg = e.vg;
     ws.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
     ws.ig = e.igf;
@@ -767,10 +801,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     ws.n_dir = e.grid.n_dir;
     ws.do_ctf = cfg.do_ctf_correction;
     ws.do_scale = cfg.do_scale_correction;
-    float *const *bpp = e.d_bp_ptrs.as<float *const>();
-    ws.bp_re = bpp;
-    ws.bp_im = bpp + K;
-    ws.bp_w = bpp + 2 * K;
+    ws.bp_slots = e.d_bp_ptrs.as<float *const>();
     ws.shell_acc = e.d_shell_acc.as<double>();
     ws.part_acc = e.d_part_acc.as<double>();
     ws.pdf_class = e.d_pdfc_acc.as<double>();
@@ -930,7 +961,7 @@ extern "C" int rln_projector_set_mdl_dim(rln_projector *p, int xdim, int ydim, i
     HIP_TRY(hipSetDevice(p->device));
     p->data.release();
     p->dims = d;
-    RLN_TRY(p->data.ensure(sizeof(float2) * d.voxels()));
+    RLN_TRY(p->data.ensure(4 * sizeof(float4) * p->bases()));
     p->version++;
     return RLN_OK;
 }
@@ -941,7 +972,7 @@ extern "C" int rln_projector_init_mdl(rln_projector *p, const float *real, const
     std::vector<float2> tmp(n);
     for (size_t i = 0; i < n; ++i) tmp[i] = make_float2(real[i], imag[i]);
     HIP_TRY(hipSetDevice(p->device));
-    HIP_TRY(hipMemcpy(p->data.ptr, tmp.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    RLN_TRY(rln::pack_into_projector(p, tmp.data()));
     p->version++;
     return RLN_OK;
 }
@@ -952,7 +983,7 @@ extern "C" int rln_projector_init_mdl_complex(rln_projector *p, const double *da
     std::vector<float2> tmp(n);
     for (size_t i = 0; i < n; ++i) tmp[i] = make_float2((float)data[2 * i], (float)data[2 * i + 1]);
     HIP_TRY(hipSetDevice(p->device));
-    HIP_TRY(hipMemcpy(p->data.ptr, tmp.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    RLN_TRY(rln::pack_into_projector(p, tmp.data()));
     p->version++;
     return RLN_OK;
 }
@@ -982,46 +1013,45 @@ extern "C" int rln_backprojector_set_mdl_dim(rln_backprojector *b, int xdim, int
     rln::MdlDims d{xdim, ydim, zdim, inity, initz, maxr, padding_factor};
     RLN_TRY(rln::check_dims(d));
     HIP_TRY(hipSetDevice(b->device));
-    if (!(d == b->dims) || !b->re.ptr) {
-        b->re.release();
-        b->im.release();
-        b->w.release();
+    if (!(d == b->dims) || !b->slots.ptr) {
+        b->slots.release();
         b->dims = d;
-        RLN_TRY(b->re.ensure(sizeof(float) * d.voxels()));
-        RLN_TRY(b->im.ensure(sizeof(float) * d.voxels()));
-        RLN_TRY(b->w.ensure(sizeof(float) * d.voxels()));
+        RLN_TRY(b->slots.ensure(sizeof(float) * rlnk::kSlot * b->bases()));
     }
-    if (bytes) *bytes = 3 * sizeof(float) * d.voxels();
+    if (bytes) *bytes = sizeof(float) * rlnk::kSlot * b->bases();
     return RLN_OK;
 }
 
 extern "C" int rln_backprojector_init_mdl(rln_backprojector *b) {
-    if (!b || !b->re.ptr) RLN_FAIL(RLN_ERR_STATE, "backprojector not dimensioned");
+    if (!b || !b->slots.ptr) RLN_FAIL(RLN_ERR_STATE, "backprojector not dimensioned");
     HIP_TRY(hipSetDevice(b->device));
-    const size_t n = b->dims.voxels() * sizeof(float);
-    HIP_TRY(hipMemset(b->re.ptr, 0, n));
-    HIP_TRY(hipMemset(b->im.ptr, 0, n));
-    HIP_TRY(hipMemset(b->w.ptr, 0, n));
+    HIP_TRY(hipMemset(b->slots.ptr, 0, sizeof(float) * rlnk::kSlot * b->bases()));
     return RLN_OK;
 }
 
 extern "C" int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real, float *imag, float *weight) {
-    if (!b || !b->re.ptr) RLN_FAIL(RLN_ERR_STATE, "backprojector not dimensioned");
+    if (!b || !b->slots.ptr) RLN_FAIL(RLN_ERR_STATE, "backprojector not dimensioned");
     HIP_TRY(hipSetDevice(b->device));
     HIP_TRY(hipDeviceSynchronize());
-    const size_t n = b->dims.voxels() * sizeof(float);
-    if (real) HIP_TRY(hipMemcpy(real, b->re.ptr, n, hipMemcpyDeviceToHost));
-    if (imag) HIP_TRY(hipMemcpy(imag, b->im.ptr, n, hipMemcpyDeviceToHost));
-    if (weight) HIP_TRY(hipMemcpy(weight, b->w.ptr, n, hipMemcpyDeviceToHost));
+    const size_t nv = b->dims.voxels();
+    rln::DevBuf plain;
+    RLN_TRY(plain.ensure(3 * sizeof(float) * nv));
+    float *d_re = plain.as<float>(), *d_im = d_re + nv, *d_w = d_im + nv;
+    const rlnk::VolGeom g{b->dims.xdim, b->dims.ydim, b->dims.zdim, b->dims.inity, b->dims.initz};
+    hipLaunchKernelGGL(rlnk::k_fold_slots, dim3((unsigned)((nv + rlnk::kBlock - 1) / rlnk::kBlock)), dim3(rlnk::kBlock),
+                       0, 0, b->slots.as<const float>(), g, d_re, d_im, d_w);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    if (real) HIP_TRY(hipMemcpy(real, d_re, sizeof(float) * nv, hipMemcpyDeviceToHost));
+    if (imag) HIP_TRY(hipMemcpy(imag, d_im, sizeof(float) * nv, hipMemcpyDeviceToHost));
+    if (weight) HIP_TRY(hipMemcpy(weight, d_w, sizeof(float) * nv, hipMemcpyDeviceToHost));
     return RLN_OK;
 }
 
 extern "C" int rln_backprojector_destroy(rln_backprojector *b) {
     if (!b) return RLN_OK;
     (void)hipSetDevice(b->device);
-    b->re.release();
-    b->im.release();
-    b->w.release();
+    b->slots.release();
     delete b;
     return RLN_OK;
 }
@@ -1084,7 +1114,7 @@ extern "C" int rln_estep_set_model(rln_estep *e, rln_projector *const *projector
             RLN_FAIL(RLN_ERR_ARG, "projector %d dims do not match current_size %d, padding %g", k, e->cfg.current_size,
                      e->cfg.padding_factor);
         const rln::MdlDims &bd = backprojectors[k]->dims;
-        if (!backprojectors[k]->re.ptr || !(bd == d))
+        if (!backprojectors[k]->slots.ptr || !(bd == d))
             RLN_FAIL(RLN_ERR_ARG, "backprojector %d dims do not match the projector", k);
         if (projectors[k]->device != e->device || backprojectors[k]->device != e->device)
             RLN_FAIL(RLN_ERR_ARG, "projector/backprojector %d on another device", k);

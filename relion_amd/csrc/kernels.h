@@ -35,9 +35,18 @@ __device__ __forceinline__ void pixel_xy(int p, int s, int xs, int &x, int &y) {
     y = (i < xs) ? i : i - s;
 }
 
+// Corner-packed reference: for every trilinear base voxel (x0, y0, z0) with x0 < xdim-1, y0 < ydim-1,
+// z0 < zdim-1, the 8 corner values are stored contiguously as 4 float4 (64 bytes):
+//   [V(z0,y0,x0), V(z0,y0,x0+1)], [V(z0,y0+1,x0), V(z0,y0+1,x0+1)],
+//   [V(z0+1,y0,x0), V(z0+1,y0,x0+1)], [V(z0+1,y0+1,x0), V(z0+1,y0+1,x0+1)]
+// so that one trilinear sample is one aligned 64-byte read instead of four scattered cache lines.
+__device__ __forceinline__ size_t packed_index(const VolGeom &g, int x0, int y0, int z0) {
+    return ((size_t)z0 * (size_t)(g.ydim - 1) + (size_t)y0) * (size_t)(g.xdim - 1) + (size_t)x0;
+}
+
 // Trilinear gather of the padded reference at (xp, yp, zp), Hermitian flip for xp < 0
 // (Projector::project TRILINEAR branch, src/projector.cpp:676-744). Out-of-box samples give 0.
-__device__ __forceinline__ float2 sample_volume(const float2 *__restrict__ vol, const VolGeom &g, float xp, float yp,
+__device__ __forceinline__ float2 sample_volume(const float4 *__restrict__ vol, const VolGeom &g, float xp, float yp,
                                                 float zp) {
     const bool neg = xp < 0.f;
     if (neg) {
@@ -52,23 +61,18 @@ __device__ __forceinline__ float2 sample_volume(const float2 *__restrict__ vol, 
     const int z0 = (int)fz0 - g.startz;
     if (x0 < 0 || x0 + 1 >= g.xdim || y0 < 0 || y0 + 1 >= g.ydim || z0 < 0 || z0 + 1 >= g.zdim)
         return make_float2(0.f, 0.f);
-    const size_t row = (size_t)g.xdim;
-    const size_t slab = row * (size_t)g.ydim;
-    const float2 *b = vol + (size_t)z0 * slab + (size_t)y0 * row + x0;
-    const float2 d000 = b[0], d001 = b[1];
-    const float2 d010 = b[row], d011 = b[row + 1];
-    const float2 d100 = b[slab], d101 = b[slab + 1];
-    const float2 d110 = b[slab + row], d111 = b[slab + row + 1];
+    const float4 *c = vol + 4 * packed_index(g, x0, y0, z0);
+    const float4 q0 = c[0], q1 = c[1], q2 = c[2], q3 = c[3];
     float2 r;
     {
-        const float dx00 = d000.x + (d001.x - d000.x) * fx, dx01 = d100.x + (d101.x - d100.x) * fx;
-        const float dx10 = d010.x + (d011.x - d010.x) * fx, dx11 = d110.x + (d111.x - d110.x) * fx;
+        const float dx00 = q0.x + (q0.z - q0.x) * fx, dx01 = q2.x + (q2.z - q2.x) * fx;
+        const float dx10 = q1.x + (q1.z - q1.x) * fx, dx11 = q3.x + (q3.z - q3.x) * fx;
         const float dxy0 = dx00 + (dx10 - dx00) * fy, dxy1 = dx01 + (dx11 - dx01) * fy;
         r.x = dxy0 + (dxy1 - dxy0) * fz;
     }
     {
-        const float dx00 = d000.y + (d001.y - d000.y) * fx, dx01 = d100.y + (d101.y - d100.y) * fx;
-        const float dx10 = d010.y + (d011.y - d010.y) * fx, dx11 = d110.y + (d111.y - d110.y) * fx;
+        const float dx00 = q0.y + (q0.w - q0.y) * fx, dx01 = q2.y + (q2.w - q2.y) * fx;
+        const float dx10 = q1.y + (q1.w - q1.y) * fx, dx11 = q3.y + (q3.w - q3.y) * fx;
         const float dxy0 = dx00 + (dx10 - dx00) * fy, dxy1 = dx01 + (dx11 - dx01) * fy;
         r.y = dxy0 + (dxy1 - dxy0) * fz;
     }
@@ -76,10 +80,62 @@ __device__ __forceinline__ float2 sample_volume(const float2 *__restrict__ vol, 
     return r;
 }
 
-__device__ __forceinline__ float2 project_pixel(const float2 *__restrict__ vol, const VolGeom &g, const Frame &f,
+__device__ __forceinline__ float2 project_pixel(const float4 *__restrict__ vol, const VolGeom &g, const Frame &f,
                                                 int x, int y) {
     const float fxp = (float)x, fyp = (float)y;
     return sample_volume(vol, g, f.e0x * fxp + f.e1x * fyp, f.e0y * fxp + f.e1y * fyp, f.e0z * fxp + f.e1z * fyp);
+}
+
+// Builds the corner-packed volume from the plain [z][y][x] float2 volume. One thread per base voxel.
+__global__ void __launch_bounds__(kBlock) k_pack_volume(const float2 *__restrict__ plain, VolGeom g,
+                                                        float4 *__restrict__ packed) {
+    const size_t n = (size_t)(g.xdim - 1) * (g.ydim - 1) * (g.zdim - 1);
+    const size_t idx = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n) return;
+    const int bx = g.xdim - 1, by = g.ydim - 1;
+    const int x0 = (int)(idx % bx);
+    const size_t zy = idx / bx;
+    const int y0 = (int)(zy % by), z0 = (int)(zy / by);
+    const size_t row = (size_t)g.xdim, slab = row * (size_t)g.ydim;
+    const float2 *b = plain + (size_t)z0 * slab + (size_t)y0 * row + x0;
+    const float2 d000 = b[0], d001 = b[1], d010 = b[row], d011 = b[row + 1];
+    const float2 d100 = b[slab], d101 = b[slab + 1], d110 = b[slab + row], d111 = b[slab + row + 1];
+    float4 *o = packed + 4 * idx;
+    o[0] = make_float4(d000.x, d000.y, d001.x, d001.y);
+    o[1] = make_float4(d010.x, d010.y, d011.x, d011.y);
+    o[2] = make_float4(d100.x, d100.y, d101.x, d101.y);
+    o[3] = make_float4(d110.x, d110.y, d111.x, d111.y);
+}
+
+// Backprojection accumulators in corner slots: per base voxel 32 floats = one 128-byte line,
+// [re c0..c7 | im c0..c7 | weight c0..c7 | unused], corner c = 4*dz + 2*dy + dx. All 24 atomics of one
+// trilinear scatter land in one line, so a wave's atomic instruction can be served line by line.
+// k_fold_slots sums the slots into plain volumes.
+constexpr int kSlot = 32;
+
+// Plain data/weight volumes from the slots: v(z,y,x) = sum over corners c of slot[base(z,y,x) - c][c].
+__global__ void __launch_bounds__(kBlock) k_fold_slots(const float *__restrict__ slots, VolGeom g,
+                                                       float *__restrict__ re, float *__restrict__ im,
+                                                       float *__restrict__ w) {
+    const size_t n = (size_t)g.xdim * g.ydim * g.zdim;
+    const size_t v = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= n) return;
+    const int x = (int)(v % g.xdim);
+    const size_t zy = v / g.xdim;
+    const int y = (int)(zy % g.ydim), z = (int)(zy / g.ydim);
+    float sr = 0.f, si = 0.f, sw = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int x0 = x - (c & 1), y0 = y - ((c >> 1) & 1), z0 = z - (c >> 2);
+        if (x0 < 0 || y0 < 0 || z0 < 0 || x0 >= g.xdim - 1 || y0 >= g.ydim - 1 || z0 >= g.zdim - 1) continue;
+        const float *s = slots + kSlot * packed_index(g, x0, y0, z0);
+        sr += s[c];
+        si += s[8 + c];
+        sw += s[16 + c];
+    }
+    re[v] = sr;
+    im[v] = si;
+    w[v] = sw;
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -92,7 +148,7 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 
 // Projections of every coarse orientation of every class at the coarse size:
 // proj[(k*n_or + o)*P + p] (class-major). One thread per pixel, grid (K*n_or, ceil(P/256)).
-__global__ void __launch_bounds__(kBlock) k_project_grid(const float2 *const *__restrict__ vols, VolGeom vg,
+__global__ void __launch_bounds__(kBlock) k_project_grid(const float4 *const *__restrict__ vols, VolGeom vg,
                                                          const Frame *__restrict__ frames, int n_or, ImgGeom ig,
                                                          float2 *__restrict__ proj) {
     const int p = blockIdx.y * kBlock + threadIdx.x;
@@ -665,11 +721,9 @@ __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__r
 }
 
 // --------------------------------------------------------------------------------------------------
-// Fine pass: one block per (job, chunk of kFineTC coarse translations); each thread owns pixels, projects the
-// 8 oversampled orientations once per pixel and accumulates the 8 x 4*kFineTC squared differences.
+// Fine pass: one block per (job, chunk of TC coarse translations); each thread owns pixels, projects the
+// 8 oversampled orientations once per pixel and accumulates the 8 x 4*TC squared differences.
 // --------------------------------------------------------------------------------------------------
-constexpr int kFineTC = 2;                 // coarse translations per block
-constexpr int kFineNF = 4 * kFineTC;       // fine translations per block
 
 struct FineArgs {
     const FineJob *jobs;
@@ -680,7 +734,7 @@ struct FineArgs {
     const int *or_off;
     const int *or_frame;           // fine frame base (coarse orientation * 8) per orientation entry
     const int *or_class;
-    const float2 *const *vols;
+    const float4 *const *vols;
     VolGeom vg;
     const Frame *fine_frames;      // [n_dir * n_psi * 8]
     ImgGeom ig;
@@ -698,7 +752,9 @@ struct FineArgs {
     float *diff2;                  // out [pair_off[b]*32 + s*32 + r*4 + u]
 };
 
+template <int kFineTC>
 __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
+    constexpr int kFineNF = 4 * kFineTC;  // fine translations per block
     __shared__ float red[kBlock / 64][8 * kFineNF];
     __shared__ int s_tix[kFineNF], s_tiy[kFineNF];
     __shared__ int s_t[kFineTC];
@@ -723,7 +779,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
     const int oe = a.or_off[b] + job.o;
     const int k = a.or_class[oe];
     const Frame *fr = a.fine_frames + a.or_frame[oe];
-    const float2 *vol = a.vols[k];
+    const float4 *vol = a.vols[k];
     const float2 *X = a.fimg + (size_t)b * a.ig.P;
     const float *C = a.fctf + (size_t)b * a.ig.P;
     const float sc = a.scale[b];
@@ -798,7 +854,7 @@ struct WsumArgs {
     const int *or_frame;
     const int *or_class;
     const int *or_dir;              // global direction index per orientation entry
-    const float2 *const *vols;
+    const float4 *const *vols;
     VolGeom vg;
     const Frame *fine_frames;
     ImgGeom ig;
@@ -825,9 +881,7 @@ struct WsumArgs {
     int n_dir;
     int do_ctf;
     int do_scale;
-    float *const *bp_re;            // [K] backprojector arrays
-    float *const *bp_im;
-    float *const *bp_w;
+    float *const *bp_slots;         // [K] corner-slot accumulators of the backprojectors
     double *shell_acc;              // [b][nshell] wsum sigma2_noise per particle
     double *part_acc;               // [b][8]: sumw, sigma2_offset, norm_corr, XA, AA, blocks, translations
     double *pdf_class;              // [b][K]
@@ -841,6 +895,9 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     __shared__ int s_n;
     __shared__ float shell[512];
     __shared__ double s_red[16];
+    __shared__ long long sx_base[kBlock];
+    __shared__ float sx_val[kBlock][3];
+    __shared__ float sx_w[kBlock][8];
     const int job_id = blockIdx.x >> 3;
     const int r = blockIdx.x & 7;
     const FineJob job = a.jobs[job_id];
@@ -893,16 +950,27 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     const int oe = a.or_off[b] + job.o;
     const int k = a.or_class[oe];
     const Frame fr = a.fine_frames[a.or_frame[oe] + r];
-    const float2 *vol = a.vols[k];
+    const float4 *vol = a.vols[k];
     const float2 *X = a.fimg + (size_t)b * a.ig.P;
     const float2 *Xn = a.fimg_nomask + (size_t)b * a.ig.P;
     const float *C = a.fctf + (size_t)b * a.ig.P;
     const float scw = a.do_scale ? a.scale_w[b] : 1.f;
-    float *bre = a.bp_re[k], *bim = a.bp_im[k], *bw = a.bp_w[k];
+    float *slots = a.bp_slots[k];
     const unsigned char *xash = a.xa_shell ? a.xa_shell + (size_t)k * a.nshell : nullptr;
     double xa = 0., aa = 0., nc = 0.;
     const int sh = a.ig.xs;
-    for (int p = tid; p < a.ig.P; p += kBlock) {
+    // start each block at a different pixel chunk so that concurrent blocks do not all scatter into the
+    // low-frequency voxels at the same time
+    const int n_chunks = (a.ig.P + kBlock - 1) / kBlock;
+    const int c0 = (int)((blockIdx.x * 2654435761u) % (unsigned)n_chunks);
+    for (int cc = 0; cc < n_chunks; ++cc) {
+        int chunk = c0 + cc;
+        if (chunk >= n_chunks) chunk -= n_chunks;
+        const int p = chunk * kBlock + tid;
+        long long sc_base = -1;  // trilinear base voxel of this pixel's scatter (-1: none)
+        float sc_val[3] = {0.f, 0.f, 0.f};
+        float sc_w[8];
+        if (p < a.ig.P) {
         int x, y;
         pixel_xy(p, a.ig.s, a.ig.xs, x, y);
         const int i = p / a.ig.xs;
@@ -960,17 +1028,38 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
             if (!(x0 < 0 || x0 + 1 >= a.vg.xdim || y0 < 0 || y0 + 1 >= a.vg.ydim || z0 < 0 ||
                   z0 + 1 >= a.vg.zdim)) {
                 const float mfx = 1.f - fx, mfy = 1.f - fy, mfz = 1.f - fz;
-                const size_t row = (size_t)a.vg.xdim, slab = row * (size_t)a.vg.ydim;
-                const size_t v0 = (size_t)z0 * slab + (size_t)y0 * row + x0;
-                const size_t off[8] = {0, 1, row, row + 1, slab, slab + 1, slab + row, slab + row + 1};
-                const float dd[8] = {mfz * mfy * mfx, mfz * mfy * fx, mfz * fy * mfx, mfz * fy * fx,
-                                     fz * mfy * mfx,  fz * mfy * fx,  fz * fy * mfx,  fz * fy * fx};
+                sc_base = (long long)packed_index(a.vg, x0, y0, z0);
+                sc_val[0] = fre;
+                sc_val[1] = vim;
+                sc_val[2] = fw;
+                sc_w[0] = mfz * mfy * mfx;
+                sc_w[1] = mfz * mfy * fx;
+                sc_w[2] = mfz * fy * mfx;
+                sc_w[3] = mfz * fy * fx;
+                sc_w[4] = fz * mfy * mfx;
+                sc_w[5] = fz * mfy * fx;
+                sc_w[6] = fz * fy * mfx;
+                sc_w[7] = fz * fy * fx;
+            }
+        }
+        }
+        // Scatter of the block's pixels into the corner slots. The 24 values of one pixel share one 128-byte
+        // slot line; lanes are regrouped through LDS so that each atomic instruction covers two pixels'
+        // lines (32 lanes each) instead of 64 lines, which is what bounds atomic throughput.
+        __syncthreads();
+        sx_base[tid] = sc_base;
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    atomicAdd(bre + v0 + off[c], dd[c] * fre);
-                    atomicAdd(bim + v0 + off[c], dd[c] * vim);
-                    atomicAdd(bw + v0 + off[c], dd[c] * fw);
-                }
+        for (int c = 0; c < 3; ++c) sx_val[tid][c] = sc_val[c];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) sx_w[tid][c] = (sc_base >= 0) ? sc_w[c] : 0.f;
+        __syncthreads();
+        {
+            const int lane = tid & 63, wave0 = tid & ~63;
+            const int f = lane & 31;
+            for (int k2 = 0; k2 < 32; ++k2) {
+                const int src = wave0 + 2 * k2 + (lane >> 5);
+                const long long sb = sx_base[src];
+                if (sb >= 0 && f < 24) atomicAdd(slots + kSlot * sb + f, sx_w[src][f & 7] * sx_val[src][f >> 3]);
             }
         }
     }

@@ -132,7 +132,14 @@ bool PoseGrid::build(const rln_estep_config &cfg) {
     n_dir = 12 << (2 * order);
     rot.resize(n_dir);
     tilt.resize(n_dir);
-    for (int p = 0; p < n_dir; ++p) healpix_direction(order, p, rot[p], tilt[p]);
+    dir_unit.resize(3 * (size_t)n_dir);
+    for (int p = 0; p < n_dir; ++p) {
+        healpix_direction(order, p, rot[p], tilt[p]);
+        const double a = rad(rot[p]), b = rad(tilt[p]);
+        dir_unit[3 * p] = sin(b) * cos(a);
+        dir_unit[3 * p + 1] = sin(b) * sin(a);
+        dir_unit[3 * p + 2] = cos(b);
+    }
     // psi (default step: src/healpix_sampling.cpp:73-74)
     double step = cfg.psi_step;
     if (step <= 0.) step = 360. / (6 * ROUND(std::pow(2., order)));
@@ -203,27 +210,38 @@ void PoseGrid::local_priors(const rln_estep_config &cfg, double prior_rot, doubl
     const double sig = std::max(cfg.sigma_rot, cfg.sigma_tilt);
     double total = 0., best = 9999.;
     int best_dir = -1;
-    for (int d = 0; d < n_dir; ++d) {
-        if (!use_dir) {
-            dir_idx.push_back(d);
-            dir_prior.push_back(1.);
-            total += 1.;
-            continue;
+    // Directions whose dot product with the prior is clearly below cos(cutoff*sigma) cannot pass the angular
+    // test below; they are skipped without the acos (the test itself is unchanged).
+    const double lim = cutoff * sig;
+    const double dot_min = (lim < 179.) ? cos((lim + 1.) * kPi / 180.) : -2.;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int d = 0; d < n_dir; ++d) {
+            if (!use_dir) {
+                dir_idx.push_back(d);
+                dir_prior.push_back(1.);
+                total += 1.;
+                continue;
+            }
+            const double *v = &dir_unit[3 * (size_t)d];
+            const double dot = v[0] * pv[0] + v[1] * pv[1] + v[2] * pv[2];
+            if (pass == 0 && dot < dot_min) continue;
+            double ang = acos(dot) * 180. / kPi;
+            if (ang > 180.) ang = std::fabs(ang - 360.);
+            if (ang < lim) {
+                const double p = normal_pdf(ang, sig);
+                dir_idx.push_back(d);
+                dir_prior.push_back(p);
+                total += p;
+            }
+            if (ang < best) {
+                best = ang;
+                best_dir = d;
+            }
         }
-        double v[3];
-        unit(rot[d], tilt[d], v);
-        double ang = acos(v[0] * pv[0] + v[1] * pv[1] + v[2] * pv[2]) * 180. / kPi;
-        if (ang > 180.) ang = std::fabs(ang - 360.);
-        if (ang < cutoff * sig) {
-            const double p = normal_pdf(ang, sig);
-            dir_idx.push_back(d);
-            dir_prior.push_back(p);
-            total += p;
-        }
-        if (ang < best) {
-            best = ang;
-            best_dir = d;
-        }
+        // the fallback to the nearest direction needs every direction's angle
+        if (!dir_idx.empty() || !use_dir) break;
+        best = 9999.;
+        best_dir = -1;
     }
     for (double &p : dir_prior) p /= total;
     if (dir_idx.empty()) {
@@ -258,6 +276,15 @@ void PoseGrid::local_priors(const rln_estep_config &cfg, double prior_rot, doubl
         psi_idx.push_back(best_psi);
         psi_prior.push_back(1.);
     }
+}
+
+void local_priors_batch(const PoseGrid &g, const rln_estep_config &cfg, const rln_particle_meta *meta, int n,
+                        std::vector<LocalSet> &out) {
+    out.resize(n);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i)
+        g.local_priors(cfg, meta[i].prior_rot, meta[i].prior_tilt, meta[i].prior_psi, out[i].dir_idx,
+                       out[i].dir_prior, out[i].psi_idx, out[i].psi_prior);
 }
 
 }  // namespace rln

@@ -15,6 +15,7 @@ struct PoseGrid {
     std::vector<double> rot, tilt;   // per direction, degrees
     std::vector<double> psi;         // per psi index, degrees
     std::vector<double> tx, ty;      // translations, Angstrom
+    std::vector<double> dir_unit;    // per direction: Euler_angles2direction (x, y, z)
 
     // Builds the C1 grid of cfg (HealpixSampling::initialise, src/healpix_sampling.cpp:51-138).
     bool build(const rln_estep_config &cfg);
@@ -27,6 +28,16 @@ struct PoseGrid {
                       std::vector<int> &dir_idx, std::vector<double> &dir_prior, std::vector<int> &psi_idx,
                       std::vector<double> &psi_prior) const;
 };
+
+// Non-zero-prior orientations of one particle.
+struct LocalSet {
+    std::vector<int> dir_idx, psi_idx;
+    std::vector<double> dir_prior, psi_prior;
+};
+
+// local_priors for n particles in parallel (OpenMP); out[i] for particle i.
+void local_priors_batch(const PoseGrid &g, const rln_estep_config &cfg, const rln_particle_meta *meta, int n,
+                        std::vector<LocalSet> &out);
 
 // HEALPix NEST primitives (Gorski et al. 2005), as in src/Healpix_2.15a/healpix_base.cc.
 void hp_nest2xyf(int order, long long pix, int &ix, int &iy, int &face);

@@ -128,7 +128,8 @@ def test_projector_roundtrip():
     pr.initMdl(p.volumes[0])
     bp = ra.AccBackprojector(0)
     nbytes = bp.setMdlDim(*dims, 2.0)
-    assert nbytes == 3 * 4 * dims[0] * dims[1] * dims[2]
+    # corner-slot accumulators: 32 floats per trilinear base voxel
+    assert nbytes == 4 * 32 * (dims[0] - 1) * (dims[1] - 1) * (dims[2] - 1)
     bp.initMdl()
     re, im, w = bp.getMdlData()
     assert not re.any() and not im.any() and not w.any()
