@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (reference for traffic counters)")
+    ap.add_argument("--traffic-kib", type=float, default=None,
+                    help="measured HBM traffic per launch of the dominant kernel (KiB, counter-calibrated)")
     return ap.parse_args()
 
 
@@ -88,6 +92,14 @@ def main():
         es.set_model([proj], [bp], [1.0], sigma2)   # per-iteration model: invalidates the grid projections
         return es.run(d_img.address, d_ctf.address, meta, on_device=True)
 
+    if args.copy_calibration_mb > 0:
+        nb = args.copy_calibration_mb << 20
+        src = hipmem.DeviceArray(np.zeros(nb, np.uint8))
+        dst = hipmem.DeviceArray(np.zeros(nb, np.uint8))
+        hipmem.copy_device(dst.address, src.address, nb)
+        hipmem.synchronize()
+        log(f"copy calibration: {nb} bytes device-to-device", args)
+        del src, dst
     for w in range(args.warmup):
         t = time.time()
         step()
@@ -130,8 +142,10 @@ def main():
         work = 8.0 * counters["wsum_pairs"] * counters["fine_pixels"]          # FLOP of the weighted sums
     achieved = work / (timings[stage] * 1e-3) / 1e12
     roofline = {"kernel": stage, "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                "stage_ms": round(timings[stage], 2)}
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": None if args.traffic_kib is None else round(args.traffic_kib * 1024.0),
+                "stage_ms": round(timings[stage], 2), "launches": counters["chunks"],
+                "avg_launch_ms": round(timings[stage] / max(1, counters["chunks"]), 3)}
 
     cpu = None
     if args.cpu_seconds > 0:

@@ -470,7 +470,7 @@ class ExpectationStep:
         return dict(zip(STAGES, ms[:n.value].tolist()))
 
     COUNTERS = ("particles", "coarse_poses", "coarse_pixels", "fine_poses", "fine_pixels", "fine_blocks",
-                "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs")
+                "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks")
 
     def counters(self):
         out = np.zeros(16, np.int64)

@@ -874,6 +874,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     }
     e.counters[9] += (long long)h.tiles.size();
     e.counters[10] += n_jobs;
+    e.counters[11] += 1;
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
         if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);

@@ -18,7 +18,7 @@ _hip.hipGetErrorString.argtypes = [C.c_int]
 _hip.hipGetErrorString.restype = C.c_char_p
 _hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
 
-H2D, D2H = 1, 2
+H2D, D2H, D2D = 1, 2, 3
 
 
 def _check(rc, what):
@@ -44,6 +44,11 @@ def mem_info():
     free, total = C.c_size_t(), C.c_size_t()
     _check(_hip.hipMemGetInfo(C.byref(free), C.byref(total)), "hipMemGetInfo")
     return free.value, total.value
+
+
+def copy_device(dst, src, nbytes):
+    """Device-to-device copy (runs the runtime's copy kernel): used to calibrate traffic counters."""
+    _check(_hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, D2D), "hipMemcpy D2D")
 
 
 class DeviceArray:
