@@ -52,7 +52,8 @@ enum {
 
 /* Precision of the squared-difference (diff2) kernels. */
 enum {
-    RLN_PREC_PLAIN_FP32 = 0      /* direct sum of |CTF*s*P - X_t|^2 in FP32 (fallback path) */
+    RLN_PREC_PLAIN_FP32 = 0,     /* direct sum of |CTF*s*P - X_t|^2 in FP32 (fallback path) */
+    RLN_PREC_SPLIT_BF16 = 1      /* coarse cross term Re(P conj(CTF*s*X_t)) on MFMA, BF16 hi+lo, FP32 accumulation */
 };
 
 /* Options of one E-step run; names follow MlOptimiser / MlModel / HealpixSampling members. */
@@ -199,7 +200,8 @@ int rln_estep_fine_size(rln_estep *e, int ip, long long *n_fine);
 int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *diff2, double *weight);
 
 /* Device time of the last run per stage, in milliseconds, from HIP events on the engine's stream:
- * prep, coarse_diff2, coarse_weights, jobs, fine_diff2, fine_weights, wsum_backproject. */
+ * prep, coarse_diff2, coarse_weights, jobs, fine_diff2, fine_weights, wsum_backproject; then host wall
+ * time of chunk preparation, job-list building, the epilogue, and the whole run. */
 int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
 /* Work counters of the last run: particles, coarse poses, coarse pixels per image, fine poses, fine pixels
  * per image, fine diff2 blocks, backprojected fine orientations, significant (orientation, translation)
@@ -257,6 +259,10 @@ int rln_synth_particles(const rln_synth_spec *spec, int nr_classes, double share
 /* CTF value of the reference model (src/ctf.h:184-240) at frequency (x, y) in 1/Angstrom. */
 double rln_ctf_value(double defU, double defV, double defAngle, double voltage, double cs, double q0,
                      double bfactor, double scale, double phase_shift, double x, double y);
+
+/* Self-test of the MFMA lane layout the split-BF16 kernels rely on: D(16x16) = A(16x32) * B(32x16), row-major
+ * float inputs rounded to BF16, one v_mfma_f32_16x16x32_bf16 on `device`. No counterpart in the reference. */
+int rln_selftest_mfma_bf16(int device, const float *A, const float *B, float *D);
 
 const char *rln_last_error(void);
 

@@ -26,6 +26,7 @@ RLN_ERR_NO_SIGNIFICANT = -5
 RLN_ERR_SUM_WEIGHT_ZERO = -6
 RLN_ERR_MEMORY = -7
 RLN_PREC_PLAIN_FP32 = 0
+RLN_PREC_SPLIT_BF16 = 1
 
 
 class RelionError(RuntimeError):
@@ -133,9 +134,11 @@ _SIGNATURES = {
     "rln_synth_particles": (C.c_int, [C.POINTER(SynthSpec), C.c_int, C.c_double, C.c_int, C.c_longlong, C.c_int,
                                       _fp, _fp, C.POINTER(ParticleMeta), _dp, _dp]),
     "rln_ctf_value": (C.c_double, [C.c_double] * 11),
+    "rln_selftest_mfma_bf16": (C.c_int, [C.c_int, _fp, _fp, _fp]),
 }
 
-STAGES = ("prep", "coarse_diff2", "coarse_weights", "jobs", "fine_diff2", "fine_weights", "wsum_backproject")
+STAGES = ("prep", "coarse_diff2", "coarse_weights", "jobs", "fine_diff2", "fine_weights", "wsum_backproject",
+          "host_chunk_prep", "host_job_lists", "host_epilogue", "run_wall")
 
 
 def _load():
@@ -192,7 +195,7 @@ def coarse_image_size(ori_size, pixel_size, healpix_order, particle_diameter, cu
 def make_config(ori_size, pixel_size, healpix_order, offset_range, offset_step, coarse_size, current_size=None,
                 nr_classes=1, padding_factor=2.0, local_search=False, sigma_ang=0.0, adaptive_fraction=0.999,
                 maximum_significants=0, sigma2_offset=None, do_ctf_correction=True, do_scale_correction=False,
-                psi_step=-1.0, sigma2_fudge=1.0):
+                psi_step=-1.0, sigma2_fudge=1.0, precision=RLN_PREC_PLAIN_FP32):
     cfg = EstepConfig()
     cfg.ori_size = ori_size
     cfg.nr_classes = nr_classes
@@ -206,7 +209,7 @@ def make_config(ori_size, pixel_size, healpix_order, offset_range, offset_step, 
     cfg.offset_range = offset_range
     cfg.offset_step = offset_step
     cfg.local_search = 1 if local_search else 0
-    cfg.precision = RLN_PREC_PLAIN_FP32
+    cfg.precision = precision
     cfg.sigma_rot = cfg.sigma_tilt = cfg.sigma_psi = sigma_ang
     cfg.sigma_cutoff = 3.0
     cfg.adaptive_fraction = adaptive_fraction
@@ -243,6 +246,15 @@ def sampling_local(cfg, prior_rot, prior_tilt, prior_psi):
                                   _ptr(dp, C.c_double), nd, C.byref(n1), _ptr(pi, C.c_int), _ptr(pp, C.c_double), npsi,
                                   C.byref(n2)))
     return di[:n1.value], dp[:n1.value], pi[:n2.value], pp[:n2.value]
+
+
+def selftest_mfma_bf16(A, B, device=0):
+    """D = A(16x32) @ B(32x16) through one BF16 MFMA with the library's lane layout."""
+    a = np.ascontiguousarray(A, np.float32)
+    b = np.ascontiguousarray(B, np.float32)
+    d = np.zeros((16, 16), np.float32)
+    _check(lib.rln_selftest_mfma_bf16(device, _ptr(a, C.c_float), _ptr(b, C.c_float), _ptr(d, C.c_float)))
+    return d
 
 
 def ctf_value(defU, defV, defAngle, voltage, cs, q0, bfactor, scale, phase_shift, x, y):

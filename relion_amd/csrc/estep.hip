@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -159,6 +160,9 @@ struct Engine {
     bool coarse_valid = false;
     // constant tables
     DevBuf d_frames_c, d_frames_f, d_proj_c;
+    DevBuf d_phi, d_plo, d_pn2, d_zero_row;         // split-BF16 coarse projections
+    DevBuf d_bhi, d_blo, d_ctf2, d_ynorm;           // split-BF16 translation operand per chunk
+    int Kp = 0;                                     // padded contraction length 2*P_coarse
     DevBuf d_q_c, d_q_f, d_minv_w, d_mresol, d_xa_shell;
     DevBuf d_ex_c, d_ey_c, d_ex_f, d_ey_f, d_tr_ix, d_tr_iy, d_over_x, d_over_y;
     DevBuf d_vols, d_bp_ptrs;
@@ -191,6 +195,7 @@ struct Engine {
     // timings
     hipEvent_t ev[kStCount + 1] = {};
     double stage_ms[kStCount] = {};
+    double host_ms[4] = {};  // host phases: chunk preparation, job lists, epilogue, whole run (wall)
     long long counters[16] = {};
     bool have_timing = false;
     std::mutex mtx;
@@ -285,7 +290,8 @@ static int engine_init(Engine &e, const rln_estep_config &cfg) {
         RLN_FAIL(RLN_ERR_ARG, "invalid image sizes (ori %d, current %d, coarse %d) or padding %g", cfg.ori_size,
                  cfg.current_size, cfg.coarse_size, cfg.padding_factor);
     if (cfg.adaptive_oversampling != 1) RLN_FAIL(RLN_ERR_ARG, "only adaptive_oversampling = 1 is supported");
-    if (cfg.precision != RLN_PREC_PLAIN_FP32) RLN_FAIL(RLN_ERR_ARG, "unsupported precision mode %d", cfg.precision);
+    if (cfg.precision != RLN_PREC_PLAIN_FP32 && cfg.precision != RLN_PREC_SPLIT_BF16)
+        RLN_FAIL(RLN_ERR_ARG, "unsupported precision mode %d", cfg.precision);
     if (!(cfg.sigma2_fudge > 0.) || !(cfg.adaptive_fraction > 0. && cfg.adaptive_fraction <= 1.))
         RLN_FAIL(RLN_ERR_ARG, "invalid sigma2_fudge or adaptive_fraction");
     if (cfg.ori_size / 2 + 1 > 512) RLN_FAIL(RLN_ERR_ARG, "ori_size %d above the supported 1022", cfg.ori_size);
@@ -356,6 +362,13 @@ static int engine_init(Engine &e, const rln_estep_config &cfg) {
     RLN_TRY(upload(e.d_over_y, e.over_y, e.stream));
     e.mresol_f = mresol_of(e.sf);
     RLN_TRY(upload(e.d_mresol, e.mresol_f, e.stream));
+    e.Kp = 32 * ((2 * e.igc.P + 31) / 32);
+    if (cfg.precision == RLN_PREC_SPLIT_BF16 && e.NT > 256)
+        RLN_FAIL(RLN_ERR_ARG, "split-BF16 coarse pass supports at most 256 translations (%d)", e.NT);
+    if (cfg.precision == RLN_PREC_SPLIT_BF16) {
+        std::vector<unsigned short> z(e.Kp, 0);
+        RLN_TRY(upload(e.d_zero_row, z, e.stream));
+    }
     HIP_TRY(hipStreamSynchronize(e.stream));
     return RLN_OK;
 }
@@ -408,12 +421,26 @@ static int engine_coarse_projections(Engine &e) {
     for (int k = 0; k < e.K; ++k)
         if (e.proj_version[k] != e.proj[k]->version) stale = true;
     if (!stale) return RLN_OK;
-    const size_t n = (size_t)e.K * e.n_or * e.igc.P;
-    RLN_TRY(e.d_proj_c.ensure(n * sizeof(float2)));
-    dim3 grid(e.K * e.n_or, (e.igc.P + rlnk::kBlock - 1) / rlnk::kBlock);
-    hipLaunchKernelGGL(rlnk::k_project_grid, grid, dim3(rlnk::kBlock), 0, e.stream, e.d_vols.as<const float4 *const>(),
-                       e.vg, e.d_frames_c.as<const rlnk::Frame>(), e.n_or, e.igc, e.d_proj_c.as<float2>());
-    RLN_TRY(launch_check("k_project_grid"));
+    if (e.cfg.precision == RLN_PREC_SPLIT_BF16) {
+        const size_t rows = (size_t)e.K * e.n_or;
+        RLN_TRY(e.d_phi.ensure(rows * e.Kp * sizeof(unsigned short)));
+        RLN_TRY(e.d_plo.ensure(rows * e.Kp * sizeof(unsigned short)));
+        RLN_TRY(e.d_pn2.ensure(rows * e.igc.P * sizeof(float)));
+        dim3 grid(e.K * e.n_or, (e.Kp / 2 + rlnk::kBlock - 1) / rlnk::kBlock);
+        hipLaunchKernelGGL(rlnk::k_project_grid_split, grid, dim3(rlnk::kBlock), 0, e.stream,
+                           e.d_vols.as<const float4 *const>(), e.vg, e.d_frames_c.as<const rlnk::Frame>(), e.n_or,
+                           e.igc, e.Kp, e.d_phi.as<unsigned short>(), e.d_plo.as<unsigned short>(),
+                           e.d_pn2.as<float>());
+        RLN_TRY(launch_check("k_project_grid_split"));
+    } else {
+        const size_t n = (size_t)e.K * e.n_or * e.igc.P;
+        RLN_TRY(e.d_proj_c.ensure(n * sizeof(float2)));
+        dim3 grid(e.K * e.n_or, (e.igc.P + rlnk::kBlock - 1) / rlnk::kBlock);
+        hipLaunchKernelGGL(rlnk::k_project_grid, grid, dim3(rlnk::kBlock), 0, e.stream,
+                           e.d_vols.as<const float4 *const>(), e.vg, e.d_frames_c.as<const rlnk::Frame>(), e.n_or,
+                           e.igc, e.d_proj_c.as<float2>());
+        RLN_TRY(launch_check("k_project_grid"));
+    }
     for (int k = 0; k < e.K; ++k) e.proj_version[k] = e.proj[k]->version;
     e.coarse_valid = true;
     e.counters[8] += (long long)e.K * e.n_or;
@@ -531,13 +558,47 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     h.total_entries = d2;
 }
 
+template <int NTT>
+static void launch_mfma_t(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
+    hipLaunchKernelGGL(rlnk::k_coarse_mfma<NTT>, dim3(n_tiles), dim3(rlnk::kBlock), 0, e.stream,
+                       e.d_tiles.as<const rlnk::CoarseTile>(), e.d_or_off.as<const int>(), e.d_or_cnt.as<const int>(),
+                       e.d_or_proj.as<const int>(), e.d_phi.as<const unsigned short>(),
+                       e.d_plo.as<const unsigned short>(), e.d_pn2.as<const float>(),
+                       e.d_bhi.as<const unsigned short>(), e.d_blo.as<const unsigned short>(),
+                       e.d_ctf2.as<const float>(), e.d_ynorm.as<const double>(), xi2, d2off, e.NT, e.igc.P, e.Kp,
+                       e.d_zero_row.as<const unsigned short>(), e.d_diff2.as<float>());
+}
+
+static int launch_coarse_mfma(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
+    const int ntt = (e.NT + 15) / 16;
+    switch (ntt) {
+#define RLN_MFMA_CASE(N)                          \
+    case N:                                       \
+        launch_mfma_t<N>(e, n_tiles, xi2, d2off); \
+        break;
+        RLN_MFMA_CASE(1) RLN_MFMA_CASE(2) RLN_MFMA_CASE(3) RLN_MFMA_CASE(4) RLN_MFMA_CASE(5) RLN_MFMA_CASE(6)
+        RLN_MFMA_CASE(7) RLN_MFMA_CASE(8) RLN_MFMA_CASE(9) RLN_MFMA_CASE(10) RLN_MFMA_CASE(11) RLN_MFMA_CASE(12)
+        RLN_MFMA_CASE(13) RLN_MFMA_CASE(14) RLN_MFMA_CASE(15) RLN_MFMA_CASE(16)
+#undef RLN_MFMA_CASE
+        default:
+            RLN_FAIL(RLN_ERR_ARG, "split-BF16 coarse pass supports at most 256 translations (%d)", e.NT);
+    }
+    return launch_check("k_coarse_mfma");
+}
+
 static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, const float *fctf, int on_device,
                      const rln_particle_meta *meta, rln_particle_result *results, rln_wsum *wsum) {
     const rln_estep_config &cfg = e.cfg;
     const int K = e.K, NT = e.NT;
     hipStream_t st = e.stream;
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t0) {
+        return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    };
+    const clk::time_point t_build = clk::now();
     ChunkHost h;
     build_chunk_host(e, B, meta, h);
+    e.host_ms[0] += ms_since(t_build);
     for (int b = 0; b < B; ++b)
         if (meta[b].scale > 10000. && cfg.do_scale_correction)
             RLN_FAIL(RLN_ERR_ARG, "rlnMicrographScaleCorrection is very high (particle %d)", b);
@@ -599,18 +660,36 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     const double *d_prior = d_old + 2 * B;
 
     // coarse: scaled images, shifted images
-    RLN_TRY(e.d_ctfq.ensure(sizeof(float) * Pc * B));
-    RLN_TRY(e.d_yq.ensure(sizeof(float2) * Pc * NT * B));
-    hipLaunchKernelGGL(rlnk::k_prep_coarse, dim3((Pc + rlnk::kBlock - 1) / rlnk::kBlock, NT, B), dim3(rlnk::kBlock), 0,
-                       st, dX, dC, d_scale, e.sf, (int)Pf, e.igc, e.d_q_c.as<const float>(),
-                       e.d_ex_c.as<const float2>(), e.d_ey_c.as<const float2>(), NT, cfg.do_ctf_correction,
-                       e.d_ctfq.as<float>(), e.d_yq.as<float2>());
-    RLN_TRY(launch_check("k_prep_coarse"));
+    const bool split = cfg.precision == RLN_PREC_SPLIT_BF16;
+    if (split) {
+        const size_t nb = (size_t)B * NT * e.Kp;
+        RLN_TRY(e.d_bhi.ensure(nb * sizeof(unsigned short)));
+        RLN_TRY(e.d_blo.ensure(nb * sizeof(unsigned short)));
+        RLN_TRY(e.d_ctf2.ensure(sizeof(float) * Pc * B));
+        RLN_TRY(e.d_ynorm.ensure(sizeof(double) * B));
+        HIP_TRY(hipMemsetAsync(e.d_ynorm.ptr, 0, sizeof(double) * B, st));
+        hipLaunchKernelGGL(rlnk::k_prep_coarse_split, dim3((e.Kp / 2 + rlnk::kBlock - 1) / rlnk::kBlock, NT, B),
+                           dim3(rlnk::kBlock), 0, st, dX, dC, d_scale, e.sf, (int)Pf, e.igc, e.d_q_c.as<const float>(),
+                           e.d_ex_c.as<const float2>(), e.d_ey_c.as<const float2>(), NT, cfg.do_ctf_correction, e.Kp,
+                           e.d_bhi.as<unsigned short>(), e.d_blo.as<unsigned short>(), e.d_ctf2.as<float>(),
+                           e.d_ynorm.as<double>());
+        RLN_TRY(launch_check("k_prep_coarse_split"));
+    } else {
+        RLN_TRY(e.d_ctfq.ensure(sizeof(float) * Pc * B));
+        RLN_TRY(e.d_yq.ensure(sizeof(float2) * Pc * NT * B));
+        hipLaunchKernelGGL(rlnk::k_prep_coarse, dim3((Pc + rlnk::kBlock - 1) / rlnk::kBlock, NT, B),
+                           dim3(rlnk::kBlock), 0, st, dX, dC, d_scale, e.sf, (int)Pf, e.igc,
+                           e.d_q_c.as<const float>(), e.d_ex_c.as<const float2>(), e.d_ey_c.as<const float2>(), NT,
+                           cfg.do_ctf_correction, e.d_ctfq.as<float>(), e.d_yq.as<float2>());
+        RLN_TRY(launch_check("k_prep_coarse"));
+    }
     HIP_TRY(hipEventRecord(e.ev[kStCoarse], st));
     RLN_TRY(e.d_diff2.ensure(sizeof(float) * std::max<long long>(1, h.total_entries)));
     RLN_TRY(e.d_weight.ensure(sizeof(double) * std::max<long long>(1, h.total_entries)));
     RLN_TRY(e.d_mask.ensure(std::max<long long>(1, h.total_entries)));
-    if (!h.tiles.empty()) {
+    if (split && !h.tiles.empty()) {
+        RLN_TRY(launch_coarse_mfma(e, (unsigned)h.tiles.size(), d_xi2, d_d2off.as<const long long>()));
+    } else if (!h.tiles.empty()) {
         hipLaunchKernelGGL(rlnk::k_coarse_diff2, dim3((unsigned)h.tiles.size(), (NT + rlnk::kCoTT - 1) / rlnk::kCoTT),
                            dim3(rlnk::kBlock), 0, st, e.d_tiles.as<const rlnk::CoarseTile>(), e.d_or_off.as<const int>(),
                            e.d_or_cnt.as<const int>(), e.d_or_proj.as<const int>(), e.d_proj_c.as<const float2>(),
@@ -677,6 +756,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     std::vector<rlnk::FineJob> jobs(n_jobs);
     if (n_jobs) HIP_TRY(hipMemcpyAsync(jobs.data(), e.d_jobs.ptr, sizeof(rlnk::FineJob) * n_jobs, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const clk::time_point t_jobs = clk::now();
     // blocks of the fine pass: jobs with <= 2 significant translations use the 2-translation variant, the
     // others blocks of 4 (the 8 projections of a pixel are recomputed once per block)
     std::vector<int> cjob, cfirst;
@@ -694,6 +774,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
             }
     RLN_TRY(upload(e.d_chunk_job, cjob, st));
     RLN_TRY(upload(e.d_chunk_first, cfirst, st));
+    e.host_ms[1] += ms_since(t_jobs);
     HIP_TRY(hipEventRecord(e.ev[kStFine], st));
     // fine diff2
     RLN_TRY(e.d_fdiff2.ensure(sizeof(float) * 32 * std::max<long long>(1, n_pairs)));
@@ -857,6 +938,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         }
     }
     HIP_TRY(hipStreamSynchronize(st));
+    const clk::time_point t_epi = clk::now();
     for (int i = 0; i < kStCount; ++i) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e.ev[i], e.ev[i + 1]);
@@ -897,11 +979,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         const int ND = h.nd[b], NP = h.np[b];
         const int k = o / (ND * NP), id = (o / NP) % ND, ip = o % NP;
         const int gd = h.dir_idx[h.dir_off[b] + id], gp = h.psi_idx[h.psi_off[b] + ip];
-        double r8[8], t8[8], p8[8];
-        e.grid.oversampled_orientations(gd, gp, r8, t8, p8);
-        r.rot = r8[rr];
-        r.tilt = t8[rr];
-        r.psi = p8[rr];
+        e.grid.oversampled_orientation(gd, gp, rr, r.rot, r.tilt, r.psi);
         r.xoff = meta[b].old_offset_x + e.over_x[t * 4 + u];
         r.yoff = meta[b].old_offset_y + e.over_y[t * 4 + u];
         r.iclass = k + 1;
@@ -928,6 +1006,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
             wsum->ave_Pmax += r.pmax;
         }
     }
+    e.host_ms[2] += ms_since(t_epi);
     return RLN_OK;
 }
 
@@ -1079,7 +1158,8 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
     if (!e) return RLN_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    rln::DevBuf *bufs[] = {&e->d_frames_c, &e->d_frames_f, &e->d_proj_c, &e->d_q_c, &e->d_q_f, &e->d_minv_w,
+    rln::DevBuf *bufs[] = {&e->d_frames_c, &e->d_frames_f, &e->d_proj_c, &e->d_phi, &e->d_plo, &e->d_pn2,
+                           &e->d_zero_row, &e->d_bhi, &e->d_blo, &e->d_ctf2, &e->d_ynorm, &e->d_q_c, &e->d_q_f, &e->d_minv_w,
                            &e->d_mresol, &e->d_xa_shell, &e->d_ex_c, &e->d_ey_c, &e->d_ex_f, &e->d_ey_f,
                            &e->d_tr_ix, &e->d_tr_iy, &e->d_over_x, &e->d_over_y, &e->d_vols, &e->d_bp_ptrs,
                            &e->d_fimg, &e->d_fimgn, &e->d_fctf, &e->d_meta_f, &e->d_meta_d, &e->d_meta_ll,
@@ -1154,7 +1234,9 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     if (!fimg_nomask) fimg_nomask = fimg;
     HIP_TRY(hipSetDevice(e->device));
     for (int i = 0; i < rln::kStCount; ++i) e->stage_ms[i] = 0.;
+    for (double &h : e->host_ms) h = 0.;
     for (long long &c : e->counters) c = 0;
+    const auto t_run = std::chrono::steady_clock::now();
     HIP_TRY(hipMemsetAsync(e->d_pdfd_acc.ptr, 0, sizeof(double) * (size_t)e->K * e->grid.n_dir, e->stream));
     RLN_TRY(rln::engine_coarse_projections(*e));
     // chunk size from a device-memory budget for the coarse intermediates
@@ -1184,6 +1266,24 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
         for (size_t i = 0; i < pd.size(); ++i) wsum->pdf_direction[i] += pd[i];
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
+    e->host_ms[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_run).count();
+    return RLN_OK;
+}
+
+extern "C" int rln_selftest_mfma_bf16(int device, const float *A, const float *B, float *D) {
+    if (!A || !B || !D) RLN_FAIL(RLN_ERR_ARG, "null argument");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) RLN_FAIL(RLN_ERR_ARG, "no HIP device %d", device);
+    HIP_TRY(hipSetDevice(device));
+    rln::DevBuf buf;
+    RLN_TRY(buf.ensure(sizeof(float) * (16 * 32 + 32 * 16 + 16 * 16)));
+    float *dA = buf.as<float>(), *dB = dA + 16 * 32, *dD = dB + 32 * 16;
+    HIP_TRY(hipMemcpy(dA, A, sizeof(float) * 16 * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dB, B, sizeof(float) * 32 * 16, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(rlnk::k_selftest_mfma, dim3(1), dim3(64), 0, 0, dA, dB, dD);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(D, dD, sizeof(float) * 16 * 16, hipMemcpyDeviceToHost));
     return RLN_OK;
 }
 
@@ -1239,8 +1339,8 @@ extern "C" int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, 
 
 extern "C" int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out) {
     if (!e || !ms) RLN_FAIL(RLN_ERR_ARG, "null argument");
-    const int n = std::min<int>(n_max, rln::kStCount);
-    for (int i = 0; i < n; ++i) ms[i] = e->stage_ms[i];
+    const int n = std::min<int>(n_max, rln::kStCount + 4);
+    for (int i = 0; i < n; ++i) ms[i] = i < rln::kStCount ? e->stage_ms[i] : e->host_ms[i - rln::kStCount];
     if (n_out) *n_out = n;
     return RLN_OK;
 }

@@ -288,6 +288,228 @@ __global__ void __launch_bounds__(kBlock) k_coarse_diff2(const CoarseTile *__res
 }
 
 // --------------------------------------------------------------------------------------------------
+// Split-BF16 coarse pass on MFMA.
+// diff2(o,t) = Xi2/2 + sum_p c_p^2 |P_o(p)|^2 + sum_p |y_t(p)|^2 - 2 sum_p Re(P_o(p) conj(c_p y_t(p))),
+// with y_t(p) = q_p X(p) exp(-2 pi i (x tx + y ty)/N) (|y_t|^2 does not depend on t) and c_p = q_p ctf_p s.
+// The cross term is a real contraction over k = 2p + {re, im} of A[o][k] (the projection, shared by all
+// particles) and B[t][k] = c_p y_t(p); both are split x = hi + lo in BF16 and the products hi*hi + hi*lo +
+// lo*hi are accumulated in FP32 by v_mfma_f32_16x16x32_bf16.
+// --------------------------------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even FP32 -> BF16 bit pattern
+__device__ __forceinline__ unsigned short f2bf(float x) {
+    unsigned int u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+__device__ __forceinline__ void split_bf16(float x, unsigned short &hi, unsigned short &lo) {
+    hi = f2bf(x);
+    lo = f2bf(x - bf2f(hi));
+}
+__device__ __forceinline__ bf16x8_t load_bf16x8(const unsigned short *p) {
+    return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(p));
+}
+
+// Lane layout of v_mfma_f32_16x16x32_bf16 used below (checked by k_selftest_mfma): lane l supplies row l%16 of
+// A and column l%16 of B, each with the 8 consecutive k = 8*(l/16) .. 8*(l/16)+7; it receives D rows
+// 4*(l/16) .. 4*(l/16)+3 of column l%16.
+__global__ void k_selftest_mfma(const float *__restrict__ A, const float *__restrict__ B, float *__restrict__ D) {
+    const int l = threadIdx.x;
+    unsigned short a[8], b[8];
+    for (int j = 0; j < 8; ++j) {
+        a[j] = f2bf(A[(l % 16) * 32 + 8 * (l / 16) + j]);
+        b[j] = f2bf(B[(8 * (l / 16) + j) * 16 + l % 16]);
+    }
+    floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(a), load_bf16x8(b), acc, 0, 0, 0);
+    for (int j = 0; j < 4; ++j) D[(4 * (l / 16) + j) * 16 + l % 16] = acc[j];
+}
+
+// Split projection tables of the coarse grid: hi/lo [K*n_or][Kp] (k = 2p + re/im, zero padded to Kp) and the
+// FP32 power |P|^2 [K*n_or][P]. grid (K*n_or, ceil(Kp/2/256)).
+__global__ void __launch_bounds__(kBlock) k_project_grid_split(const float4 *const *__restrict__ vols, VolGeom vg,
+                                                               const Frame *__restrict__ frames, int n_or, ImgGeom ig,
+                                                               int Kp, unsigned short *__restrict__ phi,
+                                                               unsigned short *__restrict__ plo,
+                                                               float *__restrict__ pn2) {
+    const int p = blockIdx.y * kBlock + threadIdx.x;
+    const int ko = blockIdx.x;
+    if (2 * p >= Kp) return;
+    float2 v = make_float2(0.f, 0.f);
+    if (p < ig.P) {
+        const int k = ko / n_or, o = ko - k * n_or;
+        int x, y;
+        pixel_xy(p, ig.s, ig.xs, x, y);
+        if (x * x + y * y <= ig.proj_r2_max) v = project_pixel(vols[k], vg, frames[o], x, y);
+        pn2[(size_t)ko * ig.P + p] = v.x * v.x + v.y * v.y;
+    }
+    unsigned short h, l;
+    split_bf16(v.x, h, l);
+    phi[(size_t)ko * Kp + 2 * p] = h;
+    plo[(size_t)ko * Kp + 2 * p] = l;
+    split_bf16(v.y, h, l);
+    phi[(size_t)ko * Kp + 2 * p + 1] = h;
+    plo[(size_t)ko * Kp + 2 * p + 1] = l;
+}
+
+// Translation-side operand: B[b][t][k] = c_p y_t(p) split in BF16, k = 2p + re/im, zero padded to Kp; ctf2[b][p] =
+// c_p^2; ynorm[b] += sum_p |y(p)|^2 (FP64). grid (ceil(Kp/2/256), NT, B).
+__global__ void __launch_bounds__(kBlock) k_prep_coarse_split(const float2 *__restrict__ fimg,
+                                                              const float *__restrict__ fctf,
+                                                              const float *__restrict__ scale, int sf, int Pf,
+                                                              ImgGeom ig, const float *__restrict__ q,
+                                                              const float2 *__restrict__ ex,
+                                                              const float2 *__restrict__ ey, int NT, int do_ctf, int Kp,
+                                                              unsigned short *__restrict__ bhi,
+                                                              unsigned short *__restrict__ blo,
+                                                              float *__restrict__ ctf2, double *__restrict__ ynorm) {
+    __shared__ double red[kBlock / 64];
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int t = blockIdx.y, b = blockIdx.z;
+    float2 v = make_float2(0.f, 0.f);
+    float yn = 0.f;
+    if (p < ig.P) {
+        const int i = p / ig.xs, x = p - i * ig.xs;
+        const int y = (i < ig.xs) ? i : i - ig.s;
+        const int isrc = (y < 0) ? y + sf : y;
+        const size_t src = (size_t)b * Pf + (size_t)isrc * (sf / 2 + 1) + x;
+        const float qp = q[p];
+        const float2 X = fimg[src];
+        const float c = qp * (do_ctf ? fctf[src] : 1.f) * scale[b];
+        const float2 ph = cmul(ex[(size_t)t * ig.xs + x], ey[(size_t)t * ig.s + i]);
+        const float2 yv = cmul(make_float2(X.x * qp, X.y * qp), ph);
+        v = make_float2(c * yv.x, c * yv.y);
+        yn = qp * qp * (X.x * X.x + X.y * X.y);
+        if (t == 0) ctf2[(size_t)b * ig.P + p] = c * c;
+    }
+    if (2 * p < Kp) {
+        const size_t base = ((size_t)b * NT + t) * Kp + 2 * p;
+        unsigned short h, l;
+        split_bf16(v.x, h, l);
+        bhi[base] = h;
+        blo[base] = l;
+        split_bf16(v.y, h, l);
+        bhi[base + 1] = h;
+        blo[base + 1] = l;
+    }
+    if (t == 0) {
+        double s = yn;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(&ynorm[b], red[0] + red[1] + red[2] + red[3]);
+    }
+}
+
+constexpr int kMfO = 64;  // orientations per block (4 MFMA row tiles)
+
+// One block (4 waves) per (particle, 64 orientations) and all translations. Wave w owns translation tiles
+// w, w+4, w+8, ... and all four orientation tiles. grid (n_tiles).
+template <int NTT>  // translation tiles of 16 in the block (16*NTT >= NT)
+__global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__restrict__ tiles,
+                                                        const int *__restrict__ or_off, const int *__restrict__ or_cnt,
+                                                        const int *__restrict__ or_proj,
+                                                        const unsigned short *__restrict__ phi,
+                                                        const unsigned short *__restrict__ plo,
+                                                        const float *__restrict__ pn2,
+                                                        const unsigned short *__restrict__ bhi,
+                                                        const unsigned short *__restrict__ blo,
+                                                        const float *__restrict__ ctf2, const double *__restrict__ ynorm,
+                                                        const float *__restrict__ xi2half,
+                                                        const long long *__restrict__ d2_off, int NT, int P, int Kp,
+                                                        const unsigned short *__restrict__ zero_row,
+                                                        float *__restrict__ diff2) {
+    constexpr int kTW = (NTT + 3) / 4;  // translation tiles per wave
+    __shared__ int og_s[kMfO];
+    __shared__ float a_norm[kMfO];
+    const CoarseTile tile = tiles[blockIdx.x];
+    const int b = tile.b;
+    const int n_o = or_cnt[b];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < kMfO) {
+        const int o = tile.o_begin + tid;
+        og_s[tid] = (o < n_o) ? or_proj[or_off[b] + o] : -1;
+    }
+    __syncthreads();
+    // sum_p c_p^2 |P_o(p)|^2 for the block's orientations: 4 lanes per orientation
+    {
+        const int oo = tid >> 2, part = tid & 3;
+        const int og = og_s[oo];
+        float s = 0.f;
+        if (og >= 0) {
+            const float *pn = pn2 + (size_t)og * P;
+            const float *c2 = ctf2 + (size_t)b * P;
+            for (int p = part; p < P; p += 4) s = fmaf(c2[p], pn[p], s);
+        }
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if (part == 0) a_norm[oo] = s;
+    }
+    __syncthreads();
+    floatx4_t acc[4][kTW];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < kTW; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    const int r16 = lane & 15, kq = 8 * (lane >> 4);
+    const unsigned short *arow_hi[4], *arow_lo[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int og = og_s[16 * i + r16];
+        arow_hi[i] = og >= 0 ? phi + (size_t)og * Kp : zero_row;
+        arow_lo[i] = og >= 0 ? plo + (size_t)og * Kp : zero_row;
+    }
+    const unsigned short *brow_hi[kTW], *brow_lo[kTW];
+#pragma unroll
+    for (int j = 0; j < kTW; ++j) {
+        const int t = 16 * (wave + 4 * j) + r16;
+        const bool ok = (wave + 4 * j) < NTT && t < NT;
+        brow_hi[j] = ok ? bhi + ((size_t)b * NT + t) * Kp : zero_row;
+        brow_lo[j] = ok ? blo + ((size_t)b * NT + t) * Kp : zero_row;
+    }
+    for (int k0 = 0; k0 < Kp; k0 += 32) {
+        bf16x8_t ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ah[i] = load_bf16x8(arow_hi[i] + k0 + kq);
+            al[i] = load_bf16x8(arow_lo[i] + k0 + kq);
+        }
+#pragma unroll
+        for (int j = 0; j < kTW; ++j) {
+            if (wave + 4 * j >= NTT) continue;
+            const bf16x8_t bh = load_bf16x8(brow_hi[j] + k0 + kq);
+            const bf16x8_t bl = load_bf16x8(brow_lo[j] + k0 + kq);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+    const float base = xi2half[b] + (float)ynorm[b];
+    float *out = diff2 + d2_off[b];
+#pragma unroll
+    for (int j = 0; j < kTW; ++j) {
+        if (wave + 4 * j >= NTT) continue;
+        const int t = 16 * (wave + 4 * j) + r16;  // D column
+        if (t >= NT) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int oo = 16 * i + (lane >> 4) * 4 + r;  // D row
+                const int o = tile.o_begin + oo;
+                if (o >= n_o) continue;
+                out[(size_t)o * NT + t] = og_s[oo] < 0 ? -999.f : base + a_norm[oo] - 2.f * acc[i][j][r];
+            }
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
 // Weights and significance (convertAllSquaredDifferencesToWeights, src/ml_optimiser.cpp:7704-8239)
 // --------------------------------------------------------------------------------------------------
 

@@ -180,6 +180,15 @@ void PoseGrid::oversampled_orientations(int idir, int ipsi, double *rot8, double
         }
 }
 
+void PoseGrid::oversampled_orientation(int idir, int ipsi, int iover_rot, double &r, double &t, double &p) const {
+    int x, y, face;
+    hp_nest2xyf(order, idir, x, y, face);
+    const int sub = iover_rot >> 1, q = iover_rot & 1;
+    const int dj = sub >> 1, di = sub & 1;
+    healpix_direction(order + 1, hp_xyf2nest(order + 1, 2 * x + di, 2 * y + dj, face), r, t);
+    p = psi[ipsi] - 0.5 * psi_step + (0.5 + q) * psi_step / 2.;
+}
+
 void PoseGrid::oversampled_translations(int itrans, double *x4, double *y4) const {
     int k = 0;
     for (int ox = 0; ox < 2; ++ox)

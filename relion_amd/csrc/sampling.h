@@ -21,6 +21,8 @@ struct PoseGrid {
     bool build(const rln_estep_config &cfg);
     // getOrientations at oversampling order 1: 8 (rot, tilt, psi), src/healpix_sampling.cpp:1832-1966.
     void oversampled_orientations(int idir, int ipsi, double *rot8, double *tilt8, double *psi8) const;
+    // one of them: iover_rot = (2*dj + di)*2 + q (sub-pixel row dj, column di, psi half q)
+    void oversampled_orientation(int idir, int ipsi, int iover_rot, double &rot, double &tilt, double &psi) const;
     // getTranslationsInPixel at oversampling order 1: 4 shifts in pixels, src/healpix_sampling.cpp:1724-1830.
     void oversampled_translations(int itrans, double *x4, double *y4) const;
     // selectOrientationsWithNonZeroPriorProbability (src/healpix_sampling.cpp:695-1076), C1, 3-D.
