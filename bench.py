@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
+PEAK_BF16_TFLOPS = 2500.0  # MI355X published dense BF16 MFMA peak
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 
 
@@ -37,6 +38,8 @@ def parse():
                     help="CPU oracle budget: batches of one particle per thread until this much time (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--paths", default="plain,split",
+                    help="comma list of E-step paths to time: plain (FP32 fallback), split (split-BF16 MFMA coarse)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference for traffic counters)")
@@ -85,12 +88,34 @@ def main():
     proj.initMdl(vol)
     bp = ra.AccBackprojector(0)
     bp.setMdlDim(*dims, 2.0)
-    es = ra.ExpectationStep(cfg, 0)
+    precisions = {"plain": ra.RLN_PREC_PLAIN_FP32, "split": ra.RLN_PREC_SPLIT_BF16}
 
-    def step():
-        bp.initMdl()
-        es.set_model([proj], [bp], [1.0], sigma2)   # per-iteration model: invalidates the grid projections
-        return es.run(d_img.address, d_ctf.address, meta, on_device=True)
+    def time_path(name):
+        cfg.precision = precisions[name]
+        es = ra.ExpectationStep(cfg, 0)
+
+        def step():
+            bp.initMdl()
+            es.set_model([proj], [bp], [1.0], sigma2)   # per-iteration model: invalidates the grid projections
+            return es.run(d_img.address, d_ctf.address, meta, on_device=True)
+
+        for w in range(args.warmup):
+            t = time.time()
+            step()
+            log(f"[{name}] warmup {w}: {time.time() - t:.2f} s", args)
+        hipmem.synchronize()
+        t_start = time.time()
+        for _ in range(args.steps):
+            res, wsum = step()
+        hipmem.synchronize()
+        elapsed = time.time() - t_start
+        timings, counters = es.timings(), es.counters()
+        log(f"[{name}] timed {args.steps} steps: {elapsed:.2f} s; stages "
+            f"{json.dumps({k: round(v, 1) for k, v in timings.items()})}", args)
+        log(f"[{name}] counters {json.dumps(counters)}", args)
+        es.close()
+        return dict(res=res, elapsed=elapsed, timings=timings, counters=counters,
+                    value=n * args.steps / elapsed, ms=1000.0 * elapsed / args.steps)
 
     if args.copy_calibration_mb > 0:
         nb = args.copy_calibration_mb << 20
@@ -100,23 +125,11 @@ def main():
         hipmem.synchronize()
         log(f"copy calibration: {nb} bytes device-to-device", args)
         del src, dst
-    for w in range(args.warmup):
-        t = time.time()
-        step()
-        log(f"warmup {w}: {time.time() - t:.2f} s", args)
-    hipmem.synchronize()
-    t_start = time.time()
-    for s in range(args.steps):
-        res, wsum = step()
-    hipmem.synchronize()
-    elapsed = time.time() - t_start
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = n * args.steps / elapsed
-    timings = es.timings()
-    counters = es.counters()
-    log(f"timed {args.steps} steps: {elapsed:.2f} s; stages {json.dumps({k: round(v, 1) for k, v in timings.items()})}",
-        args)
-    log(f"counters {json.dumps(counters)}", args)
+    runs = {name: time_path(name) for name in args.paths.split(",")}
+    head = "split" if "split" in runs else "plain"
+    R = runs[head]
+    res, timings, counters = R["res"], R["timings"], R["counters"]
+    value, ms_per_step = R["value"], R["ms"]
 
     # accuracy against the generating poses (orientation error of the max-likelihood pose)
     def rotm(a, b, c):
@@ -132,17 +145,24 @@ def main():
         errs.append(np.degrees(np.arccos(np.clip((np.trace(A @ B.T) - 1) / 2, -1, 1))))
     errs = np.array(errs)
 
-    # roofline of the dominant kernel: the fine-pass squared differences
+    # roofline of the dominant kernel of the reported path
     stage = max(("fine_diff2", "coarse_diff2", "wsum_backproject"), key=lambda k: timings[k])
-    if stage == "fine_diff2":
-        work = 4.0 * counters["fine_poses"] * counters["fine_pixels"]          # FLOP, cross-term count
-    elif stage == "coarse_diff2":
-        work = 4.0 * counters["coarse_poses"] * counters["coarse_pixels"]
+    if stage == "coarse_diff2":
+        work = 4.0 * counters["coarse_poses"] * counters["coarse_pixels"]      # FLOP, cross-term count
+        bound, unit = "mfma", "TFLOP/s"
+        peak = PEAK_BF16_TFLOPS / 3.0 if head == "split" else PEAK_FP32_TFLOPS
+        achieved = work / (timings[stage] * 1e-3) / 1e12
     else:
-        work = 8.0 * counters["wsum_pairs"] * counters["fine_pixels"]          # FLOP of the weighted sums
-    achieved = work / (timings[stage] * 1e-3) / 1e12
-    roofline = {"kernel": stage, "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        # gather-bound: 64 bytes of the corner-packed reference per (fine orientation, pixel) sample, plus the
+        # 128-byte slot line read and written per backprojected sample in the weighted sums
+        if stage == "fine_diff2":
+            work = 64.0 * 8 * counters["fine_jobs"] * counters["fine_pixels"]
+        else:
+            work = (64.0 + 256.0) * counters["wsum_orientations"] * counters["fine_pixels"]
+        bound, unit, peak = "hbm", "GB/s", PEAK_HBM_GBS
+        achieved = work / (timings[stage] * 1e-3) / 1e9
+    roofline = {"kernel": stage, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 1),
+                "unit": unit, "frac": round(achieved / peak, 4),
                 "traffic": None if args.traffic_kib is None else round(args.traffic_kib * 1024.0),
                 "stage_ms": round(timings[stage], 2), "launches": counters["chunks"],
                 "avg_launch_ms": round(timings[stage] / max(1, counters["chunks"]), 3)}
@@ -168,12 +188,17 @@ def main():
         "metric": "E-step particles/s, box 256 refine: plain FP32 vs proposal+split-BF16",
         "value": round(value, 3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"refine box{box} order{order} local-search {n} particles", "path": "plain-fp32",
+        "dtype": "bf16x3+f32" if head == "split" else "f32", "data": "synthetic",
+        "config": {"workload": f"refine box{box} order{order} local-search {n} particles",
+                   "path": "split-bf16 coarse + fp32 fine" if head == "split" else "plain-fp32",
                    "global_batch": n, "coarse_size": coarse, "sigma_ang_deg": 7.5, "particle_diameter_A": diameter,
                    "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu,
         "stages_ms": {k: round(v, 2) for k, v in timings.items()}, "counters": counters,
+        "paths": {k: {"particles_per_s": round(v["value"], 1), "ms_per_step": round(v["ms"], 2),
+                      "stages_ms": {s_: round(t_, 2) for s_, t_ in v["timings"].items()}} for k, v in runs.items()},
+        "speedup_vs_plain": (round(runs["split"]["value"] / runs["plain"]["value"], 3)
+                             if "split" in runs and "plain" in runs else None),
         "accuracy": {"median_angular_error_deg": round(float(np.median(errs)), 3),
                      "frac_within_3deg": round(float((errs < 3.0).mean()), 4)},
     }

@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -586,8 +587,10 @@ static int launch_coarse_mfma(Engine &e, unsigned n_tiles, const float *xi2, con
     return launch_check("k_coarse_mfma");
 }
 
+// The host lists of a chunk (h) are built beforehand, for the chunk after the first on a second host thread
+// while the device runs the previous chunk.
 static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, const float *fctf, int on_device,
-                     const rln_particle_meta *meta, rln_particle_result *results, rln_wsum *wsum) {
+                     const rln_particle_meta *meta, const ChunkHost &h, rln_particle_result *results, rln_wsum *wsum) {
     const rln_estep_config &cfg = e.cfg;
     const int K = e.K, NT = e.NT;
     hipStream_t st = e.stream;
@@ -595,10 +598,6 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     auto ms_since = [](clk::time_point t0) {
         return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     };
-    const clk::time_point t_build = clk::now();
-    ChunkHost h;
-    build_chunk_host(e, B, meta, h);
-    e.host_ms[0] += ms_since(t_build);
     for (int b = 0; b < B; ++b)
         if (meta[b].scale > 10000. && cfg.do_scale_correction)
             RLN_FAIL(RLN_ERR_ARG, "rlnMicrographScaleCorrection is very high (particle %d)", b);
@@ -1251,11 +1250,38 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     const double budget = 24.0e9;
     int chunk = (int)std::max(1.0, std::min(4096.0, budget / per_particle));
     if (e->keep) chunk = std::max(chunk, n);
+    // chunk c runs on the device while a second host thread builds the lists of chunk c + 1
+    using clk = std::chrono::steady_clock;
+    rln::ChunkHost cur, nxt;
+    if (n > 0) {
+        const clk::time_point t0 = clk::now();
+        rln::build_chunk_host(*e, std::min(chunk, n), meta, cur);
+        e->host_ms[0] += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    }
     for (int i0 = 0; i0 < n; i0 += chunk) {
         const int B = std::min(chunk, n - i0);
+        const int i1 = i0 + chunk;
+        bool next_ok = true;
+        std::thread builder;
+        if (i1 < n)
+            builder = std::thread([&, i1] {
+                try {
+                    rln::build_chunk_host(*e, std::min(chunk, n - i1), meta + i1, nxt);
+                } catch (...) {
+                    next_ok = false;
+                }
+            });
         const size_t off_c = (size_t)i0 * Pf;
-        RLN_TRY(rln::run_chunk(*e, B, fimg + 2 * off_c, fimg_nomask + 2 * off_c, fctf + off_c, inputs_on_device,
-                               meta + i0, results + i0, wsum));
+        const int rc = rln::run_chunk(*e, B, fimg + 2 * off_c, fimg_nomask + 2 * off_c, fctf + off_c,
+                                      inputs_on_device, meta + i0, cur, results + i0, wsum);
+        if (builder.joinable()) {
+            const clk::time_point t0 = clk::now();
+            builder.join();
+            e->host_ms[0] += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        }
+        if (rc != RLN_OK) return rc;
+        if (!next_ok) RLN_FAIL(RLN_ERR_MEMORY, "rln_estep_run: building the lists of a chunk failed");
+        std::swap(cur, nxt);
     }
     e->have_timing = true;
     if (wsum && wsum->pdf_direction) {

@@ -527,6 +527,7 @@ struct PassStats {
 };
 
 constexpr int kWBlock = 1024;
+constexpr int kCand = 2048;  // threshold candidates sorted in LDS
 
 __device__ __forceinline__ double block_sum_d(double v, double *red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -634,6 +635,10 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
     __shared__ unsigned int hcnt[256];
     __shared__ double hsum[256];
     __shared__ unsigned long long s_prefix, s_mask;
+    __shared__ double s_cand[kCand];
+    __shared__ int s_ncand, s_cand_ok;
+    __shared__ double s_cand_thr;
+    __shared__ long long s_cand_nsig;
     __shared__ double s_above_sum;
     __shared__ long long s_above_cnt;
     __shared__ int s_done;
@@ -682,8 +687,85 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
         return;
     }
 
-    // radix selection of the crossing element: 8 digits of 8 bits from the most significant
     const double target = a.adaptive_fraction * sum;
+    double thr = 0.;
+    long long nr_sign = 0;
+    bool have_thr = false;
+    // Candidates: only weights >= (1 - f) * sum / n can be the threshold, since all smaller weights together
+    // hold less than the (1 - f) tail. When they are few, sort them in LDS and accumulate in descending order
+    // exactly as the reference's loop does (src/ml_optimiser.cpp:8145-8168).
+    {
+        const double xmin = (1. - a.adaptive_fraction) * sum / (double)(n > 0 ? n : 1);
+        if (tid == 0) s_ncand = 0;
+        __syncthreads();
+        for (long long e = tid; e < n; e += kWBlock) {
+            const double v = w[e];
+            if (v > 0. && v >= xmin) {
+                const int slot = atomicAdd(&s_ncand, 1);
+                if (slot < kCand) s_cand[slot] = v;
+            }
+        }
+        __syncthreads();
+        const int nc = s_ncand;
+        if (nc <= kCand) {
+            for (int i = tid; i < kCand; i += kWBlock)
+                if (i >= nc) s_cand[i] = -1.;
+            __syncthreads();
+            // bitonic sort, ascending
+            for (int ks = 2; ks <= kCand; ks <<= 1)
+                for (int j = ks >> 1; j > 0; j >>= 1) {
+                    for (int i = tid; i < kCand; i += kWBlock) {
+                        const int l = i ^ j;
+                        if (l > i) {
+                            const double x = s_cand[i], y = s_cand[l];
+                            if (((i & ks) == 0) ? (x > y) : (x < y)) {
+                                s_cand[i] = y;
+                                s_cand[l] = x;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            // descending cumulative sum by wave 0, 64 candidates per step (inclusive lane scan)
+            if (tid < 64) {
+                double base = 0.;
+                int crossed = 0;
+                long long rank = -1;
+                for (int r0 = 0; r0 < nc; r0 += 64) {
+                    const int r = r0 + tid;
+                    const double v = (r < nc) ? s_cand[kCand - 1 - r] : 0.;
+                    double x = v;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const double y = __shfl_up(x, o);
+                        if (tid >= o) x += y;
+                    }
+                    const double cum = base + x;
+                    const unsigned long long hit = __ballot(r < nc && cum > target);
+                    if (hit) {
+                        crossed = 1;
+                        rank = r0 + (__ffsll((long long)hit) - 1);
+                        break;
+                    }
+                    base = __shfl(cum, 63);
+                }
+                if (tid == 0) {
+                    long long nsig = rank + 1;
+                    if (crossed && a.maxsig > 0 && !a.fine && nsig > a.maxsig) nsig = a.maxsig;
+                    s_cand_ok = crossed;
+                    s_cand_nsig = nsig;
+                    s_cand_thr = crossed ? s_cand[kCand - nsig] : 0.;
+                }
+            }
+            __syncthreads();
+            if (s_cand_ok) {
+                have_thr = true;
+                thr = s_cand_thr;
+                nr_sign = s_cand_nsig;
+            }
+        }
+    }
+    if (!have_thr) {
+    // radix selection of the crossing element: 8 digits of 8 bits from the most significant
     if (tid == 0) {
         s_prefix = 0ULL;
         s_mask = 0ULL;
@@ -739,9 +821,8 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
         }
         __syncthreads();
     }
-    double thr = __longlong_as_double((long long)s_prefix);
+    thr = __longlong_as_double((long long)s_prefix);
     // elements equal to thr needed to cross the target
-    long long nr_sign = 0;
     {
         const double rem = target - s_above_sum;
         long long m = (long long)floor(rem / thr) + 1;
@@ -787,6 +868,7 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
         thr = __longlong_as_double((long long)s_prefix);
         nr_sign = a.maxsig;
     }
+    }  // radix selection
     // mask, count above threshold, max weight with the reference's loop-order tie break
     long long above = 0;
     double bw = -1.;
