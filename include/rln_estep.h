@@ -189,6 +189,16 @@ int rln_estep_run(rln_estep *e, int n, const float *fimg, const float *fimg_noma
                   int inputs_on_device, const rln_particle_meta *meta,
                   rln_particle_result *results, rln_wsum *wsum);
 
+/* Explicit coarse orientation lists for the next rln_estep_run (global searches only), as produced by the
+ * learned pose proposal: particle i is searched over orientations set_orients[set_offsets[i] ..
+ * set_offsets[i+1]), each a global grid index idir * N_psi + ipsi of the coarse HEALPix grid, for all classes
+ * and translations; an empty list means the exhaustive grid (the fallback). Orientation priors are the global
+ * ones (pdf_direction normalised over the whole grid), so weights are those of the exhaustive search
+ * restricted to the list. No counterpart in the reference: the reference's coarse pass always enumerates the
+ * whole grid or the local-prior window (AccProjectorPlan::setup, src/acc/acc_projector_plan_impl.h:99-410).
+ * The lists apply to one run and are then cleared; their count must equal that run's n. */
+int rln_estep_set_orientation_sets(rln_estep *e, int n, const int *set_offsets, const int *set_orients);
+
 /* Stage hooks for parity tests. After rln_estep_run with keep_intermediates set, these copy the
  * last run's per-particle intermediates to host (particle index ip within that run):
  *   coarse: diff2 per coarse pose in ihidden order (K * N_dir_b * N_psi_b * N_tr), and weights;
@@ -225,6 +235,11 @@ int rln_sampling_grid(const rln_estep_config *cfg, double *rot, double *tilt, do
 int rln_sampling_local(const rln_estep_config *cfg, double prior_rot, double prior_tilt, double prior_psi,
                        int *dir_index, double *dir_prior, int max_dir, int *n_dir,
                        int *psi_index, double *psi_prior, int max_psi, int *n_psi);
+
+/* HEALPix NEST pixel (at order) containing the direction (rot, tilt) in degrees: ang2pix of Gorski et al.
+ * 2005 as Healpix_Base::ang2pix_z_phi (src/Healpix_2.15a/healpix_base.cc:368); the inverse of the grid's
+ * directions (HealpixSampling::getDirectionFromHealPix, src/healpix_sampling.cpp:1566-1660). */
+int rln_sampling_ang2pix(int order, double rot, double tilt, long long *pix);
 
 /* ---------------------------------------------------------------------------------------------
  * Synthetic data (seeded Gaussian-blob maps and particles; the reference's simulator is

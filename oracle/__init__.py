@@ -81,6 +81,8 @@ def ref_healpix():
     r.ref_hp_nest2xyf.argtypes = [C.c_int, C.c_longlong, _ip, _ip, _ip]
     r.ref_hp_xyf2nest.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     r.ref_hp_xyf2nest.restype = C.c_longlong
+    r.ref_hp_ang2pix_z_phi.argtypes = [C.c_int, C.c_double, C.c_double]
+    r.ref_hp_ang2pix_z_phi.restype = C.c_longlong
     r.ref_hp_npix.argtypes = [C.c_int]
     r.ref_hp_npix.restype = C.c_longlong
     return r

@@ -18,6 +18,10 @@ long long ref_hp_xyf2nest(int order, int ix, int iy, int face) {
     Healpix_Base b(order, NEST);
     return b.xyf2nest(ix, iy, face);
 }
+long long ref_hp_ang2pix_z_phi(int order, double z, double phi) {
+    Healpix_Base b(order, NEST);
+    return b.ang2pix_z_phi(z, phi);
+}
 long long ref_hp_npix(int order) {
     Healpix_Base b(order, NEST);
     return b.Npix();

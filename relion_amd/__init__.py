@@ -89,6 +89,18 @@ class SynthSpec(C.Structure):
     ]
 
 
+class ProposalConfig(C.Structure):
+    _fields_ = [("window", C.c_int), ("channels", C.c_int), ("healpix_order", C.c_int), ("n_psi_bins", C.c_int),
+                ("seed", C.c_ulonglong)]
+
+
+class ProposalGen(C.Structure):
+    _fields_ = [("ori_size", C.c_int), ("pixel_size", C.c_double), ("voltage", C.c_double), ("cs", C.c_double),
+                ("q0", C.c_double), ("defocus_min", C.c_double), ("defocus_max", C.c_double),
+                ("max_astigmatism", C.c_double), ("offset_max", C.c_double),
+                ("sigma2_noise", C.POINTER(C.c_double))]
+
+
 RESULT_DTYPE = np.dtype([(n, np.float64 if t in (C.c_double,) else (np.int64 if t is C.c_longlong else np.int32))
                          for n, t in ParticleResult._fields_])
 assert RESULT_DTYPE.itemsize == C.sizeof(ParticleResult)
@@ -119,6 +131,7 @@ _SIGNATURES = {
     "rln_estep_run": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.POINTER(ParticleMeta),
                                 C.POINTER(ParticleResult), C.POINTER(Wsum)]),
     "rln_estep_keep_intermediates": (C.c_int, [_vp, C.c_int]),
+    "rln_estep_set_orientation_sets": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "rln_estep_coarse_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _ip, _ip, _ip]),
     "rln_estep_get_coarse": (C.c_int, [_vp, C.c_int, _fp, _dp, C.POINTER(C.c_ubyte)]),
     "rln_estep_fine_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong)]),
@@ -135,6 +148,24 @@ _SIGNATURES = {
                                       _fp, _fp, C.POINTER(ParticleMeta), _dp, _dp]),
     "rln_ctf_value": (C.c_double, [C.c_double] * 11),
     "rln_selftest_mfma_bf16": (C.c_int, [C.c_int, _fp, _fp, _fp]),
+    "rln_sampling_ang2pix": (C.c_int, [C.c_int, C.c_double, C.c_double, C.POINTER(C.c_longlong)]),
+    # learned pose proposal (include/rln_proposal.h)
+    "rln_proposal_create": (C.c_int, [C.c_int, C.POINTER(ProposalConfig), C.POINTER(_vp)]),
+    "rln_proposal_destroy": (C.c_int, [_vp]),
+    "rln_proposal_num_params": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
+    "rln_proposal_num_bins": (C.c_int, [_vp, _ip]),
+    "rln_proposal_get_params": (C.c_int, [_vp, _fp]),
+    "rln_proposal_set_params": (C.c_int, [_vp, _fp]),
+    "rln_proposal_clear_training": (C.c_int, [_vp]),
+    "rln_proposal_generate": (C.c_int, [_vp, _vp, C.POINTER(ProposalGen), C.c_int, C.c_longlong, _dp]),
+    "rln_proposal_add_images": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, _dp]),
+    "rln_proposal_training_size": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
+    "rln_proposal_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
+    "rln_proposal_predict": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _ip, _fp]),
+    "rln_proposal_pose_bin": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _ip]),
+    "rln_proposal_bin_orientations": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip]),
+    "rln_proposal_images": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, _fp]),
+    "rln_proposal_forward_backward": (C.c_int, [_vp, C.c_int, _fp, _ip, _dp, _fp, _fp]),
 }
 
 STAGES = ("prep", "coarse_diff2", "coarse_weights", "jobs", "fine_diff2", "fine_weights", "wsum_backproject",
@@ -246,6 +277,13 @@ def sampling_local(cfg, prior_rot, prior_tilt, prior_psi):
                                   _ptr(dp, C.c_double), nd, C.byref(n1), _ptr(pi, C.c_int), _ptr(pp, C.c_double), npsi,
                                   C.byref(n2)))
     return di[:n1.value], dp[:n1.value], pi[:n2.value], pp[:n2.value]
+
+
+def ang2pix(order, rot, tilt):
+    """HEALPix NEST pixel at order containing direction (rot, tilt) in degrees."""
+    out = C.c_longlong()
+    _check(lib.rln_sampling_ang2pix(order, float(rot), float(tilt), C.byref(out)))
+    return out.value
 
 
 def selftest_mfma_bf16(A, B, device=0):
@@ -420,6 +458,16 @@ class ExpectationStep:
 
     def keep_intermediates(self, keep=True):
         _check(lib.rln_estep_keep_intermediates(self._h, 1 if keep else 0))
+
+    def set_orientation_sets(self, sets):
+        """Explicit coarse orientation lists for the next run: one array of global grid indices
+        (idir * n_psi + ipsi) per particle; an empty array means the exhaustive grid."""
+        counts = np.array([len(x) for x in sets], np.int32)
+        off = np.zeros(len(sets) + 1, np.int32)
+        np.cumsum(counts, out=off[1:])
+        og = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in sets]) if off[-1] else
+                                  np.zeros(1, np.int32), np.int32)
+        _check(lib.rln_estep_set_orientation_sets(self._h, len(sets), _ptr(off, C.c_int), _ptr(og, C.c_int)))
 
     def run(self, fimg, fctf, meta, fimg_nomask=None, on_device=False):
         """E-step of a batch. fimg/fctf: numpy arrays (host) or integer device addresses (on_device=True)."""

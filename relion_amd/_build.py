@@ -37,9 +37,10 @@ def build(verbose=True):
         o = os.path.join(OBJ, src.replace(".cpp", ".o"))
         _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    o = os.path.join(OBJ, "estep.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "estep.hip"), "-o", o])
-    objs.append(o)
+    for src in ("estep.hip", "proposal.hip"):
+        o = os.path.join(OBJ, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lgomp"])
     os.replace(tmp, LIB)

@@ -35,89 +35,7 @@ const char *last_error() { return g_last_error.c_str(); }
 
 }  // namespace rln
 
-#define HIP_TRY(expr)                                                                                     \
-    do {                                                                                                  \
-        hipError_t _e = (expr);                                                                           \
-        if (_e != hipSuccess) {                                                                           \
-            ::rln::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
-            return RLN_ERR_HIP;                                                                           \
-        }                                                                                                 \
-    } while (0)
-
-#define RLN_TRY(expr)             \
-    do {                          \
-        int _rc = (expr);         \
-        if (_rc != RLN_OK) return _rc; \
-    } while (0)
-
-namespace rln {
-
-// Grow-only device buffer.
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    int ensure(size_t need) {
-        if (need <= bytes) return RLN_OK;
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-        size_t want = std::max<size_t>(need, 256);
-        if (hipMalloc(&ptr, want) != hipSuccess) {
-            ptr = nullptr;
-            set_error("device allocation of %zu bytes failed", want);
-            return RLN_ERR_MEMORY;
-        }
-        bytes = want;
-        return RLN_OK;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T *as() const {
-        return reinterpret_cast<T *>(ptr);
-    }
-};
-
-struct MdlDims {
-    int xdim = 0, ydim = 0, zdim = 0, inity = 0, initz = 0, maxr = 0;
-    double pad = 0.;
-    bool operator==(const MdlDims &o) const {
-        return xdim == o.xdim && ydim == o.ydim && zdim == o.zdim && inity == o.inity && initz == o.initz &&
-               maxr == o.maxr && pad == o.pad;
-    }
-    size_t voxels() const { return (size_t)xdim * ydim * zdim; }
-};
-
-int check_dims(const MdlDims &d) {
-    if (d.xdim <= 1 || d.ydim <= 1 || d.zdim <= 1 || d.maxr <= 0 || !(d.pad >= 1.))
-        RLN_FAIL(RLN_ERR_ARG, "invalid model dimensions %d x %d x %d (maxr %d, pad %g)", d.xdim, d.ydim, d.zdim,
-                 d.maxr, d.pad);
-    return RLN_OK;
-}
-
-}  // namespace rln
-
-struct rln_projector {
-    int device = 0;
-    rln::MdlDims dims;
-    rln::DevBuf data;  // corner-packed volume: 4 float4 per trilinear base voxel
-    unsigned long long version = 0;
-    size_t bases() const { return (size_t)(dims.xdim - 1) * (dims.ydim - 1) * (dims.zdim - 1); }
-};
-
-struct rln_backprojector {
-    int device = 0;
-    rln::MdlDims dims;
-    rln::DevBuf slots;  // corner-slot accumulators: kSlot floats per trilinear base voxel
-    size_t bases() const { return (size_t)(dims.xdim - 1) * (dims.ydim - 1) * (dims.zdim - 1); }
-};
+#include "host_common.h"
 
 namespace rln {
 // Uploads a plain [z][y][x] float2 volume and packs it into the projector's corner layout.
@@ -157,13 +75,17 @@ struct Engine {
     std::vector<rln_backprojector *> bproj;
     std::vector<unsigned long long> proj_version;
     std::vector<double> pdf_class, pdf_direction, sigma2_noise, data_vs_prior;
+    double pdf_dir_mean = 1.;            // mean of pdf_direction over classes and directions
+    std::vector<int> set_off, set_og;    // explicit orientation lists for the next run (empty: none)
     bool model_set = false;
     bool coarse_valid = false;
     // constant tables
     DevBuf d_frames_c, d_frames_f, d_proj_c;
     DevBuf d_phi, d_plo, d_pn2, d_zero_row;         // split-BF16 coarse projections
-    DevBuf d_bhi, d_blo, d_ctf2, d_ynorm;           // split-BF16 translation operand per chunk
-    int Kp = 0;                                     // padded contraction length 2*P_coarse
+    DevBuf d_wc, d_ctf2, d_ynorm;                   // split-BF16 particle rows per chunk (group layout)
+    DevBuf d_grp, d_exg;                            // pixel groups (row, x0); x phases with rows padded to xs4
+    int Kp = 0;                                     // contraction length: 8 * groups, multiple of 32
+    int G = 0, xs4 = 0;                             // pixel groups with weight; coarse row length padded to 4
     DevBuf d_q_c, d_q_f, d_minv_w, d_mresol, d_xa_shell;
     DevBuf d_ex_c, d_ey_c, d_ex_f, d_ey_f, d_tr_ix, d_tr_iy, d_over_x, d_over_y;
     DevBuf d_vols, d_bp_ptrs;
@@ -363,12 +285,16 @@ static int engine_init(Engine &e, const rln_estep_config &cfg) {
     RLN_TRY(upload(e.d_over_y, e.over_y, e.stream));
     e.mresol_f = mresol_of(e.sf);
     RLN_TRY(upload(e.d_mresol, e.mresol_f, e.stream));
-    e.Kp = 32 * ((2 * e.igc.P + 31) / 32);
     if (cfg.precision == RLN_PREC_SPLIT_BF16 && e.NT > 256)
         RLN_FAIL(RLN_ERR_ARG, "split-BF16 coarse pass supports at most 256 translations (%d)", e.NT);
     if (cfg.precision == RLN_PREC_SPLIT_BF16) {
-        std::vector<unsigned short> z(e.Kp, 0);
-        RLN_TRY(upload(e.d_zero_row, z, e.stream));
+        // x phases of the coarse translations with rows padded to a multiple of 4 (zeros past xs)
+        e.xs4 = 4 * ((e.igc.xs + 3) / 4);
+        phase_rows(cx, e.sc, false, cfg.ori_size, rows);
+        std::vector<float2> rows4((size_t)e.NT * e.xs4, make_float2(0.f, 0.f));
+        for (int t = 0; t < e.NT; ++t)
+            for (int x = 0; x < e.igc.xs; ++x) rows4[(size_t)t * e.xs4 + x] = rows[(size_t)t * e.igc.xs + x];
+        RLN_TRY(upload(e.d_exg, rows4, e.stream));
     }
     HIP_TRY(hipStreamSynchronize(e.stream));
     return RLN_OK;
@@ -393,6 +319,23 @@ static int engine_model_tables(Engine &e) {
     std::vector<float> qc, qf, mw;
     q_of(e.sc, qc, nullptr);
     q_of(e.sf, qf, &mw);
+    if (cfg.precision == RLN_PREC_SPLIT_BF16) {
+        // pixel groups of the split pass: per row, x = 0 .. last pixel with non-zero Minvsigma2, in fours
+        std::vector<int2> grp;
+        for (int i = 0; i < e.igc.s; ++i) {
+            int xmax = -1;
+            for (int x = 0; x < e.igc.xs; ++x)
+                if (qc[(size_t)i * e.igc.xs + x] != 0.f) xmax = x;
+            for (int x0 = 0; x0 <= xmax; x0 += 4) grp.push_back(make_int2(i, x0));
+        }
+        e.G = (int)grp.size();
+        const int ng = 4 * ((e.G + 3) / 4);
+        grp.resize(std::max(ng, 4), make_int2(0, 0));
+        e.Kp = 8 * (int)grp.size();
+        RLN_TRY(upload(e.d_grp, grp, e.stream));
+        std::vector<unsigned int> z(e.Kp / 2, 0u);
+        RLN_TRY(upload(e.d_zero_row, z, e.stream));
+    }
     RLN_TRY(upload(e.d_q_c, qc, e.stream));
     RLN_TRY(upload(e.d_q_f, qf, e.stream));
     RLN_TRY(upload(e.d_minv_w, mw, e.stream));
@@ -426,12 +369,12 @@ static int engine_coarse_projections(Engine &e) {
         const size_t rows = (size_t)e.K * e.n_or;
         RLN_TRY(e.d_phi.ensure(rows * e.Kp * sizeof(unsigned short)));
         RLN_TRY(e.d_plo.ensure(rows * e.Kp * sizeof(unsigned short)));
-        RLN_TRY(e.d_pn2.ensure(rows * e.igc.P * sizeof(float)));
+        RLN_TRY(e.d_pn2.ensure(rows * (e.Kp / 2) * sizeof(float)));
         dim3 grid(e.K * e.n_or, (e.Kp / 2 + rlnk::kBlock - 1) / rlnk::kBlock);
         hipLaunchKernelGGL(rlnk::k_project_grid_split, grid, dim3(rlnk::kBlock), 0, e.stream,
                            e.d_vols.as<const float4 *const>(), e.vg, e.d_frames_c.as<const rlnk::Frame>(), e.n_or,
-                           e.igc, e.Kp, e.d_phi.as<unsigned short>(), e.d_plo.as<unsigned short>(),
-                           e.d_pn2.as<float>());
+                           e.igc, e.d_grp.as<const int2>(), e.G, e.Kp, e.d_phi.as<unsigned int>(),
+                           e.d_plo.as<unsigned int>(), e.d_pn2.as<float>());
         RLN_TRY(launch_check("k_project_grid_split"));
     } else {
         const size_t n = (size_t)e.K * e.n_or * e.igc.P;
@@ -465,7 +408,10 @@ struct ChunkHost {
     long long total_entries = 0;
 };
 
-static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, ChunkHost &h) {
+// set_off/set_og (optional): explicit coarse orientation lists of the chunk's particles (global grid indices
+// idir * n_psi + ipsi, offsets relative to set_off[0]); an empty list means the exhaustive grid.
+static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, ChunkHost &h,
+                             const int *set_off = nullptr, const int *set_og = nullptr) {
     const rln_estep_config &cfg = e.cfg;
     const int K = e.K, NT = e.NT;
     h = ChunkHost();
@@ -500,7 +446,32 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         const std::vector<int> &pi = cfg.local_search ? sets[b].psi_idx : all_p;
         const std::vector<double> &dp = cfg.local_search ? sets[b].dir_prior : none;
         const std::vector<double> &pp = cfg.local_search ? sets[b].psi_prior : none;
-        const int ND = (int)di.size(), NP = (int)pi.size();
+        const int n_set = set_off ? set_off[b + 1] - set_off[b] : 0;
+        if (n_set > 0) {
+            // explicit list: global-search priors, normalised by their mean over the whole grid so that weights
+            // and sums stay those of the exhaustive search restricted to the list
+            const int *og_list = set_og + (set_off[b] - set_off[0]);
+            const int base = (int)h.or_proj.size();
+            h.or_off[b] = base;
+            h.or_cnt[b] = K * n_set;
+            h.nd[b] = n_set;
+            h.np[b] = 1;
+            h.dir_off[b] = (int)h.dir_idx.size();
+            h.psi_off[b] = (int)h.psi_idx.size();
+            for (int k = 0; k < K; ++k)
+                for (int i = 0; i < n_set; ++i) {
+                    const int og = og_list[i], d = og / e.grid.n_psi;
+                    const double po = e.pdf_direction[(size_t)k * e.grid.n_dir + d] / e.pdf_dir_mean;
+                    const bool on = po > 0. && e.pdf_class[k] > 0.;
+                    h.or_proj.push_back(on ? k * e.n_or + og : -1);
+                    h.or_prior.push_back(po);
+                    h.or_frame.push_back(og * 8);
+                    h.or_class.push_back(k);
+                    h.or_dir.push_back(d);
+                }
+        }
+        const int ND = n_set > 0 ? 0 : (int)di.size(), NP = n_set > 0 ? 0 : (int)pi.size();
+        if (n_set == 0) {
         h.nd[b] = ND;
         h.np[b] = NP;
         h.dir_off[b] = (int)h.dir_idx.size();
@@ -529,6 +500,7 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         mean /= (double)((long long)K * ND * NP);
         if (mean != 0.)
             for (int i = 0; i < K * ND * NP; ++i) h.or_prior[base + i] /= mean;
+        }
         // offset priors (src/ml_optimiser.cpp:7805-7872): offsets in pixels plus translations in Angstrom
         double tmean = 0.;
         for (int t = 0; t < NT; ++t) {
@@ -563,11 +535,11 @@ template <int NTT>
 static void launch_mfma_t(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
     hipLaunchKernelGGL(rlnk::k_coarse_mfma<NTT>, dim3(n_tiles), dim3(rlnk::kBlock), 0, e.stream,
                        e.d_tiles.as<const rlnk::CoarseTile>(), e.d_or_off.as<const int>(), e.d_or_cnt.as<const int>(),
-                       e.d_or_proj.as<const int>(), e.d_phi.as<const unsigned short>(),
-                       e.d_plo.as<const unsigned short>(), e.d_pn2.as<const float>(),
-                       e.d_bhi.as<const unsigned short>(), e.d_blo.as<const unsigned short>(),
-                       e.d_ctf2.as<const float>(), e.d_ynorm.as<const double>(), xi2, d2off, e.NT, e.igc.P, e.Kp,
-                       e.d_zero_row.as<const unsigned short>(), e.d_diff2.as<float>());
+                       e.d_or_proj.as<const int>(), e.d_phi.as<const unsigned int>(),
+                       e.d_plo.as<const unsigned int>(), e.d_pn2.as<const float>(), e.d_wc.as<const float2>(),
+                       e.d_ctf2.as<const float>(), e.d_ynorm.as<const double>(), e.d_grp.as<const int2>(),
+                       e.d_exg.as<const float2>(), e.xs4, e.d_ey_c.as<const float2>(), e.igc.s, xi2, d2off, e.NT,
+                       e.Kp, e.d_zero_row.as<const unsigned int>(), e.d_diff2.as<float>());
 }
 
 static int launch_coarse_mfma(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
@@ -661,17 +633,16 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     // coarse: scaled images, shifted images
     const bool split = cfg.precision == RLN_PREC_SPLIT_BF16;
     if (split) {
-        const size_t nb = (size_t)B * NT * e.Kp;
-        RLN_TRY(e.d_bhi.ensure(nb * sizeof(unsigned short)));
-        RLN_TRY(e.d_blo.ensure(nb * sizeof(unsigned short)));
-        RLN_TRY(e.d_ctf2.ensure(sizeof(float) * Pc * B));
+        const size_t nc = (size_t)B * (e.Kp / 2);
+        RLN_TRY(e.d_wc.ensure(nc * sizeof(float2)));
+        RLN_TRY(e.d_ctf2.ensure(nc * sizeof(float)));
         RLN_TRY(e.d_ynorm.ensure(sizeof(double) * B));
         HIP_TRY(hipMemsetAsync(e.d_ynorm.ptr, 0, sizeof(double) * B, st));
-        hipLaunchKernelGGL(rlnk::k_prep_coarse_split, dim3((e.Kp / 2 + rlnk::kBlock - 1) / rlnk::kBlock, NT, B),
+        const int nthr = std::max(e.Kp / 2, (int)Pc);
+        hipLaunchKernelGGL(rlnk::k_prep_coarse_split, dim3((nthr + rlnk::kBlock - 1) / rlnk::kBlock, B),
                            dim3(rlnk::kBlock), 0, st, dX, dC, d_scale, e.sf, (int)Pf, e.igc, e.d_q_c.as<const float>(),
-                           e.d_ex_c.as<const float2>(), e.d_ey_c.as<const float2>(), NT, cfg.do_ctf_correction, e.Kp,
-                           e.d_bhi.as<unsigned short>(), e.d_blo.as<unsigned short>(), e.d_ctf2.as<float>(),
-                           e.d_ynorm.as<double>());
+                           e.d_grp.as<const int2>(), e.G, e.Kp, cfg.do_ctf_correction, e.d_wc.as<float2>(),
+                           e.d_ctf2.as<float>(), e.d_ynorm.as<double>());
         RLN_TRY(launch_check("k_prep_coarse_split"));
     } else {
         RLN_TRY(e.d_ctfq.ensure(sizeof(float) * Pc * B));
@@ -975,10 +946,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         const int c = all_pairs[pair_off[b] + (sf.best_index >> 5)];
         const int o = c / NT, t = c % NT;
         const int rr = (int)((sf.best_index >> 2) & 7), u = (int)(sf.best_index & 3);
-        const int ND = h.nd[b], NP = h.np[b];
-        const int k = o / (ND * NP), id = (o / NP) % ND, ip = o % NP;
-        const int gd = h.dir_idx[h.dir_off[b] + id], gp = h.psi_idx[h.psi_off[b] + ip];
-        e.grid.oversampled_orientation(gd, gp, rr, r.rot, r.tilt, r.psi);
+        const int og = h.or_frame[h.or_off[b] + o] / 8, k = h.or_class[h.or_off[b] + o];
+        e.grid.oversampled_orientation(og / e.grid.n_psi, og % e.grid.n_psi, rr, r.rot, r.tilt, r.psi);
         r.xoff = meta[b].old_offset_x + e.over_x[t * 4 + u];
         r.yoff = meta[b].old_offset_y + e.over_y[t * 4 + u];
         r.iclass = k + 1;
@@ -1158,7 +1127,7 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     rln::DevBuf *bufs[] = {&e->d_frames_c, &e->d_frames_f, &e->d_proj_c, &e->d_phi, &e->d_plo, &e->d_pn2,
-                           &e->d_zero_row, &e->d_bhi, &e->d_blo, &e->d_ctf2, &e->d_ynorm, &e->d_q_c, &e->d_q_f, &e->d_minv_w,
+                           &e->d_zero_row, &e->d_wc, &e->d_grp, &e->d_exg, &e->d_ctf2, &e->d_ynorm, &e->d_q_c, &e->d_q_f, &e->d_minv_w,
                            &e->d_mresol, &e->d_xa_shell, &e->d_ex_c, &e->d_ey_c, &e->d_ex_f, &e->d_ey_f,
                            &e->d_tr_ix, &e->d_tr_iy, &e->d_over_x, &e->d_over_y, &e->d_vols, &e->d_bp_ptrs,
                            &e->d_fimg, &e->d_fimgn, &e->d_fctf, &e->d_meta_f, &e->d_meta_d, &e->d_meta_ll,
@@ -1209,6 +1178,10 @@ extern "C" int rln_estep_set_model(rln_estep *e, rln_projector *const *projector
         e->pdf_direction.assign(pdf_direction, pdf_direction + (size_t)K * nd);
     else
         e->pdf_direction.assign((size_t)K * nd, 1.);
+    double pm = 0.;
+    for (double v : e->pdf_direction) pm += v;
+    pm /= (double)e->pdf_direction.size();
+    e->pdf_dir_mean = pm > 0. ? pm : 1.;
     e->sigma2_noise.assign(sigma2_noise, sigma2_noise + e->nshell);
     if (data_vs_prior)
         e->data_vs_prior.assign(data_vs_prior, data_vs_prior + (size_t)K * e->nshell);
@@ -1220,6 +1193,24 @@ extern "C" int rln_estep_set_model(rln_estep *e, rln_projector *const *projector
     RLN_TRY(rln::engine_model_tables(*e));
     e->coarse_valid = false;
     e->model_set = true;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_set_orientation_sets(rln_estep *e, int n, const int *set_offsets, const int *set_orients) {
+    if (!e || n < 0 || (n > 0 && !set_offsets)) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_orientation_sets: null argument");
+    std::lock_guard<std::mutex> lk(e->mtx);
+    if (e->cfg.local_search) RLN_FAIL(RLN_ERR_ARG, "orientation lists replace the global search; local_search is set");
+    const int n_or = e->grid.n_dir * e->grid.n_psi;
+    if (set_offsets[0] != 0) RLN_FAIL(RLN_ERR_ARG, "set_offsets[0] must be 0");
+    for (int i = 0; i < n; ++i)
+        if (set_offsets[i + 1] < set_offsets[i]) RLN_FAIL(RLN_ERR_ARG, "set_offsets not ascending at %d", i);
+    const int total = set_offsets[n];
+    if (total > 0 && !set_orients) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_orientation_sets: null orientation list");
+    for (int j = 0; j < total; ++j)
+        if (set_orients[j] < 0 || set_orients[j] >= n_or)
+            RLN_FAIL(RLN_ERR_ARG, "orientation %d out of range [0, %d)", set_orients[j], n_or);
+    e->set_off.assign(set_offsets, set_offsets + n + 1);
+    e->set_og.assign(set_orients, set_orients + total);
     return RLN_OK;
 }
 
@@ -1246,16 +1237,25 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     } else {
         max_entries = (long long)e->K * e->n_or * e->NT;
     }
-    const double per_particle = 13.0 * (double)max_entries + 8.0 * Pc * e->NT + 20.0 * Pf + 64.0 * 1024;
+    const bool split = e->cfg.precision == RLN_PREC_SPLIT_BF16;
+    const double per_particle = 13.0 * (double)max_entries + (split ? 6.0 * e->Kp : 8.0 * Pc * e->NT) + 20.0 * Pf +
+                                64.0 * 1024;
     const double budget = 24.0e9;
     int chunk = (int)std::max(1.0, std::min(4096.0, budget / per_particle));
     if (e->keep) chunk = std::max(chunk, n);
     // chunk c runs on the device while a second host thread builds the lists of chunk c + 1
     using clk = std::chrono::steady_clock;
     rln::ChunkHost cur, nxt;
+    std::vector<int> set_off, set_og;
+    set_off.swap(e->set_off);  // explicit lists apply to this run only
+    set_og.swap(e->set_og);
+    const bool have_sets = !set_off.empty();
+    if (have_sets && (int)set_off.size() != n + 1) RLN_FAIL(RLN_ERR_ARG, "orientation lists were set for %d particles, run has %d", (int)set_off.size() - 1, n);
+    auto so = [&](int i) { return have_sets ? set_off.data() + i : nullptr; };
+    const int *sog = have_sets ? set_og.data() : nullptr;
     if (n > 0) {
         const clk::time_point t0 = clk::now();
-        rln::build_chunk_host(*e, std::min(chunk, n), meta, cur);
+        rln::build_chunk_host(*e, std::min(chunk, n), meta, cur, so(0), sog ? sog + set_off[0] : nullptr);
         e->host_ms[0] += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     }
     for (int i0 = 0; i0 < n; i0 += chunk) {
@@ -1266,7 +1266,8 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
         if (i1 < n)
             builder = std::thread([&, i1] {
                 try {
-                    rln::build_chunk_host(*e, std::min(chunk, n - i1), meta + i1, nxt);
+                    rln::build_chunk_host(*e, std::min(chunk, n - i1), meta + i1, nxt, so(i1),
+                                          sog ? sog + set_off[i1] : nullptr);
                 } catch (...) {
                     next_ok = false;
                 }

@@ -8,83 +8,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_common.h"
+
 namespace rlnk {
-
-constexpr int kBlock = 256;
-
-// Geometry of one image size, precomputed on the host in double (exact integer tests).
-struct ImgGeom {
-    int s, xs, P;
-    int proj_r2_max;  // pixels projected: x^2+y^2 <= proj_r2_max (Projector::project circle and r_max test)
-    int bp_r2_max;    // pixels backprojected: x^2+y^2 <= bp_r2_max (backproject2Dto3D ellipse/r_max test)
-};
-
-struct VolGeom {
-    int xdim, ydim, zdim, starty, startz;
-};
-
-// Per-orientation projection frame: 3-D coordinates of pixel (x, y) are x*e0 + y*e1 (padding included),
-// e0 = pad * row 0 of A, e1 = pad * row 1 of A (Ainv = A^T in Projector::project, src/projector.cpp:638-640).
-struct Frame {
-    float e0x, e0y, e0z, e1x, e1y, e1z;
-};
-
-__device__ __forceinline__ void pixel_xy(int p, int s, int xs, int &x, int &y) {
-    const int i = p / xs;
-    x = p - i * xs;
-    y = (i < xs) ? i : i - s;
-}
-
-// Corner-packed reference: for every trilinear base voxel (x0, y0, z0) with x0 < xdim-1, y0 < ydim-1,
-// z0 < zdim-1, the 8 corner values are stored contiguously as 4 float4 (64 bytes):
-//   [V(z0,y0,x0), V(z0,y0,x0+1)], [V(z0,y0+1,x0), V(z0,y0+1,x0+1)],
-//   [V(z0+1,y0,x0), V(z0+1,y0,x0+1)], [V(z0+1,y0+1,x0), V(z0+1,y0+1,x0+1)]
-// so that one trilinear sample is one aligned 64-byte read instead of four scattered cache lines.
-__device__ __forceinline__ size_t packed_index(const VolGeom &g, int x0, int y0, int z0) {
-    return ((size_t)z0 * (size_t)(g.ydim - 1) + (size_t)y0) * (size_t)(g.xdim - 1) + (size_t)x0;
-}
-
-// Trilinear gather of the padded reference at (xp, yp, zp), Hermitian flip for xp < 0
-// (Projector::project TRILINEAR branch, src/projector.cpp:676-744). Out-of-box samples give 0.
-__device__ __forceinline__ float2 sample_volume(const float4 *__restrict__ vol, const VolGeom &g, float xp, float yp,
-                                                float zp) {
-    const bool neg = xp < 0.f;
-    if (neg) {
-        xp = -xp;
-        yp = -yp;
-        zp = -zp;
-    }
-    const float fx0 = floorf(xp), fy0 = floorf(yp), fz0 = floorf(zp);
-    const float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
-    const int x0 = (int)fx0;
-    const int y0 = (int)fy0 - g.starty;
-    const int z0 = (int)fz0 - g.startz;
-    if (x0 < 0 || x0 + 1 >= g.xdim || y0 < 0 || y0 + 1 >= g.ydim || z0 < 0 || z0 + 1 >= g.zdim)
-        return make_float2(0.f, 0.f);
-    const float4 *c = vol + 4 * packed_index(g, x0, y0, z0);
-    const float4 q0 = c[0], q1 = c[1], q2 = c[2], q3 = c[3];
-    float2 r;
-    {
-        const float dx00 = q0.x + (q0.z - q0.x) * fx, dx01 = q2.x + (q2.z - q2.x) * fx;
-        const float dx10 = q1.x + (q1.z - q1.x) * fx, dx11 = q3.x + (q3.z - q3.x) * fx;
-        const float dxy0 = dx00 + (dx10 - dx00) * fy, dxy1 = dx01 + (dx11 - dx01) * fy;
-        r.x = dxy0 + (dxy1 - dxy0) * fz;
-    }
-    {
-        const float dx00 = q0.y + (q0.w - q0.y) * fx, dx01 = q2.y + (q2.w - q2.y) * fx;
-        const float dx10 = q1.y + (q1.w - q1.y) * fx, dx11 = q3.y + (q3.w - q3.y) * fx;
-        const float dxy0 = dx00 + (dx10 - dx00) * fy, dxy1 = dx01 + (dx11 - dx01) * fy;
-        r.y = dxy0 + (dxy1 - dxy0) * fz;
-    }
-    if (neg) r.y = -r.y;
-    return r;
-}
-
-__device__ __forceinline__ float2 project_pixel(const float4 *__restrict__ vol, const VolGeom &g, const Frame &f,
-                                                int x, int y) {
-    const float fxp = (float)x, fyp = (float)y;
-    return sample_volume(vol, g, f.e0x * fxp + f.e1x * fyp, f.e0y * fxp + f.e1y * fyp, f.e0z * fxp + f.e1z * fyp);
-}
 
 // Builds the corner-packed volume from the plain [z][y][x] float2 volume. One thread per base voxel.
 __global__ void __launch_bounds__(kBlock) k_pack_volume(const float2 *__restrict__ plain, VolGeom g,
@@ -138,9 +64,6 @@ __global__ void __launch_bounds__(kBlock) k_fold_slots(const float *__restrict__
     w[v] = sw;
 }
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 // --------------------------------------------------------------------------------------------------
 // Coarse pass
@@ -297,6 +220,8 @@ __global__ void __launch_bounds__(kBlock) k_coarse_diff2(const CoarseTile *__res
 // --------------------------------------------------------------------------------------------------
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float floatx2_t __attribute__((ext_vector_type(2)));
 
 // round-to-nearest-even FP32 -> BF16 bit pattern
 __device__ __forceinline__ unsigned short f2bf(float x) {
@@ -328,99 +253,123 @@ __global__ void k_selftest_mfma(const float *__restrict__ A, const float *__rest
     for (int j = 0; j < 4; ++j) D[(4 * (l / 16) + j) * 16 + l % 16] = acc[j];
 }
 
-// Split projection tables of the coarse grid: hi/lo [K*n_or][Kp] (k = 2p + re/im, zero padded to Kp) and the
-// FP32 power |P|^2 [K*n_or][P]. grid (K*n_or, ceil(Kp/2/256)).
-__global__ void __launch_bounds__(kBlock) k_project_grid_split(const float4 *const *__restrict__ vols, VolGeom vg,
-                                                               const Frame *__restrict__ frames, int n_or, ImgGeom ig,
-                                                               int Kp, unsigned short *__restrict__ phi,
-                                                               unsigned short *__restrict__ plo,
-                                                               float *__restrict__ pn2) {
-    const int p = blockIdx.y * kBlock + threadIdx.x;
-    const int ko = blockIdx.x;
-    if (2 * p >= Kp) return;
-    float2 v = make_float2(0.f, 0.f);
-    if (p < ig.P) {
-        const int k = ko / n_or, o = ko - k * n_or;
-        int x, y;
-        pixel_xy(p, ig.s, ig.xs, x, y);
-        if (x * x + y * y <= ig.proj_r2_max) v = project_pixel(vols[k], vg, frames[o], x, y);
-        pn2[(size_t)ko * ig.P + p] = v.x * v.x + v.y * v.y;
-    }
-    unsigned short h, l;
-    split_bf16(v.x, h, l);
-    phi[(size_t)ko * Kp + 2 * p] = h;
-    plo[(size_t)ko * Kp + 2 * p] = l;
-    split_bf16(v.y, h, l);
-    phi[(size_t)ko * Kp + 2 * p + 1] = h;
-    plo[(size_t)ko * Kp + 2 * p + 1] = l;
+// Contraction layout of the split-BF16 pass: pixel groups of 4 consecutive x in one row. Group g = (row i, x0)
+// holds pixels (i, x0 .. x0+3) at k = 8g + 2j + {re, im}; the groups cover, row by row, x = 0 up to the last
+// pixel whose Minvsigma2 is non-zero, so rows and corners that carry no weight are not contracted. Kp = 8 *
+// (groups rounded up to a multiple of 4); slots past the last group and pixels with x >= xs hold zeros.
+
+// hi + lo split of two floats with the hardware round-to-nearest-even conversion; hi and lo are packed pairs
+__device__ __forceinline__ void split2_bf16(float a, float b, unsigned int &hi, unsigned int &lo) {
+    const bf16x2_t h = __builtin_convertvector((floatx2_t){a, b}, bf16x2_t);
+    hi = __builtin_bit_cast(unsigned int, h);
+    const float ra = a - __uint_as_float(hi << 16), rb = b - __uint_as_float(hi & 0xffff0000u);
+    lo = __builtin_bit_cast(unsigned int, __builtin_convertvector((floatx2_t){ra, rb}, bf16x2_t));
 }
 
-// Translation-side operand: B[b][t][k] = c_p y_t(p) split in BF16, k = 2p + re/im, zero padded to Kp; ctf2[b][p] =
-// c_p^2; ynorm[b] += sum_p |y(p)|^2 (FP64). grid (ceil(Kp/2/256), NT, B).
+// Split projection tables of the coarse grid in the group layout: hi/lo [K*n_or][Kp] and the FP32 power
+// |P|^2 [K*n_or][Kp/2] (one per complex slot). grid (K*n_or, ceil(Kp/2/256)).
+__global__ void __launch_bounds__(kBlock) k_project_grid_split(const float4 *const *__restrict__ vols, VolGeom vg,
+                                                               const Frame *__restrict__ frames, int n_or, ImgGeom ig,
+                                                               const int2 *__restrict__ grp, int G, int Kp,
+                                                               unsigned int *__restrict__ phi,
+                                                               unsigned int *__restrict__ plo,
+                                                               float *__restrict__ pn2) {
+    const int idx = blockIdx.y * kBlock + threadIdx.x;  // complex slot 4g + j
+    const int ko = blockIdx.x;
+    if (2 * idx >= Kp) return;
+    const int g = idx >> 2, j = idx & 3;
+    float2 v = make_float2(0.f, 0.f);
+    if (g < G) {
+        const int2 gr = grp[g];
+        const int x = gr.y + j, i = gr.x;
+        const int y = (i < ig.xs) ? i : i - ig.s;
+        if (x < ig.xs && x * x + y * y <= ig.proj_r2_max) {
+            const int k = ko / n_or, o = ko - k * n_or;
+            v = project_pixel(vols[k], vg, frames[o], x, y);
+        }
+    }
+    unsigned int h, l;
+    split2_bf16(v.x, v.y, h, l);
+    phi[(size_t)ko * (Kp / 2) + idx] = h;
+    plo[(size_t)ko * (Kp / 2) + idx] = l;
+    pn2[(size_t)ko * (Kp / 2) + idx] = v.x * v.x + v.y * v.y;
+}
+
+// Particle side of the split pass, in the group layout: w[b][4g+j] = c_p q_p X(p) with c_p = q_p ctf_p s (the
+// translation phase is applied inside k_coarse_mfma), ctf2[b][4g+j] = c_p^2, and ynorm[b] = sum_p |q_p X(p)|^2
+// over all pixels (FP64). grid (ceil(max(Kp/2, P)/256), B).
 __global__ void __launch_bounds__(kBlock) k_prep_coarse_split(const float2 *__restrict__ fimg,
                                                               const float *__restrict__ fctf,
                                                               const float *__restrict__ scale, int sf, int Pf,
                                                               ImgGeom ig, const float *__restrict__ q,
-                                                              const float2 *__restrict__ ex,
-                                                              const float2 *__restrict__ ey, int NT, int do_ctf, int Kp,
-                                                              unsigned short *__restrict__ bhi,
-                                                              unsigned short *__restrict__ blo,
-                                                              float *__restrict__ ctf2, double *__restrict__ ynorm) {
+                                                              const int2 *__restrict__ grp, int G, int Kp, int do_ctf,
+                                                              float2 *__restrict__ w, float *__restrict__ ctf2,
+                                                              double *__restrict__ ynorm) {
     __shared__ double red[kBlock / 64];
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    const int t = blockIdx.y, b = blockIdx.z;
-    float2 v = make_float2(0.f, 0.f);
-    float yn = 0.f;
-    if (p < ig.P) {
-        const int i = p / ig.xs, x = p - i * ig.xs;
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    const int b = blockIdx.y;
+    const float sc = scale[b];
+    auto src_of = [&](int i, int x) {
         const int y = (i < ig.xs) ? i : i - ig.s;
         const int isrc = (y < 0) ? y + sf : y;
-        const size_t src = (size_t)b * Pf + (size_t)isrc * (sf / 2 + 1) + x;
-        const float qp = q[p];
-        const float2 X = fimg[src];
-        const float c = qp * (do_ctf ? fctf[src] : 1.f) * scale[b];
-        const float2 ph = cmul(ex[(size_t)t * ig.xs + x], ey[(size_t)t * ig.s + i]);
-        const float2 yv = cmul(make_float2(X.x * qp, X.y * qp), ph);
-        v = make_float2(c * yv.x, c * yv.y);
+        return (size_t)b * Pf + (size_t)isrc * (sf / 2 + 1) + x;
+    };
+    if (idx < Kp / 2) {
+        const int g = idx >> 2, j = idx & 3;
+        float2 v = make_float2(0.f, 0.f);
+        float c2 = 0.f;
+        if (g < G) {
+            const int2 gr = grp[g];
+            const int x = gr.y + j, i = gr.x;
+            if (x < ig.xs) {
+                const size_t src = src_of(i, x);
+                const float qp = q[i * ig.xs + x];
+                const float c = qp * (do_ctf ? fctf[src] : 1.f) * sc;
+                const float2 X = fimg[src];
+                v = make_float2(c * qp * X.x, c * qp * X.y);
+                c2 = c * c;
+            }
+        }
+        w[(size_t)b * (Kp / 2) + idx] = v;
+        ctf2[(size_t)b * (Kp / 2) + idx] = c2;
+    }
+    float yn = 0.f;
+    if (idx < ig.P) {
+        const int i = idx / ig.xs, x = idx - i * ig.xs;
+        const float qp = q[idx];
+        const float2 X = fimg[src_of(i, x)];
         yn = qp * qp * (X.x * X.x + X.y * X.y);
-        if (t == 0) ctf2[(size_t)b * ig.P + p] = c * c;
     }
-    if (2 * p < Kp) {
-        const size_t base = ((size_t)b * NT + t) * Kp + 2 * p;
-        unsigned short h, l;
-        split_bf16(v.x, h, l);
-        bhi[base] = h;
-        blo[base] = l;
-        split_bf16(v.y, h, l);
-        bhi[base + 1] = h;
-        blo[base + 1] = l;
-    }
-    if (t == 0) {
-        double s = yn;
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) atomicAdd(&ynorm[b], red[0] + red[1] + red[2] + red[3]);
+    double s = yn;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double t = red[0] + red[1] + red[2] + red[3];
+        if (t != 0.) atomicAdd(&ynorm[b], t);
     }
 }
 
 constexpr int kMfO = 64;  // orientations per block (4 MFMA row tiles)
 
 // One block (4 waves) per (particle, 64 orientations) and all translations. Wave w owns translation tiles
-// w, w+4, w+8, ... and all four orientation tiles. grid (n_tiles).
+// w, w+4, w+8, ... and all four orientation tiles. The translation operand B[t][k] = w(p) e^{-2 pi i (x tx + y
+// ty)/N} is formed in registers from the particle's w row and the phase tables exg [NT][xs4] (x phases, rows
+// padded to a multiple of 4) and ey [NT][s] (y phases), then split in BF16. grid (n_tiles).
 template <int NTT>  // translation tiles of 16 in the block (16*NTT >= NT)
 __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__restrict__ tiles,
                                                         const int *__restrict__ or_off, const int *__restrict__ or_cnt,
                                                         const int *__restrict__ or_proj,
-                                                        const unsigned short *__restrict__ phi,
-                                                        const unsigned short *__restrict__ plo,
+                                                        const unsigned int *__restrict__ phi,
+                                                        const unsigned int *__restrict__ plo,
                                                         const float *__restrict__ pn2,
-                                                        const unsigned short *__restrict__ bhi,
-                                                        const unsigned short *__restrict__ blo,
+                                                        const float2 *__restrict__ w,
                                                         const float *__restrict__ ctf2, const double *__restrict__ ynorm,
+                                                        const int2 *__restrict__ grp, const float2 *__restrict__ exg,
+                                                        int xs4, const float2 *__restrict__ ey, int s,
                                                         const float *__restrict__ xi2half,
-                                                        const long long *__restrict__ d2_off, int NT, int P, int Kp,
-                                                        const unsigned short *__restrict__ zero_row,
+                                                        const long long *__restrict__ d2_off, int NT, int Kp,
+                                                        const unsigned int *__restrict__ zero_row,
                                                         float *__restrict__ diff2) {
     constexpr int kTW = (NTT + 3) / 4;  // translation tiles per wave
     __shared__ int og_s[kMfO];
@@ -429,6 +378,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
     const int b = tile.b;
     const int n_o = or_cnt[b];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int Kc = Kp / 2;  // complex slots per row
     if (tid < kMfO) {
         const int o = tile.o_begin + tid;
         og_s[tid] = (o < n_o) ? or_proj[or_off[b] + o] : -1;
@@ -438,15 +388,21 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
     {
         const int oo = tid >> 2, part = tid & 3;
         const int og = og_s[oo];
-        float s = 0.f;
+        float acc_n = 0.f;
         if (og >= 0) {
-            const float *pn = pn2 + (size_t)og * P;
-            const float *c2 = ctf2 + (size_t)b * P;
-            for (int p = part; p < P; p += 4) s = fmaf(c2[p], pn[p], s);
+            const float4 *pn = reinterpret_cast<const float4 *>(pn2 + (size_t)og * Kc);
+            const float4 *c2 = reinterpret_cast<const float4 *>(ctf2 + (size_t)b * Kc);
+            for (int g = part; g < Kc / 4; g += 4) {
+                const float4 a = pn[g], c = c2[g];
+                acc_n = fmaf(a.x, c.x, acc_n);
+                acc_n = fmaf(a.y, c.y, acc_n);
+                acc_n = fmaf(a.z, c.z, acc_n);
+                acc_n = fmaf(a.w, c.w, acc_n);
+            }
         }
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if (part == 0) a_norm[oo] = s;
+        acc_n += __shfl_xor(acc_n, 1);
+        acc_n += __shfl_xor(acc_n, 2);
+        if (part == 0) a_norm[oo] = acc_n;
     }
     __syncthreads();
     floatx4_t acc[4][kTW];
@@ -454,34 +410,54 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < kTW; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-    const int r16 = lane & 15, kq = 8 * (lane >> 4);
-    const unsigned short *arow_hi[4], *arow_lo[4];
+    const int r16 = lane & 15, gq = lane >> 4;
+    const uint4 *arow_hi[4], *arow_lo[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int og = og_s[16 * i + r16];
-        arow_hi[i] = og >= 0 ? phi + (size_t)og * Kp : zero_row;
-        arow_lo[i] = og >= 0 ? plo + (size_t)og * Kp : zero_row;
+        arow_hi[i] = reinterpret_cast<const uint4 *>(og >= 0 ? phi + (size_t)og * Kc : zero_row);
+        arow_lo[i] = reinterpret_cast<const uint4 *>(og >= 0 ? plo + (size_t)og * Kc : zero_row);
     }
-    const unsigned short *brow_hi[kTW], *brow_lo[kTW];
+    int tt[kTW];
 #pragma unroll
     for (int j = 0; j < kTW; ++j) {
         const int t = 16 * (wave + 4 * j) + r16;
-        const bool ok = (wave + 4 * j) < NTT && t < NT;
-        brow_hi[j] = ok ? bhi + ((size_t)b * NT + t) * Kp : zero_row;
-        brow_lo[j] = ok ? blo + ((size_t)b * NT + t) * Kp : zero_row;
+        tt[j] = ((wave + 4 * j) < NTT && t < NT) ? t : -1;
     }
+    const float4 *wrow = reinterpret_cast<const float4 *>(w + (size_t)b * Kc);
     for (int k0 = 0; k0 < Kp; k0 += 32) {
+        const int g = (k0 >> 3) + gq;  // this lane's pixel group
         bf16x8_t ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ah[i] = load_bf16x8(arow_hi[i] + k0 + kq);
-            al[i] = load_bf16x8(arow_lo[i] + k0 + kq);
+            ah[i] = __builtin_bit_cast(bf16x8_t, arow_hi[i][g]);
+            al[i] = __builtin_bit_cast(bf16x8_t, arow_lo[i][g]);
         }
+        const float4 w01 = wrow[2 * g], w23 = wrow[2 * g + 1];
+        const int2 gr = grp[g];
 #pragma unroll
         for (int j = 0; j < kTW; ++j) {
             if (wave + 4 * j >= NTT) continue;
-            const bf16x8_t bh = load_bf16x8(brow_hi[j] + k0 + kq);
-            const bf16x8_t bl = load_bf16x8(brow_lo[j] + k0 + kq);
+            unsigned int hh[4], ll[4];
+            if (tt[j] >= 0) {
+                const float4 *exr = reinterpret_cast<const float4 *>(exg + (size_t)tt[j] * xs4 + gr.y);
+                const float4 e01 = exr[0], e23 = exr[1];
+                const float2 fy = ey[(size_t)tt[j] * s + gr.x];
+                const float2 wv[4] = {make_float2(w01.x, w01.y), make_float2(w01.z, w01.w),
+                                      make_float2(w23.x, w23.y), make_float2(w23.z, w23.w)};
+                const float2 ev[4] = {make_float2(e01.x, e01.y), make_float2(e01.z, e01.w),
+                                      make_float2(e23.x, e23.y), make_float2(e23.z, e23.w)};
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float2 v = cmul(wv[m], cmul(ev[m], fy));
+                    split2_bf16(v.x, v.y, hh[m], ll[m]);
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) hh[m] = ll[m] = 0u;
+            }
+            const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, make_uint4(hh[0], hh[1], hh[2], hh[3]));
+            const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, make_uint4(ll[0], ll[1], ll[2], ll[3]));
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
@@ -494,14 +470,13 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
     float *out = diff2 + d2_off[b];
 #pragma unroll
     for (int j = 0; j < kTW; ++j) {
-        if (wave + 4 * j >= NTT) continue;
-        const int t = 16 * (wave + 4 * j) + r16;  // D column
-        if (t >= NT) continue;
+        if (tt[j] < 0) continue;
+        const int t = tt[j];  // D column
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int oo = 16 * i + (lane >> 4) * 4 + r;  // D row
+                const int oo = 16 * i + gq * 4 + r;  // D row
                 const int o = tile.o_begin + oo;
                 if (o >= n_o) continue;
                 out[(size_t)o * NT + t] = og_s[oo] < 0 ? -999.f : base + a_norm[oo] - 2.f * acc[i][j][r];

@@ -1,0 +1,505 @@
+// Device kernels of the learned pose proposal for CDNA4 (gfx950). Included by proposal.hip only.
+//
+// The network maps a W x W real image (the particle's central Fourier window |k| < W/2, inverse transformed
+// and normalised) to logits over direction x psi bins. Six 3x3 convolutions (pad 1, stride 1/2 alternating,
+// C channels, ReLU) and one fully connected layer; activations are NHWC BF16, weights BF16 copies of FP32
+// master weights, all products on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+//
+// Convolution as a product: col[m][k] (m = (image, oy, ox), k = tap * Cin + c with tap = 3*ky + kx, zero
+// padded to Kp) times W[Cout][Kp]^T gives the NHWC output. Backward: dW = dZ^T col (split over m with FP32
+// atomics), dcol = dZ W, then a gather col2im fused with the previous layer's ReLU mask.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.h"
+
+namespace rlnp {
+
+using rlnk::Frame;
+using rlnk::VolGeom;
+constexpr int kB = 256;
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16;  // BF16 bit pattern in memory
+
+__device__ __forceinline__ bf16 to_bf16(float x) { return __builtin_bit_cast(bf16, (__bf16)x); }
+__device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float((unsigned int)h << 16); }
+
+// counter-based normal deviates (splitmix64 finaliser + Box-Muller)
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float2 normal2(uint64_t key) {
+    const uint64_t h = mix64(key);
+    const float u1 = ((float)(h >> 40) + 0.5f) * (1.f / 16777216.f);
+    const float u2 = ((float)((h >> 16) & 0xffffffULL) + 0.5f) * (1.f / 16777216.f);
+    const float r = sqrtf(-2.f * logf(u1));
+    float s, c;
+    sincosf(6.283185307179586f * u2, &s, &c);
+    return make_float2(r * c, r * s);
+}
+
+// --------------------------------------------------------------------------------------------------
+// Training samples: the W x (W/2+1) Fourier window of a projection of the reference at a known pose, with the
+// acquisition's CTF, a shift and Gaussian noise of the model's spectrum (per-component variance
+// sigma2[ires], ires = round(|k|) in the original box).
+// --------------------------------------------------------------------------------------------------
+struct GenSample {
+    Frame fr;                 // projection frame of the pose (padding included)
+    float dx, dy;             // shift in pixels of the original box
+    float defU, defV, defAng; // CTF (Angstrom, degrees)
+};
+
+struct CtfConst {
+    float K1, K2, K3;  // pi*lambda, pi/2*Cs*lambda^3, atan(Q0/sqrt(1-Q0^2)) (src/ctf.h:184-240)
+    float inv_box_A;   // 1 / (ori_size * pixel_size)
+    float ori_size;
+};
+
+__device__ __forceinline__ float ctf_at(const CtfConst &k, float defU, float defV, float defAng, float x, float y) {
+    const float fx = x * k.inv_box_A, fy = y * k.inv_box_A;
+    float s, c;
+    sincosf(defAng * 0.017453292519943295f, &s, &c);
+    const float axx = -defU * c * c - defV * s * s;
+    const float axy = -defU * c * s + defV * s * c;
+    const float ayy = -defU * s * s - defV * c * c;
+    const float u2 = fx * fx + fy * fy;
+    const float gamma = k.K1 * (axx * fx * fx + 2.f * axy * fx * fy + ayy * fy * fy) + k.K2 * u2 * u2 - k.K3;
+    return -sinf(gamma);
+}
+
+// grid (ceil(W*(W/2+1)/256), n); out [n][W][W/2+1]
+__global__ void __launch_bounds__(kB) k_gen_window(const float4 *__restrict__ vol, VolGeom vg, int proj_r2_max,
+                                                   const GenSample *__restrict__ smp, CtfConst ck,
+                                                   const float *__restrict__ sigma2, int nshell, int W,
+                                                   unsigned long long seed, long long first,
+                                                   float2 *__restrict__ out) {
+    const int xw = W / 2 + 1;
+    const int idx = blockIdx.x * kB + threadIdx.x;
+    const int b = blockIdx.y;
+    if (idx >= W * xw) return;
+    const int r = idx / xw, kx = idx - r * xw;
+    const int ky = (r < W / 2) ? r : r - W;
+    const GenSample g = smp[b];
+    const int r2 = kx * kx + ky * ky;
+    float2 v = make_float2(0.f, 0.f);
+    if (r2 <= proj_r2_max) v = rlnk::project_pixel(vol, vg, g.fr, kx, ky);
+    const float ctf = ctf_at(ck, g.defU, g.defV, g.defAng, (float)kx, (float)ky);
+    float sn, cs;
+    sincosf(6.283185307179586f * ((float)kx * g.dx + (float)ky * g.dy) / ck.ori_size, &sn, &cs);
+    float2 f = rlnk::cmul(v, make_float2(cs, sn));
+    f.x *= ctf;
+    f.y *= ctf;
+    const int ires = (int)rintf(sqrtf((float)r2));
+    const float sg = (ires < nshell) ? sqrtf(sigma2[ires]) : 0.f;
+    const float2 nz = normal2((seed * 0x100000001b3ULL) ^ ((uint64_t)(first + b) << 20) ^ (uint64_t)idx);
+    out[(size_t)b * W * xw + idx] = make_float2(f.x + sg * nz.x, f.y + sg * nz.y);
+}
+
+// --------------------------------------------------------------------------------------------------
+// Network input: img(u, v) = sum_{ky=-W/2}^{W/2-1} sum_{kx=0}^{W/2} h(kx) Re[F(kx,ky) e^{2 pi i (kx u + ky v)/W}],
+// h(0) = h(W/2) = 1, else 2, from a half-complex source of s_src rows of xs_src (row of ky: ky >= 0 ? ky :
+// ky + s_src), normalised to zero mean and unit variance. One block per image; out bf16 [n][W][W] (v-major),
+// optional FP32 copy.
+// --------------------------------------------------------------------------------------------------
+constexpr int kMaxW = 64;
+
+__global__ void __launch_bounds__(kB) k_window_image(const float2 *__restrict__ src, long long src_stride, int s_src,
+                                                     int xs_src, int W, bf16 *__restrict__ out,
+                                                     float *__restrict__ out_f32) {
+    __shared__ float2 Fw[kMaxW][kMaxW / 2 + 1];
+    __shared__ float2 G[kMaxW / 2 + 1][kMaxW];
+    __shared__ float2 tw[kMaxW];
+    __shared__ float img[kMaxW * kMaxW];
+    __shared__ double red[2][kB / 64];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int xw = W / 2 + 1;
+    const float2 *F = src + (size_t)b * src_stride;
+    for (int j = tid; j < W; j += kB) {
+        float s, c;
+        sincosf(6.283185307179586f * (float)j / (float)W, &s, &c);
+        tw[j] = make_float2(c, s);
+    }
+    for (int idx = tid; idx < W * xw; idx += kB) {
+        const int r = idx / xw, kx = idx - r * xw;
+        const int ky = (r < W / 2) ? r : r - W;
+        const int row = (ky >= 0) ? ky : ky + s_src;
+        Fw[r][kx] = F[(size_t)row * xs_src + kx];
+    }
+    __syncthreads();
+    // G(kx, v) = sum_ky F(kx, ky) e^{2 pi i ky v / W}
+    for (int idx = tid; idx < xw * W; idx += kB) {
+        const int kx = idx / W, v = idx - kx * W;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int r = 0; r < W; ++r) {
+            const int ky = (r < W / 2) ? r : r - W;
+            const float2 t = tw[((ky * v) % W + W) % W];
+            const float2 f = Fw[r][kx];
+            acc.x += f.x * t.x - f.y * t.y;
+            acc.y += f.x * t.y + f.y * t.x;
+        }
+        G[kx][v] = acc;
+    }
+    __syncthreads();
+    double s1 = 0., s2 = 0.;
+    for (int idx = tid; idx < W * W; idx += kB) {
+        const int v = idx / W, u = idx - v * W;
+        float acc = 0.f;
+        for (int kx = 0; kx < xw; ++kx) {
+            const float h = (kx == 0 || kx == W / 2) ? 1.f : 2.f;
+            const float2 t = tw[(kx * u) % W];
+            const float2 g = G[kx][v];
+            acc += h * (g.x * t.x - g.y * t.y);
+        }
+        img[idx] = acc;
+        s1 += acc;
+        s2 += (double)acc * acc;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+    }
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = s1;
+        red[1][tid >> 6] = s2;
+    }
+    __syncthreads();
+    double t1 = 0., t2 = 0.;
+    for (int w = 0; w < kB / 64; ++w) {
+        t1 += red[0][w];
+        t2 += red[1][w];
+    }
+    const double mean = t1 / (double)(W * W);
+    const double var = t2 / (double)(W * W) - mean * mean;
+    const float inv = var > 0. ? (float)(1. / sqrt(var)) : 0.f;
+    const float mf = (float)mean;
+    for (int idx = tid; idx < W * W; idx += kB) {
+        const float y = (img[idx] - mf) * inv;
+        out[(size_t)b * W * W + idx] = to_bf16(y);
+        if (out_f32) out_f32[(size_t)b * W * W + idx] = y;
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
+// im2col for a 3x3 convolution, pad 1: X [n][H][H][C] bf16 -> col [n*Ho*Ho][Kp], k = tap*C + c (k >= 9C: 0).
+// C % 8 == 0: one thread per (m, tap, 8 channels); C == 1: one thread per (m, k).
+// --------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kB) k_im2col(const bf16 *__restrict__ X, int n, int H, int C, int stride, int Ho,
+                                               int Kp, bf16 *__restrict__ col) {
+    const long long idx = (long long)blockIdx.x * kB + threadIdx.x;
+    const long long M = (long long)n * Ho * Ho;
+    if (C % 8 == 0) {
+        const int c8 = C / 8;
+        if (idx >= M * 9 * c8) return;
+        const long long m = idx / (9 * c8);
+        const int rem = (int)(idx - m * 9 * c8), tap = rem / c8, cg = rem - tap * c8;
+        const int img = (int)(m / (Ho * Ho)), pix = (int)(m - (long long)img * Ho * Ho);
+        const int oy = pix / Ho, ox = pix - oy * Ho;
+        const int iy = oy * stride + tap / 3 - 1, ix = ox * stride + tap % 3 - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (iy >= 0 && iy < H && ix >= 0 && ix < H)
+            v = *reinterpret_cast<const uint4 *>(X + (((size_t)img * H + iy) * H + ix) * C + cg * 8);
+        *reinterpret_cast<uint4 *>(col + (size_t)m * Kp + tap * C + cg * 8) = v;
+    } else {
+        if (idx >= M * Kp) return;
+        const long long m = idx / Kp;
+        const int k = (int)(idx - m * Kp);
+        bf16 v = 0;
+        if (k < 9 * C) {
+            const int tap = k / C, c = k - tap * C;
+            const int img = (int)(m / (Ho * Ho)), pix = (int)(m - (long long)img * Ho * Ho);
+            const int oy = pix / Ho, ox = pix - oy * Ho;
+            const int iy = oy * stride + tap / 3 - 1, ix = ox * stride + tap % 3 - 1;
+            if (iy >= 0 && iy < H && ix >= 0 && ix < H) v = X[(((size_t)img * H + iy) * H + ix) * C + c];
+        }
+        col[idx] = v;
+    }
+}
+
+// dX[img][y][x][c] = sum over taps of dcol[(img, oy, ox)][tap*C + c] with oy*stride + ky - 1 = y (and x), times
+// the ReLU mask of the layer below (Y > 0), as bf16. One thread per element of dX.
+__global__ void __launch_bounds__(kB) k_col2im_mask(const float *__restrict__ dcol, int n, int H, int C, int stride,
+                                                    int Ho, int Kp, const bf16 *__restrict__ Y,
+                                                    bf16 *__restrict__ dZ) {
+    const long long idx = (long long)blockIdx.x * kB + threadIdx.x;
+    const long long total = (long long)n * H * H * C;
+    if (idx >= total) return;
+    const int c = (int)(idx % C);
+    const long long pix = idx / C;
+    const int x = (int)(pix % H), y = (int)((pix / H) % H), img = (int)(pix / ((long long)H * H));
+    float acc = 0.f;
+    for (int ky = 0; ky < 3; ++ky) {
+        const int ty = y + 1 - ky;
+        if (ty < 0 || ty % stride) continue;
+        const int oy = ty / stride;
+        if (oy >= Ho) continue;
+        for (int kx = 0; kx < 3; ++kx) {
+            const int tx = x + 1 - kx;
+            if (tx < 0 || tx % stride) continue;
+            const int ox = tx / stride;
+            if (ox >= Ho) continue;
+            acc += dcol[(((size_t)img * Ho + oy) * Ho + ox) * Kp + (3 * ky + kx) * C + c];
+        }
+    }
+    dZ[idx] = to_bf16(from_bf16(Y[idx]) > 0.f ? acc : 0.f);
+}
+
+// dZ = bf16(d * (Y > 0)) elementwise
+__global__ void __launch_bounds__(kB) k_mask_relu(const float *__restrict__ d, const bf16 *__restrict__ Y, long long n,
+                                                  bf16 *__restrict__ dZ) {
+    const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+    if (i < n) dZ[i] = to_bf16(from_bf16(Y[i]) > 0.f ? d[i] : 0.f);
+}
+
+// out[j] += sum_i Z[i][j] over rows [blockIdx.y*rows, +rows): bias gradients. grid (ceil(N/256), ceil(M/rows)).
+__global__ void __launch_bounds__(kB) k_col_sum(const bf16 *__restrict__ Z, int M, int N, int rows,
+                                                float *__restrict__ out) {
+    const int j = blockIdx.x * kB + threadIdx.x;
+    if (j >= N) return;
+    const int r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
+    float s = 0.f;
+    for (int i = r0; i < r1; ++i) s += from_bf16(Z[(size_t)i * N + j]);
+    atomicAdd(&out[j], s);
+}
+
+// --------------------------------------------------------------------------------------------------
+// GEMM on MFMA: C(m, n) = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn], bf16.
+// AK: A is k-contiguous (sak == 1, K % 8 == 0) else m-contiguous (sam == 1, M % 8 == 0); BK likewise with n.
+// Tile 64 x 64 x 32 per block of 4 waves (each 32 x 32: 2 x 2 MFMA tiles), LDS-staged; blockIdx.z splits k
+// into chunks of kchunk (a multiple of 32).
+// --------------------------------------------------------------------------------------------------
+enum { kEpiF32 = 0, kEpiAtomicF32 = 1, kEpiBiasReluBf16 = 2 };
+
+struct GemmArgs {
+    const bf16 *A, *B;
+    long long sam, sak, sbk, sbn;
+    int M, N, K, kchunk;
+    const float *bias;  // [N] or null
+    float *Cf;          // kEpiF32 / kEpiAtomicF32
+    bf16 *Cb;           // kEpiBiasReluBf16
+    long long ldc;
+};
+
+template <bool AK, bool BK, int EPI>
+__global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
+    constexpr int kPitch = 40;  // bf16 per LDS row (32 + 8): 80-byte rows keep 16-byte alignment
+    __shared__ __attribute__((aligned(16))) bf16 As[64 * kPitch];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[64 * kPitch];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int r16 = lane & 15, gq = lane >> 4;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    const int kb = blockIdx.z * g.kchunk, ke = min(g.K, kb + g.kchunk);
+    floatx4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        if (AK) {
+            const int m = tid >> 2, kk = (tid & 3) * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (m0 + m < g.M && k0 + kk < ke)
+                v = *reinterpret_cast<const uint4 *>(g.A + (size_t)(m0 + m) * g.sam + (k0 + kk));
+            *reinterpret_cast<uint4 *>(&As[m * kPitch + kk]) = v;
+        } else {
+            const int k = tid >> 3, mm = (tid & 7) * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (k0 + k < ke && m0 + mm < g.M)
+                v = *reinterpret_cast<const uint4 *>(g.A + (size_t)(k0 + k) * g.sak + (m0 + mm));
+            const bf16 *e = reinterpret_cast<const bf16 *>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) As[(mm + j) * kPitch + k] = e[j];
+        }
+        if (BK) {
+            const int n = tid >> 2, kk = (tid & 3) * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (n0 + n < g.N && k0 + kk < ke)
+                v = *reinterpret_cast<const uint4 *>(g.B + (size_t)(n0 + n) * g.sbn + (k0 + kk));
+            *reinterpret_cast<uint4 *>(&Bs[n * kPitch + kk]) = v;
+        } else {
+            const int k = tid >> 3, nn = (tid & 7) * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (k0 + k < ke && n0 + nn < g.N)
+                v = *reinterpret_cast<const uint4 *>(g.B + (size_t)(k0 + k) * g.sbk + (n0 + nn));
+            const bf16 *e = reinterpret_cast<const bf16 *>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Bs[(nn + j) * kPitch + k] = e[j];
+        }
+        __syncthreads();
+        bf16x8_t a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            a[i] = *reinterpret_cast<const bf16x8_t *>(&As[(32 * wm + 16 * i + r16) * kPitch + 8 * gq]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            b[j] = *reinterpret_cast<const bf16x8_t *>(&Bs[(32 * wn + 16 * j + r16) * kPitch + 8 * gq]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int n = n0 + 32 * wn + 16 * j + r16;
+            if (n >= g.N) continue;
+            const float bias = (g.bias && blockIdx.z == 0) ? g.bias[n] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + 32 * wm + 16 * i + 4 * gq + r;
+                if (m >= g.M) continue;
+                const float v = acc[i][j][r] + bias;
+                if (EPI == kEpiF32) g.Cf[(size_t)m * g.ldc + n] = v;
+                else if (EPI == kEpiAtomicF32) atomicAdd(&g.Cf[(size_t)m * g.ldc + n], v);
+                else g.Cb[(size_t)m * g.ldc + n] = to_bf16(fmaxf(v, 0.f));
+            }
+        }
+}
+
+// --------------------------------------------------------------------------------------------------
+// Loss and prediction over NB bins: one block per image.
+// --------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float block_max(float v, float *red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = red[0];
+    for (int w = 1; w < kB / 64; ++w) r = fmaxf(r, red[w]);
+    return r;
+}
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.;
+    for (int w = 0; w < kB / 64; ++w) r += red[w];
+    return r;
+}
+
+// loss[b] = logsumexp(z) - z[label]; dz = (softmax(z) - onehot) * scale (bf16)
+__global__ void __launch_bounds__(kB) k_softmax_xent(const float *__restrict__ z, const int *__restrict__ labels,
+                                                     int NB, float scale, double *__restrict__ loss,
+                                                     bf16 *__restrict__ dz) {
+    __shared__ float redf[kB / 64];
+    __shared__ double redd[kB / 64];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const float *zb = z + (size_t)b * NB;
+    float mx = -3.0e38f;
+    for (int j = tid; j < NB; j += kB) mx = fmaxf(mx, zb[j]);
+    mx = block_max(mx, redf);
+    double s = 0.;
+    for (int j = tid; j < NB; j += kB) s += exp((double)(zb[j] - mx));
+    s = block_sum(s, redd);
+    const int lab = labels[b];
+    if (tid == 0) loss[b] = log(s) + (double)mx - (double)zb[lab];
+    const double inv = 1. / s;
+    for (int j = tid; j < NB; j += kB) {
+        const float p = (float)(exp((double)(zb[j] - mx)) * inv);
+        dz[(size_t)b * NB + j] = to_bf16(((j == lab) ? p - 1.f : p) * scale);
+    }
+}
+
+constexpr int kMaxTopK = 64;
+
+// top-k bins of softmax(z) per image (descending probability, ties to the lower bin)
+__global__ void __launch_bounds__(kB) k_topk(const float *__restrict__ z, int NB, int topk, int *__restrict__ bins,
+                                             float *__restrict__ probs) {
+    extern __shared__ float zs[];
+    __shared__ float redf[kB / 64];
+    __shared__ double redd[kB / 64];
+    __shared__ int redi[kB / 64];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const float *zb = z + (size_t)b * NB;
+    float mx = -3.0e38f;
+    for (int j = tid; j < NB; j += kB) {
+        zs[j] = zb[j];
+        mx = fmaxf(mx, zs[j]);
+    }
+    mx = block_max(mx, redf);
+    double s = 0.;
+    for (int j = tid; j < NB; j += kB) s += exp((double)(zs[j] - mx));
+    s = block_sum(s, redd);
+    for (int t = 0; t < topk; ++t) {
+        float bv = -3.0e38f;
+        int bi = 0x7fffffff;
+        for (int j = tid; j < NB; j += kB)
+            if (zs[j] > bv || (zs[j] == bv && j < bi)) {
+                bv = zs[j];
+                bi = j;
+            }
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(bv, o);
+            const int oi = __shfl_xor(bi, o);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        __syncthreads();
+        if ((tid & 63) == 0) {
+            redf[tid >> 6] = bv;
+            redi[tid >> 6] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < kB / 64; ++w)
+                if (redf[w] > bv || (redf[w] == bv && redi[w] < bi)) {
+                    bv = redf[w];
+                    bi = redi[w];
+                }
+            bins[(size_t)b * topk + t] = bi;
+            probs[(size_t)b * topk + t] = (float)(exp((double)(bv - mx)) / s);
+            zs[bi] = -3.0e38f;
+        }
+        __syncthreads();
+    }
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights, refreshing the BF16 copies; grads are cleared.
+__global__ void __launch_bounds__(kB) k_adam(float *__restrict__ w, float *__restrict__ grad, float *__restrict__ m1,
+                                             float *__restrict__ m2, bf16 *__restrict__ wb, long long n, float lr,
+                                             float b1, float b2, float eps, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const float gr = grad[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gr;
+    const float v = b2 * m2[i] + (1.f - b2) * gr * gr;
+    m1[i] = a;
+    m2[i] = v;
+    const float wi = w[i] - lr * (a * c1) / (sqrtf(v * c2) + eps);
+    w[i] = wi;
+    wb[i] = to_bf16(wi);
+    grad[i] = 0.f;
+}
+
+__global__ void __launch_bounds__(kB) k_to_bf16(const float *__restrict__ x, long long n, bf16 *__restrict__ y) {
+    const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+    if (i < n) y[i] = to_bf16(x[i]);
+}
+
+// gather a batch of stored images: dst[i] = src[idx[i]] (W*W bf16 each)
+__global__ void __launch_bounds__(kB) k_gather_images(const bf16 *__restrict__ src, const int *__restrict__ idx,
+                                                      int WW, bf16 *__restrict__ dst) {
+    const int b = blockIdx.y;
+    const int j = blockIdx.x * kB + threadIdx.x;
+    if (j < WW) dst[(size_t)b * WW + j] = src[(size_t)idx[b] * WW + j];
+}
+
+__global__ void __launch_bounds__(kB) k_gather_labels(const int *__restrict__ src, const int *__restrict__ idx, int n,
+                                                      int *__restrict__ dst) {
+    const int i = blockIdx.x * kB + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+}  // namespace rlnp

@@ -106,6 +106,44 @@ void hp_pix2ang_z_phi(int order, long long pix, double &z, double &phi) {
     phi = ((double)jp - (double)(shift + 1) * 0.5) * ((kPi / 2.) / (double)nr);
 }
 
+// HEALPix NEST pixel containing direction (z = cos(theta), phi): the ang2pix of Gorski et al. 2005 (ApJ 622,
+// 759), as Healpix_Base::ang2pix_z_phi (src/Healpix_2.15a/healpix_base.cc:368).
+long long hp_ang2pix_nest(int order, double z, double phi) {
+    const int nside = 1 << order;
+    const double za = std::fabs(z);
+    double tt = std::fmod(phi, 2. * kPi);
+    if (tt < 0.) tt += 2. * kPi;
+    tt *= 2. / kPi;  // in [0, 4)
+    int face, ix, iy;
+    if (za <= 2. / 3.) {
+        const double t1 = nside * (0.5 + tt), t2 = nside * z * 0.75;
+        const int jp = (int)(t1 - t2), jm = (int)(t1 + t2);
+        const int ifp = jp / nside, ifm = jm / nside;
+        face = (ifp == ifm) ? (ifp | 4) : ((ifp < ifm) ? ifp : ifm + 8);
+        ix = jm & (nside - 1);
+        iy = nside - (jp & (nside - 1)) - 1;
+    } else {
+        int ntt = (int)tt;
+        if (ntt >= 4) ntt = 3;
+        const double tp = tt - ntt;
+        const double tmp = nside * std::sqrt(3. * (1. - za));
+        int jp = (int)(tp * tmp), jm = (int)((1. - tp) * tmp);
+        jp = std::min(jp, nside - 1);
+        jm = std::min(jm, nside - 1);
+        if (z >= 0.) {
+            face = ntt;
+            ix = nside - jm - 1;
+            iy = nside - jp - 1;
+        } else {
+            face = ntt + 8;
+            ix = jp;
+            iy = jm;
+        }
+    }
+    return hp_xyf2nest(order, ix, iy, face);
+}
+
+
 void euler_angles2matrix(double rot, double tilt, double psi, double *A) {
     const double a = rad(rot), b = rad(tilt), g = rad(psi);
     const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
@@ -119,6 +157,23 @@ void euler_angles2matrix(double rot, double tilt, double psi, double *A) {
     A[6] = sc;
     A[7] = ss;
     A[8] = cb;
+}
+
+// ZYZ angles of a rotation matrix in the convention of Euler_angles2matrix (src/euler.cpp:52-91).
+void matrix_to_angles(const double *A, double &rot, double &tilt, double &psi) {
+    const double sb = std::sqrt(A[2] * A[2] + A[5] * A[5]);
+    if (sb > 1e-9) {
+        psi = std::atan2(A[5], -A[2]);
+        rot = std::atan2(A[7], A[6]);
+        tilt = std::atan2(sb, A[8]);
+    } else {
+        rot = 0.;
+        tilt = A[8] > 0 ? 0. : kPi;
+        psi = std::atan2(-A[3], A[0]);
+    }
+    rot *= 180. / kPi;
+    tilt *= 180. / kPi;
+    psi *= 180. / kPi;
 }
 
 bool PoseGrid::build(const rln_estep_config &cfg) {
@@ -355,5 +410,15 @@ extern "C" int rln_sampling_local(const rln_estep_config *cfg, double prior_rot,
     std::copy(dp.begin(), dp.end(), dir_prior);
     std::copy(pi.begin(), pi.end(), psi_index);
     std::copy(pp.begin(), pp.end(), psi_prior);
+    return RLN_OK;
+}
+
+extern "C" int rln_sampling_ang2pix(int order, double rot, double tilt, long long *pix) {
+    if (order < 0 || order > 12 || !pix) {
+        rln::set_error("rln_sampling_ang2pix: invalid argument");
+        return RLN_ERR_ARG;
+    }
+    const double a = rot * rln::kPi / 180., b = tilt * rln::kPi / 180.;
+    *pix = rln::hp_ang2pix_nest(order, std::cos(b), std::atan2(std::sin(b) * std::sin(a), std::sin(b) * std::cos(a)));
     return RLN_OK;
 }

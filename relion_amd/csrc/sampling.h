@@ -45,8 +45,10 @@ void local_priors_batch(const PoseGrid &g, const rln_estep_config &cfg, const rl
 void hp_nest2xyf(int order, long long pix, int &ix, int &iy, int &face);
 long long hp_xyf2nest(int order, int ix, int iy, int face);
 void hp_pix2ang_z_phi(int order, long long pix, double &z, double &phi);
+long long hp_ang2pix_nest(int order, double z, double phi);
 
 // Euler_angles2matrix (src/euler.cpp:52-91), row-major.
 void euler_angles2matrix(double rot, double tilt, double psi, double *A);
+void matrix_to_angles(const double *A, double &rot, double &tilt, double &psi);
 
 }  // namespace rln

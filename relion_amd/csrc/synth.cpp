@@ -84,23 +84,6 @@ inline cd axis_factor(double sigma, double c, double k, double n) {
     return cd(env * cos(ph), env * sin(ph));
 }
 
-// ZYZ angles of a rotation matrix in the convention of Euler_angles2matrix (src/euler.cpp:52-91).
-void matrix_to_angles(const double *A, double &rot, double &tilt, double &psi) {
-    const double sb = sqrt(A[2] * A[2] + A[5] * A[5]);
-    if (sb > 1e-9) {
-        psi = atan2(A[5], -A[2]);
-        rot = atan2(A[7], A[6]);
-        tilt = atan2(sb, A[8]);
-    } else {
-        rot = 0.;
-        tilt = A[8] > 0 ? 0. : kPi;
-        psi = atan2(-A[3], A[0]);
-    }
-    rot *= 180. / kPi;
-    tilt *= 180. / kPi;
-    psi *= 180. / kPi;
-}
-
 }  // namespace
 
 // CTF::initialise + CTF::getCTF (src/ctf.cpp:207-261, src/ctf.h:184-240); x, y in 1/Angstrom.

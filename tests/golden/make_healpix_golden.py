@@ -1,5 +1,6 @@
 """Writes tests/golden/healpix_ref.npz from the reference's own HEALPix (src/Healpix_2.15a compiled as-is into
-oracle/_ref by oracle/Makefile): (z, phi) of every NEST pixel and the NEST children of every pixel, orders 0-4.
+oracle/_ref by oracle/Makefile): (z, phi) of every NEST pixel and the NEST children of every pixel, orders 0-4, and
+the NEST pixel (ang2pix_z_phi) of 4000 seeded random directions plus the pole and seam cases, orders 0-4.
 
 Run in the development container, where /root/reference exists: `python tests/golden/make_healpix_golden.py`.
 """
@@ -30,5 +31,12 @@ for order in range(5):
         ix[p], iy[p], face[p] = i.value, j.value, f.value
     out[f"z{order}"], out[f"phi{order}"] = z, phi
     out[f"ix{order}"], out[f"iy{order}"], out[f"face{order}"] = ix, iy, face
+rng = np.random.default_rng(20241)
+az = np.concatenate([rng.uniform(-1, 1, 4000), [1.0, -1.0, 2.0 / 3.0, -2.0 / 3.0, 0.0, 0.0]])
+aphi = np.concatenate([rng.uniform(0, 2 * np.pi, 4000), [0.3, 5.0, 1.0, 2.0, 0.0, np.pi / 2]])
+out["a_z"], out["a_phi"] = az, aphi
+for order in range(5):
+    out[f"a_pix{order}"] = np.array([ref.ref_hp_ang2pix_z_phi(order, float(a), float(b)) for a, b in zip(az, aphi)],
+                                    np.int64)
 np.savez_compressed(os.path.join(HERE, "healpix_ref.npz"), **out)
 print("wrote", os.path.join(HERE, "healpix_ref.npz"))

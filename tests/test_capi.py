@@ -8,18 +8,22 @@ import pytest
 
 import relion_amd as ra
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "rln_estep.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+HEADERS = sorted(os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE) if f.endswith(".h"))
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(rln_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(rln_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_symbols_exported():
     names = declared_functions()
-    assert len(names) >= 25
+    assert len(HEADERS) >= 2 and len(names) >= 45
+    assert "rln_proposal_train" in names and "rln_estep_set_orientation_sets" in names
     lib = C.CDLL(ra.LIB_PATH)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
