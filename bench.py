@@ -1,13 +1,24 @@
 #!/usr/bin/env python3
 """E-step throughput on one MI355X: particles/s of the expectation step at box 256, HEALPix order 3.
 
-Workload (stated choices marked *): one 256^3 map of 150 Gaussian blobs at 1.0 A/pixel, 10,000 particles
-(256x256, SNR 0.03, defocus U(0.8, 2.5) um, 300 kV, Cs 2.7 mm, Q0 0.1, offsets U(+-3 px)), HEALPix order 3
-local search (*sigma_ang 7.5 deg = twice the fine angular step, cut at 3 sigma; *priors = true pose + 3 deg
-Gaussian noise), offsets 4 A range, 0.5 A step, oversampling 1, padding 2, adaptive_fraction 0.999,
-*particle diameter 200 A (coarse image 48 px), *current_size = box. Data are synthetic (seeded).
+Data (stated choices marked *): one 256^3 map of 150 Gaussian blobs at 1.0 A/pixel, 10,000 particles (256x256,
+SNR 0.03, defocus U(0.8, 2.5) um, 300 kV, Cs 2.7 mm, Q0 0.1, offsets U(+-3 px)), offsets 4 A range, 0.5 A
+step, oversampling 1, padding 2, adaptive_fraction 0.999, *particle diameter 200 A (coarse image 48 px),
+*current_size = box. Synthetic and seeded.
 
-A step is one E-step over all particles: projections of the coarse grid, coarse pass, weights and
+Headline (the learned proposal on this data): HEALPix order 3 global search.
+  * plain: the exhaustive coarse search in plain FP32 (36,864 orientations x 197 translations per particle),
+    timed on the first --plain-particles particles (particles are independent, so particles/s carries over);
+  * proposal: the pose-proposal network (64x64 input, 6 conv x 64 channels, 192 x 24 bins) trained on
+    --train-samples projections of the reference with CTF and the model's noise, its top-16 bins seed the coarse
+    search (exhaustive fallback below top-1 probability 0.2), coarse cross term in split-BF16 on MFMA. Timed
+    on all particles; the step includes the network's inference.
+  Reported beside it: the miss rate (particles whose max-likelihood pose differs from the exhaustive search's)
+  and the FSC between the two paths' backprojections, both over the plain subset; training time (untimed).
+Secondary (precision variant): HEALPix order 3 local search (*sigma_ang 7.5 deg, *priors = true pose + 3 deg
+noise), plain FP32 vs split-BF16 coarse pass on all particles.
+
+A step is one E-step over the particles: projections of the coarse grid, coarse pass, weights and
 significance, fine pass, weighted sums and backprojection; particle images are resident in HBM before the
 timed region. Prints one JSON line.
 """
@@ -33,13 +44,19 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=10000)
+    ap.add_argument("--plain-particles", type=int, default=2000,
+                    help="particles of the exhaustive plain-FP32 baseline (and of the miss-rate / FSC check)")
     ap.add_argument("--box", type=int, default=256)
+    ap.add_argument("--train-samples", type=int, default=200000)
+    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--topk", type=int, default=16)
+    ap.add_argument("--min-top1", type=float, default=0.2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU oracle budget: batches of one particle per thread until this much time (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--paths", default="plain,split",
-                    help="comma list of E-step paths to time: plain (FP32 fallback), split (split-BF16 MFMA coarse)")
+    ap.add_argument("--paths", default="proposal,plain,local",
+                    help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (reference for traffic counters)")
@@ -53,69 +70,81 @@ def log(msg, args):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def rotm(a, b, c):
+    a, b, c = np.radians([a, b, c])
+    ca, sa, cb, sb, cg, sg = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    return np.array([[cg * cb * ca - sg * sa, cg * cb * sa + sg * ca, -cg * sb],
+                     [-sg * cb * ca - cg * sa, -sg * cb * sa + cg * ca, sg * sb],
+                     [sb * ca, sb * sa, cb]])
+
+
+def angle_between(r0, r1):
+    A, B = rotm(*r0), rotm(*r1)
+    return float(np.degrees(np.arccos(np.clip((np.trace(A @ B.T) - 1) / 2, -1, 1))))
+
+
+def fsc(v0, v1, box, pad, r_max):
+    """FSC per shell (original-box Fourier pixels) between two reconstructions data / weight."""
+    (re0, im0, w0), (re1, im1, w1) = v0, v1
+    F0 = np.where(w0 > 0, (re0 + 1j * im0) / np.maximum(w0, 1e-30), 0)
+    F1 = np.where(w1 > 0, (re1 + 1j * im1) / np.maximum(w1, 1e-30), 0)
+    zd, yd, xd = re0.shape
+    z = np.arange(zd) - zd // 2
+    y = np.arange(yd) - yd // 2
+    x = np.arange(xd)
+    r = np.sqrt(z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2) / pad
+    shell = np.rint(r).astype(np.int64).ravel()
+    keep = shell <= r_max
+    sh = shell[keep]
+    a, b = F0.ravel()[keep], F1.ravel()[keep]
+    num = np.bincount(sh, np.real(a * np.conj(b)), r_max + 1)
+    d0 = np.bincount(sh, np.abs(a) ** 2, r_max + 1)
+    d1 = np.bincount(sh, np.abs(b) ** 2, r_max + 1)
+    return num / np.sqrt(np.maximum(d0 * d1, 1e-300))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs the E-step on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "this build measures one GPU: run with --gpus 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not run)"}))
         return 2
     import relion_amd as ra
     from relion_amd import hipmem
+    from relion_amd.proposal import PoseProposal
 
     if hipmem.device_count() < 1:
         print(json.dumps({"error": "no HIP device visible"}))
         return 2
     hipmem.set_device(0)
+    paths = args.paths.split(",")
     box, px, order = args.box, 256.0 / args.box if args.box != 256 else 1.0, 3
     diameter = 200.0 * box / 256.0
     coarse = ra.coarse_image_size(box, px, order, diameter)
-    cfg = ra.make_config(box, px, order, offset_range=4.0, offset_step=0.5, coarse_size=coarse, local_search=True,
-                         sigma_ang=7.5)
+    cfg_g = ra.make_config(box, px, order, offset_range=4.0, offset_step=0.5, coarse_size=coarse)
+    cfg_l = ra.make_config(box, px, order, offset_range=4.0, offset_step=0.5, coarse_size=coarse, local_search=True,
+                           sigma_ang=7.5)
+    n_dir, n_psi, n_tr = ra.sampling_counts(cfg_g)
     spec = ra.synth_spec(args.seed, box, px, 150, snr=0.03, defocus=(8000.0, 25000.0), offset_max=3.0,
                          prior_sigma_deg=3.0)
     t0 = time.time()
     vol = ra.synth_reference(spec, 0, box)
     n = args.particles
+    n_plain = min(args.plain_particles, n)
     fimg, fctf, meta, pose, nvar = ra.synth_particles(spec, n, box)
     log(f"generated reference + {n} particles in {time.time() - t0:.1f} s (coarse {coarse}px)", args)
     sigma2 = np.full(box // 2 + 1, float(nvar.mean()))
+    pdf_dir = np.full(n_dir, 1.0 / n_dir)
     d_img = hipmem.DeviceArray(fimg)
     d_ctf = hipmem.DeviceArray(fctf)
     del fimg, fctf
+    meta_plain = (ra.ParticleMeta * n_plain)(*[meta[i] for i in range(n_plain)])
     dims = ra.volume_dims(box)
     proj = ra.AccProjector(0)
     proj.setMdlDim(*dims, 2.0)
     proj.initMdl(vol)
     bp = ra.AccBackprojector(0)
     bp.setMdlDim(*dims, 2.0)
-    precisions = {"plain": ra.RLN_PREC_PLAIN_FP32, "split": ra.RLN_PREC_SPLIT_BF16}
-
-    def time_path(name):
-        cfg.precision = precisions[name]
-        es = ra.ExpectationStep(cfg, 0)
-
-        def step():
-            bp.initMdl()
-            es.set_model([proj], [bp], [1.0], sigma2)   # per-iteration model: invalidates the grid projections
-            return es.run(d_img.address, d_ctf.address, meta, on_device=True)
-
-        for w in range(args.warmup):
-            t = time.time()
-            step()
-            log(f"[{name}] warmup {w}: {time.time() - t:.2f} s", args)
-        hipmem.synchronize()
-        t_start = time.time()
-        for _ in range(args.steps):
-            res, wsum = step()
-        hipmem.synchronize()
-        elapsed = time.time() - t_start
-        timings, counters = es.timings(), es.counters()
-        log(f"[{name}] timed {args.steps} steps: {elapsed:.2f} s; stages "
-            f"{json.dumps({k: round(v, 1) for k, v in timings.items()})}", args)
-        log(f"[{name}] counters {json.dumps(counters)}", args)
-        es.close()
-        return dict(res=res, elapsed=elapsed, timings=timings, counters=counters,
-                    value=n * args.steps / elapsed, ms=1000.0 * elapsed / args.steps)
 
     if args.copy_calibration_mb > 0:
         nb = args.copy_calibration_mb << 20
@@ -125,36 +154,131 @@ def main():
         hipmem.synchronize()
         log(f"copy calibration: {nb} bytes device-to-device", args)
         del src, dst
-    runs = {name: time_path(name) for name in args.paths.split(",")}
-    head = "split" if "split" in runs else "plain"
-    R = runs[head]
-    res, timings, counters = R["res"], R["timings"], R["counters"]
-    value, ms_per_step = R["value"], R["ms"]
 
-    # accuracy against the generating poses (orientation error of the max-likelihood pose)
-    def rotm(a, b, c):
-        a, b, c = np.radians([a, b, c])
-        ca, sa, cb, sb, cg, sg = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
-        return np.array([[cg * cb * ca - sg * sa, cg * cb * sa + sg * ca, -cg * sb],
-                         [-sg * cb * ca - cg * sa, -sg * cb * sa + cg * ca, sg * sb],
-                         [sb * ca, sb * sa, cb]])
-    errs = []
-    for i in range(0, n, max(1, n // 500)):
-        A = rotm(*pose[i, :3])
-        B = rotm(res[i]["rot"], res[i]["tilt"], res[i]["psi"])
-        errs.append(np.degrees(np.arccos(np.clip((np.trace(A @ B.T) - 1) / 2, -1, 1))))
-    errs = np.array(errs)
+    def timed(name, step, n_part, steps, warmup, es):
+        for w in range(warmup):
+            t = time.time()
+            step()
+            log(f"[{name}] warmup {w}: {time.time() - t:.2f} s", args)
+        hipmem.synchronize()
+        t_start = time.time()
+        for _ in range(steps):
+            out = step()
+        hipmem.synchronize()
+        elapsed = time.time() - t_start
+        timings, counters = es.timings(), es.counters()
+        log(f"[{name}] timed {steps} steps: {elapsed:.2f} s; stages "
+            f"{json.dumps({k: round(v, 1) for k, v in timings.items()})}", args)
+        log(f"[{name}] counters {json.dumps(counters)}", args)
+        return dict(out=out, elapsed=elapsed, timings=timings, counters=counters, value=n_part * steps / elapsed,
+                    ms=1000.0 * elapsed / steps, steps=steps, warmup=warmup, particles=n_part)
+
+    runs, extra = {}, {}
+    pp = None
+    # ---------------------------------------------------------------- learned proposal (training is untimed)
+    if "proposal" in paths:
+        pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=args.seed)
+        t = time.time()
+        pp.generate(proj, args.train_samples, ori_size=box, pixel_size=px, sigma2_noise=sigma2,
+                    defocus=(8000.0, 25000.0), max_astigmatism=300.0, offset_max=3.0)
+        hipmem.synchronize()
+        t_gen = time.time() - t
+        t = time.time()
+        loss = pp.train(epochs=args.train_epochs, batch=256, lr=1e-3)
+        t_train = time.time() - t
+        log(f"proposal: {args.train_samples} projections in {t_gen:.1f} s, {args.train_epochs} epochs in "
+            f"{t_train:.1f} s, last-epoch loss {loss:.3f}", args)
+        extra["proposal_training"] = {"samples": args.train_samples, "epochs": args.train_epochs, "batch": 256,
+                                      "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+                                      "last_epoch_loss": round(loss, 4)}
+        cfg_g.precision = ra.RLN_PREC_SPLIT_BF16
+        es_p = ra.ExpectationStep(cfg_g, 0)
+        pred_ms = []
+
+        def proposal_step(count=n, mt=meta):
+            bp.initMdl()
+            es_p.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
+            t1 = time.time()
+            bins, probs = pp.predict(d_img.address, count, box, topk=args.topk, on_device=True)
+            off, og = pp.orientation_lists(bins, probs, order, n_psi, min_top1=args.min_top1)
+            es_p.set_orientation_lists(off, og)
+            pred_ms.append(1000.0 * (time.time() - t1))
+            res, _ = es_p.run(d_img.address, d_ctf.address, mt, on_device=True)
+            return res, off
+
+        runs["proposal"] = timed("proposal", proposal_step, n, args.steps, args.warmup, es_p)
+        extra["proposal_predict_ms"] = round(float(np.mean(pred_ms[-args.steps:])), 2)
+        off = runs["proposal"]["out"][1]
+        extra["proposal_fallback_fraction"] = round(float(np.mean(np.diff(off) == 0)), 4)
+        extra["proposal_orientations_per_particle"] = round(float(np.mean(np.diff(off)[np.diff(off) > 0])), 1)
+    # ---------------------------------------------------------------- exhaustive plain FP32 baseline
+    if "plain" in paths:
+        cfg_g.precision = ra.RLN_PREC_PLAIN_FP32
+        es_x = ra.ExpectationStep(cfg_g, 0)
+
+        def plain_step():
+            bp.initMdl()
+            es_x.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
+            res, _ = es_x.run(d_img.address, d_ctf.address, meta_plain, on_device=True)
+            return res
+
+        runs["plain"] = timed("plain", plain_step, n_plain, 1, 0, es_x)
+        vol_plain = bp.getMdlData()
+        es_x.close()
+        if pp is not None:
+            # the proposal path on the same particles (untimed): misses against the exhaustive search, FSC
+            res_p, off_p = proposal_step(n_plain, meta_plain)
+            vol_prop = bp.getMdlData()
+            res_x = runs["plain"]["out"]
+            ang = np.array([angle_between((a["rot"], a["tilt"], a["psi"]), (b["rot"], b["tilt"], b["psi"]))
+                            for a, b in zip(res_x, res_p)])
+            same = (ang < 1e-3) & (np.abs(res_x["xoff"] - res_p["xoff"]) < 1e-6) & \
+                   (np.abs(res_x["yoff"] - res_p["yoff"]) < 1e-6)
+            f = fsc(vol_plain, vol_prop, box, 2.0, box // 2)
+            lim = int(0.8 * (box // 2))
+            extra["proposal_vs_exhaustive"] = {
+                "particles": n_plain, "kept_best_pose": round(float(same.mean()), 4),
+                "miss_rate": round(float(1.0 - same.mean()), 4),
+                "misses_within_fine_step_deg": round(float(np.mean(ang[~same] < 3.75)) if (~same).any() else 1.0, 4),
+                "fallback_fraction": round(float(np.mean(np.diff(off_p) == 0)), 4),
+                "fsc_min_to_0.8_nyquist": round(float(f[1:lim + 1].min()), 5),
+                "fsc_mean_to_0.8_nyquist": round(float(f[1:lim + 1].mean()), 5)}
+            log(f"proposal vs exhaustive: {json.dumps(extra['proposal_vs_exhaustive'])}", args)
+    if pp is not None:
+        es_p.close()
+    # ---------------------------------------------------------------- precision variant: local search
+    if "local" in paths:
+        for name, prec in (("local_plain", ra.RLN_PREC_PLAIN_FP32), ("local_split", ra.RLN_PREC_SPLIT_BF16)):
+            cfg_l.precision = prec
+            es_l = ra.ExpectationStep(cfg_l, 0)
+
+            def local_step():
+                bp.initMdl()
+                es_l.set_model([proj], [bp], [1.0], sigma2)
+                res, _ = es_l.run(d_img.address, d_ctf.address, meta, on_device=True)
+                return res
+
+            runs[name] = timed(name, local_step, n, args.steps, args.warmup, es_l)
+            es_l.close()
+
+    head = "proposal" if "proposal" in runs else ("local_split" if "local_split" in runs else next(iter(runs)))
+    R = runs[head]
+    timings, counters = R["timings"], R["counters"]
+    res = R["out"][0] if head == "proposal" else R["out"]
+    errs = np.array([angle_between(pose[i, :3], (res[i]["rot"], res[i]["tilt"], res[i]["psi"]))
+                     for i in range(0, len(res), max(1, len(res) // 500))])
 
     # roofline of the dominant kernel of the reported path
     stage = max(("fine_diff2", "coarse_diff2", "wsum_backproject"), key=lambda k: timings[k])
+    split = head in ("proposal", "local_split")
     if stage == "coarse_diff2":
         work = 4.0 * counters["coarse_poses"] * counters["coarse_pixels"]      # FLOP, cross-term count
         bound, unit = "mfma", "TFLOP/s"
-        peak = PEAK_BF16_TFLOPS / 3.0 if head == "split" else PEAK_FP32_TFLOPS
+        peak = PEAK_BF16_TFLOPS / 3.0 if split else PEAK_FP32_TFLOPS
         achieved = work / (timings[stage] * 1e-3) / 1e12
     else:
         # gather-bound: 64 bytes of the corner-packed reference per (fine orientation, pixel) sample, plus the
-        # 128-byte slot line read and written per backprojected sample in the weighted sums
+        # 256 bytes of slot line read and written per backprojected sample in the weighted sums
         if stage == "fine_diff2":
             work = 64.0 * 8 * counters["fine_jobs"] * counters["fine_pixels"]
         else:
@@ -171,37 +295,52 @@ def main():
     if args.cpu_seconds > 0:
         import oracle
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        ccfg = cfg_g if head == "proposal" else cfg_l
         m, dt = 0, 0.0
         while dt < args.cpu_seconds and m < n:
             k = min(threads, n - m)
             sub_img, sub_ctf, sub_meta, _, _ = ra.synth_particles(spec, k, box, first=m)
             t = time.time()
-            oracle.estep(cfg, [vol], [1.0], sigma2, sub_meta, sub_img, sub_ctf, threads=threads)
+            oracle.estep(ccfg, [vol], [1.0], sigma2, sub_meta, sub_img, sub_ctf,
+                         pdf_direction=None if ccfg.local_search else pdf_dir[None, :], threads=threads)
             dt += time.time() - t
             m += k
+        search = "exhaustive global" if head == "proposal" else "local"
         cpu = {"value": round(m / dt, 4), "unit": "particles/s", "cores": threads, "kind": "port",
-               "sample": f"first {m} particles of the same workload ({dt:.1f} s), FP64 oracle, one particle per "
-                         "OpenMP thread"}
+               "sample": f"first {m} particles of the same data, {search} search ({dt:.1f} s), FP64 oracle, one "
+                         "particle per OpenMP thread"}
         log(f"cpu oracle: {m} particles in {dt:.1f} s on {threads} threads", args)
 
+    def summary(v):
+        return {"particles_per_s": round(v["value"], 2), "ms_per_step": round(v["ms"], 2),
+                "particles": v["particles"], "steps": v["steps"], "warmup": v["warmup"],
+                "stages_ms": {s_: round(t_, 2) for s_, t_ in v["timings"].items()},
+                "coarse_poses": v["counters"]["coarse_poses"]}
+
+    speed = {}
+    if "proposal" in runs and "plain" in runs:
+        speed["proposal_vs_exhaustive_plain"] = round(runs["proposal"]["value"] / runs["plain"]["value"], 2)
+    if "local_split" in runs and "local_plain" in runs:
+        speed["local_split_vs_local_plain"] = round(runs["local_split"]["value"] / runs["local_plain"]["value"], 3)
     out = {
         "metric": "E-step particles/s, box 256 refine: plain FP32 vs proposal+split-BF16",
-        "value": round(value, 3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3+f32" if head == "split" else "f32", "data": "synthetic",
-        "config": {"workload": f"refine box{box} order{order} local-search {n} particles",
-                   "path": "split-bf16 coarse + fp32 fine" if head == "split" else "plain-fp32",
-                   "global_batch": n, "coarse_size": coarse, "sigma_ang_deg": 7.5, "particle_diameter_A": diameter,
-                   "parallelism": "1 GPU"},
+        "value": round(R["value"], 3), "unit": "particles/s", "n_gpus": 1, "steps": R["steps"],
+        "warmup": R["warmup"], "ms_per_step": round(R["ms"], 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16x3+f32" if split else "f32", "data": "synthetic",
+        "config": {"workload": (f"refine box{box} order{order} "
+                                f"{'global search seeded by the pose proposal' if head == 'proposal' else 'local search'}"
+                                f" {n} particles"),
+                   "path": {"proposal": "proposal top-16 + split-bf16 coarse + fp32 fine",
+                            "local_split": "split-bf16 coarse + fp32 fine"}.get(head, head),
+                   "global_batch": n, "coarse_size": coarse, "particle_diameter_A": diameter, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu,
         "stages_ms": {k: round(v, 2) for k, v in timings.items()}, "counters": counters,
-        "paths": {k: {"particles_per_s": round(v["value"], 1), "ms_per_step": round(v["ms"], 2),
-                      "stages_ms": {s_: round(t_, 2) for s_, t_ in v["timings"].items()}} for k, v in runs.items()},
-        "speedup_vs_plain": (round(runs["split"]["value"] / runs["plain"]["value"], 3)
-                             if "split" in runs and "plain" in runs else None),
-        "accuracy": {"median_angular_error_deg": round(float(np.median(errs)), 3),
-                     "frac_within_3deg": round(float((errs < 3.0).mean()), 4)},
+        "paths": {k: summary(v) for k, v in runs.items()},
+        "speedup": speed,
+        "accuracy_vs_true_pose": {"median_angular_error_deg": round(float(np.median(errs)), 3),
+                                  "frac_within_3deg": round(float((errs < 3.0).mean()), 4)},
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
     return 0
 

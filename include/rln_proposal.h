@@ -85,6 +85,14 @@ int rln_proposal_pose_bin(const rln_proposal *p, double rot, double tilt, double
 int rln_proposal_bin_orientations(const rln_proposal *p, int grid_order, int n_psi_grid, int bin, int *out,
                                   int capacity, int *count);
 
+/* Coarse orientation lists of n particles from their top-k bins (rln_proposal_predict output), in the format of
+ * rln_estep_set_orientation_sets: the sorted union of the bins' grid orientations, or an empty list (exhaustive
+ * search) when the particle's top-1 probability is below min_top1. set_offsets[n+1]; set_orients holds at most
+ * capacity entries (n * topk * the orientations of one bin always suffices). */
+int rln_proposal_orientation_sets(const rln_proposal *p, int n, const int *bins, const float *probs, int topk,
+                                  int grid_order, int n_psi_grid, double min_top1, int *set_offsets,
+                                  int *set_orients, long long capacity);
+
 /* Check hooks: the network input images (n x W x W FP32) of particle images, and one forward + backward
  * pass over given input images (FP32, rounded to BF16) with labels: loss = mean cross entropy, grads in the
  * parameter layout (FP32), logits [n][NB]. The weights are not updated. */

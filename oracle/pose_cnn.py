@@ -89,31 +89,43 @@ def col2im(dcol, n, H, cin, s, Ho):
     return dXp[:, 1:H + 1, 1:H + 1, :]
 
 
-def forward(w, images, C, NB, bf=False):
+def _rounder(bf, jitter, rng):
+    """BF16 rounding (or identity); with jitter > 0 the value is first perturbed by a relative N(0, jitter) to
+    model FP32 accumulation error, which moves some values across BF16 rounding boundaries."""
+    if not bf:
+        return lambda a: np.asarray(a, np.float64)
+    if jitter <= 0.0:
+        return bf16
+    return lambda a: bf16(np.asarray(a, np.float64) * (1.0 + jitter * rng.standard_normal(np.shape(a))))
+
+
+def forward(w, images, C, NB, bf=False, jitter=0.0, rng=None):
     """Logits [n][NB] and the activations needed by backward."""
-    r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    r = _rounder(bf, jitter, rng)
     n, W, _ = images.shape
     conv, Wfc, bfc = unpack(w, W, C, NB)
-    X = r(images)[..., None]
+    q = bf16 if bf else (lambda a: np.asarray(a, np.float64))  # exact roundings of stored inputs and weights
+    X = q(images)[..., None]
     cache = []
     for (Wl, bl), l in zip(conv, layers(W, C)):
         col = im2col(X, l["s"], l["Ho"], l["Kp"])
-        Z = col @ r(Wl).T + bl
+        Z = col @ q(Wl).T + bl
         Xn = r(np.maximum(Z, 0.0)).reshape(n, l["Ho"], l["Ho"], C)
         cache.append((X, col))
         X = Xn
     feats = X.reshape(n, -1)
-    logits = feats @ r(Wfc).T + bfc
+    logits = feats @ q(Wfc).T + bfc
     return logits, (cache, X, feats)
 
 
-def loss_and_grads(w, images, labels, C, NB, bf=False):
+def loss_and_grads(w, images, labels, C, NB, bf=False, jitter=0.0, seed=0):
     """Mean cross entropy over the images, its gradient in the parameter layout, and the logits."""
-    r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    rng = np.random.default_rng(seed)
+    r = _rounder(bf, jitter, rng)
     images = np.asarray(images, np.float64)
     n, W, _ = images.shape
     conv, Wfc, bfc = unpack(w, W, C, NB)
-    logits, (cache, X6, feats) = forward(w, images, C, NB, bf)
+    logits, (cache, X6, feats) = forward(w, images, C, NB, bf, jitter, rng)
     m = logits.max(1, keepdims=True)
     e = np.exp(logits - m)
     s = e.sum(1, keepdims=True)
@@ -124,7 +136,8 @@ def loss_and_grads(w, images, labels, C, NB, bf=False):
     dlog = r(p / n)
     gWfc = dlog.T @ feats
     gbfc = dlog.sum(0)
-    dX = (dlog @ r(Wfc)).reshape(X6.shape)
+    q = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    dX = (dlog @ q(Wfc)).reshape(X6.shape)
     dZ = r(dX * (X6 > 0))
     gconv = [None] * len(conv)
     L = layers(W, C)
@@ -136,7 +149,7 @@ def loss_and_grads(w, images, labels, C, NB, bf=False):
         gconv[i] = (dZm.T @ col, dZm.sum(0))
         if i == 0:
             break
-        dcol = dZm @ r(Wl)
+        dcol = dZm @ q(Wl)
         dXin = col2im(dcol, n, l["H"], l["cin"], l["s"], l["Ho"])
         dZ = r(dXin * (Xin > 0))
     return loss, pack(gconv, gWfc, gbfc), logits

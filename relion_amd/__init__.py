@@ -164,6 +164,8 @@ _SIGNATURES = {
     "rln_proposal_predict": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _ip, _fp]),
     "rln_proposal_pose_bin": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _ip]),
     "rln_proposal_bin_orientations": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip]),
+    "rln_proposal_orientation_sets": (C.c_int, [_vp, C.c_int, _ip, _fp, C.c_int, C.c_int, C.c_int, C.c_double, _ip,
+                                                _ip, C.c_longlong]),
     "rln_proposal_images": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, _fp]),
     "rln_proposal_forward_backward": (C.c_int, [_vp, C.c_int, _fp, _ip, _dp, _fp, _fp]),
 }
@@ -458,6 +460,12 @@ class ExpectationStep:
 
     def keep_intermediates(self, keep=True):
         _check(lib.rln_estep_keep_intermediates(self._h, 1 if keep else 0))
+
+    def set_orientation_lists(self, offsets, orients):
+        """set_orientation_sets in the flat form: offsets[n+1], orients[offsets[n]] (int32)."""
+        off = np.ascontiguousarray(offsets, np.int32)
+        og = np.ascontiguousarray(orients if len(orients) else np.zeros(1), np.int32)
+        _check(lib.rln_estep_set_orientation_sets(self._h, len(off) - 1, _ptr(off, C.c_int), _ptr(og, C.c_int)))
 
     def set_orientation_sets(self, sets):
         """Explicit coarse orientation lists for the next run: one array of global grid indices

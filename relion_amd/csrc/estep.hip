@@ -728,7 +728,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     HIP_TRY(hipStreamSynchronize(st));
     const clk::time_point t_jobs = clk::now();
     // blocks of the fine pass: jobs with <= 2 significant translations use the 2-translation variant, the
-    // others blocks of 4 (the 8 projections of a pixel are recomputed once per block)
+    // others the wide variant, one block per 16 translations (the 8 projections of a pixel are computed once
+    // per block)
     std::vector<int> cjob, cfirst;
     for (int j = 0; j < n_jobs; ++j)
         if (jobs[j].s_count <= 2) {
@@ -738,7 +739,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     const size_t n_small = cjob.size();
     for (int j = 0; j < n_jobs; ++j)
         if (jobs[j].s_count > 2)
-            for (int f = 0; f < jobs[j].s_count; f += 4) {
+            for (int f = 0; f < jobs[j].s_count; f += rlnk::kWideTC) {
                 cjob.push_back(j);
                 cfirst.push_back(f);
             }
@@ -782,9 +783,9 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         rlnk::FineArgs fb = fa;
         fb.chunk_job += n_small;
         fb.chunk_first += n_small;
-        hipLaunchKernelGGL(rlnk::k_fine_diff2<4>, dim3((unsigned)(cjob.size() - n_small)), dim3(rlnk::kBlock), 0, st,
-                           fb);
-        RLN_TRY(launch_check("k_fine_diff2<4>"));
+        hipLaunchKernelGGL(rlnk::k_fine_diff2_wide, dim3((unsigned)(cjob.size() - n_small)), dim3(rlnk::kBlock), 0,
+                           st, fb);
+        RLN_TRY(launch_check("k_fine_diff2_wide"));
     }
     HIP_TRY(hipEventRecord(e.ev[kStFineW], st));
     // fine weights
@@ -1178,10 +1179,14 @@ extern "C" int rln_estep_set_model(rln_estep *e, rln_projector *const *projector
         e->pdf_direction.assign(pdf_direction, pdf_direction + (size_t)K * nd);
     else
         e->pdf_direction.assign((size_t)K * nd, 1.);
+    // mean of the global-search orientation priors, summed in the order of the exhaustive enumeration (class,
+    // direction, psi) so that explicit orientation lists reproduce its normalisation bit for bit
     double pm = 0.;
-    for (double v : e->pdf_direction) pm += v;
-    pm /= (double)e->pdf_direction.size();
-    e->pdf_dir_mean = pm > 0. ? pm : 1.;
+    for (int k = 0; k < K; ++k)
+        for (int d = 0; d < nd; ++d)
+            for (int ip = 0; ip < e->grid.n_psi; ++ip) pm += e->pdf_direction[(size_t)k * nd + d];
+    pm /= (double)((long long)K * nd * e->grid.n_psi);
+    e->pdf_dir_mean = pm != 0. ? pm : 1.;
     e->sigma2_noise.assign(sigma2_noise, sigma2_noise + e->nshell);
     if (data_vs_prior)
         e->data_vs_prior.assign(data_vs_prior, data_vs_prior + (size_t)K * e->nshell);
@@ -1229,49 +1234,63 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     const auto t_run = std::chrono::steady_clock::now();
     HIP_TRY(hipMemsetAsync(e->d_pdfd_acc.ptr, 0, sizeof(double) * (size_t)e->K * e->grid.n_dir, e->stream));
     RLN_TRY(rln::engine_coarse_projections(*e));
-    // chunk size from a device-memory budget for the coarse intermediates
+    // explicit orientation lists apply to this run only
+    std::vector<int> set_off, set_og;
+    set_off.swap(e->set_off);
+    set_og.swap(e->set_og);
+    const bool have_sets = !set_off.empty();
+    if (have_sets && (int)set_off.size() != n + 1)
+        RLN_FAIL(RLN_ERR_ARG, "orientation lists were set for %d particles, run has %d", (int)set_off.size() - 1, n);
+    // chunk boundaries from a device-memory budget for the coarse intermediates (13 bytes per coarse pose:
+    // diff2, FP64 weight, mask), with per-particle pose counts: the explicit list, the whole grid, or a fraction
+    // of it for local searches
     const size_t Pc = e->igc.P, Pf = e->igf.P;
-    long long max_entries = 1;
-    if (e->cfg.local_search) {
-        max_entries = (long long)e->K * e->n_or * e->NT / 8;  // local searches: a fraction of the grid
-    } else {
-        max_entries = (long long)e->K * e->n_or * e->NT;
-    }
     const bool split = e->cfg.precision == RLN_PREC_SPLIT_BF16;
-    const double per_particle = 13.0 * (double)max_entries + (split ? 6.0 * e->Kp : 8.0 * Pc * e->NT) + 20.0 * Pf +
-                                64.0 * 1024;
+    const double fixed = (split ? 6.0 * e->Kp : 8.0 * Pc * e->NT) + 20.0 * Pf + 64.0 * 1024;
     const double budget = 24.0e9;
-    int chunk = (int)std::max(1.0, std::min(4096.0, budget / per_particle));
-    if (e->keep) chunk = std::max(chunk, n);
+    std::vector<int> starts;
+    {
+        double used = 0.;
+        int cnt = 0;
+        for (int i = 0; i < n; ++i) {
+            long long orients = e->cfg.local_search ? std::max(1, e->n_or / 8) : e->n_or;
+            if (have_sets && set_off[i + 1] > set_off[i]) orients = set_off[i + 1] - set_off[i];
+            const double need = 13.0 * (double)e->K * (double)orients * e->NT + fixed;
+            if (cnt == 0 || (!e->keep && (used + need > budget || cnt >= 4096))) {
+                starts.push_back(i);
+                used = 0.;
+                cnt = 0;
+            }
+            used += need;
+            ++cnt;
+        }
+        starts.push_back(n);
+    }
     // chunk c runs on the device while a second host thread builds the lists of chunk c + 1
     using clk = std::chrono::steady_clock;
     rln::ChunkHost cur, nxt;
-    std::vector<int> set_off, set_og;
-    set_off.swap(e->set_off);  // explicit lists apply to this run only
-    set_og.swap(e->set_og);
-    const bool have_sets = !set_off.empty();
-    if (have_sets && (int)set_off.size() != n + 1) RLN_FAIL(RLN_ERR_ARG, "orientation lists were set for %d particles, run has %d", (int)set_off.size() - 1, n);
     auto so = [&](int i) { return have_sets ? set_off.data() + i : nullptr; };
-    const int *sog = have_sets ? set_og.data() : nullptr;
-    if (n > 0) {
+    auto sg = [&](int i) { return have_sets ? set_og.data() + set_off[i] : nullptr; };
+    const int n_chunks = (int)starts.size() - 1;
+    if (n_chunks > 0) {
         const clk::time_point t0 = clk::now();
-        rln::build_chunk_host(*e, std::min(chunk, n), meta, cur, so(0), sog ? sog + set_off[0] : nullptr);
+        rln::build_chunk_host(*e, starts[1] - starts[0], meta, cur, so(0), sg(0));
         e->host_ms[0] += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     }
-    for (int i0 = 0; i0 < n; i0 += chunk) {
-        const int B = std::min(chunk, n - i0);
-        const int i1 = i0 + chunk;
+    for (int c = 0; c < n_chunks; ++c) {
+        const int i0 = starts[c], B = starts[c + 1] - starts[c];
         bool next_ok = true;
         std::thread builder;
-        if (i1 < n)
-            builder = std::thread([&, i1] {
+        if (c + 1 < n_chunks) {
+            const int i1 = starts[c + 1], B1 = starts[c + 2] - starts[c + 1];
+            builder = std::thread([&, i1, B1] {
                 try {
-                    rln::build_chunk_host(*e, std::min(chunk, n - i1), meta + i1, nxt, so(i1),
-                                          sog ? sog + set_off[i1] : nullptr);
+                    rln::build_chunk_host(*e, B1, meta + i1, nxt, so(i1), sg(i1));
                 } catch (...) {
                     next_ok = false;
                 }
             });
+        }
         const size_t off_c = (size_t)i0 * Pf;
         const int rc = rln::run_chunk(*e, B, fimg + 2 * off_c, fimg_nomask + 2 * off_c, fctf + off_c,
                                       inputs_on_device, meta + i0, cur, results + i0, wsum);

@@ -1119,6 +1119,132 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
     }
 }
 
+// Wide variant for jobs with more than 2 significant translations: one block per (job, 4 * kWideCW coarse
+// translations). Per tile of 64 pixels the four waves first project the 8 fine orientations together (two per
+// thread) into LDS; then wave w accumulates the squared differences of its kWideCW coarse translations (4 * kWideCW
+// fine translations) over the tile's pixels. A job with up to kWideTC significant translations reads each
+// trilinear sample once.
+constexpr int kWideCW = 2;             // coarse translations per wave
+constexpr int kWideTC = 4 * kWideCW;   // coarse translations per block
+
+__global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
+    constexpr int kNF = 4 * kWideCW;  // fine translations per wave
+    __shared__ float2 s_A[8][64];
+    __shared__ float2 s_X[64];
+    __shared__ int s_p[64];
+    __shared__ int s_tix[4][kNF], s_tiy[4][kNF];  // per wave
+    const int job_id = a.chunk_job[blockIdx.x];
+    const int first = a.chunk_first[blockIdx.x];
+    const FineJob job = a.jobs[job_id];
+    const int b = job.b;
+    const int nc = min(kWideTC, job.s_count - first);  // coarse translations of this block
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < 4 * kNF) {
+        const int w = tid / kNF, j = tid % kNF;  // wave w, fine translation j = 4c + u
+        const int c = kWideCW * w + (j >> 2), u = j & 3;
+        int t = 0;
+        if (c < nc) t = a.pairs[a.pair_off[b] + job.s_begin + first + c] % a.NT;
+        s_tix[w][j] = a.tr_ix[t * 4 + u];
+        s_tiy[w][j] = a.tr_iy[t * 4 + u];
+    }
+    const int oe = a.or_off[b] + job.o;
+    const int k = a.or_class[oe];
+    const Frame *fr = a.fine_frames + a.or_frame[oe];
+    const float4 *vol = a.vols[k];
+    const float2 *X = a.fimg + (size_t)b * a.ig.P;
+    const float *C = a.fctf + (size_t)b * a.ig.P;
+    const float sc = a.scale[b];
+    const bool active = kWideCW * wave < nc;  // this wave has translations
+    float acc[8][kNF];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int j = 0; j < kNF; ++j) acc[r][j] = 0.f;
+    for (int p0 = 0; p0 < a.ig.P; p0 += 64) {
+        __syncthreads();  // previous tile consumed (and the translation tables written, first time)
+        // phase A: 8 projections of 64 pixels, 2 per thread
+        {
+            const int p = p0 + lane;
+            float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
+            float c = 0.f;
+            if (p < a.ig.P) {
+                const float qp = a.q[p];
+                if (qp != 0.f) {
+                    int x, y;
+                    pixel_xy(p, a.ig.s, a.ig.xs, x, y);
+                    c = qp * (a.do_ctf ? C[p] : 1.f) * sc;
+                    if (x * x + y * y <= a.ig.proj_r2_max) {
+                        v0 = project_pixel(vol, a.vg, fr[2 * wave], x, y);
+                        v1 = project_pixel(vol, a.vg, fr[2 * wave + 1], x, y);
+                    }
+                    if (wave == 0) {
+                        const float2 Xp = X[p];
+                        s_X[lane] = make_float2(qp * Xp.x, qp * Xp.y);
+                        s_p[lane] = p;
+                    }
+                } else if (wave == 0) {
+                    s_p[lane] = -1;
+                }
+            } else if (wave == 0) {
+                s_p[lane] = -1;
+            }
+            s_A[2 * wave][lane] = make_float2(c * v0.x, c * v0.y);
+            s_A[2 * wave + 1][lane] = make_float2(c * v1.x, c * v1.y);
+        }
+        __syncthreads();
+        // phase B: this wave's 16 fine translations over the tile
+        if (active) {
+            const int p = s_p[lane];
+            if (p >= 0) {
+                const int i = p / a.ig.xs, x = p - i * a.ig.xs;
+                float2 A[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) A[r] = s_A[r][lane];
+                const float2 Xq = s_X[lane];
+#pragma unroll
+                for (int j = 0; j < kNF; ++j) {
+                    const float2 ph = cmul(a.ex[(size_t)s_tix[wave][j] * a.ig.xs + x], a.ey[(size_t)s_tiy[wave][j] * a.ig.s + i]);
+                    const float2 Y = cmul(Xq, ph);
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        const float dr = A[r].x - Y.x, di = A[r].y - Y.y;
+                        acc[r][j] = fmaf(dr, dr, acc[r][j]);
+                        acc[r][j] = fmaf(di, di, acc[r][j]);
+                    }
+                }
+            }
+        }
+    }
+    if (!active) return;
+    // wave reduction; lane 0 writes the wave's translations
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int j = 0; j < kNF; ++j) {
+            float v = acc[r][j];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            acc[r][j] = v;
+        }
+    if (lane < kWideCW) {
+        const int c = kWideCW * wave + lane;  // one coarse translation per lane
+        if (c < nc) {
+            const long long s = job.s_begin + first + c;
+            float *out = a.diff2 + (a.pair_off[b] + s) * 32;
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    // acc index j = 4 * lane + u, selected without dynamic register indexing
+                    float v = acc[r][u];
+#pragma unroll
+                    for (int q = 1; q < kWideCW; ++q)
+                        if (lane == q) v = acc[r][4 * q + u];
+                    out[r * 4 + u] = a.xi2half[b] + v;
+                }
+        }
+    }
+}
+
 // --------------------------------------------------------------------------------------------------
 // Weighted sums and backprojection (storeWeightedSums, src/ml_optimiser.cpp:8241-9156, with
 // BackProjector::backproject2Dto3D, src/backprojector.cpp:55-357). One block per (job, iover_rot).

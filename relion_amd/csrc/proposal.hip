@@ -548,14 +548,13 @@ extern "C" int rln_proposal_pose_bin(const rln_proposal *p, double rot, double t
     return RLN_OK;
 }
 
-extern "C" int rln_proposal_bin_orientations(const rln_proposal *p, int grid_order, int n_psi_grid, int bin, int *out,
-                                             int capacity, int *count) {
-    if (!p || !count || (capacity > 0 && !out)) RLN_FAIL(RLN_ERR_ARG, "null argument");
-    if (grid_order < p->cfg.healpix_order || grid_order > 12 || n_psi_grid < 1 || bin < 0 || bin >= p->NB)
-        RLN_FAIL(RLN_ERR_ARG, "rln_proposal_bin_orientations: invalid argument");
-    const int d = bin / p->cfg.n_psi_bins, ib = bin % p->cfg.n_psi_bins;
-    const int nchild = 1 << (2 * (grid_order - p->cfg.healpix_order));
-    const double sb = 360. / p->cfg.n_psi_bins, sg = 360. / n_psi_grid;
+namespace {
+// grid orientations of one bin: NEST children of its direction x grid psi within half a bin of its centre
+void bin_orients(const rln_proposal &p, int grid_order, int n_psi_grid, int bin, std::vector<int> &out) {
+    out.clear();
+    const int d = bin / p.cfg.n_psi_bins, ib = bin % p.cfg.n_psi_bins;
+    const int nchild = 1 << (2 * (grid_order - p.cfg.healpix_order));
+    const double sb = 360. / p.cfg.n_psi_bins, sg = 360. / n_psi_grid;
     std::vector<int> psis;
     for (int j = 0; j < n_psi_grid; ++j) {
         double dd = std::fmod(j * sg - ib * sb, 360.);
@@ -563,13 +562,53 @@ extern "C" int rln_proposal_bin_orientations(const rln_proposal *p, int grid_ord
         if (dd < -180.) dd += 360.;
         if (std::fabs(dd) <= 0.5 * sb + 1e-9) psis.push_back(j);
     }
-    int c = 0;
     for (int k = 0; k < nchild; ++k)
-        for (int j : psis) {
-            if (c < capacity) out[c] = (d * nchild + k) * n_psi_grid + j;
-            ++c;
+        for (int j : psis) out.push_back((d * nchild + k) * n_psi_grid + j);
+}
+}  // namespace
+
+extern "C" int rln_proposal_bin_orientations(const rln_proposal *p, int grid_order, int n_psi_grid, int bin, int *out,
+                                             int capacity, int *count) {
+    if (!p || !count || (capacity > 0 && !out)) RLN_FAIL(RLN_ERR_ARG, "null argument");
+    if (grid_order < p->cfg.healpix_order || grid_order > 12 || n_psi_grid < 1 || bin < 0 || bin >= p->NB)
+        RLN_FAIL(RLN_ERR_ARG, "rln_proposal_bin_orientations: invalid argument");
+    std::vector<int> o;
+    bin_orients(*p, grid_order, n_psi_grid, bin, o);
+    for (int c = 0; c < (int)o.size() && c < capacity; ++c) out[c] = o[c];
+    *count = (int)o.size();
+    return RLN_OK;
+}
+
+extern "C" int rln_proposal_orientation_sets(const rln_proposal *p, int n, const int *bins, const float *probs,
+                                             int topk, int grid_order, int n_psi_grid, double min_top1,
+                                             int *set_offsets, int *set_orients, long long capacity) {
+    if (!p || n < 0 || (n > 0 && (!bins || !probs || !set_offsets || !set_orients)) || topk < 1)
+        RLN_FAIL(RLN_ERR_ARG, "rln_proposal_orientation_sets: null argument");
+    if (grid_order < p->cfg.healpix_order || grid_order > 12 || n_psi_grid < 1)
+        RLN_FAIL(RLN_ERR_ARG, "rln_proposal_orientation_sets: invalid grid");
+    std::vector<std::vector<int>> cache(p->NB);
+    std::vector<int> acc;
+    long long total = 0;
+    set_offsets[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        acc.clear();
+        if (probs[(size_t)i * topk] >= min_top1) {
+            for (int t = 0; t < topk; ++t) {
+                const int b = bins[(size_t)i * topk + t];
+                if (b < 0 || b >= p->NB) RLN_FAIL(RLN_ERR_ARG, "bin %d out of range", b);
+                if (cache[b].empty()) bin_orients(*p, grid_order, n_psi_grid, b, cache[b]);
+                acc.insert(acc.end(), cache[b].begin(), cache[b].end());
+            }
+            std::sort(acc.begin(), acc.end());
+            acc.erase(std::unique(acc.begin(), acc.end()), acc.end());
         }
-    *count = c;
+        if (total + (long long)acc.size() > capacity)
+            RLN_FAIL(RLN_ERR_ARG, "orientation list capacity %lld exceeded", capacity);
+        std::copy(acc.begin(), acc.end(), set_orients + total);
+        total += (long long)acc.size();
+        if (total > 0x7fffffffLL) RLN_FAIL(RLN_ERR_ARG, "orientation lists exceed 2^31 entries");
+        set_offsets[i + 1] = (int)total;
+    }
     return RLN_OK;
 }
 

@@ -98,23 +98,24 @@ class PoseProposal:
                                                  C.byref(cnt)))
         return out
 
+    def orientation_lists(self, bins, probs, grid_order, n_psi_grid, min_top1=0.2):
+        """Coarse orientation lists from the top bins as (offsets[n+1], orientations); a particle whose top-1
+        probability is below min_top1 gets an empty list (exhaustive search)."""
+        bins = np.ascontiguousarray(bins, np.int32)
+        probs = np.ascontiguousarray(probs, np.float32)
+        n, topk = bins.shape
+        cap = n * topk * len(self.bin_orientations(grid_order, n_psi_grid, 0))
+        off = np.zeros(n + 1, np.int32)
+        og = np.zeros(max(cap, 1), np.int32)
+        _check(lib.rln_proposal_orientation_sets(self._h, n, _ptr(bins, C.c_int), _ptr(probs, C.c_float), topk,
+                                                 grid_order, n_psi_grid, min_top1, _ptr(off, C.c_int),
+                                                 _ptr(og, C.c_int), cap))
+        return off, og[:off[-1]]
+
     def orientation_sets(self, bins, probs, grid_order, n_psi_grid, min_top1=0.2):
-        """Coarse orientation lists from the top bins; an empty list (exhaustive search) when the top-1
-        probability is below min_top1."""
-        cache = {}
-        sets = []
-        for b_row, p_row in zip(bins, probs):
-            if p_row[0] < min_top1:
-                sets.append(np.zeros(0, np.int32))
-                continue
-            parts = []
-            for b in b_row:
-                b = int(b)
-                if b not in cache:
-                    cache[b] = self.bin_orientations(grid_order, n_psi_grid, b)
-                parts.append(cache[b])
-            sets.append(np.unique(np.concatenate(parts)).astype(np.int32))
-        return sets
+        """orientation_lists as one array per particle."""
+        off, og = self.orientation_lists(bins, probs, grid_order, n_psi_grid, min_top1)
+        return [og[off[i]:off[i + 1]] for i in range(len(off) - 1)]
 
     # check hooks ------------------------------------------------------------------------------
     def images(self, fimg, n, current_size, on_device=False):

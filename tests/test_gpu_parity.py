@@ -162,3 +162,23 @@ def test_maxsig_cap():
     rg, ro = check_run(p)
     assert (ro["nr_significant"] <= 5).all()
     assert np.array_equal(rg["nr_significant"], ro["nr_significant"])
+
+
+def test_four_classes_global():
+    """classification-like: K = 4 classes sharing 30% of their blobs, HEALPix order 2 global search, offsets 6 px
+    step 1 (113 translations)."""
+    p = make_problem(box=48, px=3.0, order=2, offset_range_px=6, offset_step_px=1, n=6, snr=0.2, nr_classes=4,
+                     nr_blobs=30, seed=31)
+    rg, ro = check_run(p)
+    assert len(np.unique(ro["iclass"])) > 1
+
+
+def test_uneven_snr_fine_pass():
+    """uneven work: per-particle SNR log-uniform in [0.01, 0.3] spreads the significant-pose counts of the fine
+    pass over orders of magnitude."""
+    p = make_problem(box=48, px=3.0, order=2, offset_range_px=3, offset_step_px=1, n=8, snr=(0.01, 0.3),
+                     nr_blobs=30, seed=47)
+    rg, ro = check_run(p, debug=(0, 1, 2))
+    fe = rg["nr_fine_evaluated"]
+    print(f"[uneven] fine poses per particle: min {fe.min()}, max {fe.max()}")
+    assert fe.max() >= 10 * fe.min()

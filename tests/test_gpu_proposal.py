@@ -3,7 +3,9 @@
   * network input images of particles within 2e-4 (absolute, unit-variance images) of the FP64 inverse DFT;
   * one forward + backward pass of the full-size network (64 x 64 input, 6 x 64 channels, 192 x 24 bins):
     logits, loss and every parameter tensor's gradient against the oracle with BF16 rounding at the same points,
-    within 5e-3 relative (norm-wise per tensor; the rest is FP32 versus FP64 accumulation);
+    within twice the BF16 noise floor (norm-wise per tensor): the difference between two oracle runs whose
+    pre-rounding values differ by 3e-7 relative, i.e. less than FP32 accumulation error (a few percent for the
+    convolution gradients, ~1e-3 for the logits; a wrong index or transpose gives errors of order 1);
   * bins cover the coarse grid: every orientation of an order-3 grid belongs to one or two bins (two on psi bin
     boundaries) and a grid orientation's own bin lists it;
   * the gain: trained on projections of the reference, the proposal seeds an order-3 search that evaluates a
@@ -12,6 +14,7 @@
 import numpy as np
 import pytest
 
+import oracle
 import relion_amd as ra
 from oracle import pose_cnn as pc
 from problems import make_problem
@@ -59,20 +62,25 @@ def test_proposal_gradients_match_oracle():
     imgs = np.stack([pc.input_image(p.fimg[i], 64, W) for i in range(4)]).astype(np.float32)
     labels = rng.integers(0, NB, 4)
     loss, g, logits = pp.forward_backward(imgs, labels)
-    lo, go, logo = pc.loss_and_grads(w.astype(np.float64), imgs.astype(np.float64), labels, C, NB, bf=True)
-    e_log = rel_norm(logits, logo)
-    print(f"[proposal] logits rel err {e_log:.2e}, loss {loss:.6f} vs {lo:.6f}")
-    assert e_log < 5e-3 and abs(loss - lo) < 5e-3 * max(1.0, abs(lo))
+    w64, i64 = w.astype(np.float64), imgs.astype(np.float64)
+    lo, go, logo = pc.loss_and_grads(w64, i64, labels, C, NB, bf=True)
+    # noise floor of BF16 arithmetic: the same oracle with its pre-rounding values perturbed by 3e-7 relative
+    # (below FP32 accumulation error), which moves some values across BF16 rounding boundaries
+    ln, gn, logn = pc.loss_and_grads(w64, i64, labels, C, NB, bf=True, jitter=3e-7, seed=1)
+    e_log, n_log = rel_norm(logits, logo), rel_norm(logn, logo)
+    print(f"[proposal] logits rel err {e_log:.2e} (BF16 noise floor {n_log:.2e}), loss {loss:.6f} vs {lo:.6f}")
+    assert e_log < max(2.0 * n_log, 1e-3) and abs(loss - lo) < 2e-3 * max(1.0, abs(lo))
     gc, gW, gb = pc.unpack(g, W, C, NB)
     oc, oW, ob = pc.unpack(go, W, C, NB)
+    nc, nW, nb = pc.unpack(gn, W, C, NB)
     errs = []
-    for l, ((a, ab), (b, bb)) in enumerate(zip(gc, oc)):
-        errs.append((f"conv{l}.W", rel_norm(a, b)))
-        errs.append((f"conv{l}.b", rel_norm(ab, bb)))
-    errs += [("fc.W", rel_norm(gW, oW)), ("fc.b", rel_norm(gb, ob))]
-    print("[proposal] gradient rel err: " + ", ".join(f"{k} {v:.1e}" for k, v in errs))
-    for k, v in errs:
-        assert v < 5e-3, f"{k}: gradient rel err {v:.2e}"
+    for l in range(len(gc)):
+        errs.append((f"conv{l}.W", rel_norm(gc[l][0], oc[l][0]), rel_norm(nc[l][0], oc[l][0])))
+        errs.append((f"conv{l}.b", rel_norm(gc[l][1], oc[l][1]), rel_norm(nc[l][1], oc[l][1])))
+    errs += [("fc.W", rel_norm(gW, oW), rel_norm(nW, oW)), ("fc.b", rel_norm(gb, ob), rel_norm(nb, ob))]
+    print("[proposal] gradient rel err (noise floor): " + ", ".join(f"{k} {v:.1e} ({f:.1e})" for k, v, f in errs))
+    for k, v, f in errs:
+        assert v < max(2.0 * f, 2e-3), f"{k}: gradient rel err {v:.2e} above twice the BF16 noise floor {f:.2e}"
     # the check hook leaves the weights unchanged
     assert np.array_equal(pp.get_params(), w)
     pp.close()
@@ -95,4 +103,55 @@ def test_proposal_bins_cover_the_grid():
         d, j = divmod(int(og), n_psi)
         b = pp.pose_bin(rot[d], tilt[d], psi[j])
         assert og in pp.bin_orientations(3, n_psi, b)
+    pp.close()
+
+
+def angular_distance(r0, r1):
+    """Rotation angle (degrees) between the poses of two result arrays."""
+    out = []
+    for a, b in zip(r0, r1):
+        A = oracle.euler(a["rot"], a["tilt"], a["psi"])
+        B = oracle.euler(b["rot"], b["tilt"], b["psi"])
+        c = (np.trace(A @ B.T) - 1.0) / 2.0
+        out.append(np.degrees(np.arccos(np.clip(c, -1.0, 1.0))))
+    return np.array(out)
+
+
+def test_proposal_seeds_order3_search_and_keeps_best_pose():
+    n = 64
+    p = make_problem(box=64, px=2.0, order=3, n=n, snr=0.1, seed=93, offset_range_px=3, offset_step_px=1)
+    cfg = p.cfg
+    print(f"[proposal] gain test: box 64, order 3 global search, {n} particles", flush=True)
+    dims = ra.volume_dims(64)
+    pr = ra.AccProjector(0)
+    pr.setMdlDim(*dims, 2.0)
+    pr.initMdl(p.volumes[0])
+    bp = ra.AccBackprojector(0)
+    bp.setMdlDim(*dims, 2.0)
+    bp.initMdl()
+    es = ra.ExpectationStep(cfg, 0)
+    es.set_model([pr], [bp], p.pdf_class, p.sigma2_noise, pdf_direction=p.pdf_direction)
+    res0, _ = es.run(p.fimg, p.fctf, p.meta)
+    poses0 = es.counters()["coarse_poses"]
+    pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=5)
+    pp.generate(pr, 40000, ori_size=64, pixel_size=2.0, sigma2_noise=p.sigma2_noise, defocus=(1.0e4, 3.0e4),
+                max_astigmatism=300.0, offset_max=3.0)
+    loss = pp.train(epochs=4, batch=256, lr=1e-3)
+    bins, probs = pp.predict(p.fimg, n, 64, topk=16)
+    sets = pp.orientation_sets(bins, probs, 3, 48, min_top1=0.2)
+    fallback = np.mean([len(s) == 0 for s in sets])
+    es.set_orientation_sets(sets)
+    res1, _ = es.run(p.fimg, p.fctf, p.meta)
+    poses1 = es.counters()["coarse_poses"]
+    dist = angular_distance(res0, res1)
+    kept = np.mean((dist < 1e-3) & (res0["xoff"] == res1["xoff"]) & (res0["yoff"] == res1["yoff"]))
+    # true-bin rank of the generated-data classifier on these particles
+    tb = np.array([pp.pose_bin(*p.pose[i, :3]) for i in range(n)])
+    in_top = np.mean([tb[i] in bins[i] for i in range(n)])
+    print(f"[proposal] final training loss {loss:.3f}; true bin in top-16 for {in_top:.0%}; fallback {fallback:.0%}; "
+          f"coarse poses {poses1:,} vs exhaustive {poses0:,} ({poses0 / poses1:.1f}x fewer); "
+          f"exhaustive best pose kept for {kept:.1%}")
+    assert poses1 * 3 < poses0
+    assert kept >= 0.9
+    es.close()
     pp.close()
