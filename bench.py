@@ -178,18 +178,23 @@ def main():
     # ---------------------------------------------------------------- learned proposal (training is untimed)
     if "proposal" in paths:
         pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=args.seed)
-        t = time.time()
-        pp.generate(proj, args.train_samples, ori_size=box, pixel_size=px, sigma2_noise=sigma2,
-                    defocus=(8000.0, 25000.0), max_astigmatism=300.0, offset_max=3.0)
-        hipmem.synchronize()
-        t_gen = time.time() - t
-        t = time.time()
-        loss = pp.train(epochs=args.train_epochs, batch=256, lr=1e-3)
-        t_train = time.time() - t
-        log(f"proposal: {args.train_samples} projections in {t_gen:.1f} s, {args.train_epochs} epochs in "
-            f"{t_train:.1f} s, last-epoch loss {loss:.3f}", args)
-        extra["proposal_training"] = {"samples": args.train_samples, "epochs": args.train_epochs, "batch": 256,
-                                      "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
+        # each epoch sees fresh projections (new poses, CTFs, shifts and noise)
+        t_gen = t_train = 0.0
+        for ep in range(args.train_epochs):
+            t = time.time()
+            pp.clear_training()
+            pp.generate(proj, args.train_samples, ori_size=box, pixel_size=px, sigma2_noise=sigma2,
+                        defocus=(8000.0, 25000.0), max_astigmatism=300.0, offset_max=3.0,
+                        first=ep * args.train_samples)
+            hipmem.synchronize()
+            t_gen += time.time() - t
+            t = time.time()
+            loss = pp.train(epochs=1, batch=256, lr=1e-3)
+            t_train += time.time() - t
+            log(f"proposal epoch {ep}: {args.train_samples} fresh projections, loss {loss:.3f}", args)
+        log(f"proposal: generation {t_gen:.1f} s, training {t_train:.1f} s", args)
+        extra["proposal_training"] = {"samples_per_epoch": args.train_samples, "epochs": args.train_epochs,
+                                      "batch": 256, "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
                                       "last_epoch_loss": round(loss, 4)}
         cfg_g.precision = ra.RLN_PREC_SPLIT_BF16
         es_p = ra.ExpectationStep(cfg_g, 0)
@@ -199,7 +204,7 @@ def main():
             bp.initMdl()
             es_p.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
             t1 = time.time()
-            bins, probs = pp.predict(d_img.address, count, box, topk=args.topk, on_device=True)
+            bins, probs = pp.predict(d_img.address, d_ctf.address, count, box, topk=args.topk, on_device=True)
             off, og = pp.orientation_lists(bins, probs, order, n_psi, min_top1=args.min_top1)
             es_p.set_orientation_lists(off, og)
             pred_ms.append(1000.0 * (time.time() - t1))

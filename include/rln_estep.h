@@ -214,9 +214,10 @@ int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *dif
  * time of chunk preparation, job-list building, the epilogue, and the whole run. */
 int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
 /* Work counters of the last run: particles, coarse poses, coarse pixels per image, fine poses, fine pixels
- * per image, fine diff2 blocks, backprojected fine orientations, significant (orientation, translation)
+ * per image, fine diff2 work items, backprojected fine orientations, significant (orientation, translation)
  * pairs summed in the weighted sums, coarse grid projections computed, coarse tiles, fine jobs, particle chunks
- * (one launch of each per-chunk kernel per chunk). */
+ * (one launch of each per-chunk kernel per chunk), launches of the captured fine-pass graph, fine-pass
+ * capacity retries, and the tuned persistent grid sizes of the two fine diff2 kernels. */
 int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out);
 
 /* ---------------------------------------------------------------------------------------------

@@ -62,12 +62,14 @@ int rln_proposal_set_params(rln_proposal *p, const float *in);
 
 /* Training set: generated projections (poses, if not null, receive rot, tilt, psi per sample) and particle
  * images labelled with poses (e.g. the previous iteration's max-likelihood assignments). fimg: n FFTW
- * half-complex images of current_size (host, or device if on_device). */
+ * half-complex images of current_size (host, or device if on_device); fctf: their CTFs in the same layout, or
+ * null. The network sees phase-flipped windows: particle windows are multiplied by the sign of fctf (when
+ * given), generated windows by the sign of their own CTF. */
 int rln_proposal_clear_training(rln_proposal *p);
 int rln_proposal_generate(rln_proposal *p, const rln_projector *reference, const rln_proposal_gen *gen, int n,
                           long long first, double *poses);
-int rln_proposal_add_images(rln_proposal *p, int n, const float *fimg, int current_size, int on_device,
-                            const double *poses);
+int rln_proposal_add_images(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
+                            int on_device, const double *poses);
 int rln_proposal_training_size(const rln_proposal *p, long long *n);
 
 /* Adam over the training set in shuffled minibatches; mean_loss receives the mean cross entropy of the last
@@ -75,8 +77,8 @@ int rln_proposal_training_size(const rln_proposal *p, long long *n);
 int rln_proposal_train(rln_proposal *p, int epochs, int batch, double lr, double *mean_loss);
 
 /* Top-k bins and their softmax probabilities per particle (bins[n][topk], probs[n][topk], descending). */
-int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, int current_size, int on_device, int topk,
-                         int *bins, float *probs);
+int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
+                         int on_device, int topk, int *bins, float *probs);
 
 /* The bin of a pose, and the coarse grid orientations (idir * n_psi_grid + ipsi at grid_order >=
  * healpix_order, grid psi step 360 / n_psi_grid) that a bin covers: the NEST children of its direction times
@@ -96,7 +98,8 @@ int rln_proposal_orientation_sets(const rln_proposal *p, int n, const int *bins,
 /* Check hooks: the network input images (n x W x W FP32) of particle images, and one forward + backward
  * pass over given input images (FP32, rounded to BF16) with labels: loss = mean cross entropy, grads in the
  * parameter layout (FP32), logits [n][NB]. The weights are not updated. */
-int rln_proposal_images(rln_proposal *p, int n, const float *fimg, int current_size, int on_device, float *out);
+int rln_proposal_images(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size, int on_device,
+                        float *out);
 int rln_proposal_forward_backward(rln_proposal *p, int n, const float *images, const int *labels, double *loss,
                                   float *grads, float *logits);
 

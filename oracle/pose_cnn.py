@@ -167,18 +167,22 @@ def init_params(W, C, NB, seed):
     return pack(conv, rng.normal(0, np.sqrt(1.0 / F), (NB, F)), np.zeros(NB))
 
 
-def window(fimg, s, W):
-    """The W x (W/2+1) low-frequency window of an s x (s/2+1) FFTW half-complex image (complex)."""
+def window(fimg, s, W, ctf=None):
+    """The W x (W/2+1) low-frequency window of an s x (s/2+1) FFTW half-complex image (complex), multiplied by
+    the sign of the CTF (same layout) when given."""
     xs = s // 2 + 1
     F = np.asarray(fimg, np.float64).reshape(s, xs, 2)
     F = F[..., 0] + 1j * F[..., 1]
+    if ctf is not None:
+        F = np.where(np.asarray(ctf).reshape(s, xs) < 0, -F, F)
     rows = [ky if ky >= 0 else ky + s for ky in list(range(W // 2)) + list(range(-W // 2, 0))]
     return F[rows, :W // 2 + 1]
 
 
-def input_image(fimg, s, W):
-    """Network input: normalised sum_{ky, kx} h(kx) Re[F e^{2 pi i (kx u + ky v)/W}] (v-major)."""
-    img = np.fft.irfft2(window(fimg, s, W), s=(W, W)) * (W * W)
+def input_image(fimg, s, W, ctf=None):
+    """Network input: normalised sum_{ky, kx} h(kx) Re[F e^{2 pi i (kx u + ky v)/W}] (v-major) of the
+    (phase-flipped) window."""
+    img = np.fft.irfft2(window(fimg, s, W, ctf), s=(W, W)) * (W * W)
     img = img - img.mean()
     return img / np.sqrt((img ** 2).mean())
 

@@ -158,15 +158,15 @@ _SIGNATURES = {
     "rln_proposal_set_params": (C.c_int, [_vp, _fp]),
     "rln_proposal_clear_training": (C.c_int, [_vp]),
     "rln_proposal_generate": (C.c_int, [_vp, _vp, C.POINTER(ProposalGen), C.c_int, C.c_longlong, _dp]),
-    "rln_proposal_add_images": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, _dp]),
+    "rln_proposal_add_images": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _dp]),
     "rln_proposal_training_size": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
     "rln_proposal_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
-    "rln_proposal_predict": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _ip, _fp]),
+    "rln_proposal_predict": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _ip, _fp]),
     "rln_proposal_pose_bin": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _ip]),
     "rln_proposal_bin_orientations": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip]),
     "rln_proposal_orientation_sets": (C.c_int, [_vp, C.c_int, _ip, _fp, C.c_int, C.c_int, C.c_int, C.c_double, _ip,
                                                 _ip, C.c_longlong]),
-    "rln_proposal_images": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, _fp]),
+    "rln_proposal_images": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _fp]),
     "rln_proposal_forward_backward": (C.c_int, [_vp, C.c_int, _fp, _ip, _dp, _fp, _fp]),
 }
 
@@ -537,8 +537,9 @@ class ExpectationStep:
         _check(lib.rln_estep_last_timings(self._h, _ptr(ms, C.c_double), len(STAGES), C.byref(n)))
         return dict(zip(STAGES, ms[:n.value].tolist()))
 
-    COUNTERS = ("particles", "coarse_poses", "coarse_pixels", "fine_poses", "fine_pixels", "fine_blocks",
-                "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks")
+    COUNTERS = ("particles", "coarse_poses", "coarse_pixels", "fine_poses", "fine_pixels", "fine_items",
+                "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks",
+                "fine_graph_launches", "fine_capacity_retries", "fine_grid_small", "fine_grid_wide")
 
     def counters(self):
         out = np.zeros(16, np.int64)

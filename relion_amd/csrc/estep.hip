@@ -102,6 +102,19 @@ struct Engine {
         d_chunk_first;
     DevBuf d_fdiff2, d_fweight, d_fine_n, d_nent, d_foff;
     DevBuf d_shell_acc, d_part_acc, d_pdfc_acc, d_pdfd_acc;
+    // fine pass planned on the device: capacities (elements) the buffers were sized for, the chunk capacity B,
+    // the plan header, per-chunk image pointers, work items, the tuned persistent grids and the captured graph
+    int cap_B = 0;
+    long long need_pairs = 0;
+    int need_jobs = 0, need_small = 0, need_wide = 0;
+    DevBuf d_plan, d_cp, d_items_s, d_items_w;
+    rlnk::ChunkPtrs cp_host{};
+    bool tuned = false;
+    int grid_small = 1024, grid_wide = 1024, grid_wsum = 2048;
+    hipGraph_t fine_graph = nullptr;
+    hipGraphExec_t fine_exec = nullptr;
+    std::vector<unsigned char> fine_key;  // kernel arguments the graph was captured with
+    bool graph_off = false;               // capture unsupported: launch directly
     // intermediates kept for tests
     bool keep = false;
     struct Kept {
@@ -531,6 +544,172 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     h.total_entries = d2;
 }
 
+
+// ------------------------------------------------------------------------------------------------
+// Fine pass planned on the device (persistent kernels, one captured graph)
+// ------------------------------------------------------------------------------------------------
+static int ensure_fine_capacity(Engine &e) {
+    const int cB = e.cap_B;
+    if (e.need_pairs == 0) e.need_pairs = 64LL * cB;
+    if (e.need_jobs == 0) e.need_jobs = 16 * cB;
+    if (e.need_small == 0) e.need_small = 16 * cB;
+    if (e.need_wide == 0) e.need_wide = 16 * cB;
+    RLN_TRY(e.d_pairs.ensure(sizeof(int) * e.need_pairs));
+    RLN_TRY(e.d_jobs.ensure(sizeof(rlnk::FineJob) * e.need_jobs));
+    RLN_TRY(e.d_items_s.ensure(sizeof(int2) * e.need_small));
+    RLN_TRY(e.d_items_w.ensure(sizeof(int2) * e.need_wide));
+    RLN_TRY(e.d_fdiff2.ensure(sizeof(float) * 32 * e.need_pairs));
+    RLN_TRY(e.d_fweight.ensure(sizeof(double) * 32 * e.need_pairs));
+    RLN_TRY(e.d_npairs.ensure(sizeof(int) * 2 * cB));
+    RLN_TRY(e.d_pair_off.ensure(sizeof(long long) * cB));
+    RLN_TRY(e.d_job_off.ensure(sizeof(int) * cB));
+    RLN_TRY(e.d_fine_n.ensure(sizeof(long long) * cB));
+    RLN_TRY(e.d_foff.ensure(sizeof(long long) * cB));
+    RLN_TRY(e.d_stats_f.ensure(sizeof(rlnk::PassStats) * cB));
+    RLN_TRY(e.d_shell_acc.ensure(sizeof(double) * e.nshell * cB));
+    RLN_TRY(e.d_part_acc.ensure(sizeof(double) * 8 * cB));
+    RLN_TRY(e.d_pdfc_acc.ensure(sizeof(double) * e.K * cB));
+    RLN_TRY(e.d_plan.ensure(sizeof(rlnk::FinePlan)));
+    RLN_TRY(e.d_cp.ensure(sizeof(rlnk::ChunkPtrs)));
+    return RLN_OK;
+}
+
+// All kernel arguments of the fine pass of a chunk; the graph is reused while they are unchanged.
+struct FineLaunch {
+    const int *np, *nj;
+    const rlnk::ChunkPtrs *cp;
+    long long cap_pairs;
+    int cap_jobs, cap_small, cap_wide, grid_B;
+    long long *pair_off, *fine_n, *foff;
+    int *job_off, *pairs;
+    rlnk::FinePlan *plan;
+    const unsigned char *mask;
+    const long long *d2off;
+    const int *or_cnt;
+    int NT;
+    rlnk::FineJob *jobs;
+    int2 *items_s, *items_w;
+    rlnk::FineArgs fs, fw;
+    rlnk::WeightArgs wf;
+    rlnk::WsumArgs ws;
+    double *shell, *part, *pdfc;
+    long long n_shell, n_part, n_pdfc;
+    int grid_small, grid_wide, grid_wsum;
+};
+
+static int launch_fine(Engine &e, const FineLaunch &L, bool events) {
+    hipStream_t st = e.stream;
+    hipLaunchKernelGGL(rlnk::k_fine_plan, dim3(1), dim3(rlnk::kWBlock), 0, st, L.np, L.nj, L.cp, L.cap_pairs,
+                       L.cap_jobs, L.pair_off, L.job_off, L.fine_n, L.foff, L.plan);
+    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt, L.NT,
+                       (const long long *)L.pair_off, (const int *)L.job_off, L.pairs, L.jobs,
+                       (const rlnk::FinePlan *)L.plan);
+    hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs, 2,
+                       rlnk::kWideTC, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
+    hipLaunchKernelGGL(rlnk::k_fine_diff2<2>, dim3(L.grid_small), dim3(rlnk::kBlock), 0, st, L.fs);
+    hipLaunchKernelGGL(rlnk::k_fine_diff2_wide, dim3(L.grid_wide), dim3(rlnk::kBlock), 0, st, L.fw);
+    if (events) HIP_TRY(hipEventRecord(e.ev[kStFineW], st));
+    hipLaunchKernelGGL(rlnk::k_weights, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.wf);
+    hipLaunchKernelGGL(rlnk::k_zero_d, dim3(64), dim3(rlnk::kBlock), 0, st, L.shell, L.n_shell);
+    hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.part, L.n_part);
+    hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.pdfc, L.n_pdfc);
+    if (events) HIP_TRY(hipEventRecord(e.ev[kStWsum], st));
+    hipLaunchKernelGGL(rlnk::k_wsum_bp, dim3(L.grid_wsum), dim3(rlnk::kBlock), 0, st, L.ws);
+    return launch_check("fine pass");
+}
+
+// Persistent grid sizes of the two fine diff2 kernels: each candidate timed once on the first chunk (the
+// kernels only write fine diff2, so repeating them is harmless), the fastest kept for the engine's life.
+static int tune_fine(Engine &e, FineLaunch L) {
+    hipStream_t st = e.stream;
+    hipLaunchKernelGGL(rlnk::k_fine_plan, dim3(1), dim3(rlnk::kWBlock), 0, st, L.np, L.nj, L.cp, L.cap_pairs,
+                       L.cap_jobs, L.pair_off, L.job_off, L.fine_n, L.foff, L.plan);
+    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt, L.NT,
+                       (const long long *)L.pair_off, (const int *)L.job_off, L.pairs, L.jobs,
+                       (const rlnk::FinePlan *)L.plan);
+    hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs, 2,
+                       rlnk::kWideTC, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
+    RLN_TRY(launch_check("fine plan"));
+    rlnk::FinePlan plan{};
+    HIP_TRY(hipMemcpyAsync(&plan, L.plan, sizeof(plan), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (plan.overflow || plan.n_small + plan.n_wide < 512) return RLN_OK;  // retried with more room / too small
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    const int cand[] = {256, 512, 1024, 2048, 4096};
+    float best_s = 1e30f, best_w = 1e30f;
+    for (int g : cand) {
+        float ms = 0.f;
+        HIP_TRY(hipEventRecord(a, st));
+        hipLaunchKernelGGL(rlnk::k_fine_diff2<2>, dim3(g), dim3(rlnk::kBlock), 0, st, L.fs);
+        HIP_TRY(hipEventRecord(b, st));
+        HIP_TRY(hipEventSynchronize(b));
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        if (plan.n_small && ms < best_s) {
+            best_s = ms;
+            e.grid_small = g;
+        }
+        HIP_TRY(hipEventRecord(a, st));
+        hipLaunchKernelGGL(rlnk::k_fine_diff2_wide, dim3(g), dim3(rlnk::kBlock), 0, st, L.fw);
+        HIP_TRY(hipEventRecord(b, st));
+        HIP_TRY(hipEventSynchronize(b));
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        if (plan.n_wide && ms < best_w) {
+            best_w = ms;
+            e.grid_wide = g;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    RLN_TRY(launch_check("fine tuning"));
+    e.tuned = true;
+    e.counters[14] = e.grid_small;
+    e.counters[15] = e.grid_wide;
+    return RLN_OK;
+}
+
+static int run_fine_stage(Engine &e, const FineLaunch &L0) {
+    FineLaunch L = L0;
+    if (!e.tuned) {
+        RLN_TRY(tune_fine(e, L));
+        L.grid_small = e.grid_small;
+        L.grid_wide = e.grid_wide;
+    }
+    if (!e.graph_off) {
+        const unsigned char *kb = reinterpret_cast<const unsigned char *>(&L);
+        std::vector<unsigned char> key(kb, kb + sizeof(L));
+        if (!e.fine_exec || key != e.fine_key) {
+            if (e.fine_exec) (void)hipGraphExecDestroy(e.fine_exec);
+            if (e.fine_graph) (void)hipGraphDestroy(e.fine_graph);
+            e.fine_exec = nullptr;
+            e.fine_graph = nullptr;
+            bool ok = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                const int rc = launch_fine(e, L, true);
+                hipGraph_t g = nullptr;
+                ok = hipStreamEndCapture(e.stream, &g) == hipSuccess && rc == RLN_OK && g;
+                if (ok) ok = hipGraphInstantiate(&e.fine_exec, g, nullptr, nullptr, 0) == hipSuccess;
+                if (ok) e.fine_graph = g;
+                else if (g) (void)hipGraphDestroy(g);
+            }
+            (void)hipGetLastError();
+            if (!ok) {
+                e.fine_exec = nullptr;
+                e.graph_off = true;  // capture unsupported here: launch directly from now on
+            } else {
+                e.fine_key = key;
+            }
+        }
+        if (e.fine_exec) {
+            HIP_TRY(hipGraphLaunch(e.fine_exec, e.stream));
+            e.counters[12] += 1;
+            return RLN_OK;
+        }
+    }
+    return launch_fine(e, L, true);
+}
+
 template <int NTT>
 static void launch_mfma_t(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
     hipLaunchKernelGGL(rlnk::k_coarse_mfma<NTT>, dim3(n_tiles), dim3(rlnk::kBlock), 0, e.stream,
@@ -611,24 +790,26 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     DevBuf &d_d2off = e.d_meta_ll;
     DevBuf &d_nent_tmp = e.d_nent;  // entries per particle
     RLN_TRY(upload(d_nent_tmp, h.n_entries, st));
-    std::vector<float> mf(3 * (size_t)B);
+    // per-particle scalars with the chunk capacity as stride, so that their addresses do not depend on B
+    const int cB = e.cap_B;
+    std::vector<float> mf(3 * (size_t)cB, 0.f);
     for (int b = 0; b < B; ++b) {
         mf[b] = h.xi2half[b];
-        mf[B + b] = h.scale[b];
-        mf[2 * B + b] = h.scale_w[b];
+        mf[cB + b] = h.scale[b];
+        mf[2 * cB + b] = h.scale_w[b];
     }
     RLN_TRY(upload(e.d_meta_f, mf, st));
-    std::vector<double> md(4 * (size_t)B);
+    std::vector<double> md(4 * (size_t)cB, 0.);
     for (int b = 0; b < 2 * B; ++b) {
         md[b] = h.old_off[b];
-        md[2 * B + b] = h.prior_off[b];
+        md[2 * cB + b] = h.prior_off[b];
     }
     RLN_TRY(upload(e.d_meta_d, md, st));
     const float *d_xi2 = e.d_meta_f.as<const float>();
-    const float *d_scale = d_xi2 + B;
-    const float *d_scale_w = d_xi2 + 2 * B;
+    const float *d_scale = d_xi2 + cB;
+    const float *d_scale_w = d_xi2 + 2 * cB;
     const double *d_old = e.d_meta_d.as<const double>();
-    const double *d_prior = d_old + 2 * B;
+    const double *d_prior = d_old + 2 * cB;
 
     // coarse: scaled images, shifted images
     const bool split = cfg.precision == RLN_PREC_SPLIT_BF16;
@@ -669,7 +850,6 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     }
     HIP_TRY(hipEventRecord(e.ev[kStCoarseW], st));
     RLN_TRY(e.d_stats_c.ensure(sizeof(rlnk::PassStats) * B));
-    RLN_TRY(e.d_stats_f.ensure(sizeof(rlnk::PassStats) * B));
     rlnk::WeightArgs wa{};
     wa.diff2 = e.d_diff2.as<const float>();
     wa.d2_off = d_d2off.as<const long long>();
@@ -687,182 +867,164 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     hipLaunchKernelGGL(rlnk::k_weights, dim3(B), dim3(rlnk::kWBlock), 0, st, wa);
     RLN_TRY(launch_check("k_weights(coarse)"));
     HIP_TRY(hipEventRecord(e.ev[kStJobs], st));
-    // fine-pass job lists
-    RLN_TRY(e.d_npairs.ensure(sizeof(int) * 3 * B));
-    int *d_np = e.d_npairs.as<int>();
-    int *d_nj = d_np + B;
+    // ---------------------------------------------------------------- fine pass, planned on the device
+    // per-particle counts; everything after runs from device-side counts (FinePlan) without host round trips,
+    // as persistent kernels, captured once into a graph and replayed for every chunk
+    RLN_TRY(ensure_fine_capacity(e));
     hipLaunchKernelGGL(rlnk::k_count_jobs, dim3(B), dim3(rlnk::kWBlock), 0, st, e.d_mask.as<const unsigned char>(),
-                       d_d2off.as<const long long>(), e.d_or_cnt.as<const int>(), NT, d_np, d_nj);
+                       d_d2off.as<const long long>(), e.d_or_cnt.as<const int>(), NT, e.d_npairs.as<int>(),
+                       e.d_npairs.as<int>() + e.cap_B);
     RLN_TRY(launch_check("k_count_jobs"));
-    std::vector<int> counts(2 * B);
+    e.cp_host.fimg = dX;
+    e.cp_host.fimg_nomask = dXn;
+    e.cp_host.fctf = dC;
+    e.cp_host.B = B;
+    HIP_TRY(hipMemcpyAsync(e.d_cp.ptr, &e.cp_host, sizeof(rlnk::ChunkPtrs), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(e.ev[kStFine], st));
+    rlnk::FinePlan plan{};
+    for (int attempt = 0;; ++attempt) {
+        RLN_TRY(ensure_fine_capacity(e));
+        FineLaunch L;
+        std::memset(&L, 0, sizeof(L));  // padding included: the bytes are the graph's key
+        L.np = e.d_npairs.as<const int>();
+        L.nj = e.d_npairs.as<const int>() + cB;
+        L.cp = e.d_cp.as<const rlnk::ChunkPtrs>();
+        L.cap_pairs = e.need_pairs;
+        L.cap_jobs = e.need_jobs;
+        L.cap_small = e.need_small;
+        L.cap_wide = e.need_wide;
+        L.grid_B = cB;
+        L.pair_off = e.d_pair_off.as<long long>();
+        L.fine_n = e.d_fine_n.as<long long>();
+        L.foff = e.d_foff.as<long long>();
+        L.job_off = e.d_job_off.as<int>();
+        L.pairs = e.d_pairs.as<int>();
+        L.plan = e.d_plan.as<rlnk::FinePlan>();
+        L.mask = e.d_mask.as<const unsigned char>();
+        L.d2off = d_d2off.as<const long long>();
+        L.or_cnt = e.d_or_cnt.as<const int>();
+        L.NT = NT;
+        L.jobs = e.d_jobs.as<rlnk::FineJob>();
+        L.items_s = e.d_items_s.as<int2>();
+        L.items_w = e.d_items_w.as<int2>();
+        rlnk::FineArgs &fa = L.fs;
+        fa.jobs = e.d_jobs.as<const rlnk::FineJob>();
+        fa.items = e.d_items_s.as<const int2>();
+        fa.plan = e.d_plan.as<const rlnk::FinePlan>();
+        fa.which = 0;
+        fa.cp = e.d_cp.as<const rlnk::ChunkPtrs>();
+        fa.pairs = e.d_pairs.as<const int>();
+        fa.pair_off = e.d_pair_off.as<const long long>();
+        fa.or_off = e.d_or_off.as<const int>();
+        fa.or_frame = e.d_or_frame.as<const int>();
+        fa.or_class = e.d_or_class.as<const int>();
+        fa.vols = e.d_vols.as<const float4 *const>();
+        fa.vg = e.vg;
+        fa.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
+        fa.ig = e.igf;
+        fa.scale = d_scale;
+        fa.q = e.d_q_f.as<const float>();
+        fa.ex = e.d_ex_f.as<const float2>();
+        fa.ey = e.d_ey_f.as<const float2>();
+        fa.tr_ix = e.d_tr_ix.as<const int>();
+        fa.tr_iy = e.d_tr_iy.as<const int>();
+        fa.xi2half = d_xi2;
+        fa.NT = NT;
+        fa.do_ctf = cfg.do_ctf_correction;
+        fa.diff2 = e.d_fdiff2.as<float>();
+        L.fw = fa;
+        L.fw.items = e.d_items_w.as<const int2>();
+        L.fw.which = 1;
+        rlnk::WeightArgs &wf = L.wf;
+        wf = wa;
+        wf.diff2 = e.d_fdiff2.as<const float>();
+        wf.d2_off = e.d_foff.as<const long long>();
+        wf.n_entries = e.d_fine_n.as<const long long>();
+        wf.pairs = e.d_pairs.as<const int>();
+        wf.pair_off = e.d_pair_off.as<const long long>();
+        wf.fine = 1;
+        wf.maxsig = 0;
+        wf.weight = e.d_fweight.as<double>();
+        wf.mask = nullptr;
+        wf.stats = e.d_stats_f.as<rlnk::PassStats>();
+        wf.plan = e.d_plan.as<const rlnk::FinePlan>();
+        L.shell = e.d_shell_acc.as<double>();
+        L.part = e.d_part_acc.as<double>();
+        L.pdfc = e.d_pdfc_acc.as<double>();
+        L.n_shell = (long long)e.nshell * cB;
+        L.n_part = 8LL * cB;
+        L.n_pdfc = (long long)K * cB;
+        rlnk::WsumArgs &ws = L.ws;
+        ws.jobs = e.d_jobs.as<const rlnk::FineJob>();
+        ws.pairs = e.d_pairs.as<const int>();
+        ws.pair_off = e.d_pair_off.as<const long long>();
+        ws.or_off = e.d_or_off.as<const int>();
+        ws.or_frame = e.d_or_frame.as<const int>();
+        ws.or_class = e.d_or_class.as<const int>();
+        ws.or_dir = e.d_or_dir.as<const int>();
+        ws.vols = e.d_vols.as<const float4 *const>();
+        ws.vg = e.vg;
+        ws.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
+        ws.ig = e.igf;
+        ws.cp = e.d_cp.as<const rlnk::ChunkPtrs>();
+        ws.plan = e.d_plan.as<const rlnk::FinePlan>();
+        ws.scale_w = d_scale_w;
+        ws.minv_w = e.d_minv_w.as<const float>();
+        ws.mresol = e.d_mresol.as<const int>();
+        ws.xa_shell = (cfg.do_scale_correction && e.d_xa_shell.ptr) ? e.d_xa_shell.as<const unsigned char>() : nullptr;
+        ws.ex = e.d_ex_f.as<const float2>();
+        ws.ey = e.d_ey_f.as<const float2>();
+        ws.tr_ix = e.d_tr_ix.as<const int>();
+        ws.tr_iy = e.d_tr_iy.as<const int>();
+        ws.over_x = e.d_over_x.as<const double>();
+        ws.over_y = e.d_over_y.as<const double>();
+        ws.weight = e.d_fweight.as<const double>();
+        ws.stats = e.d_stats_f.as<const rlnk::PassStats>();
+        ws.old_off = d_old;
+        ws.prior_off = d_prior;
+        ws.pixel_size = cfg.pixel_size;
+        ws.NT = NT;
+        ws.nshell = e.nshell;
+        ws.n_dir = e.grid.n_dir;
+        ws.do_ctf = cfg.do_ctf_correction;
+        ws.do_scale = cfg.do_scale_correction;
+        ws.bp_slots = e.d_bp_ptrs.as<float *const>();
+        ws.shell_acc = e.d_shell_acc.as<double>();
+        ws.part_acc = e.d_part_acc.as<double>();
+        ws.pdf_class = e.d_pdfc_acc.as<double>();
+        ws.pdf_direction = e.d_pdfd_acc.as<double>();
+        ws.K = K;
+        L.grid_small = e.grid_small;
+        L.grid_wide = e.grid_wide;
+        L.grid_wsum = e.grid_wsum;
+        RLN_TRY(run_fine_stage(e, L));
+        HIP_TRY(hipMemcpyAsync(&plan, e.d_plan.ptr, sizeof(plan), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (!plan.overflow) break;
+        if (attempt >= 2) RLN_FAIL(RLN_ERR_MEMORY, "fine pass capacity not reached after %d attempts", attempt + 1);
+        // grow to what the plan asked for, with headroom, and run the pass again
+        e.need_pairs = std::max(e.need_pairs, plan.n_pairs + plan.n_pairs / 4 + 1024);
+        e.need_jobs = std::max(e.need_jobs, plan.n_jobs + plan.n_jobs / 4 + 256);
+        e.need_small = std::max(e.need_small, plan.n_small + plan.n_small / 4 + 256);
+        e.need_wide = std::max(e.need_wide, plan.n_wide + plan.n_wide / 4 + 256);
+        e.counters[13] += 1;
+    }
+    const long long n_pairs = plan.n_pairs;
+    const int n_jobs = plan.n_jobs;
+    std::vector<int> counts(B);
+    std::vector<long long> pair_off(B), foff(B);
     std::vector<rlnk::PassStats> stc(B);
-    HIP_TRY(hipMemcpyAsync(counts.data(), d_np, sizeof(int) * 2 * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(counts.data(), e.d_npairs.ptr, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(pair_off.data(), e.d_pair_off.ptr, sizeof(long long) * B, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(stc.data(), e.d_stats_c.ptr, sizeof(rlnk::PassStats) * B, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    for (int b = 0; b < B; ++b) foff[b] = 32LL * pair_off[b];
     for (int b = 0; b < B; ++b)
         if (stc[b].status != 0) {
             int code = stc[b].status;
             RLN_FAIL(code, "coarse pass of particle %d: %s", b,
                      code == RLN_ERR_SUM_WEIGHT_ZERO ? "zero or NaN sum of weights" : "no significant samples");
         }
-    std::vector<long long> pair_off(B);
-    std::vector<int> job_off(B);
-    long long n_pairs = 0;
-    int n_jobs = 0;
-    for (int b = 0; b < B; ++b) {
-        pair_off[b] = n_pairs;
-        job_off[b] = n_jobs;
-        n_pairs += counts[b];
-        n_jobs += counts[B + b];
-    }
-    RLN_TRY(upload(e.d_pair_off, pair_off, st));
-    RLN_TRY(upload(e.d_job_off, job_off, st));
-    RLN_TRY(e.d_pairs.ensure(sizeof(int) * std::max<long long>(1, n_pairs)));
-    RLN_TRY(e.d_jobs.ensure(sizeof(rlnk::FineJob) * std::max(1, n_jobs)));
-    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(B), dim3(rlnk::kWBlock), 0, st, e.d_mask.as<const unsigned char>(),
-                       d_d2off.as<const long long>(), e.d_or_cnt.as<const int>(), NT, e.d_pair_off.as<const long long>(),
-                       e.d_job_off.as<const int>(), e.d_pairs.as<int>(), e.d_jobs.as<rlnk::FineJob>());
-    RLN_TRY(launch_check("k_write_jobs"));
-    std::vector<rlnk::FineJob> jobs(n_jobs);
-    if (n_jobs) HIP_TRY(hipMemcpyAsync(jobs.data(), e.d_jobs.ptr, sizeof(rlnk::FineJob) * n_jobs, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    const clk::time_point t_jobs = clk::now();
-    // blocks of the fine pass: jobs with <= 2 significant translations use the 2-translation variant, the
-    // others the wide variant, one block per 16 translations (the 8 projections of a pixel are computed once
-    // per block)
-    std::vector<int> cjob, cfirst;
-    for (int j = 0; j < n_jobs; ++j)
-        if (jobs[j].s_count <= 2) {
-            cjob.push_back(j);
-            cfirst.push_back(0);
-        }
-    const size_t n_small = cjob.size();
-    for (int j = 0; j < n_jobs; ++j)
-        if (jobs[j].s_count > 2)
-            for (int f = 0; f < jobs[j].s_count; f += rlnk::kWideTC) {
-                cjob.push_back(j);
-                cfirst.push_back(f);
-            }
-    RLN_TRY(upload(e.d_chunk_job, cjob, st));
-    RLN_TRY(upload(e.d_chunk_first, cfirst, st));
-    e.host_ms[1] += ms_since(t_jobs);
-    HIP_TRY(hipEventRecord(e.ev[kStFine], st));
-    // fine diff2
-    RLN_TRY(e.d_fdiff2.ensure(sizeof(float) * 32 * std::max<long long>(1, n_pairs)));
-    RLN_TRY(e.d_fweight.ensure(sizeof(double) * 32 * std::max<long long>(1, n_pairs)));
-    rlnk::FineArgs fa{};
-    fa.jobs = e.d_jobs.as<const rlnk::FineJob>();
-    fa.chunk_job = e.d_chunk_job.as<const int>();
-    fa.chunk_first = e.d_chunk_first.as<const int>();
-    fa.pairs = e.d_pairs.as<const int>();
-    fa.pair_off = e.d_pair_off.as<const long long>();
-    fa.or_off = e.d_or_off.as<const int>();
-    fa.or_frame = e.d_or_frame.as<const int>();
-    fa.or_class = e.d_or_class.as<const int>();
-    fa.vols = e.d_vols.as<const float4 *const>();
-    fa.vg = e.vg;
-    fa.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
-    fa.ig = e.igf;
-    fa.fimg = dX;
-    fa.fctf = dC;
-    fa.scale = d_scale;
-    fa.q = e.d_q_f.as<const float>();
-    fa.ex = e.d_ex_f.as<const float2>();
-    fa.ey = e.d_ey_f.as<const float2>();
-    fa.tr_ix = e.d_tr_ix.as<const int>();
-    fa.tr_iy = e.d_tr_iy.as<const int>();
-    fa.xi2half = d_xi2;
-    fa.NT = NT;
-    fa.do_ctf = cfg.do_ctf_correction;
-    fa.diff2 = e.d_fdiff2.as<float>();
-    if (n_small) {
-        hipLaunchKernelGGL(rlnk::k_fine_diff2<2>, dim3((unsigned)n_small), dim3(rlnk::kBlock), 0, st, fa);
-        RLN_TRY(launch_check("k_fine_diff2<2>"));
-    }
-    if (cjob.size() > n_small) {
-        rlnk::FineArgs fb = fa;
-        fb.chunk_job += n_small;
-        fb.chunk_first += n_small;
-        hipLaunchKernelGGL(rlnk::k_fine_diff2_wide, dim3((unsigned)(cjob.size() - n_small)), dim3(rlnk::kBlock), 0,
-                           st, fb);
-        RLN_TRY(launch_check("k_fine_diff2_wide"));
-    }
-    HIP_TRY(hipEventRecord(e.ev[kStFineW], st));
-    // fine weights
-    std::vector<long long> fent(B), foff(B);
-    for (int b = 0; b < B; ++b) {
-        fent[b] = 32LL * counts[b];
-        foff[b] = 32LL * pair_off[b];
-    }
-    RLN_TRY(upload(e.d_fine_n, fent, st));
-    DevBuf &d_foff = e.d_foff;
-    RLN_TRY(upload(d_foff, foff, st));
-    rlnk::WeightArgs wf = wa;
-    wf.diff2 = e.d_fdiff2.as<const float>();
-    wf.d2_off = d_foff.as<const long long>();
-    wf.n_entries = e.d_fine_n.as<const long long>();
-    wf.pairs = e.d_pairs.as<const int>();
-    wf.pair_off = e.d_pair_off.as<const long long>();
-    wf.fine = 1;
-    wf.maxsig = 0;
-    wf.weight = e.d_fweight.as<double>();
-    wf.mask = nullptr;
-    wf.stats = e.d_stats_f.as<rlnk::PassStats>();
-    hipLaunchKernelGGL(rlnk::k_weights, dim3(B), dim3(rlnk::kWBlock), 0, st, wf);
-    RLN_TRY(launch_check("k_weights(fine)"));
-    // weighted sums and backprojection
-    RLN_TRY(e.d_shell_acc.ensure(sizeof(double) * e.nshell * B));
-    RLN_TRY(e.d_part_acc.ensure(sizeof(double) * 8 * B));
-    RLN_TRY(e.d_pdfc_acc.ensure(sizeof(double) * K * B));
-    HIP_TRY(hipMemsetAsync(e.d_shell_acc.ptr, 0, sizeof(double) * e.nshell * B, st));
-    HIP_TRY(hipMemsetAsync(e.d_part_acc.ptr, 0, sizeof(double) * 8 * B, st));
-    HIP_TRY(hipMemsetAsync(e.d_pdfc_acc.ptr, 0, sizeof(double) * K * B, st));
-    HIP_TRY(hipEventRecord(e.ev[kStWsum], st));
-    rlnk::WsumArgs ws{};
-    ws.jobs = e.d_jobs.as<const rlnk::FineJob>();
-    ws.pairs = e.d_pairs.as<const int>();
-    ws.pair_off = e.d_pair_off.as<const long long>();
-    ws.or_off = e.d_or_off.as<const int>();
-    ws.or_frame = e.d_or_frame.as<const int>();
-    ws.or_class = e.d_or_class.as<const int>();
-    ws.or_dir = e.d_or_dir.as<const int>();
-    ws.vols = e.d_vols.as<const float4 *const>();
-    ws.vg = e.vg;
-    ws.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
-    ws.ig = e.igf;
-    ws.fimg = dX;
-    ws.fimg_nomask = dXn;
-    ws.fctf = dC;
-    ws.scale_w = d_scale_w;
-    ws.minv_w = e.d_minv_w.as<const float>();
-    ws.mresol = e.d_mresol.as<const int>();
-    ws.xa_shell = (cfg.do_scale_correction && e.d_xa_shell.ptr) ? e.d_xa_shell.as<const unsigned char>() : nullptr;
-    ws.ex = e.d_ex_f.as<const float2>();
-    ws.ey = e.d_ey_f.as<const float2>();
-    ws.tr_ix = e.d_tr_ix.as<const int>();
-    ws.tr_iy = e.d_tr_iy.as<const int>();
-    ws.over_x = e.d_over_x.as<const double>();
-    ws.over_y = e.d_over_y.as<const double>();
-    ws.weight = e.d_fweight.as<const double>();
-    ws.stats = e.d_stats_f.as<const rlnk::PassStats>();
-    ws.old_off = d_old;
-    ws.prior_off = d_prior;
-    ws.pixel_size = cfg.pixel_size;
-    ws.NT = NT;
-    ws.nshell = e.nshell;
-    ws.n_dir = e.grid.n_dir;
-    ws.do_ctf = cfg.do_ctf_correction;
-    ws.do_scale = cfg.do_scale_correction;
-    ws.bp_slots = e.d_bp_ptrs.as<float *const>();
-    ws.shell_acc = e.d_shell_acc.as<double>();
-    ws.part_acc = e.d_part_acc.as<double>();
-    ws.pdf_class = e.d_pdfc_acc.as<double>();
-    ws.pdf_direction = e.d_pdfd_acc.as<double>();
-    ws.K = K;
-    if (n_jobs) {
-        hipLaunchKernelGGL(rlnk::k_wsum_bp, dim3((unsigned)n_jobs * 8), dim3(rlnk::kBlock), 0, st, ws);
-        RLN_TRY(launch_check("k_wsum_bp"));
-    }
     HIP_TRY(hipEventRecord(e.ev[kStCount], st));
     // results
     std::vector<rlnk::PassStats> stf(B);
@@ -920,7 +1082,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[2] = (long long)Pc;
     e.counters[3] += 32 * n_pairs;
     e.counters[4] = (long long)Pf;
-    e.counters[5] += (long long)cjob.size();
+    e.counters[5] += (long long)plan.n_small + plan.n_wide;
     for (int b = 0; b < B; ++b) {
         e.counters[6] += (long long)pacc[8 * b + 5];
         e.counters[7] += (long long)pacc[8 * b + 6];
@@ -928,6 +1090,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[9] += (long long)h.tiles.size();
     e.counters[10] += n_jobs;
     e.counters[11] += 1;
+    e.counters[14] = e.grid_small;
+    e.counters[15] = e.grid_wide;
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
         if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
@@ -1137,7 +1301,10 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
                            &e->d_diff2, &e->d_weight, &e->d_mask, &e->d_stats_c, &e->d_stats_f, &e->d_npairs,
                            &e->d_njobs, &e->d_nchunks, &e->d_pair_off, &e->d_job_off, &e->d_chunk_off, &e->d_pairs,
                            &e->d_jobs, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fweight,
-                           &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc};
+                           &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc,
+                           &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w};
+    if (e->fine_exec) (void)hipGraphExecDestroy(e->fine_exec);
+    if (e->fine_graph) (void)hipGraphDestroy(e->fine_graph);
     for (rln::DevBuf *b : bufs) b->release();
     for (int i = 0; i <= rln::kStCount; ++i)
         if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
@@ -1266,6 +1433,7 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
         }
         starts.push_back(n);
     }
+    for (size_t c = 0; c + 1 < starts.size(); ++c) e->cap_B = std::max(e->cap_B, starts[c + 1] - starts[c]);
     // chunk c runs on the device while a second host thread builds the lists of chunk c + 1
     using clk = std::chrono::steady_clock;
     rln::ChunkHost cur, nxt;

@@ -556,6 +556,29 @@ __device__ __forceinline__ long long block_sum_ll(long long v, long long *red) {
     return r;
 }
 
+// Device-side header of a chunk's fine pass, written by k_fine_plan / k_fine_items and read by the persistent
+// kernels, so that the fine pass needs no host round trip. When a count exceeds the capacity the host sized
+// the buffers for, overflow is set, every later kernel of the pass returns at once, and the host grows the
+// buffers and runs the pass again.
+struct FinePlan {
+    long long n_pairs;   // significant coarse poses of the chunk
+    int n_jobs;          // (particle, orientation) jobs
+    int n_small, n_wide; // work items of k_fine_diff2<2> and k_fine_diff2_wide
+    int overflow;
+    int B;               // particles in the chunk
+    int pad_;
+};
+
+// Per-chunk input images (device pointers), read by the fine-pass kernels through one indirection so that a
+// captured graph of the pass serves every chunk.
+struct ChunkPtrs {
+    const float2 *fimg;
+    const float2 *fimg_nomask;
+    const float *fctf;
+    int B;  // particles in the chunk
+    int pad_;
+};
+
 // Mode of k_weights: coarse entries are e = o*NT + t; fine entries are e = s*32 + r*4 + u where pair s
 // holds the coarse index (o*NT + t) in pairs[].
 struct WeightArgs {
@@ -574,6 +597,7 @@ struct WeightArgs {
     double *weight;              // out, same indexing as diff2
     unsigned char *mask;         // out (coarse mode), weight >= threshold
     PassStats *stats;            // out [b]
+    const FinePlan *plan;        // fine mode: chunk header (overflow: skip; blocks >= plan->B: idle); else null
 };
 
 __device__ __forceinline__ double entry_prior(const WeightArgs &a, int b, long long e) {
@@ -618,6 +642,7 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
     __shared__ long long s_above_cnt;
     __shared__ int s_done;
     const int b = blockIdx.x;
+    if (a.plan && (a.plan->overflow || b >= a.plan->B)) return;
     const long long n = a.n_entries[b];
     const float *d2 = a.diff2 + a.d2_off[b];
     double *w = a.weight + a.d2_off[b];
@@ -913,6 +938,7 @@ struct FineJob {
     int s_count;    // number of significant translations
 };
 
+
 __device__ __forceinline__ int block_exclusive_scan_i(int v, int *tmp, int &total) {
     // wave-level inclusive scan
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -963,15 +989,51 @@ __global__ void __launch_bounds__(kWBlock) k_count_jobs(const unsigned char *__r
     }
 }
 
+// Offsets of the chunk's fine pass from the per-particle counts: exclusive scans over particles of pairs and
+// jobs, fine-weight entry counts and offsets, totals and the capacity check. One block of 1024.
+__global__ void __launch_bounds__(kWBlock) k_fine_plan(const int *__restrict__ n_pairs_b, const int *__restrict__ n_jobs_b,
+                                                       const ChunkPtrs *__restrict__ cp, long long cap_pairs, int cap_jobs,
+                                                       long long *__restrict__ pair_off, int *__restrict__ job_off,
+                                                       long long *__restrict__ fine_n, long long *__restrict__ fine_off,
+                                                       FinePlan *__restrict__ plan) {
+    __shared__ int tmp[24];
+    const int B = cp->B;
+    long long pbase = 0;
+    int jbase = 0;
+    for (int b0 = 0; b0 < B; b0 += kWBlock) {
+        const int b = b0 + threadIdx.x;
+        const int np = b < B ? n_pairs_b[b] : 0, nj = b < B ? n_jobs_b[b] : 0;
+        int tp, tj;
+        const int ps = block_exclusive_scan_i(np, tmp, tp);
+        const int js = block_exclusive_scan_i(nj, tmp, tj);
+        if (b < B) {
+            pair_off[b] = pbase + ps;
+            job_off[b] = jbase + js;
+            fine_n[b] = 32LL * np;
+            fine_off[b] = 32LL * (pbase + ps);
+        }
+        pbase += tp;
+        jbase += tj;
+    }
+    if (threadIdx.x == 0) {
+        plan->n_pairs = pbase;
+        plan->n_jobs = jbase;
+        plan->n_small = plan->n_wide = 0;
+        plan->B = B;
+        plan->overflow = (pbase > cap_pairs || jbase > cap_jobs) ? 1 : 0;
+    }
+}
+
 // Writes pairs and jobs at the offsets computed from k_count_jobs. grid (B), block 1024.
 __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__restrict__ mask,
                                                         const long long *__restrict__ d2_off,
                                                         const int *__restrict__ or_cnt, int NT,
                                                         const long long *__restrict__ pair_off,
                                                         const int *__restrict__ job_off, int *__restrict__ pairs,
-                                                        FineJob *__restrict__ jobs) {
+                                                        FineJob *__restrict__ jobs, const FinePlan *__restrict__ plan) {
     __shared__ int tmp[24];
     const int b = blockIdx.x;
+    if (plan && (plan->overflow || b >= plan->B)) return;
     const unsigned char *m = mask + d2_off[b];
     int pair_base = 0, job_base = 0;
     const int n_o = or_cnt[b];
@@ -1004,10 +1066,46 @@ __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__r
 // 8 oversampled orientations once per pixel and accumulates the 8 x 4*TC squared differences.
 // --------------------------------------------------------------------------------------------------
 
+// Work items of the fine pass: jobs with at most small_max significant translations are one item of
+// k_fine_diff2<2>; the others one item of k_fine_diff2_wide per kWideTC translations. One block of 1024.
+__global__ void __launch_bounds__(kWBlock) k_fine_items(const FineJob *__restrict__ jobs, int small_max, int wide_tc,
+                                                        int cap_small, int cap_wide, int2 *__restrict__ small_items,
+                                                        int2 *__restrict__ wide_items, FinePlan *__restrict__ plan) {
+    __shared__ int tmp[24];
+    if (plan->overflow) return;
+    const int n_jobs = plan->n_jobs;
+    int sbase = 0, wbase = 0;
+    for (int j0 = 0; j0 < n_jobs; j0 += kWBlock) {
+        const int j = j0 + threadIdx.x;
+        int ns = 0, nw = 0, sc = 0;
+        if (j < n_jobs) {
+            sc = jobs[j].s_count;
+            if (sc <= small_max) ns = 1;
+            else nw = (sc + wide_tc - 1) / wide_tc;
+        }
+        int ts, tw;
+        const int ss = block_exclusive_scan_i(ns, tmp, ts);
+        const int ws = block_exclusive_scan_i(nw, tmp, tw);
+        if (ns && sbase + ss < cap_small) small_items[sbase + ss] = make_int2(j, 0);
+        for (int q = 0; q < nw; ++q)
+            if (wbase + ws + q < cap_wide) wide_items[wbase + ws + q] = make_int2(j, q * wide_tc);
+        sbase += ts;
+        wbase += tw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        plan->n_small = sbase;
+        plan->n_wide = wbase;
+        if (sbase > cap_small || wbase > cap_wide) plan->overflow = 1;
+    }
+}
+
 struct FineArgs {
     const FineJob *jobs;
-    const int *chunk_job;          // job of each block
-    const int *chunk_first;        // first job-local pair of each block
+    const int2 *items;             // (job, first job-local pair) per work item
+    const FinePlan *plan;
+    int which;                     // 0: items counted by plan->n_small, 1: by plan->n_wide
+    const ChunkPtrs *cp;           // this chunk's images
     const int *pairs;
     const long long *pair_off;
     const int *or_off;
@@ -1017,8 +1115,6 @@ struct FineArgs {
     VolGeom vg;
     const Frame *fine_frames;      // [n_dir * n_psi * 8]
     ImgGeom ig;
-    const float2 *fimg;            // [b][Pf]
-    const float *fctf;
     const float *scale;
     const float *q;                // sqrt(Minvsigma2/2) of the fine size
     const float2 *ex;              // [n_xs][xs] phase factors of the fine x shifts
@@ -1037,8 +1133,11 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
     __shared__ float red[kBlock / 64][8 * kFineNF];
     __shared__ int s_tix[kFineNF], s_tiy[kFineNF];
     __shared__ int s_t[kFineTC];
-    const int job_id = a.chunk_job[blockIdx.x];
-    const int first = a.chunk_first[blockIdx.x];
+    if (a.plan->overflow) return;
+    const int n_items = a.which ? a.plan->n_wide : a.plan->n_small;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int job_id = a.items[item].x;
+    const int first = a.items[item].y;
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
     const int nc = min(kFineTC, job.s_count - first);
@@ -1059,8 +1158,8 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
     const int k = a.or_class[oe];
     const Frame *fr = a.fine_frames + a.or_frame[oe];
     const float4 *vol = a.vols[k];
-    const float2 *X = a.fimg + (size_t)b * a.ig.P;
-    const float *C = a.fctf + (size_t)b * a.ig.P;
+    const float2 *X = a.cp->fimg + (size_t)b * a.ig.P;
+    const float *C = a.cp->fctf + (size_t)b * a.ig.P;
     const float sc = a.scale[b];
     float acc[8][kFineNF];
 #pragma unroll
@@ -1117,6 +1216,8 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
             a.diff2[(a.pair_off[b] + s) * 32 + r * 4 + u] = a.xi2half[b] + v;
         }
     }
+    __syncthreads();  // LDS of this item consumed before the next item's writes
+    }
 }
 
 // Wide variant for jobs with more than 2 significant translations: one block per (job, 4 * kWideCW coarse
@@ -1133,8 +1234,12 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
     __shared__ float2 s_X[64];
     __shared__ int s_p[64];
     __shared__ int s_tix[4][kNF], s_tiy[4][kNF];  // per wave
-    const int job_id = a.chunk_job[blockIdx.x];
-    const int first = a.chunk_first[blockIdx.x];
+    if (a.plan->overflow) return;
+    const int n_items = a.which ? a.plan->n_wide : a.plan->n_small;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    __syncthreads();  // previous item's translation tables consumed
+    const int job_id = a.items[item].x;
+    const int first = a.items[item].y;
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
     const int nc = min(kWideTC, job.s_count - first);  // coarse translations of this block
@@ -1151,8 +1256,8 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
     const int k = a.or_class[oe];
     const Frame *fr = a.fine_frames + a.or_frame[oe];
     const float4 *vol = a.vols[k];
-    const float2 *X = a.fimg + (size_t)b * a.ig.P;
-    const float *C = a.fctf + (size_t)b * a.ig.P;
+    const float2 *X = a.cp->fimg + (size_t)b * a.ig.P;
+    const float *C = a.cp->fctf + (size_t)b * a.ig.P;
     const float sc = a.scale[b];
     const bool active = kWideCW * wave < nc;  // this wave has translations
     float acc[8][kNF];
@@ -1215,8 +1320,8 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
             }
         }
     }
-    if (!active) return;
-    // wave reduction; lane 0 writes the wave's translations
+    if (active) {
+    // wave reduction; lanes 0 .. kWideCW-1 write the wave's translations
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
@@ -1243,6 +1348,13 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
                 }
         }
     }
+    }  // active
+    }  // items
+}
+
+// Zero fill used inside captured graphs instead of memset nodes.
+__global__ void __launch_bounds__(kBlock) k_zero_d(double *__restrict__ p, long long n) {
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) p[i] = 0.;
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -1263,9 +1375,8 @@ struct WsumArgs {
     VolGeom vg;
     const Frame *fine_frames;
     ImgGeom ig;
-    const float2 *fimg;
-    const float2 *fimg_nomask;
-    const float *fctf;
+    const ChunkPtrs *cp;            // this chunk's images
+    const FinePlan *plan;           // items: plan->n_jobs * 8
     const float *scale_w;           // max(scale, 0.001) (src/ml_optimiser.cpp:8402-8416)
     const float *minv_w;            // Minvsigma2 with the origin restored
     const int *mresol;              // resolution shell per pixel, -1 excluded
@@ -1303,8 +1414,11 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     __shared__ long long sx_base[kBlock];
     __shared__ float sx_val[kBlock][3];
     __shared__ float sx_w[kBlock][8];
-    const int job_id = blockIdx.x >> 3;
-    const int r = blockIdx.x & 7;
+    if (a.plan->overflow) return;
+    const int n_items = 8 * a.plan->n_jobs;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int job_id = item >> 3;
+    const int r = item & 7;
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
     const int tid = threadIdx.x;
@@ -1351,14 +1465,17 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     }
     __syncthreads();
     const int nsig = s_n;
-    if (nsig == 0) return;
+    if (nsig == 0) {
+        __syncthreads();
+        continue;
+    }
     const int oe = a.or_off[b] + job.o;
     const int k = a.or_class[oe];
     const Frame fr = a.fine_frames[a.or_frame[oe] + r];
     const float4 *vol = a.vols[k];
-    const float2 *X = a.fimg + (size_t)b * a.ig.P;
-    const float2 *Xn = a.fimg_nomask + (size_t)b * a.ig.P;
-    const float *C = a.fctf + (size_t)b * a.ig.P;
+    const float2 *X = a.cp->fimg + (size_t)b * a.ig.P;
+    const float2 *Xn = a.cp->fimg_nomask + (size_t)b * a.ig.P;
+    const float *C = a.cp->fctf + (size_t)b * a.ig.P;
     const float scw = a.do_scale ? a.scale_w[b] : 1.f;
     float *slots = a.bp_slots[k];
     const unsigned char *xash = a.xa_shell ? a.xa_shell + (size_t)k * a.nshell : nullptr;
@@ -1367,7 +1484,7 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     // start each block at a different pixel chunk so that concurrent blocks do not all scatter into the
     // low-frequency voxels at the same time
     const int n_chunks = (a.ig.P + kBlock - 1) / kBlock;
-    const int c0 = (int)((blockIdx.x * 2654435761u) % (unsigned)n_chunks);
+    const int c0 = (int)(((unsigned)item * 2654435761u) % (unsigned)n_chunks);
     for (int cc = 0; cc < n_chunks; ++cc) {
         int chunk = c0 + cc;
         if (chunk >= n_chunks) chunk -= n_chunks;
@@ -1480,6 +1597,8 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
         atomicAdd(&a.part_acc[8 * b + 3], xa);
         atomicAdd(&a.part_acc[8 * b + 4], aa);
     }
+    __syncthreads();  // LDS of this item consumed before the next item
+    }  // items
 }
 
 }  // namespace rlnk

@@ -46,7 +46,7 @@ struct rln_proposal {
     // activations for a batch of up to cap images
     int cap = 0;
     rln::DevBuf x[rln::kLayers + 1], col[rln::kLayers], logits, dlog, dtmp, dz_a, dz_b, loss, labels, idx;
-    rln::DevBuf src;  // particle images staged from the host
+    rln::DevBuf src, src_ctf;  // particle images and CTFs staged from the host
     // training set
     rln::DevBuf train_img;  // bf16 [n_train][W][W]
     std::vector<int> train_lab;
@@ -178,21 +178,28 @@ int refresh_bf16(rln_proposal &p) {
 }
 
 // network input images of n particle images (half-complex at current_size) into dst (bf16), optional FP32
-int particle_images(rln_proposal &p, int n, const float *fimg, int current_size, int on_device, rlnp::bf16 *dst,
-                    float *dst_f32) {
+int particle_images(rln_proposal &p, int n, const float *fimg, const float *fctf, int current_size, int on_device,
+                    rlnp::bf16 *dst, float *dst_f32) {
     if (current_size < p.W || (current_size & 1)) RLN_FAIL(RLN_ERR_ARG, "current_size %d below the window %d", current_size, p.W);
     const int xs = current_size / 2 + 1;
     const size_t per = (size_t)current_size * xs;
     const float2 *srcp;
+    const float *ctfp = nullptr;
     if (on_device) {
         srcp = reinterpret_cast<const float2 *>(fimg);
+        ctfp = fctf;
     } else {
         RLN_TRY(p.src.ensure(sizeof(float2) * per * n));
         HIP_TRY(hipMemcpyAsync(p.src.ptr, fimg, sizeof(float2) * per * n, hipMemcpyHostToDevice, p.stream));
         srcp = p.src.as<const float2>();
+        if (fctf) {
+            RLN_TRY(p.src_ctf.ensure(sizeof(float) * per * n));
+            HIP_TRY(hipMemcpyAsync(p.src_ctf.ptr, fctf, sizeof(float) * per * n, hipMemcpyHostToDevice, p.stream));
+            ctfp = p.src_ctf.as<const float>();
+        }
     }
-    hipLaunchKernelGGL(rlnp::k_window_image, dim3(n), dim3(rlnp::kB), 0, p.stream, srcp, (long long)per, current_size,
-                       xs, p.W, dst, dst_f32);
+    hipLaunchKernelGGL(rlnp::k_window_image, dim3(n), dim3(rlnp::kB), 0, p.stream, srcp, ctfp, (long long)per,
+                       current_size, xs, p.W, dst, dst_f32);
     HIP_TRY(hipGetLastError());
     return RLN_OK;
 }
@@ -435,7 +442,7 @@ extern "C" int rln_proposal_generate(rln_proposal *p, const rln_projector *ref, 
                            d_win.as<float2>());
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(rlnp::k_window_image, dim3(m), dim3(rlnp::kB), 0, p->stream, d_win.as<const float2>(),
-                           (long long)W * xw, W, xw, W,
+                           (const float *)nullptr, (long long)W * xw, W, xw, W,
                            p->train_img.as<rlnp::bf16>() + (size_t)(p->n_train + i0) * W * W, (float *)nullptr);
         HIP_TRY(hipGetLastError());
     }
@@ -444,8 +451,8 @@ extern "C" int rln_proposal_generate(rln_proposal *p, const rln_projector *ref, 
     return RLN_OK;
 }
 
-extern "C" int rln_proposal_add_images(rln_proposal *p, int n, const float *fimg, int current_size, int on_device,
-                                       const double *poses) {
+extern "C" int rln_proposal_add_images(rln_proposal *p, int n, const float *fimg, const float *fctf,
+                                       int current_size, int on_device, const double *poses) {
     if (!p || n < 0 || (n > 0 && (!fimg || !poses))) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_add_images: null argument");
     std::lock_guard<std::mutex> lk(p->mtx);
     HIP_TRY(hipSetDevice(p->device));
@@ -454,7 +461,7 @@ extern "C" int rln_proposal_add_images(rln_proposal *p, int n, const float *fimg
     const size_t per = (size_t)current_size * (current_size / 2 + 1);
     for (int i0 = 0; i0 < n; i0 += chunk) {
         const int m = std::min(chunk, n - i0);
-        RLN_TRY(particle_images(*p, m, fimg + 2 * per * i0, current_size, on_device,
+        RLN_TRY(particle_images(*p, m, fimg + 2 * per * i0, fctf ? fctf + per * i0 : nullptr, current_size, on_device,
                                 p->train_img.as<rlnp::bf16>() + (size_t)(p->n_train + i0) * p->W * p->W, nullptr));
         HIP_TRY(hipStreamSynchronize(p->stream));
     }
@@ -514,8 +521,8 @@ extern "C" int rln_proposal_train(rln_proposal *p, int epochs, int batch, double
     return RLN_OK;
 }
 
-extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, int current_size, int on_device,
-                                    int topk, int *bins, float *probs) {
+extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
+                                    int on_device, int topk, int *bins, float *probs) {
     if (!p || n < 0 || (n > 0 && (!fimg || !bins || !probs))) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_predict: null argument");
     if (topk < 1 || topk > rlnp::kMaxTopK || topk > p->NB) RLN_FAIL(RLN_ERR_ARG, "topk %d out of range", topk);
     std::lock_guard<std::mutex> lk(p->mtx);
@@ -528,7 +535,8 @@ extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, i
     const size_t per = (size_t)current_size * (current_size / 2 + 1);
     for (int i0 = 0; i0 < n; i0 += chunk) {
         const int m = std::min(chunk, n - i0);
-        RLN_TRY(particle_images(*p, m, fimg + 2 * per * i0, current_size, on_device, p->x[0].as<rlnp::bf16>(), nullptr));
+        RLN_TRY(particle_images(*p, m, fimg + 2 * per * i0, fctf ? fctf + per * i0 : nullptr, current_size, on_device,
+                                p->x[0].as<rlnp::bf16>(), nullptr));
         RLN_TRY(forward(*p, m));
         hipLaunchKernelGGL(rlnp::k_topk, dim3(m), dim3(rlnp::kB), sizeof(float) * p->NB, p->stream,
                            p->logits.as<const float>(), p->NB, topk, d_bins.as<int>(), d_probs.as<float>());
@@ -612,15 +620,15 @@ extern "C" int rln_proposal_orientation_sets(const rln_proposal *p, int n, const
     return RLN_OK;
 }
 
-extern "C" int rln_proposal_images(rln_proposal *p, int n, const float *fimg, int current_size, int on_device,
-                                   float *out) {
+extern "C" int rln_proposal_images(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
+                                   int on_device, float *out) {
     if (!p || n < 0 || (n > 0 && (!fimg || !out))) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_images: null argument");
     std::lock_guard<std::mutex> lk(p->mtx);
     HIP_TRY(hipSetDevice(p->device));
     RLN_TRY(ensure_batch(*p, n));
     DevBuf f;
     RLN_TRY(f.ensure(sizeof(float) * (size_t)n * p->W * p->W));
-    RLN_TRY(particle_images(*p, n, fimg, current_size, on_device, p->x[0].as<rlnp::bf16>(), f.as<float>()));
+    RLN_TRY(particle_images(*p, n, fimg, fctf, current_size, on_device, p->x[0].as<rlnp::bf16>(), f.as<float>()));
     HIP_TRY(hipMemcpyAsync(out, f.ptr, sizeof(float) * (size_t)n * p->W * p->W, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
     return RLN_OK;

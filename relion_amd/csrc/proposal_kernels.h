@@ -48,7 +48,7 @@ __device__ __forceinline__ float2 normal2(uint64_t key) {
 // --------------------------------------------------------------------------------------------------
 // Training samples: the W x (W/2+1) Fourier window of a projection of the reference at a known pose, with the
 // acquisition's CTF, a shift and Gaussian noise of the model's spectrum (per-component variance
-// sigma2[ires], ires = round(|k|) in the original box).
+// sigma2[ires], ires = round(|k|) in the original box), phase flipped (multiplied by the sign of the CTF).
 // --------------------------------------------------------------------------------------------------
 struct GenSample {
     Frame fr;                 // projection frame of the pose (padding included)
@@ -99,20 +99,22 @@ __global__ void __launch_bounds__(kB) k_gen_window(const float4 *__restrict__ vo
     const int ires = (int)rintf(sqrtf((float)r2));
     const float sg = (ires < nshell) ? sqrtf(sigma2[ires]) : 0.f;
     const float2 nz = normal2((seed * 0x100000001b3ULL) ^ ((uint64_t)(first + b) << 20) ^ (uint64_t)idx);
-    out[(size_t)b * W * xw + idx] = make_float2(f.x + sg * nz.x, f.y + sg * nz.y);
+    // phase flipped, as the particle windows (k_window_image with the particle's CTF)
+    const float sgn = ctf < 0.f ? -1.f : 1.f;
+    out[(size_t)b * W * xw + idx] = make_float2(sgn * (f.x + sg * nz.x), sgn * (f.y + sg * nz.y));
 }
 
 // --------------------------------------------------------------------------------------------------
 // Network input: img(u, v) = sum_{ky=-W/2}^{W/2-1} sum_{kx=0}^{W/2} h(kx) Re[F(kx,ky) e^{2 pi i (kx u + ky v)/W}],
 // h(0) = h(W/2) = 1, else 2, from a half-complex source of s_src rows of xs_src (row of ky: ky >= 0 ? ky :
-// ky + s_src), normalised to zero mean and unit variance. One block per image; out bf16 [n][W][W] (v-major),
-// optional FP32 copy.
+// ky + s_src), phase flipped by the sign of ctf (same layout, optional), normalised to zero mean and unit
+// variance. One block per image; out bf16 [n][W][W] (v-major), optional FP32 copy.
 // --------------------------------------------------------------------------------------------------
 constexpr int kMaxW = 64;
 
-__global__ void __launch_bounds__(kB) k_window_image(const float2 *__restrict__ src, long long src_stride, int s_src,
-                                                     int xs_src, int W, bf16 *__restrict__ out,
-                                                     float *__restrict__ out_f32) {
+__global__ void __launch_bounds__(kB) k_window_image(const float2 *__restrict__ src, const float *__restrict__ ctf,
+                                                     long long src_stride, int s_src, int xs_src, int W,
+                                                     bf16 *__restrict__ out, float *__restrict__ out_f32) {
     __shared__ float2 Fw[kMaxW][kMaxW / 2 + 1];
     __shared__ float2 G[kMaxW / 2 + 1][kMaxW];
     __shared__ float2 tw[kMaxW];
@@ -130,7 +132,9 @@ __global__ void __launch_bounds__(kB) k_window_image(const float2 *__restrict__ 
         const int r = idx / xw, kx = idx - r * xw;
         const int ky = (r < W / 2) ? r : r - W;
         const int row = (ky >= 0) ? ky : ky + s_src;
-        Fw[r][kx] = F[(size_t)row * xs_src + kx];
+        float2 f = F[(size_t)row * xs_src + kx];
+        if (ctf && ctf[(size_t)b * src_stride + (size_t)row * xs_src + kx] < 0.f) f = make_float2(-f.x, -f.y);
+        Fw[r][kx] = f;
     }
     __syncthreads();
     // G(kx, v) = sum_ky F(kx, ky) e^{2 pi i ky v / W}

@@ -49,14 +49,16 @@ class PoseProposal:
         _check(lib.rln_proposal_generate(self._h, projector._h, C.byref(g), n, first, _ptr(poses, C.c_double)))
         return poses
 
-    def add_images(self, fimg, current_size, poses, on_device=False):
-        """Adds particle images labelled with poses (rot, tilt, psi) -- e.g. max-likelihood assignments."""
+    def add_images(self, fimg, fctf, current_size, poses, on_device=False):
+        """Adds particle images (and CTFs, or None) labelled with poses (rot, tilt, psi) -- e.g. max-likelihood
+        assignments."""
         poses = np.ascontiguousarray(poses, np.float64)
         n = len(poses)
         a, keep = self._images_arg(fimg, on_device)
-        _check(lib.rln_proposal_add_images(self._h, n, a, current_size, 1 if on_device else 0,
+        c, keep_c = self._images_arg(fctf, on_device)
+        _check(lib.rln_proposal_add_images(self._h, n, a, c, current_size, 1 if on_device else 0,
                                            _ptr(poses, C.c_double)))
-        del keep
+        del keep, keep_c
 
     def training_size(self):
         n = C.c_longlong()
@@ -71,18 +73,22 @@ class PoseProposal:
     # inference --------------------------------------------------------------------------------
     @staticmethod
     def _images_arg(fimg, on_device):
+        if fimg is None:
+            return None, None
         if on_device:
             return _vp(int(fimg)), None
         fa = np.ascontiguousarray(fimg, np.float32)
         return _vp(fa.ctypes.data), fa
 
-    def predict(self, fimg, n, current_size, topk=16, on_device=False):
+    def predict(self, fimg, fctf, n, current_size, topk=16, on_device=False):
+        """Top-k bins and probabilities of n particle images (fctf: their CTFs for phase flipping, or None)."""
         bins = np.zeros((n, topk), np.int32)
         probs = np.zeros((n, topk), np.float32)
         a, keep = self._images_arg(fimg, on_device)
-        _check(lib.rln_proposal_predict(self._h, n, a, current_size, 1 if on_device else 0, topk,
+        c, keep_c = self._images_arg(fctf, on_device)
+        _check(lib.rln_proposal_predict(self._h, n, a, c, current_size, 1 if on_device else 0, topk,
                                         _ptr(bins, C.c_int), _ptr(probs, C.c_float)))
-        del keep
+        del keep, keep_c
         return bins, probs
 
     def pose_bin(self, rot, tilt, psi):
@@ -118,12 +124,13 @@ class PoseProposal:
         return [og[off[i]:off[i + 1]] for i in range(len(off) - 1)]
 
     # check hooks ------------------------------------------------------------------------------
-    def images(self, fimg, n, current_size, on_device=False):
+    def images(self, fimg, fctf, n, current_size, on_device=False):
         W = self.cfg.window
         out = np.zeros((n, W, W), np.float32)
         a, keep = self._images_arg(fimg, on_device)
-        _check(lib.rln_proposal_images(self._h, n, a, current_size, 1 if on_device else 0, _ptr(out, C.c_float)))
-        del keep
+        c, keep_c = self._images_arg(fctf, on_device)
+        _check(lib.rln_proposal_images(self._h, n, a, c, current_size, 1 if on_device else 0, _ptr(out, C.c_float)))
+        del keep, keep_c
         return out
 
     def forward_backward(self, images, labels):

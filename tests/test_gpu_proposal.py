@@ -27,16 +27,20 @@ def test_proposal_input_images_match_oracle():
     p = make_problem(box=64, px=2.0, order=2, n=4, snr=0.1, seed=91)
     print("[proposal] input images, box 64, 4 particles", flush=True)
     pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=1)
-    img = pp.images(p.fimg, 4, 64)
+    img = pp.images(p.fimg, None, 4, 64)
     for i in range(4):
         ref = pc.input_image(p.fimg[i], 64, 64)
         err = np.abs(img[i] - ref).max()
         assert err < 2e-4, f"particle {i}: input image max abs err {err:.2e}"
+    # phase flipped by the particle's CTF
+    img = pp.images(p.fimg, p.fctf, 4, 64)
+    for i in range(4):
+        assert np.abs(img[i] - pc.input_image(p.fimg[i], 64, 64, p.fctf[i])).max() < 2e-4
     # a smaller window of a larger image
     pp32 = PoseProposal(window=32, channels=8, healpix_order=1, n_psi_bins=12, seed=1)
-    img = pp32.images(p.fimg, 4, 64)
+    img = pp32.images(p.fimg, p.fctf, 4, 64)
     for i in range(4):
-        assert np.abs(img[i] - pc.input_image(p.fimg[i], 64, 32)).max() < 2e-4
+        assert np.abs(img[i] - pc.input_image(p.fimg[i], 64, 32, p.fctf[i])).max() < 2e-4
     pp.close()
     pp32.close()
 
@@ -134,10 +138,13 @@ def test_proposal_seeds_order3_search_and_keeps_best_pose():
     res0, _ = es.run(p.fimg, p.fctf, p.meta)
     poses0 = es.counters()["coarse_poses"]
     pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=5)
-    pp.generate(pr, 40000, ori_size=64, pixel_size=2.0, sigma2_noise=p.sigma2_noise, defocus=(1.0e4, 3.0e4),
-                max_astigmatism=300.0, offset_max=3.0)
-    loss = pp.train(epochs=4, batch=256, lr=1e-3)
-    bins, probs = pp.predict(p.fimg, n, 64, topk=16)
+    for ep in range(3):  # fresh projections every epoch
+        pp.clear_training()
+        pp.generate(pr, 100000, ori_size=64, pixel_size=2.0, sigma2_noise=p.sigma2_noise, defocus=(1.0e4, 3.0e4),
+                    max_astigmatism=300.0, offset_max=3.0, first=ep * 100000)
+        loss = pp.train(epochs=1, batch=256, lr=1e-3)
+        print(f"[proposal] epoch {ep}: loss {loss:.3f}", flush=True)
+    bins, probs = pp.predict(p.fimg, p.fctf, n, 64, topk=16)
     sets = pp.orientation_sets(bins, probs, 3, 48, min_top1=0.2)
     fallback = np.mean([len(s) == 0 for s in sets])
     es.set_orientation_sets(sets)
