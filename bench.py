@@ -83,11 +83,11 @@ def angle_between(r0, r1):
     return float(np.degrees(np.arccos(np.clip((np.trace(A @ B.T) - 1) / 2, -1, 1))))
 
 
-def fsc(v0, v1, box, pad, r_max):
-    """FSC per shell (original-box Fourier pixels) between two reconstructions data / weight."""
+def fsc(v0, v1, pad, r_max):
+    """FSC per shell (original-box Fourier pixels) between two reconstructions data / (weight + tau), tau = the
+    mean weight of the voxel's shell (a Wiener-like regularisation: voxels with near-zero weight do not
+    dominate the shell sums)."""
     (re0, im0, w0), (re1, im1, w1) = v0, v1
-    F0 = np.where(w0 > 0, (re0 + 1j * im0) / np.maximum(w0, 1e-30), 0)
-    F1 = np.where(w1 > 0, (re1 + 1j * im1) / np.maximum(w1, 1e-30), 0)
     zd, yd, xd = re0.shape
     z = np.arange(zd) - zd // 2
     y = np.arange(yd) - yd // 2
@@ -96,7 +96,13 @@ def fsc(v0, v1, box, pad, r_max):
     shell = np.rint(r).astype(np.int64).ravel()
     keep = shell <= r_max
     sh = shell[keep]
-    a, b = F0.ravel()[keep], F1.ravel()[keep]
+    cnt = np.maximum(np.bincount(sh, minlength=r_max + 1), 1)
+
+    def recon(re, im, w):
+        wv = w.ravel()[keep].astype(np.float64)
+        tau = (np.bincount(sh, wv, r_max + 1) / cnt)[sh]
+        return (re.ravel()[keep] + 1j * im.ravel()[keep]) / (wv + tau + 1e-30)
+    a, b = recon(re0, im0, w0), recon(re1, im1, w1)
     num = np.bincount(sh, np.real(a * np.conj(b)), r_max + 1)
     d0 = np.bincount(sh, np.abs(a) ** 2, r_max + 1)
     d1 = np.bincount(sh, np.abs(b) ** 2, r_max + 1)
@@ -239,7 +245,7 @@ def main():
                             for a, b in zip(res_x, res_p)])
             same = (ang < 1e-3) & (np.abs(res_x["xoff"] - res_p["xoff"]) < 1e-6) & \
                    (np.abs(res_x["yoff"] - res_p["yoff"]) < 1e-6)
-            f = fsc(vol_plain, vol_prop, box, 2.0, box // 2)
+            f = fsc(vol_plain, vol_prop, 2.0, box // 2)
             lim = int(0.8 * (box // 2))
             extra["proposal_vs_exhaustive"] = {
                 "particles": n_plain, "kept_best_pose": round(float(same.mean()), 4),

@@ -189,6 +189,12 @@ int rln_estep_run(rln_estep *e, int n, const float *fimg, const float *fimg_noma
                   int inputs_on_device, const rln_particle_meta *meta,
                   rln_particle_result *results, rln_wsum *wsum);
 
+/* The fine pass (job lists, fine diff2, fine weights, weighted sums and backprojection) runs from device-side
+ * counts as persistent kernels captured into HIP graphs and replayed for every chunk (default, enable = 1), or
+ * launched directly (enable = 0; same kernels and results). No counterpart in the reference, whose fine pass
+ * is planned on the host per particle (src/acc/acc_ml_optimiser_impl.h:1420-1890). */
+int rln_estep_use_graphs(rln_estep *e, int enable);
+
 /* Explicit coarse orientation lists for the next rln_estep_run (global searches only), as produced by the
  * learned pose proposal: particle i is searched over orientations set_orients[set_offsets[i] ..
  * set_offsets[i+1]), each a global grid index idir * N_psi + ipsi of the coarse HEALPix grid, for all classes

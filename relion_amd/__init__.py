@@ -132,6 +132,7 @@ _SIGNATURES = {
                                 C.POINTER(ParticleResult), C.POINTER(Wsum)]),
     "rln_estep_keep_intermediates": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_orientation_sets": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "rln_estep_use_graphs": (C.c_int, [_vp, C.c_int]),
     "rln_estep_coarse_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _ip, _ip, _ip]),
     "rln_estep_get_coarse": (C.c_int, [_vp, C.c_int, _fp, _dp, C.POINTER(C.c_ubyte)]),
     "rln_estep_fine_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong)]),
@@ -460,6 +461,9 @@ class ExpectationStep:
 
     def keep_intermediates(self, keep=True):
         _check(lib.rln_estep_keep_intermediates(self._h, 1 if keep else 0))
+
+    def use_graphs(self, enable=True):
+        _check(lib.rln_estep_use_graphs(self._h, 1 if enable else 0))
 
     def set_orientation_lists(self, offsets, orients):
         """set_orientation_sets in the flat form: offsets[n+1], orients[offsets[n]] (int32)."""

@@ -108,11 +108,12 @@ struct Engine {
     long long need_pairs = 0;
     int need_jobs = 0, need_small = 0, need_wide = 0;
     DevBuf d_plan, d_cp, d_items_s, d_items_w;
+    DevBuf d_wsegs, d_seg_first, d_wpart, d_hcnt, d_hsum, d_wcand, d_segbest, d_ocnt;  // segmented coarse weights
     rlnk::ChunkPtrs cp_host{};
     bool tuned = false;
     int grid_small = 1024, grid_wide = 1024, grid_wsum = 2048;
-    hipGraph_t fine_graph = nullptr;
-    hipGraphExec_t fine_exec = nullptr;
+    hipGraph_t fine_graph[3] = {};
+    hipGraphExec_t fine_exec[3] = {};
     std::vector<unsigned char> fine_key;  // kernel arguments the graph was captured with
     bool graph_off = false;               // capture unsupported: launch directly
     // intermediates kept for tests
@@ -418,6 +419,8 @@ struct ChunkHost {
     std::vector<float> xi2half, scale, scale_w;
     std::vector<double> old_off, prior_off;
     std::vector<rlnk::CoarseTile> tiles;
+    std::vector<rlnk::WSeg> segs;  // coarse-weight segments (kSegEntries entries each)
+    std::vector<int> seg_first;    // [B+1] first segment of each particle
     long long total_entries = 0;
 };
 
@@ -542,6 +545,15 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         h.prior_off[2 * b + 1] = m.prior_offset_y;
     }
     h.total_entries = d2;
+    h.seg_first.assign(B + 1, 0);
+    for (int b = 0; b < B; ++b) {
+        h.seg_first[b] = (int)h.segs.size();
+        const long long n = h.n_entries[b];
+        const long long ns = std::max(1LL, (n + rlnk::kSegEntries - 1) / rlnk::kSegEntries);
+        for (long long q = 0; q < ns; ++q)
+            h.segs.push_back(rlnk::WSeg{b, 0, q * rlnk::kSegEntries, std::min(n, (q + 1) * rlnk::kSegEntries)});
+    }
+    h.seg_first[B] = (int)h.segs.size();
 }
 
 
@@ -585,7 +597,7 @@ struct FineLaunch {
     rlnk::FinePlan *plan;
     const unsigned char *mask;
     const long long *d2off;
-    const int *or_cnt;
+    const int *or_cnt, *or_off, *ocnt;
     int NT;
     rlnk::FineJob *jobs;
     int2 *items_s, *items_w;
@@ -597,24 +609,31 @@ struct FineLaunch {
     int grid_small, grid_wide, grid_wsum;
 };
 
-static int launch_fine(Engine &e, const FineLaunch &L, bool events) {
+// part 0: plan, job lists, work items, fine diff2; part 1: fine weights; part 2: weighted sums and
+// backprojection. Each part is captured into its own graph so that stage events sit between graph launches.
+static int launch_fine(Engine &e, const FineLaunch &L, int part) {
     hipStream_t st = e.stream;
+    if (part == 1) {
+        hipLaunchKernelGGL(rlnk::k_weights, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.wf);
+        return launch_check("fine weights");
+    }
+    if (part == 2) {
+        hipLaunchKernelGGL(rlnk::k_zero_d, dim3(64), dim3(rlnk::kBlock), 0, st, L.shell, L.n_shell);
+        hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.part, L.n_part);
+        hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.pdfc, L.n_pdfc);
+        hipLaunchKernelGGL(rlnk::k_wsum_bp, dim3(L.grid_wsum), dim3(rlnk::kBlock), 0, st, L.ws);
+        return launch_check("weighted sums");
+    }
     hipLaunchKernelGGL(rlnk::k_fine_plan, dim3(1), dim3(rlnk::kWBlock), 0, st, L.np, L.nj, L.cp, L.cap_pairs,
                        L.cap_jobs, L.pair_off, L.job_off, L.fine_n, L.foff, L.plan);
-    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt, L.NT,
+    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt,
+                       L.or_off, L.ocnt, L.NT,
                        (const long long *)L.pair_off, (const int *)L.job_off, L.pairs, L.jobs,
                        (const rlnk::FinePlan *)L.plan);
     hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs, 2,
                        rlnk::kWideTC, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
     hipLaunchKernelGGL(rlnk::k_fine_diff2<2>, dim3(L.grid_small), dim3(rlnk::kBlock), 0, st, L.fs);
     hipLaunchKernelGGL(rlnk::k_fine_diff2_wide, dim3(L.grid_wide), dim3(rlnk::kBlock), 0, st, L.fw);
-    if (events) HIP_TRY(hipEventRecord(e.ev[kStFineW], st));
-    hipLaunchKernelGGL(rlnk::k_weights, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.wf);
-    hipLaunchKernelGGL(rlnk::k_zero_d, dim3(64), dim3(rlnk::kBlock), 0, st, L.shell, L.n_shell);
-    hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.part, L.n_part);
-    hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.pdfc, L.n_pdfc);
-    if (events) HIP_TRY(hipEventRecord(e.ev[kStWsum], st));
-    hipLaunchKernelGGL(rlnk::k_wsum_bp, dim3(L.grid_wsum), dim3(rlnk::kBlock), 0, st, L.ws);
     return launch_check("fine pass");
 }
 
@@ -624,7 +643,8 @@ static int tune_fine(Engine &e, FineLaunch L) {
     hipStream_t st = e.stream;
     hipLaunchKernelGGL(rlnk::k_fine_plan, dim3(1), dim3(rlnk::kWBlock), 0, st, L.np, L.nj, L.cp, L.cap_pairs,
                        L.cap_jobs, L.pair_off, L.job_off, L.fine_n, L.foff, L.plan);
-    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt, L.NT,
+    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt,
+                       L.or_off, L.ocnt, L.NT,
                        (const long long *)L.pair_off, (const int *)L.job_off, L.pairs, L.jobs,
                        (const rlnk::FinePlan *)L.plan);
     hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs, 2,
@@ -679,35 +699,49 @@ static int run_fine_stage(Engine &e, const FineLaunch &L0) {
     if (!e.graph_off) {
         const unsigned char *kb = reinterpret_cast<const unsigned char *>(&L);
         std::vector<unsigned char> key(kb, kb + sizeof(L));
-        if (!e.fine_exec || key != e.fine_key) {
-            if (e.fine_exec) (void)hipGraphExecDestroy(e.fine_exec);
-            if (e.fine_graph) (void)hipGraphDestroy(e.fine_graph);
-            e.fine_exec = nullptr;
-            e.fine_graph = nullptr;
-            bool ok = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-            if (ok) {
-                const int rc = launch_fine(e, L, true);
+        if (!e.fine_exec[0] || key != e.fine_key) {
+            for (int q = 0; q < 3; ++q) {
+                if (e.fine_exec[q]) (void)hipGraphExecDestroy(e.fine_exec[q]);
+                if (e.fine_graph[q]) (void)hipGraphDestroy(e.fine_graph[q]);
+                e.fine_exec[q] = nullptr;
+                e.fine_graph[q] = nullptr;
+            }
+            bool ok = true;
+            for (int q = 0; q < 3 && ok; ++q) {
+                ok = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+                if (!ok) break;
+                const int rc = launch_fine(e, L, q);
                 hipGraph_t g = nullptr;
                 ok = hipStreamEndCapture(e.stream, &g) == hipSuccess && rc == RLN_OK && g;
-                if (ok) ok = hipGraphInstantiate(&e.fine_exec, g, nullptr, nullptr, 0) == hipSuccess;
-                if (ok) e.fine_graph = g;
+                if (ok) ok = hipGraphInstantiate(&e.fine_exec[q], g, nullptr, nullptr, 0) == hipSuccess;
+                if (ok) e.fine_graph[q] = g;
                 else if (g) (void)hipGraphDestroy(g);
             }
             (void)hipGetLastError();
             if (!ok) {
-                e.fine_exec = nullptr;
+                for (int q = 0; q < 3; ++q) {
+                    if (e.fine_exec[q]) (void)hipGraphExecDestroy(e.fine_exec[q]);
+                    if (e.fine_graph[q]) (void)hipGraphDestroy(e.fine_graph[q]);
+                    e.fine_exec[q] = nullptr;
+                    e.fine_graph[q] = nullptr;
+                }
                 e.graph_off = true;  // capture unsupported here: launch directly from now on
             } else {
                 e.fine_key = key;
             }
         }
-        if (e.fine_exec) {
-            HIP_TRY(hipGraphLaunch(e.fine_exec, e.stream));
+    }
+    for (int q = 0; q < 3; ++q) {
+        if (q == 1) HIP_TRY(hipEventRecord(e.ev[kStFineW], e.stream));
+        if (q == 2) HIP_TRY(hipEventRecord(e.ev[kStWsum], e.stream));
+        if (e.fine_exec[q]) {
+            HIP_TRY(hipGraphLaunch(e.fine_exec[q], e.stream));
             e.counters[12] += 1;
-            return RLN_OK;
+        } else {
+            RLN_TRY(launch_fine(e, L, q));
         }
     }
-    return launch_fine(e, L, true);
+    return RLN_OK;
 }
 
 template <int NTT>
@@ -864,17 +898,54 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     wa.weight = e.d_weight.as<double>();
     wa.mask = e.d_mask.as<unsigned char>();
     wa.stats = e.d_stats_c.as<rlnk::PassStats>();
-    hipLaunchKernelGGL(rlnk::k_weights, dim3(B), dim3(rlnk::kWBlock), 0, st, wa);
-    RLN_TRY(launch_check("k_weights(coarse)"));
+    // coarse weights over segments of kSegEntries poses (multi-block per particle)
+    {
+        const unsigned nseg = (unsigned)h.segs.size();
+        const unsigned gB = (unsigned)((B + rlnk::kBlock - 1) / rlnk::kBlock);
+        RLN_TRY(upload(e.d_wsegs, h.segs, st));
+        RLN_TRY(upload(e.d_seg_first, h.seg_first, st));
+        RLN_TRY(e.d_wpart.ensure(sizeof(rlnk::WPart) * B));
+        RLN_TRY(e.d_hcnt.ensure(sizeof(unsigned int) * 256 * B));
+        RLN_TRY(e.d_hsum.ensure(sizeof(double) * 256 * B));
+        RLN_TRY(e.d_wcand.ensure(sizeof(double) * rlnk::kGather * B));
+        RLN_TRY(e.d_segbest.ensure(sizeof(rlnk::WSegBest) * nseg));
+        HIP_TRY(hipMemsetAsync(e.d_hcnt.ptr, 0, sizeof(unsigned int) * 256 * B, st));
+        // per-orientation significant counts and per-particle job counts of the fine pass
+        RLN_TRY(ensure_fine_capacity(e));
+        RLN_TRY(e.d_ocnt.ensure(sizeof(int) * std::max<size_t>(1, h.or_proj.size())));
+        HIP_TRY(hipMemsetAsync(e.d_ocnt.ptr, 0, sizeof(int) * std::max<size_t>(1, h.or_proj.size()), st));
+        HIP_TRY(hipMemsetAsync(e.d_npairs.as<int>() + cB, 0, sizeof(int) * B, st));
+        HIP_TRY(hipMemsetAsync(e.d_hsum.ptr, 0, sizeof(double) * 256 * B, st));
+        const rlnk::WSeg *sg = e.d_wsegs.as<const rlnk::WSeg>();
+        rlnk::WPart *wp = e.d_wpart.as<rlnk::WPart>();
+        unsigned int *hc = e.d_hcnt.as<unsigned int>();
+        double *hs = e.d_hsum.as<double>();
+        hipLaunchKernelGGL(rlnk::k_wpart_init, dim3(gB), dim3(rlnk::kBlock), 0, st, wp, B);
+        hipLaunchKernelGGL(rlnk::k_wseg_min, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp);
+        hipLaunchKernelGGL(rlnk::k_wseg_exp, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp);
+        hipLaunchKernelGGL(rlnk::k_wsel_start, dim3(gB), dim3(rlnk::kBlock), 0, st, wa, wp, B);
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            hipLaunchKernelGGL(rlnk::k_wseg_hist, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg,
+                               (const rlnk::WPart *)wp, shift, hc, hs);
+            hipLaunchKernelGGL(rlnk::k_wsel_pick, dim3(gB), dim3(rlnk::kBlock), 0, st, wp, B, shift, hc, hs);
+        }
+        hipLaunchKernelGGL(rlnk::k_wseg_gather, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp,
+                           e.d_wcand.as<double>());
+        hipLaunchKernelGGL(rlnk::k_wsel_final, dim3(B), dim3(rlnk::kWBlock), 0, st, wa, wp,
+                           e.d_wcand.as<const double>());
+        if (wa.maxsig > 0) hipLaunchKernelGGL(rlnk::k_wsel_maxsig, dim3(B), dim3(rlnk::kWBlock), 0, st, wa, wp);
+        hipLaunchKernelGGL(rlnk::k_wseg_mask, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp,
+                           e.d_segbest.as<rlnk::WSegBest>(), e.d_ocnt.as<int>(), e.d_npairs.as<int>() + cB);
+        hipLaunchKernelGGL(rlnk::k_wsel_stats, dim3(gB), dim3(rlnk::kBlock), 0, st, (const rlnk::WPart *)wp,
+                           e.d_seg_first.as<const int>(), e.d_segbest.as<const rlnk::WSegBest>(), B, 0,
+                           e.d_stats_c.as<rlnk::PassStats>(), e.d_npairs.as<int>());
+        RLN_TRY(launch_check("coarse weights"));
+    }
     HIP_TRY(hipEventRecord(e.ev[kStJobs], st));
     // ---------------------------------------------------------------- fine pass, planned on the device
     // per-particle counts; everything after runs from device-side counts (FinePlan) without host round trips,
     // as persistent kernels, captured once into a graph and replayed for every chunk
-    RLN_TRY(ensure_fine_capacity(e));
-    hipLaunchKernelGGL(rlnk::k_count_jobs, dim3(B), dim3(rlnk::kWBlock), 0, st, e.d_mask.as<const unsigned char>(),
-                       d_d2off.as<const long long>(), e.d_or_cnt.as<const int>(), NT, e.d_npairs.as<int>(),
-                       e.d_npairs.as<int>() + e.cap_B);
-    RLN_TRY(launch_check("k_count_jobs"));
+    // (per-particle pair and job counts were counted by k_wseg_mask / k_wsel_stats)
     e.cp_host.fimg = dX;
     e.cp_host.fimg_nomask = dXn;
     e.cp_host.fctf = dC;
@@ -903,6 +974,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         L.mask = e.d_mask.as<const unsigned char>();
         L.d2off = d_d2off.as<const long long>();
         L.or_cnt = e.d_or_cnt.as<const int>();
+        L.or_off = e.d_or_off.as<const int>();
+        L.ocnt = e.d_ocnt.as<const int>();
         L.NT = NT;
         L.jobs = e.d_jobs.as<rlnk::FineJob>();
         L.items_s = e.d_items_s.as<int2>();
@@ -1074,9 +1147,10 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     const clk::time_point t_epi = clk::now();
     for (int i = 0; i < kStCount; ++i) {
         float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e.ev[i], e.ev[i + 1]);
+        if (hipEventElapsedTime(&ms, e.ev[i], e.ev[i + 1]) != hipSuccess) ms = 0.f;
         e.stage_ms[i] += ms;
     }
+    (void)hipGetLastError();  // a stage without its events leaves no sticky error for the next launch check
     e.counters[0] += B;
     e.counters[1] += h.total_entries;
     e.counters[2] = (long long)Pc;
@@ -1302,9 +1376,12 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
                            &e->d_njobs, &e->d_nchunks, &e->d_pair_off, &e->d_job_off, &e->d_chunk_off, &e->d_pairs,
                            &e->d_jobs, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fweight,
                            &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc,
-                           &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w};
-    if (e->fine_exec) (void)hipGraphExecDestroy(e->fine_exec);
-    if (e->fine_graph) (void)hipGraphDestroy(e->fine_graph);
+                           &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w, &e->d_wsegs, &e->d_seg_first,
+                           &e->d_wpart, &e->d_hcnt, &e->d_hsum, &e->d_wcand, &e->d_segbest, &e->d_ocnt};
+    for (int q = 0; q < 3; ++q) {
+        if (e->fine_exec[q]) (void)hipGraphExecDestroy(e->fine_exec[q]);
+        if (e->fine_graph[q]) (void)hipGraphDestroy(e->fine_graph[q]);
+    }
     for (rln::DevBuf *b : bufs) b->release();
     for (int i = 0; i <= rln::kStCount; ++i)
         if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
@@ -1365,6 +1442,22 @@ extern "C" int rln_estep_set_model(rln_estep *e, rln_projector *const *projector
     RLN_TRY(rln::engine_model_tables(*e));
     e->coarse_valid = false;
     e->model_set = true;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_use_graphs(rln_estep *e, int enable) {
+    if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_use_graphs: null engine");
+    std::lock_guard<std::mutex> lk(e->mtx);
+    e->graph_off = !enable;
+    if (!enable) {
+        for (int q = 0; q < 3; ++q) {
+            if (e->fine_exec[q]) (void)hipGraphExecDestroy(e->fine_exec[q]);
+            if (e->fine_graph[q]) (void)hipGraphDestroy(e->fine_graph[q]);
+            e->fine_exec[q] = nullptr;
+            e->fine_graph[q] = nullptr;
+        }
+        e->fine_key.clear();
+    }
     return RLN_OK;
 }
 

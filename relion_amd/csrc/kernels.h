@@ -925,6 +925,430 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
 }
 
 // --------------------------------------------------------------------------------------------------
+// Coarse weights over segments: the same results as k_weights (min, FP64 weights, sum, significance
+// threshold, mask, max weight), with every pass over the entries split into blocks of kSegEntries, so that a
+// particle with millions of coarse poses (an exhaustive search) is spread over the whole GPU. The threshold is
+// found by a multi-block radix selection over the weights' bit patterns (8-bit digits from the top; positive
+// doubles order like their uint64 patterns) until the crossing digit bin holds at most kGather elements, which
+// are then sorted in LDS and scanned exactly.
+// --------------------------------------------------------------------------------------------------
+constexpr long long kSegEntries = 32768;  // entries per segment block
+constexpr int kGather = 4096;             // crossing-bin elements sorted exactly
+
+struct WSeg {
+    int b;             // particle
+    int pad_;
+    long long begin;   // entry range [begin, end) within the particle
+    long long end;
+};
+
+struct WPart {
+    unsigned int mn_bits;   // min diff2 (non-negative float bits)
+    int status;
+    double sum;             // sum of weights
+    double target;          // adaptive_fraction * sum
+    unsigned long long prefix, mask;  // radix selection state
+    double above_sum;       // weights in digit bins above the crossing bin
+    long long above_cnt;
+    int gather;             // 1: crossing bin small enough, elements gathered
+    int ncand;
+    double thr;             // significance threshold
+    long long nr_sign;
+    long long n_above;      // entries >= thr
+};
+
+struct WSegBest {
+    double w;
+    long long key, idx;
+};
+
+__global__ void __launch_bounds__(kBlock) k_wpart_init(WPart *__restrict__ part, int B) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B) return;
+    WPart p{};
+    p.mn_bits = 0x7f800000u;  // +inf
+    part[b] = p;
+}
+
+__global__ void __launch_bounds__(kBlock) k_wseg_min(WeightArgs a, const WSeg *__restrict__ segs,
+                                                     WPart *__restrict__ part) {
+    __shared__ float red[kBlock / 64];
+    const WSeg sg = segs[blockIdx.x];
+    const float *d2 = a.diff2 + a.d2_off[sg.b];
+    float mn = 3.0e38f;
+    for (long long e = sg.begin + threadIdx.x; e < sg.end; e += kBlock) {
+        const float v = d2[e];
+        if (v >= 0.f) mn = fminf(mn, v);
+    }
+    for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) mn = fminf(mn, red[w]);
+        if (mn < 3.0e38f) atomicMin(&part[sg.b].mn_bits, __float_as_uint(mn));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_wseg_exp(WeightArgs a, const WSeg *__restrict__ segs,
+                                                     WPart *__restrict__ part) {
+    __shared__ double red[kBlock / 64];
+    const WSeg sg = segs[blockIdx.x];
+    const int b = sg.b;
+    const float *d2 = a.diff2 + a.d2_off[b];
+    double *w = a.weight + a.d2_off[b];
+    const double mind = (double)__uint_as_float(part[b].mn_bits);
+    double sum = 0.;
+    for (long long e = sg.begin + threadIdx.x; e < sg.end; e += kBlock) {
+        const float v = d2[e];
+        double wt = 0.;
+        if (v >= 0.f) {
+            const double diff = (double)v - mind;
+            wt = entry_prior(a, b, e);
+            wt = (diff > 700.) ? 0. : wt * exp(-diff);
+        }
+        w[e] = wt;
+        sum += wt;
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kBlock / 64; ++i) sum += red[i];
+        atomicAdd(&part[b].sum, sum);
+    }
+}
+
+// per particle: status and selection start
+__global__ void __launch_bounds__(kBlock) k_wsel_start(WeightArgs a, WPart *__restrict__ part, int B) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B) return;
+    WPart &p = part[b];
+    if (!(p.sum > 0.)) {
+        p.status = -6;  // zero or NaN sum (src/ml_optimiser.cpp:8054-8104)
+        p.gather = 2;   // no selection
+        return;
+    }
+    p.target = a.adaptive_fraction * p.sum;
+}
+
+// one radix pass: per-particle digit histograms (count, sum) of the weights that match the prefix
+__global__ void __launch_bounds__(kBlock) k_wseg_hist(WeightArgs a, const WSeg *__restrict__ segs,
+                                                      const WPart *__restrict__ part, int shift,
+                                                      unsigned int *__restrict__ hcnt, double *__restrict__ hsum) {
+    __shared__ unsigned int c[256];
+    __shared__ double sm[256];
+    const WSeg sg = segs[blockIdx.x];
+    const WPart p = part[sg.b];
+    if (p.gather) return;
+    c[threadIdx.x] = 0;
+    sm[threadIdx.x] = 0.;
+    __syncthreads();
+    const double *w = a.weight + a.d2_off[sg.b];
+    for (long long e = sg.begin + threadIdx.x; e < sg.end; e += kBlock) {
+        const double v = w[e];
+        if (!(v > 0.)) continue;
+        const unsigned long long key = (unsigned long long)__double_as_longlong(v);
+        if ((key & p.mask) != p.prefix) continue;
+        const int bin = (int)((key >> shift) & 255ULL);
+        atomicAdd(&c[bin], 1u);
+        atomicAdd(&sm[bin], v);
+    }
+    __syncthreads();
+    if (c[threadIdx.x]) {
+        atomicAdd(&hcnt[(size_t)sg.b * 256 + threadIdx.x], c[threadIdx.x]);
+        atomicAdd(&hsum[(size_t)sg.b * 256 + threadIdx.x], sm[threadIdx.x]);
+    }
+}
+
+// per particle: the digit bin where the descending cumulative sum crosses the target; histograms cleared
+__global__ void __launch_bounds__(kBlock) k_wsel_pick(WPart *__restrict__ part, int B, int shift,
+                                                      unsigned int *__restrict__ hcnt, double *__restrict__ hsum) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B) return;
+    WPart &p = part[b];
+    unsigned int *hc = hcnt + (size_t)b * 256;
+    double *hs = hsum + (size_t)b * 256;
+    if (!p.gather) {
+        double acc = p.above_sum;
+        long long cnt = p.above_cnt;
+        int bin = -1;
+        for (int k = 255; k >= 0; --k) {
+            if (hc[k] == 0) continue;
+            if (acc + hs[k] > p.target) {
+                bin = k;
+                break;
+            }
+            acc += hs[k];
+            cnt += hc[k];
+        }
+        if (bin < 0) {  // rounding: the total never exceeds the target; take the smallest present bin
+            for (int k = 0; k < 256; ++k)
+                if (hc[k]) {
+                    bin = k;
+                    break;
+                }
+            acc -= hs[bin];
+            cnt -= hc[bin];
+        }
+        p.above_sum = acc;
+        p.above_cnt = cnt;
+        p.prefix |= ((unsigned long long)bin << shift);
+        p.mask |= (255ULL << shift);
+        if (hc[bin] <= (unsigned)kGather || shift == 0) p.gather = 1;
+    }
+    for (int k = 0; k < 256; ++k) {
+        hc[k] = 0;
+        hs[k] = 0.;
+    }
+}
+
+// elements of the crossing bin (weights matching the prefix) into the per-particle candidate buffer
+__global__ void __launch_bounds__(kBlock) k_wseg_gather(WeightArgs a, const WSeg *__restrict__ segs,
+                                                        WPart *__restrict__ part, double *__restrict__ cand) {
+    const WSeg sg = segs[blockIdx.x];
+    const WPart p = part[sg.b];
+    if (p.gather != 1) return;
+    const double *w = a.weight + a.d2_off[sg.b];
+    for (long long e = sg.begin + threadIdx.x; e < sg.end; e += kBlock) {
+        const double v = w[e];
+        if (!(v > 0.)) continue;
+        const unsigned long long key = (unsigned long long)__double_as_longlong(v);
+        if ((key & p.mask) != p.prefix) continue;
+        const int slot = atomicAdd(&part[sg.b].ncand, 1);
+        if (slot < kGather) cand[(size_t)sg.b * kGather + slot] = v;
+    }
+}
+
+// per particle (one block): exact threshold from the sorted crossing-bin elements, maxsig cap by rank
+__global__ void __launch_bounds__(kWBlock) k_wsel_final(WeightArgs a, WPart *__restrict__ part,
+                                                        const double *__restrict__ cand) {
+    __shared__ double s_c[kGather];
+    __shared__ long long s_rank;
+    __shared__ int s_ok;
+    const int b = blockIdx.x, tid = threadIdx.x;
+    WPart &p = part[b];
+    if (p.gather == 2) return;
+    const int nc = min(p.ncand, kGather);
+    for (int i = tid; i < kGather; i += kWBlock) s_c[i] = (i < nc) ? cand[(size_t)b * kGather + i] : -1.;
+    __syncthreads();
+    for (int ks = 2; ks <= kGather; ks <<= 1)  // bitonic sort, ascending
+        for (int j = ks >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < kGather; i += kWBlock) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const double x = s_c[i], y = s_c[l];
+                    if (((i & ks) == 0) ? (x > y) : (x < y)) {
+                        s_c[i] = y;
+                        s_c[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // descending cumulative sum from above_sum, 64 elements per step (wave 0)
+    if (tid < 64) {
+        double base = p.above_sum;
+        int crossed = 0;
+        long long rank = nc - 1;
+        for (int r0 = 0; r0 < nc; r0 += 64) {
+            const int r = r0 + tid;
+            double x = (r < nc) ? s_c[kGather - 1 - r] : 0.;
+            for (int o = 1; o < 64; o <<= 1) {
+                const double y = __shfl_up(x, o);
+                if (tid >= o) x += y;
+            }
+            const double cum = base + x;
+            const unsigned long long hit = __ballot(r < nc && cum > p.target);
+            if (hit) {
+                crossed = 1;
+                rank = r0 + (__ffsll((long long)hit) - 1);
+                break;
+            }
+            base = __shfl(cum, 63);
+        }
+        if (tid == 0) {
+            s_rank = rank;
+            s_ok = crossed;
+        }
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double thr;
+    long long nr_sign;
+    if (p.ncand <= kGather && nc > 0) {
+        // rank within the bin; without a crossing (rounding) the smallest element, as the reference's loop ends
+        const long long rank = s_ok ? s_rank : (long long)nc - 1;
+        thr = s_c[kGather - 1 - rank];
+        nr_sign = p.above_cnt + rank + 1;
+    } else {
+        // all 64 bits fixed and more than kGather equal elements: the bin is one value
+        thr = __longlong_as_double((long long)p.prefix);
+        long long m = (long long)floor((p.target - p.above_sum) / thr) + 1;
+        if (m < 1) m = 1;
+        nr_sign = p.above_cnt + m;
+    }
+    p.thr = thr;
+    p.nr_sign = nr_sign;
+}
+
+// maxsig cap (src/ml_optimiser.cpp:8147-8157): the maxsig-th largest weight becomes the threshold. One block
+// per capped particle, radix selection by counts over the particle's entries.
+__global__ void __launch_bounds__(kWBlock) k_wsel_maxsig(WeightArgs a, WPart *__restrict__ part) {
+    __shared__ unsigned int hc[256];
+    __shared__ unsigned long long s_prefix, s_mask;
+    __shared__ long long s_cnt;
+    const int b = blockIdx.x, tid = threadIdx.x;
+    WPart &p = part[b];
+    if (p.gather == 2 || a.maxsig <= 0 || p.nr_sign <= a.maxsig) return;
+    const long long n = a.n_entries[b];
+    const double *w = a.weight + a.d2_off[b];
+    if (tid == 0) {
+        s_prefix = 0ULL;
+        s_mask = 0ULL;
+        s_cnt = 0;
+    }
+    __syncthreads();
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 256; i += kWBlock) hc[i] = 0;
+        __syncthreads();
+        const unsigned long long pre = s_prefix, msk = s_mask;
+        for (long long e = tid; e < n; e += kWBlock) {
+            const double v = w[e];
+            if (!(v > 0.)) continue;
+            const unsigned long long key = (unsigned long long)__double_as_longlong(v);
+            if ((key & msk) != pre) continue;
+            atomicAdd(&hc[(int)((key >> shift) & 255ULL)], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            long long cnt = s_cnt;
+            int bin = 0;
+            for (int k = 255; k >= 0; --k) {
+                if (cnt + (long long)hc[k] >= a.maxsig) {
+                    bin = k;
+                    break;
+                }
+                cnt += hc[k];
+            }
+            s_cnt = cnt;
+            s_prefix |= ((unsigned long long)bin << shift);
+            s_mask |= (255ULL << shift);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        p.thr = __longlong_as_double((long long)s_prefix);
+        p.nr_sign = a.maxsig;
+    }
+}
+
+// mask, count above threshold, and each segment's max weight with the wsum-order tie break
+// Also counts the significant translations of every orientation entry (ocnt) and, per particle, the orientations
+// with at least one (njobs): the inputs of the fine-pass job lists.
+__global__ void __launch_bounds__(kBlock) k_wseg_mask(WeightArgs a, const WSeg *__restrict__ segs,
+                                                      WPart *__restrict__ part, WSegBest *__restrict__ best,
+                                                      int *__restrict__ ocnt, int *__restrict__ njobs) {
+    __shared__ double s_bw[kBlock / 64];
+    __shared__ long long s_bk[kBlock / 64], s_bi[kBlock / 64], s_ab[kBlock / 64];
+    const WSeg sg = segs[blockIdx.x];
+    const int b = sg.b;
+    const WPart p = part[b];
+    const double thr = p.thr;
+    const bool ok = p.gather != 2;
+    const double *w = a.weight + a.d2_off[b];
+    long long above = 0;
+    double bw = -1.;
+    long long bkey = 0x7fffffffffffffffLL, bidx = -1;
+    for (long long e = sg.begin + threadIdx.x; e < sg.end; e += kBlock) {
+        const double v = w[e];
+        const bool sig = ok && v >= thr && v > 0.;
+        a.mask[a.d2_off[b] + e] = sig ? 1 : 0;
+        above += sig ? 1 : 0;
+        if (sig) {
+            const int o = (int)(e / a.NT);
+            if (atomicAdd(&ocnt[a.or_off[b] + o], 1) == 0) atomicAdd(&njobs[b], 1);
+        }
+        if (v > 0.) {
+            const long long key = wsum_order_key(a, b, e);
+            if (v > bw || (v == bw && key < bkey)) {
+                bw = v;
+                bkey = key;
+                bidx = e;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        above += __shfl_xor(above, o);
+        const double ow = __shfl_xor(bw, o);
+        const long long ok2 = __shfl_xor(bkey, o), oi = __shfl_xor(bidx, o);
+        if (ow > bw || (ow == bw && ok2 < bkey)) {
+            bw = ow;
+            bkey = ok2;
+            bidx = oi;
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_bw[wv] = bw;
+        s_bk[wv] = bkey;
+        s_bi[wv] = bidx;
+        s_ab[wv] = above;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kBlock / 64; ++i) {
+            above += s_ab[i];
+            if (s_bw[i] > bw || (s_bw[i] == bw && s_bk[i] < bkey)) {
+                bw = s_bw[i];
+                bkey = s_bk[i];
+                bidx = s_bi[i];
+            }
+        }
+        if (above) atomicAdd((unsigned long long *)&part[b].n_above, (unsigned long long)above);
+        best[blockIdx.x] = WSegBest{bw, bkey, bidx};
+    }
+}
+
+// per particle: PassStats from the particle state and its segments' maxima
+__global__ void __launch_bounds__(kBlock) k_wsel_stats(const WPart *__restrict__ part, const int *__restrict__ seg_first,
+                                                       const WSegBest *__restrict__ best, int B, int fine,
+                                                       PassStats *__restrict__ stats, int *__restrict__ npairs) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= B) return;
+    const WPart p = part[b];
+    npairs[b] = (int)p.n_above;
+    PassStats st;
+    st.min_diff2 = (double)__uint_as_float(p.mn_bits);
+    st.sum_weight = p.sum;
+    st.threshold = p.thr;
+    st.nr_sign = p.nr_sign;
+    st.n_above = p.n_above;
+    st.status = p.status;
+    st.pad_ = 0;
+    double bw = -1.;
+    long long bkey = 0x7fffffffffffffffLL, bidx = -1;
+    for (int s = seg_first[b]; s < seg_first[b + 1]; ++s) {
+        const WSegBest q = best[s];
+        if (q.w > bw || (q.w == bw && q.key < bkey)) {
+            bw = q.w;
+            bkey = q.key;
+            bidx = q.idx;
+        }
+    }
+    st.max_weight = p.status ? 0. : bw;
+    st.best_key = p.status ? -1 : bkey;
+    st.best_index = p.status ? -1 : bidx;
+    if (!p.status) {
+        if (!fine && p.nr_sign == 0) st.status = -5;
+    } else {
+        st.threshold = 0.;
+        st.nr_sign = 0;
+        st.n_above = 0;
+    }
+    stats[b] = st;
+}
+
+// --------------------------------------------------------------------------------------------------
 // Fine-pass job lists from the coarse mask (the reference builds them on the host:
 // makeJobsForDiff2Fine, src/acc/acc_helper_functions_impl.h:26-102).
 // A job is one coarse orientation with >= 1 significant translation; its pairs are consecutive in
@@ -966,28 +1390,6 @@ __device__ __forceinline__ int block_exclusive_scan_i(int v, int *tmp, int &tota
     return r;
 }
 
-// Counts per particle: significant pairs and jobs. grid (B), block 1024.
-__global__ void __launch_bounds__(kWBlock) k_count_jobs(const unsigned char *__restrict__ mask,
-                                                        const long long *__restrict__ d2_off,
-                                                        const int *__restrict__ or_cnt, int NT,
-                                                        int *__restrict__ n_pairs, int *__restrict__ n_jobs) {
-    __shared__ long long red[16];
-    const int b = blockIdx.x;
-    const unsigned char *m = mask + d2_off[b];
-    long long np = 0, nj = 0;
-    for (int o = threadIdx.x; o < or_cnt[b]; o += kWBlock) {
-        int c = 0;
-        for (int t = 0; t < NT; ++t) c += m[(size_t)o * NT + t];
-        np += c;
-        nj += (c > 0);
-    }
-    np = block_sum_ll(np, red);
-    nj = block_sum_ll(nj, red);
-    if (threadIdx.x == 0) {
-        n_pairs[b] = (int)np;
-        n_jobs[b] = (int)nj;
-    }
-}
 
 // Offsets of the chunk's fine pass from the per-particle counts: exclusive scans over particles of pairs and
 // jobs, fine-weight entry counts and offsets, totals and the capacity check. One block of 1024.
@@ -1024,10 +1426,12 @@ __global__ void __launch_bounds__(kWBlock) k_fine_plan(const int *__restrict__ n
     }
 }
 
-// Writes pairs and jobs at the offsets computed from k_count_jobs. grid (B), block 1024.
+// Writes pairs and jobs at the offsets computed by k_fine_plan from the per-orientation significant counts
+// (ocnt, counted by k_wseg_mask). grid (B), block 1024.
 __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__restrict__ mask,
                                                         const long long *__restrict__ d2_off,
-                                                        const int *__restrict__ or_cnt, int NT,
+                                                        const int *__restrict__ or_cnt, const int *__restrict__ or_off,
+                                                        const int *__restrict__ ocnt, int NT,
                                                         const long long *__restrict__ pair_off,
                                                         const int *__restrict__ job_off, int *__restrict__ pairs,
                                                         FineJob *__restrict__ jobs, const FinePlan *__restrict__ plan) {
@@ -1039,9 +1443,7 @@ __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__r
     const int n_o = or_cnt[b];
     for (int o0 = 0; o0 < n_o; o0 += kWBlock) {
         const int o = o0 + threadIdx.x;
-        int c = 0;
-        if (o < n_o)
-            for (int t = 0; t < NT; ++t) c += m[(size_t)o * NT + t];
+        const int c = (o < n_o) ? ocnt[or_off[b] + o] : 0;  // significant translations of orientation o
         int tot_p, tot_j;
         const int ps = block_exclusive_scan_i(c, tmp, tot_p);
         const int js = block_exclusive_scan_i(c > 0 ? 1 : 0, tmp, tot_j);

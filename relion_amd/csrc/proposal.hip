@@ -96,11 +96,31 @@ int gemm(rln_proposal &p, const rlnp::bf16 *A, long long sam, long long sak, con
 
 const rlnp::bf16 *wbf(const rln_proposal &p, size_t off) { return p.wb.as<const rlnp::bf16>() + off; }
 
-// logits of the n images in x[0]
-int forward(rln_proposal &p, int n) {
+template <int NT>
+void conv_fwd_t(rln_proposal &p, const ConvLayer &L, int n) {
+    const long long M = (long long)n * L.Ho * L.Ho;
+    hipLaunchKernelGGL((rlnp::k_conv_fwd<NT>), dim3((unsigned)((M + 63) / 64)), dim3(rlnp::kB), 0, p.stream,
+                       p.x[&L - p.conv].as<const rlnp::bf16>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp,
+                       p.wb.as<const rlnp::bf16>() + L.w_off, p.w.as<const float>() + L.b_off,
+                       p.x[&L - p.conv + 1].as<rlnp::bf16>());
+}
+
+// logits of the n images in x[0]; with keep_col the im2col buffers are written (the backward pass needs them),
+// otherwise the convolutions run as implicit GEMMs straight from the activations
+int forward(rln_proposal &p, int n, bool keep_col = true) {
     for (int l = 0; l < kLayers; ++l) {
         const ConvLayer &L = p.conv[l];
         const long long M = (long long)n * L.Ho * L.Ho;
+        if (!keep_col && L.cout % 16 == 0 && L.cout <= 64) {
+            switch (L.cout / 16) {
+                case 1: conv_fwd_t<1>(p, L, n); break;
+                case 2: conv_fwd_t<2>(p, L, n); break;
+                case 3: conv_fwd_t<3>(p, L, n); break;
+                default: conv_fwd_t<4>(p, L, n); break;
+            }
+            HIP_TRY(hipGetLastError());
+            continue;
+        }
         const long long threads = (L.cin % 8 == 0) ? M * 9 * (L.cin / 8) : M * L.Kp;
         hipLaunchKernelGGL(rlnp::k_im2col, dim3(grid1(threads)), dim3(rlnp::kB), 0, p.stream,
                            p.x[l].as<const rlnp::bf16>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp, p.col[l].as<rlnp::bf16>());
@@ -537,7 +557,7 @@ extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, c
         const int m = std::min(chunk, n - i0);
         RLN_TRY(particle_images(*p, m, fimg + 2 * per * i0, fctf ? fctf + per * i0 : nullptr, current_size, on_device,
                                 p->x[0].as<rlnp::bf16>(), nullptr));
-        RLN_TRY(forward(*p, m));
+        RLN_TRY(forward(*p, m, false));
         hipLaunchKernelGGL(rlnp::k_topk, dim3(m), dim3(rlnp::kB), sizeof(float) * p->NB, p->stream,
                            p->logits.as<const float>(), p->NB, topk, d_bins.as<int>(), d_probs.as<float>());
         HIP_TRY(hipGetLastError());

@@ -370,6 +370,78 @@ __global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
 }
 
 // --------------------------------------------------------------------------------------------------
+// Forward 3x3 convolution as an implicit GEMM (inference): the A operand of each MFMA (8 consecutive k =
+// 8 channels of one tap of one output pixel) is read straight from the NHWC input, so no im2col buffer is
+// written or read. Block = 4 waves x 16 output pixels x all Cout (Cout / 16 MFMA column tiles, <= 4).
+// Cin % 8 == 0, or Cin == 1 (then k = tap, 9 taps padded to Kp = 16 in the weights).
+// --------------------------------------------------------------------------------------------------
+template <int NT>  // column tiles of 16 output channels
+__global__ void __launch_bounds__(kB) k_conv_fwd(const bf16 *__restrict__ X, int n, int H, int Cin, int stride, int Ho,
+                                                 int Kp, const bf16 *__restrict__ Wt, const float *__restrict__ bias,
+                                                 bf16 *__restrict__ Y) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, gq = lane >> 4;
+    const long long M = (long long)n * Ho * Ho;
+    const long long m = ((long long)blockIdx.x * 4 + wave) * 16 + r16;  // this lane's output pixel (A row)
+    int img = 0, oy = 0, ox = 0;
+    const bool mok = m < M;
+    if (mok) {
+        img = (int)(m / (Ho * Ho));
+        const int pix = (int)(m - (long long)img * Ho * Ho);
+        oy = pix / Ho;
+        ox = pix - oy * Ho;
+    }
+    floatx4_t acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    const int Kloop = (Cin == 1) ? 32 : Kp;  // Cin == 1: one K step of 32, k >= 9 are zeros
+    for (int k0 = 0; k0 < Kloop; k0 += 32) {
+        const int k = k0 + 8 * gq;
+        uint4 av = make_uint4(0u, 0u, 0u, 0u);
+        if (Cin == 1) {
+            bf16 e[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int tap = k + q;
+                bf16 v = 0;
+                if (mok && tap < 9) {
+                    const int iy = oy * stride + tap / 3 - 1, ix = ox * stride + tap % 3 - 1;
+                    if (iy >= 0 && iy < H && ix >= 0 && ix < H) v = X[((size_t)img * H + iy) * H + ix];
+                }
+                e[q] = v;
+            }
+            av = make_uint4((unsigned)e[0] | ((unsigned)e[1] << 16), (unsigned)e[2] | ((unsigned)e[3] << 16),
+                            (unsigned)e[4] | ((unsigned)e[5] << 16), (unsigned)e[6] | ((unsigned)e[7] << 16));
+        } else if (mok) {
+            const int tap = k / Cin, c = k - tap * Cin;
+            const int iy = oy * stride + tap / 3 - 1, ix = ox * stride + tap % 3 - 1;
+            if (iy >= 0 && iy < H && ix >= 0 && ix < H)
+                av = *reinterpret_cast<const uint4 *>(X + (((size_t)img * H + iy) * H + ix) * Cin + c);
+        }
+        const bf16x8_t a = __builtin_bit_cast(bf16x8_t, av);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int col = 16 * j + r16;  // B column: output channel
+            uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+            if (k < Kp) bv = *reinterpret_cast<const uint4 *>(Wt + (size_t)col * Kp + k);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8_t, bv), acc[j], 0, 0, 0);
+        }
+    }
+    // D rows 4*gq + r of the wave's 16 pixels, column r16 of each tile
+    const int Cout = 16 * NT;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + r16;
+        const float bj = bias[col];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long long mm = ((long long)blockIdx.x * 4 + wave) * 16 + 4 * gq + r;
+            if (mm < M) Y[(size_t)mm * Cout + col] = to_bf16(fmaxf(acc[j][r] + bj, 0.f));
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
 // Loss and prediction over NB bins: one block per image.
 // --------------------------------------------------------------------------------------------------
 __device__ __forceinline__ float block_max(float v, float *red) {
