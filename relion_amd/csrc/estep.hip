@@ -411,9 +411,7 @@ static int engine_coarse_projections(Engine &e) {
 struct ChunkHost {
     int B = 0;
     std::vector<int> or_off, or_cnt, or_proj, or_frame, or_class, or_dir;
-    std::vector<int> nd, np;  // per particle: directions, psi
-    std::vector<int> dir_idx, psi_idx;  // flattened, offsets in dir_off/psi_off
-    std::vector<int> dir_off, psi_off;
+    std::vector<int> nd, np;  // per particle: directions, psi (an explicit list counts as nd = its length)
     std::vector<double> or_prior, tr_prior;
     std::vector<long long> d2_off, n_entries;
     std::vector<float> xi2half, scale, scale_w;
@@ -426,6 +424,7 @@ struct ChunkHost {
 
 // set_off/set_og (optional): explicit coarse orientation lists of the chunk's particles (global grid indices
 // idir * n_psi + ipsi, offsets relative to set_off[0]); an empty list means the exhaustive grid.
+// Two passes: sizes per particle, then every particle fills its own slices (OpenMP over particles).
 static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, ChunkHost &h,
                              const int *set_off = nullptr, const int *set_og = nullptr) {
     const rln_estep_config &cfg = e.cfg;
@@ -436,8 +435,6 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     h.or_cnt.resize(B);
     h.nd.resize(B);
     h.np.resize(B);
-    h.dir_off.resize(B);
-    h.psi_off.resize(B);
     h.tr_prior.resize((size_t)B * NT);
     h.d2_off.resize(B);
     h.n_entries.resize(B);
@@ -451,71 +448,91 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     const double offset_norm = 2. * kPi * sqrt(s2) * sqrt(s2);
     std::vector<LocalSet> sets;
     if (cfg.local_search) local_priors_batch(e.grid, cfg, meta, B, sets);
-    std::vector<int> all_d(e.grid.n_dir), all_p(e.grid.n_psi);
-    for (int i = 0; i < e.grid.n_dir; ++i) all_d[i] = i;
-    for (int i = 0; i < e.grid.n_psi; ++i) all_p[i] = i;
-    const std::vector<double> none;
-    long long d2 = 0;
+    const int n_dir = e.grid.n_dir, n_psi = e.grid.n_psi;
+    auto n_set = [&](int b) { return set_off ? set_off[b + 1] - set_off[b] : 0; };
+    // pass 1: sizes
+    for (int b = 0; b < B; ++b) {
+        int nd, np;
+        if (n_set(b) > 0) {
+            nd = n_set(b);
+            np = 1;
+        } else if (cfg.local_search) {
+            nd = (int)sets[b].dir_idx.size();
+            np = (int)sets[b].psi_idx.size();
+        } else {
+            nd = n_dir;
+            np = n_psi;
+        }
+        h.nd[b] = nd;
+        h.np[b] = np;
+        h.or_cnt[b] = K * nd * np;
+    }
+    long long tot_or = 0, d2 = 0;
+    std::vector<int> tile_off(B + 1, 0), seg_off(B + 1, 0);
+    for (int b = 0; b < B; ++b) {
+        h.or_off[b] = (int)tot_or;
+        tot_or += h.or_cnt[b];
+        h.n_entries[b] = (long long)h.or_cnt[b] * NT;
+        h.d2_off[b] = d2;
+        d2 += h.n_entries[b];
+        tile_off[b + 1] = tile_off[b] + (h.or_cnt[b] + rlnk::kCoTO - 1) / rlnk::kCoTO;
+        const long long ns = std::max(1LL, (h.n_entries[b] + rlnk::kSegEntries - 1) / rlnk::kSegEntries);
+        seg_off[b + 1] = seg_off[b] + (int)ns;
+    }
+    h.total_entries = d2;
+    h.or_proj.resize(tot_or);
+    h.or_prior.resize(tot_or);
+    h.or_frame.resize(tot_or);
+    h.or_class.resize(tot_or);
+    h.or_dir.resize(tot_or);
+    h.tiles.resize(tile_off[B]);
+    h.segs.resize(seg_off[B]);
+    h.seg_first.assign(seg_off.begin(), seg_off.end());
+    // pass 2: every particle fills its slices
+#pragma omp parallel for schedule(dynamic, 16)
     for (int b = 0; b < B; ++b) {
         const rln_particle_meta &m = meta[b];
-        const std::vector<int> &di = cfg.local_search ? sets[b].dir_idx : all_d;
-        const std::vector<int> &pi = cfg.local_search ? sets[b].psi_idx : all_p;
-        const std::vector<double> &dp = cfg.local_search ? sets[b].dir_prior : none;
-        const std::vector<double> &pp = cfg.local_search ? sets[b].psi_prior : none;
-        const int n_set = set_off ? set_off[b + 1] - set_off[b] : 0;
-        if (n_set > 0) {
+        const int base = h.or_off[b];
+        int q = base;
+        if (n_set(b) > 0) {
             // explicit list: global-search priors, normalised by their mean over the whole grid so that weights
             // and sums stay those of the exhaustive search restricted to the list
             const int *og_list = set_og + (set_off[b] - set_off[0]);
-            const int base = (int)h.or_proj.size();
-            h.or_off[b] = base;
-            h.or_cnt[b] = K * n_set;
-            h.nd[b] = n_set;
-            h.np[b] = 1;
-            h.dir_off[b] = (int)h.dir_idx.size();
-            h.psi_off[b] = (int)h.psi_idx.size();
             for (int k = 0; k < K; ++k)
-                for (int i = 0; i < n_set; ++i) {
-                    const int og = og_list[i], d = og / e.grid.n_psi;
-                    const double po = e.pdf_direction[(size_t)k * e.grid.n_dir + d] / e.pdf_dir_mean;
+                for (int i = 0; i < n_set(b); ++i, ++q) {
+                    const int og = og_list[i], d = og / n_psi;
+                    const double po = e.pdf_direction[(size_t)k * n_dir + d] / e.pdf_dir_mean;
                     const bool on = po > 0. && e.pdf_class[k] > 0.;
-                    h.or_proj.push_back(on ? k * e.n_or + og : -1);
-                    h.or_prior.push_back(po);
-                    h.or_frame.push_back(og * 8);
-                    h.or_class.push_back(k);
-                    h.or_dir.push_back(d);
+                    h.or_proj[q] = on ? k * e.n_or + og : -1;
+                    h.or_prior[q] = po;
+                    h.or_frame[q] = og * 8;
+                    h.or_class[q] = k;
+                    h.or_dir[q] = d;
                 }
-        }
-        const int ND = n_set > 0 ? 0 : (int)di.size(), NP = n_set > 0 ? 0 : (int)pi.size();
-        if (n_set == 0) {
-        h.nd[b] = ND;
-        h.np[b] = NP;
-        h.dir_off[b] = (int)h.dir_idx.size();
-        h.psi_off[b] = (int)h.psi_idx.size();
-        h.dir_idx.insert(h.dir_idx.end(), di.begin(), di.end());
-        h.psi_idx.insert(h.psi_idx.end(), pi.begin(), pi.end());
-        // orientation priors, normalised by their mean over classes, directions and psi (src/ml_optimiser.cpp:7790-7803)
-        const int base = (int)h.or_proj.size();
-        h.or_off[b] = base;
-        h.or_cnt[b] = K * ND * NP;
-        double mean = 0.;
-        for (int k = 0; k < K; ++k)
-            for (int id = 0; id < ND; ++id)
-                for (int ip = 0; ip < NP; ++ip) {
-                    const double po = cfg.local_search ? dp[id] * pp[ip]
-                                                       : e.pdf_direction[(size_t)k * e.grid.n_dir + di[id]];
-                    mean += po;
-                    const int og = di[id] * e.grid.n_psi + pi[ip];
-                    const bool on = po > 0. && e.pdf_class[k] > 0.;
-                    h.or_proj.push_back(on ? k * e.n_or + og : -1);
-                    h.or_prior.push_back(po);
-                    h.or_frame.push_back(og * 8);
-                    h.or_class.push_back(k);
-                    h.or_dir.push_back(di[id]);
-                }
-        mean /= (double)((long long)K * ND * NP);
-        if (mean != 0.)
-            for (int i = 0; i < K * ND * NP; ++i) h.or_prior[base + i] /= mean;
+        } else {
+            // orientation priors, normalised by their mean over classes, directions and psi
+            // (src/ml_optimiser.cpp:7790-7803)
+            const int ND = h.nd[b], NP = h.np[b];
+            double mean = 0.;
+            for (int k = 0; k < K; ++k)
+                for (int id = 0; id < ND; ++id)
+                    for (int ip = 0; ip < NP; ++ip, ++q) {
+                        const int di = cfg.local_search ? sets[b].dir_idx[id] : id;
+                        const int pi = cfg.local_search ? sets[b].psi_idx[ip] : ip;
+                        const double po = cfg.local_search ? sets[b].dir_prior[id] * sets[b].psi_prior[ip]
+                                                           : e.pdf_direction[(size_t)k * n_dir + di];
+                        mean += po;
+                        const int og = di * n_psi + pi;
+                        const bool on = po > 0. && e.pdf_class[k] > 0.;
+                        h.or_proj[q] = on ? k * e.n_or + og : -1;
+                        h.or_prior[q] = po;
+                        h.or_frame[q] = og * 8;
+                        h.or_class[q] = k;
+                        h.or_dir[q] = di;
+                    }
+            mean /= (double)((long long)K * ND * NP);
+            if (mean != 0.)
+                for (int i = 0; i < K * ND * NP; ++i) h.or_prior[base + i] /= mean;
         }
         // offset priors (src/ml_optimiser.cpp:7805-7872): offsets in pixels plus translations in Angstrom
         double tmean = 0.;
@@ -531,10 +548,13 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         tmean = tmean * K / (double)(K * NT);  // mean over classes x translations (same per class)
         if (tmean > 0.)
             for (int t = 0; t < NT; ++t) h.tr_prior[(size_t)b * NT + t] /= tmean;
-        h.d2_off[b] = d2;
-        h.n_entries[b] = (long long)h.or_cnt[b] * NT;
-        d2 += h.n_entries[b];
-        for (int o = 0; o < h.or_cnt[b]; o += rlnk::kCoTO) h.tiles.push_back(rlnk::CoarseTile{b, o});
+        int ti = tile_off[b];
+        for (int o = 0; o < h.or_cnt[b]; o += rlnk::kCoTO) h.tiles[ti++] = rlnk::CoarseTile{b, o};
+        const long long n = h.n_entries[b];
+        for (int sI = seg_off[b]; sI < seg_off[b + 1]; ++sI) {
+            const long long qs = sI - seg_off[b];
+            h.segs[sI] = rlnk::WSeg{b, 0, qs * rlnk::kSegEntries, std::min(n, (qs + 1) * rlnk::kSegEntries)};
+        }
         h.xi2half[b] = (float)(m.highres_Xi2 / 2.);
         const double sc = cfg.do_scale_correction ? m.scale : 1.;
         h.scale[b] = (float)sc;
@@ -544,18 +564,20 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         h.prior_off[2 * b] = m.prior_offset_x;
         h.prior_off[2 * b + 1] = m.prior_offset_y;
     }
-    h.total_entries = d2;
-    h.seg_first.assign(B + 1, 0);
-    for (int b = 0; b < B; ++b) {
-        h.seg_first[b] = (int)h.segs.size();
-        const long long n = h.n_entries[b];
-        const long long ns = std::max(1LL, (n + rlnk::kSegEntries - 1) / rlnk::kSegEntries);
-        for (long long q = 0; q < ns; ++q)
-            h.segs.push_back(rlnk::WSeg{b, 0, q * rlnk::kSegEntries, std::min(n, (q + 1) * rlnk::kSegEntries)});
+    // Global searches enumerate the same grid for every particle: order the coarse tiles by orientation block
+    // first, so that blocks running together read the same projection rows (L2 hits) instead of each particle
+    // streaming the whole grid from HBM. Tiles of particles searched over the whole grid (all of them, or the
+    // exhaustive fallbacks among explicit lists) come first, by orientation block; tiles of explicit lists after
+    // them in particle order.
+    if (!cfg.local_search) {
+        auto full = [&](const rlnk::CoarseTile &t) { return !set_off || set_off[t.b + 1] == set_off[t.b]; };
+        std::stable_sort(h.tiles.begin(), h.tiles.end(), [&](const rlnk::CoarseTile &a, const rlnk::CoarseTile &b) {
+            const bool fa = full(a), fb = full(b);
+            if (fa != fb) return fa;
+            return fa && a.o_begin < b.o_begin;
+        });
     }
-    h.seg_first[B] = (int)h.segs.size();
 }
-
 
 // ------------------------------------------------------------------------------------------------
 // Fine pass planned on the device (persistent kernels, one captured graph)

@@ -1,6 +1,11 @@
-"""Summarise a rocprofv3 kernel-trace database (rocpd SQLite) into a short per-kernel table.
+"""Summarise rocprofv3 databases (rocpd SQLite) into short per-kernel tables.
 
-Usage: python -m relion_amd.profile_summary <results.db> [--command "..."] > summary.md
+Kernel trace:  python relion_amd/profile_summary.py <kt_results.db> [--command "..."] > summary.md
+HBM traffic:   python relion_amd/profile_summary.py --fetch <fetch.db> --write <write.db> [--fetch-scale 2.0]
+               [--write-scale 1.0] [--command "..."] > traffic.md
+The traffic mode averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, multiplies them by the
+calibration scales (what the counters read for a device-to-device copy of known size) and reports GB per
+launch and the resulting bandwidth over the dispatch durations of the counter runs.
 """
 import argparse
 import sqlite3
@@ -15,22 +20,64 @@ def summarise(db):
     return rows, total
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("db")
-    ap.add_argument("--command", default="")
-    ap.add_argument("--top", type=int, default=12)
-    a = ap.parse_args()
+def counter(db, name):
+    """{kernel: (dispatches, mean counter value, mean duration ns)} of one PMC counter."""
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select name, count(*), avg(counter_value), avg(duration) from pmc_events where counter_name = ? "
+        "group by name", (name,)).fetchall()
+    return {r[0]: (r[1], r[2], r[3]) for r in rows}
+
+
+def short(name):
+    return name.split("(")[0]
+
+
+def kernel_table(a):
     rows, total = summarise(a.db)
     if a.command:
         print(f"command: `{a.command}`\n")
     print("| kernel | calls | total ms | avg us | min us | max us | % | VGPR | LDS B | scratch B |")
     print("|---|---|---|---|---|---|---|---|---|---|")
     for name, n, tot, avg, mn, mx, vgpr, lds, scr in rows[:a.top]:
-        short = name.split("(")[0]
-        print(f"| {short} | {n} | {tot / 1e6:.2f} | {avg / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | "
+        print(f"| {short(name)} | {n} | {tot / 1e6:.2f} | {avg / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | "
               f"{100.0 * tot / total:.1f} | {vgpr} | {lds} | {scr} |")
     print(f"\ntotal kernel time: {total / 1e6:.2f} ms")
+
+
+def traffic_table(a):
+    f = counter(a.fetch, "FETCH_SIZE")
+    w = counter(a.write, "WRITE_SIZE")
+    if a.command:
+        print(f"command: `{a.command}`\n")
+    print(f"FETCH_SIZE x {a.fetch_scale:.3f}, WRITE_SIZE x {a.write_scale:.3f} (copy calibration)\n")
+    print("| kernel | launches | avg us | fetch GB/launch | write GB/launch | GB/s |")
+    print("|---|---|---|---|---|---|")
+    rows = []
+    for name, (n, fk, dur) in f.items():
+        wk = w.get(name, (0, 0.0, dur))[1]
+        fb = fk * 1024.0 * a.fetch_scale
+        wb = wk * 1024.0 * a.write_scale
+        rows.append((n * dur, name, n, dur, fb, wb))
+    for _, name, n, dur, fb, wb in sorted(rows, reverse=True)[:a.top]:
+        bw = (fb + wb) / (dur * 1e-9) / 1e9 if dur else 0.0
+        print(f"| {short(name)} | {n} | {dur / 1e3:.1f} | {fb / 1e9:.3f} | {wb / 1e9:.3f} | {bw:.0f} |")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("db", nargs="?")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--fetch-scale", type=float, default=2.0)
+    ap.add_argument("--write-scale", type=float, default=1.0)
+    ap.add_argument("--command", default="")
+    ap.add_argument("--top", type=int, default=12)
+    a = ap.parse_args()
+    if a.fetch and a.write:
+        traffic_table(a)
+    else:
+        kernel_table(a)
 
 
 if __name__ == "__main__":

@@ -3,7 +3,8 @@
 Three E-step runs replay the captured graphs (with host work and synchronisation between the runs); each is
 compared with a run that launches the same kernels directly: identical max-likelihood poses, diff2 and fine
 weights; sums of weights within 1e-12 relative (FP64 atomics add in a different order); backprojected volumes
-within 1e-5 relative (FP32 atomics).
+within 1e-5 relative (FP32 atomics); the noise power per shell within 1e-6 relative (each item sums its pixels
+into FP32 shared-memory bins with atomics, whose order differs between runs).
 """
 import numpy as np
 import pytest
@@ -48,5 +49,5 @@ def test_graph_replays_match_direct_launches():
             assert np.allclose(r0[f], r1[f], rtol=1e-12, atol=0), f
         for a, b in zip(v0, v1):
             assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a)
-        assert np.allclose(w0["sigma2_noise"], w1["sigma2_noise"], rtol=1e-9)
+        assert np.allclose(w0["sigma2_noise"], w1["sigma2_noise"], rtol=1e-6, atol=0)
     es.close()
