@@ -101,6 +101,22 @@ class ProposalGen(C.Structure):
                 ("sigma2_noise", C.POINTER(C.c_double))]
 
 
+class ImgPrepConfig(C.Structure):
+    _fields_ = [("ori_size", C.c_int), ("current_size", C.c_int), ("pixel_size", C.c_double),
+                ("particle_diameter", C.c_double), ("width_mask_edge", C.c_double), ("zero_mask", C.c_int),
+                ("sigma2_fudge", C.c_double), ("random_seed", C.c_ulonglong), ("avg_norm_correction", C.c_double),
+                ("do_ctf_correction", C.c_int), ("ctf_phase_flipped", C.c_int), ("only_flip_phases", C.c_int),
+                ("intact_ctf_first_peak", C.c_int), ("ctf_premultiplied", C.c_int), ("voltage", C.c_double),
+                ("cs", C.c_double), ("q0", C.c_double)]
+
+
+class ImageMeta(C.Structure):
+    _fields_ = [("norm_correction", C.c_double), ("old_offset_x", C.c_double), ("old_offset_y", C.c_double),
+                ("defocus_u", C.c_double), ("defocus_v", C.c_double), ("defocus_angle", C.c_double),
+                ("bfactor", C.c_double), ("kfactor", C.c_double), ("phase_shift", C.c_double),
+                ("part_id", C.c_longlong)]
+
+
 RESULT_DTYPE = np.dtype([(n, np.float64 if t in (C.c_double,) else (np.int64 if t is C.c_longlong else np.int32))
                          for n, t in ParticleResult._fields_])
 assert RESULT_DTYPE.itemsize == C.sizeof(ParticleResult)
@@ -169,6 +185,13 @@ _SIGNATURES = {
                                                 _ip, C.c_longlong]),
     "rln_proposal_images": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _fp]),
     "rln_proposal_forward_backward": (C.c_int, [_vp, C.c_int, _fp, _ip, _dp, _fp, _fp]),
+    # particle image preparation (include/rln_imgprep.h)
+    "rln_imgprep_create": (C.c_int, [C.c_int, C.POINTER(ImgPrepConfig), C.POINTER(_vp)]),
+    "rln_imgprep_destroy": (None, [_vp]),
+    "rln_imgprep_set_sigma2_noise": (C.c_int, [_vp, _dp, C.c_int]),
+    "rln_imgprep_run": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.POINTER(ImageMeta), _vp, _vp, _vp, C.c_int, _dp, _dp,
+                                  _dp]),
+    "rln_imgprep_timings": (C.c_int, [_vp, _dp]),
 }
 
 STAGES = ("prep", "coarse_diff2", "coarse_weights", "jobs", "fine_diff2", "fine_weights", "wsum_backproject",
@@ -554,6 +577,94 @@ class ExpectationStep:
     def close(self):
         if self._h:
             lib.rln_estep_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ImagePreparation:
+    """Particle image preparation on one device: the image part of
+    ``MlOptimiser::getFourierTransformsAndCtfs`` (src/ml_optimiser.cpp:5840-6531) for 2-D single-image particles
+    (norm correction, integer re-centring, soft mask, transforms, power spectrum, CTF image)."""
+
+    META_FIELDS = tuple(n for n, _ in ImageMeta._fields_)
+
+    def __init__(self, ori_size, current_size, pixel_size, particle_diameter, width_mask_edge=5.0, zero_mask=True,
+                 sigma2_fudge=1.0, random_seed=0, avg_norm_correction=0.0, do_ctf_correction=True,
+                 ctf_phase_flipped=False, only_flip_phases=False, intact_ctf_first_peak=False,
+                 ctf_premultiplied=False, voltage=300.0, cs=2.7, q0=0.1, device=0):
+        c = ImgPrepConfig()
+        c.ori_size, c.current_size, c.pixel_size = ori_size, current_size, pixel_size
+        c.particle_diameter, c.width_mask_edge, c.zero_mask = particle_diameter, width_mask_edge, int(zero_mask)
+        c.sigma2_fudge, c.random_seed, c.avg_norm_correction = sigma2_fudge, random_seed, avg_norm_correction
+        c.do_ctf_correction, c.ctf_phase_flipped = int(do_ctf_correction), int(ctf_phase_flipped)
+        c.only_flip_phases, c.intact_ctf_first_peak = int(only_flip_phases), int(intact_ctf_first_peak)
+        c.ctf_premultiplied, c.voltage, c.cs, c.q0 = int(ctf_premultiplied), voltage, cs, q0
+        self.cfg = c
+        self._h = _vp()
+        _check(lib.rln_imgprep_create(device, C.byref(c), C.byref(self._h)))
+
+    def config_dict(self):
+        return {n: getattr(self.cfg, n) for n, _ in ImgPrepConfig._fields_}
+
+    def set_sigma2_noise(self, sigma2):
+        s2 = np.ascontiguousarray(sigma2, np.float64)
+        _check(lib.rln_imgprep_set_sigma2_noise(self._h, _ptr(s2, C.c_double), len(s2)))
+
+    @classmethod
+    def meta_array(cls, metas):
+        arr = (ImageMeta * max(len(metas), 1))()
+        for i, m in enumerate(metas):
+            for k in cls.META_FIELDS:
+                setattr(arr[i], k, m[k])
+        return arr
+
+    def run_device(self, n, images, metas, fimg, fimg_nomask, fctf, images_on_device=True):
+        """Device-resident variant: images/fimg/fimg_nomask/fctf are device addresses (ints); returns the host
+        outputs highres_Xi2 [n], power [n][N/2+1], old_offset [n][2]."""
+        N = self.cfg.ori_size
+        meta = metas if isinstance(metas, C.Array) else self.meta_array(metas)
+        hx = np.zeros(n)
+        power = np.zeros((n, N // 2 + 1))
+        off = np.zeros((n, 2))
+        _check(lib.rln_imgprep_run(self._h, n, _vp(images), int(images_on_device), meta, _vp(fimg), _vp(fimg_nomask),
+                                   _vp(fctf), 1, _ptr(hx, C.c_double), _ptr(power, C.c_double),
+                                   _ptr(off, C.c_double)))
+        return hx, power, off
+
+    def getFourierTransformsAndCtfs(self, images, metas):
+        """Host arrays in and out: images [n][N][N] float32 -> dict of fimg, fimg_nomask (complex64 [n][s][s/2+1]),
+        fctf (float32), highres_Xi2, power, old_offset."""
+        img = np.ascontiguousarray(images, np.float32)
+        n = img.shape[0]
+        N, s = self.cfg.ori_size, self.cfg.current_size
+        if img.shape[1:] != (N, N):
+            raise ValueError(f"images must be [n][{N}][{N}], got {img.shape}")
+        xs = s // 2 + 1
+        fimg = np.zeros((n, s, xs), np.complex64)
+        fnm = np.zeros((n, s, xs), np.complex64)
+        fctf = np.zeros((n, s, xs), np.float32)
+        hx = np.zeros(n)
+        power = np.zeros((n, N // 2 + 1))
+        off = np.zeros((n, 2))
+        meta = metas if isinstance(metas, C.Array) else self.meta_array(metas)
+        _check(lib.rln_imgprep_run(self._h, n, img.ctypes.data, 0, meta, fimg.ctypes.data, fnm.ctypes.data,
+                                   fctf.ctypes.data, 0, _ptr(hx, C.c_double), _ptr(power, C.c_double),
+                                   _ptr(off, C.c_double)))
+        return dict(fimg=fimg, fimg_nomask=fnm, fctf=fctf, highres_Xi2=hx, power=power, old_offset=off)
+
+    def timings(self):
+        ms = np.zeros(5)
+        _check(lib.rln_imgprep_timings(self._h, _ptr(ms, C.c_double)))
+        return dict(zip(("real", "fft", "fourier", "noise", "total"), ms.tolist()))
+
+    def close(self):
+        if self._h:
+            lib.rln_imgprep_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

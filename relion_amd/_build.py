@@ -1,4 +1,5 @@
-"""Build step of librln_estep.so: hipcc for gfx950 (device kernels + engine), g++ for the OpenMP host modules.
+"""Build step of librln_estep.so: hipcc for gfx950 (device kernels + engine), g++ for the OpenMP host modules; the
+image preparation links hipFFT.
 
 Run as ``python relion_amd/_build.py`` (by path, so the package and a stale library are not imported) or through ``__graft_entry__.build()``. Outputs go to
 ``relion_amd/librln_estep.so`` (objects under ``relion_amd/_build/``).
@@ -14,6 +15,7 @@ LIB = os.path.join(HERE, "librln_estep.so")
 ARCH = os.environ.get("RLN_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_LIB = "/opt/rocm/lib"
 CXX = os.environ.get("CXX_HOST", "g++")
 
 HIP_FLAGS = [
@@ -37,12 +39,13 @@ def build(verbose=True):
         o = os.path.join(OBJ, src.replace(".cpp", ".o"))
         _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    for src in ("estep.hip", "proposal.hip"):
+    for src in ("estep.hip", "proposal.hip", "imgprep.hip"):
         o = os.path.join(OBJ, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lgomp"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lgomp",
+          f"-L{ROCM_LIB}", "-lhipfft", f"-Wl,-rpath,{ROCM_LIB}"])
     os.replace(tmp, LIB)
     return LIB
 
