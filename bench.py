@@ -18,6 +18,9 @@ Headline (the learned proposal on this data): HEALPix order 3 global search.
 Secondary (precision variant): HEALPix order 3 local search (*sigma_ang 7.5 deg, *priors = true pose + 3 deg
 noise), plain FP32 vs split-BF16 coarse pass on all particles.
 
+Also reported (extra block): image preparation (real-space particles -> masked/unmasked transforms, power
+spectra and CTF images) on --prep-particles device-resident particles.
+
 A step is one E-step over the particles: projections of the coarse grid, coarse pass, weights and
 significance, fine pass, weighted sums and backprojection; particle images are resident in HBM before the
 timed region. Prints one JSON line.
@@ -55,8 +58,10 @@ def parse():
                     help="CPU oracle budget: batches of one particle per thread until this much time (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--paths", default="proposal,plain,local",
-                    help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant)")
+    ap.add_argument("--prep-particles", type=int, default=8192)
+    ap.add_argument("--paths", default="proposal,plain,local,prep",
+                    help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant), "
+                         "prep (image preparation before the E-step)")
     ap.add_argument("--workload", default="refine256", choices=["refine256", "uneven192"],
                     help="refine256: the headline (box 256); uneven192: box 192, 20,000 particles of log-uniform "
                          "SNR in [0.01, 0.3], global order-3 coarse search, order-4 fine pass (batched fine pass)")
@@ -171,6 +176,63 @@ def uneven(args):
     print(json.dumps(out), flush=True)
     es.close()
     return 0
+
+
+def prep_leg(args, box, px, diameter, d_img, nvar):
+    """Image preparation on device-resident real-space particles: norm correction, re-centring, soft mask with a
+    noise background, forward transforms, power spectra, CTF images. The real-space stack is the inverse
+    transform of the first 512 synthetic particles, tiled to --prep-particles. HBM roofline of the three
+    streaming kernels and the library FFT: 4 N^2 + 8 N^2 (real) + 8 N^2 + 16 N (N/2+1) (FFT) + 16 N (N/2+1) +
+    20 s (s/2+1) (Fourier) bytes per particle, plus the noise images (8 N (N/2+1) + 4 N^2 + 4 N^2 read)."""
+    import relion_amd as ra
+    from relion_amd import hipmem
+
+    N = s = box
+    xs = s // 2 + 1
+    k = 512
+    sub = hipmem.DeviceArray(np.zeros((k, N, xs, 2), np.float32))
+    hipmem.copy_device(sub.address, d_img.address, sub.nbytes)
+    F = sub.to_host()
+    sub.free()
+    F = F[..., 0].astype(np.float64) + 1j * F[..., 1]
+    sign = np.where(((np.arange(N)[:, None] + np.arange(xs)[None, :]) & 1) == 1, -1.0, 1.0)
+    real = (np.fft.irfft2(F * sign, s=(N, N)) * (N * N)).astype(np.float32)
+    n = args.prep_particles
+    reps = -(-n // k)
+    d_real = hipmem.DeviceArray(np.tile(real, (reps, 1, 1))[:n])
+    d_f = hipmem.DeviceArray(np.zeros((n, s, xs, 2), np.float32))
+    d_u = hipmem.DeviceArray(np.zeros((n, s, xs, 2), np.float32))
+    d_c = hipmem.DeviceArray(np.zeros((n, s, xs), np.float32))
+    rng = np.random.default_rng(args.seed)
+    metas = ra.ImagePreparation.meta_array([dict(norm_correction=rng.uniform(0.8, 1.2), old_offset_x=rng.uniform(-3, 3),
+                                                 old_offset_y=rng.uniform(-3, 3), defocus_u=rng.uniform(8e3, 2.5e4),
+                                                 defocus_v=rng.uniform(8e3, 2.5e4), defocus_angle=rng.uniform(0, 180),
+                                                 bfactor=0.0, kfactor=1.0, phase_shift=0.0, part_id=i)
+                                            for i in range(n)])
+    prep = ra.ImagePreparation(N, s, px, diameter, zero_mask=False, random_seed=args.seed, avg_norm_correction=1.0)
+    prep.set_sigma2_noise(np.full(N // 2 + 1, float(np.mean(nvar))))
+    prep.run_device(n, d_real.address, metas, d_f.address, d_u.address, d_c.address)  # warm-up (FFT plans)
+    hipmem.synchronize()
+    steps = max(1, args.steps)
+    t0 = time.time()
+    ms = {"real": 0.0, "fft": 0.0, "fourier": 0.0, "noise": 0.0, "total": 0.0}
+    for _ in range(steps):
+        prep.run_device(n, d_real.address, metas, d_f.address, d_u.address, d_c.address)
+        for kk, v in prep.timings().items():
+            ms[kk] += v / steps
+    hipmem.synchronize()
+    wall = (time.time() - t0) / steps
+    hw = N * (N // 2 + 1)
+    per = (4 * N * N + 8 * N * N) + (8 * N * N + 16 * hw) + (16 * hw + 20 * s * xs) + (8 * hw + 8 * N * N)
+    kernel_ms = ms["real"] + ms["fft"] + ms["fourier"] + ms["noise"]
+    gbs = per * n / (kernel_ms * 1e-3) / 1e9
+    log(f"[prep] {n} particles: {json.dumps({kk: round(v, 2) for kk, v in ms.items()})} ms, wall {wall * 1e3:.1f} ms",
+        args)
+    prep.close()
+    return {"particles": n, "box": N, "window": s, "background": "noise", "particles_per_s": round(n / (ms["total"] * 1e-3), 1),
+            "stages_ms": {kk: round(v, 3) for kk, v in ms.items()}, "wall_ms": round(wall * 1e3, 2),
+            "roofline": {"bound": "hbm", "bytes_per_particle": per, "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4)}}
 
 
 def main():
@@ -338,6 +400,9 @@ def main():
             runs[name] = timed(name, local_step, n, args.steps, args.warmup, es_l)
             es_l.close()
 
+    # ---------------------------------------------------------------- image preparation (the stage before the E-step)
+    if "prep" in paths:
+        extra["image_preparation"] = prep_leg(args, box, px, diameter, d_img, nvar)
     head = "proposal" if "proposal" in runs else ("local_split" if "local_split" in runs else next(iter(runs)))
     R = runs[head]
     timings, counters = R["timings"], R["counters"]
