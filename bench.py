@@ -59,6 +59,13 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--prep-particles", type=int, default=8192)
+    ap.add_argument("--lists-out", default=None,
+                    help="save the proposal's orientation lists of the last timed step (npz), for profiling runs")
+    ap.add_argument("--lists-in", default=None,
+                    help="replay saved orientation lists instead of training and running the proposal network "
+                         "(the E-step workload of the run that saved them; for counter-collection runs)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the fine stage directly instead of through captured graphs (same kernels)")
     ap.add_argument("--paths", default="proposal,plain,local,prep",
                     help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant), "
                          "prep (image preparation before the E-step)")
@@ -310,7 +317,29 @@ def main():
     runs, extra = {}, {}
     pp = None
     # ---------------------------------------------------------------- learned proposal (training is untimed)
-    if "proposal" in paths:
+    replay = None
+    if "proposal" in paths and args.lists_in:
+        z = np.load(args.lists_in)
+        replay = (z["off"], z["og"])
+        if len(replay[0]) != n + 1:
+            raise SystemExit(f"{args.lists_in} holds lists for {len(replay[0]) - 1} particles, not {n}")
+    if "proposal" in paths and replay is not None:
+        cfg_g.precision = ra.RLN_PREC_SPLIT_BF16
+        es_p = ra.ExpectationStep(cfg_g, 0)
+        if args.no_graphs:
+            es_p.use_graphs(False)
+
+        def proposal_step(count=n, mt=meta):
+            bp.initMdl()
+            es_p.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
+            off, og = replay
+            es_p.set_orientation_lists(off[:count + 1], og[:off[count]])
+            res, _ = es_p.run(d_img.address, d_ctf.address, mt, on_device=True)
+            return res, off[:count + 1]
+
+        runs["proposal"] = timed("proposal (replayed lists)", proposal_step, n, args.steps, args.warmup, es_p)
+        extra["proposal_lists_replayed_from"] = args.lists_in
+    elif "proposal" in paths:
         pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=args.seed)
         # each epoch sees fresh projections (new poses, CTFs, shifts and noise)
         t_gen = t_train = 0.0
@@ -332,6 +361,8 @@ def main():
                                       "last_epoch_loss": round(loss, 4)}
         cfg_g.precision = ra.RLN_PREC_SPLIT_BF16
         es_p = ra.ExpectationStep(cfg_g, 0)
+        if args.no_graphs:
+            es_p.use_graphs(False)
         pred_ms = []
 
         def proposal_step(count=n, mt=meta):
@@ -348,12 +379,19 @@ def main():
         runs["proposal"] = timed("proposal", proposal_step, n, args.steps, args.warmup, es_p)
         extra["proposal_predict_ms"] = round(float(np.mean(pred_ms[-args.steps:])), 2)
         off = runs["proposal"]["out"][1]
+        if args.lists_out:
+            bins, probs = pp.predict(d_img.address, d_ctf.address, n, box, topk=args.topk, on_device=True)
+            o2, g2 = pp.orientation_lists(bins, probs, order, n_psi, min_top1=args.min_top1)
+            np.savez(args.lists_out, off=o2, og=g2)
+            log(f"orientation lists saved to {args.lists_out}", args)
         extra["proposal_fallback_fraction"] = round(float(np.mean(np.diff(off) == 0)), 4)
         extra["proposal_orientations_per_particle"] = round(float(np.mean(np.diff(off)[np.diff(off) > 0])), 1)
     # ---------------------------------------------------------------- exhaustive plain FP32 baseline
     if "plain" in paths:
         cfg_g.precision = ra.RLN_PREC_PLAIN_FP32
         es_x = ra.ExpectationStep(cfg_g, 0)
+        if args.no_graphs:
+            es_x.use_graphs(False)
 
         def plain_step():
             bp.initMdl()
@@ -383,13 +421,15 @@ def main():
                 "fsc_min_to_0.8_nyquist": round(float(f[1:lim + 1].min()), 5),
                 "fsc_mean_to_0.8_nyquist": round(float(f[1:lim + 1].mean()), 5)}
             log(f"proposal vs exhaustive: {json.dumps(extra['proposal_vs_exhaustive'])}", args)
-    if pp is not None:
+    if "proposal" in runs:
         es_p.close()
     # ---------------------------------------------------------------- precision variant: local search
     if "local" in paths:
         for name, prec in (("local_plain", ra.RLN_PREC_PLAIN_FP32), ("local_split", ra.RLN_PREC_SPLIT_BF16)):
             cfg_l.precision = prec
             es_l = ra.ExpectationStep(cfg_l, 0)
+            if args.no_graphs:
+                es_l.use_graphs(False)
 
             def local_step():
                 bp.initMdl()
@@ -421,10 +461,11 @@ def main():
     else:
         # gather-bound: 64 bytes of the corner-packed reference per (fine orientation, pixel) sample, plus the
         # 256 bytes of slot line read and written per backprojected sample in the weighted sums
+        # pixels counted: those sampled (non-zero Minvsigma2 inside the projection radius)
         if stage == "fine_diff2":
-            work = 64.0 * 8 * counters["fine_jobs"] * counters["fine_pixels"]
+            work = 64.0 * 8 * counters["fine_jobs"] * counters["fine_pixels_sampled"]
         else:
-            work = (64.0 + 256.0) * counters["wsum_orientations"] * counters["fine_pixels"]
+            work = (64.0 + 256.0) * counters["wsum_orientations"] * counters["fine_pixels_sampled"]
         bound, unit, peak = "hbm", "GB/s", PEAK_HBM_GBS
         achieved = work / (timings[stage] * 1e-3) / 1e9
     roofline = {"kernel": stage, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 1),

@@ -216,14 +216,17 @@ int rln_estep_fine_size(rln_estep *e, int ip, long long *n_fine);
 int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *diff2, double *weight);
 
 /* Device time of the last run per stage, in milliseconds, from HIP events on the engine's stream:
- * prep, coarse_diff2, coarse_weights, jobs, fine_diff2, fine_weights, wsum_backproject; then host wall
+ * prep, coarse_diff2, coarse_weights, jobs (fine-pass plan and work items), fine_diff2 (the fine diff2 kernel),
+ * fine_weights, wsum_backproject; then host wall
  * time of chunk preparation, job-list building, the epilogue, and the whole run. */
 int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
 /* Work counters of the last run: particles, coarse poses, coarse pixels per image, fine poses, fine pixels
  * per image, fine diff2 work items, backprojected fine orientations, significant (orientation, translation)
  * pairs summed in the weighted sums, coarse grid projections computed, coarse tiles, fine jobs, particle chunks
  * (one launch of each per-chunk kernel per chunk), launches of the captured fine-pass graph, fine-pass
- * capacity retries, and the tuned persistent grid sizes of the two fine diff2 kernels. */
+ * capacity retries, the persistent grid sizes of the fine diff2 kernel (tuned) and of the weighted sums (resident
+ * blocks), and the fine-image pixels that are sampled per fine orientation (non-zero Minvsigma2 inside the
+ * projection radius). */
 int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out);
 
 /* ---------------------------------------------------------------------------------------------

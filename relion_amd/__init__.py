@@ -566,12 +566,13 @@ class ExpectationStep:
 
     COUNTERS = ("particles", "coarse_poses", "coarse_pixels", "fine_poses", "fine_pixels", "fine_items",
                 "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks",
-                "fine_graph_launches", "fine_capacity_retries", "fine_grid_small", "fine_grid_wide")
+                "fine_graph_launches", "fine_capacity_retries", "fine_grid", "wsum_grid",
+                "fine_pixels_sampled")
 
     def counters(self):
-        out = np.zeros(16, np.int64)
+        out = np.zeros(len(self.COUNTERS), np.int64)
         n = C.c_int()
-        _check(lib.rln_estep_last_counters(self._h, _ptr(out, C.c_longlong), 16, C.byref(n)))
+        _check(lib.rln_estep_last_counters(self._h, _ptr(out, C.c_longlong), len(out), C.byref(n)))
         return dict(zip(self.COUNTERS, out.tolist()))
 
     def close(self):

@@ -30,7 +30,7 @@ struct Frame {
     float e0x, e0y, e0z, e1x, e1y, e1z;
 };
 
-__device__ __forceinline__ void pixel_xy(int p, int s, int xs, int &x, int &y) {
+__host__ __device__ __forceinline__ void pixel_xy(int p, int s, int xs, int &x, int &y) {
     const int i = p / xs;
     x = p - i * xs;
     y = (i < xs) ? i : i - s;

@@ -58,6 +58,8 @@ namespace rln {
 
 // Stage timers of one run (HIP events on the engine's stream).
 enum Stage { kStPrep, kStCoarse, kStCoarseW, kStJobs, kStFine, kStFineW, kStWsum, kStCount };
+// fine stage graphs: plan and work items; fine diff2; fine weights; weighted sums and backprojection
+constexpr int kFineGraphs = 4;
 
 struct Engine {
     int device = 0;
@@ -96,6 +98,7 @@ struct Engine {
     DevBuf d_fimg, d_fimgn, d_fctf;
     DevBuf d_meta_f, d_meta_d, d_meta_ll;
     DevBuf d_or_off, d_or_cnt, d_or_proj, d_or_prior, d_or_frame, d_or_class, d_or_dir, d_tr_prior;
+    DevBuf d_oc_off;  // per-particle offsets of per-orientation state
     DevBuf d_ctfq, d_yq, d_tiles;
     DevBuf d_diff2, d_weight, d_mask, d_stats_c, d_stats_f;
     DevBuf d_npairs, d_njobs, d_nchunks, d_pair_off, d_job_off, d_chunk_off, d_pairs, d_jobs, d_chunk_job,
@@ -111,9 +114,10 @@ struct Engine {
     DevBuf d_wsegs, d_seg_first, d_wpart, d_hcnt, d_hsum, d_wcand, d_segbest, d_ocnt;  // segmented coarse weights
     rlnk::ChunkPtrs cp_host{};
     bool tuned = false;
-    int grid_small = 1024, grid_wide = 1024, grid_wsum = 2048;
-    hipGraph_t fine_graph[3] = {};
-    hipGraphExec_t fine_exec[3] = {};
+    int grid_fine = 1024, grid_wsum = 2048;  // persistent grids (fine: tuned; weighted sums: resident blocks)
+    int resident_fine = 0;                   // resident blocks of k_fine_diff2 on the device
+    hipGraph_t fine_graph[kFineGraphs] = {};
+    hipGraphExec_t fine_exec[kFineGraphs] = {};
     std::vector<unsigned char> fine_key;  // kernel arguments the graph was captured with
     bool graph_off = false;               // capture unsupported: launch directly
     // intermediates kept for tests
@@ -133,7 +137,8 @@ struct Engine {
     hipEvent_t ev[kStCount + 1] = {};
     double stage_ms[kStCount] = {};
     double host_ms[4] = {};  // host phases: chunk preparation, job lists, epilogue, whole run (wall)
-    long long counters[16] = {};
+    long long counters[17] = {};
+    long long pf_sampled = 0;  // fine pixels gathered per fine orientation
     bool have_timing = false;
     std::mutex mtx;
 };
@@ -246,6 +251,15 @@ static int engine_init(Engine &e, const rln_estep_config &cfg) {
     HIP_TRY(hipSetDevice(e.device));
     HIP_TRY(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
     for (int i = 0; i <= kStCount; ++i) HIP_TRY(hipEventCreate(&e.ev[i]));
+    // persistent grids sized to the blocks the device holds at once (dynamic work fetch inside)
+    {
+        int cus = 0, nb_f = 0, nb_w = 0;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e.device));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_f, rlnk::k_fine_diff2<0>, rlnk::kBlock, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_w, rlnk::k_wsum_bp, rlnk::kBlock, 0));
+        e.resident_fine = cus * nb_f;
+        if (cus * nb_w > 0) e.grid_wsum = cus * nb_w;
+    }
     // orientation frames of the coarse grid and of its order-1 oversampling
     std::vector<rlnk::Frame> fc(e.n_or), ff((size_t)e.n_or * 8);
     double r8[8], t8[8], p8[8];
@@ -352,6 +366,13 @@ static int engine_model_tables(Engine &e) {
     }
     RLN_TRY(upload(e.d_q_c, qc, e.stream));
     RLN_TRY(upload(e.d_q_f, qf, e.stream));
+    // fine pixels gathered per fine orientation: non-zero Minvsigma2 inside the projection radius
+    e.pf_sampled = 0;
+    for (int n = 0; n < e.igf.P; ++n) {
+        int x, y;
+        rlnk::pixel_xy(n, e.igf.s, e.igf.xs, x, y);
+        if (qf[n] != 0.f && x * x + y * y <= e.igf.proj_r2_max) ++e.pf_sampled;
+    }
     RLN_TRY(upload(e.d_minv_w, mw, e.stream));
     // logsigma2 of the dLL (src/ml_optimiser.cpp:9024-9033)
     e.logsigma2 = 0.;
@@ -411,6 +432,8 @@ static int engine_coarse_projections(Engine &e) {
 struct ChunkHost {
     int B = 0;
     std::vector<int> or_off, or_cnt, or_proj, or_frame, or_class, or_dir;
+    std::vector<int> oc_off;       // per particle: offset of its per-orientation state (significant counts)
+    long long total_oc = 0;
     std::vector<int> nd, np;  // per particle: directions, psi (an explicit list counts as nd = its length)
     std::vector<double> or_prior, tr_prior;
     std::vector<long long> d2_off, n_entries;
@@ -432,6 +455,7 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     h = ChunkHost();
     h.B = B;
     h.or_off.resize(B);
+    h.oc_off.resize(B);
     h.or_cnt.resize(B);
     h.nd.resize(B);
     h.np.resize(B);
@@ -450,6 +474,11 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     if (cfg.local_search) local_priors_batch(e.grid, cfg, meta, B, sets);
     const int n_dir = e.grid.n_dir, n_psi = e.grid.n_psi;
     auto n_set = [&](int b) { return set_off ? set_off[b + 1] - set_off[b] : 0; };
+    // particles searched over the whole grid share one block of orientation entries at offset 0
+    auto full_grid = [&](int b) { return !cfg.local_search && n_set(b) == 0; };
+    bool any_full = false;
+    for (int b = 0; b < B && !any_full; ++b) any_full = full_grid(b);
+    const int n_grid = K * n_dir * n_psi;
     // pass 1: sizes
     for (int b = 0; b < B; ++b) {
         int nd, np;
@@ -467,11 +496,17 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         h.np[b] = np;
         h.or_cnt[b] = K * nd * np;
     }
-    long long tot_or = 0, d2 = 0;
+    long long tot_or = any_full ? n_grid : 0, d2 = 0, tot_oc = 0;
     std::vector<int> tile_off(B + 1, 0), seg_off(B + 1, 0);
     for (int b = 0; b < B; ++b) {
-        h.or_off[b] = (int)tot_or;
-        tot_or += h.or_cnt[b];
+        h.oc_off[b] = (int)tot_oc;
+        tot_oc += h.or_cnt[b];
+        if (full_grid(b)) {
+            h.or_off[b] = 0;
+        } else {
+            h.or_off[b] = (int)tot_or;
+            tot_or += h.or_cnt[b];
+        }
         h.n_entries[b] = (long long)h.or_cnt[b] * NT;
         h.d2_off[b] = d2;
         d2 += h.n_entries[b];
@@ -480,6 +515,7 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         seg_off[b + 1] = seg_off[b] + (int)ns;
     }
     h.total_entries = d2;
+    h.total_oc = tot_oc;
     h.or_proj.resize(tot_or);
     h.or_prior.resize(tot_or);
     h.or_frame.resize(tot_or);
@@ -488,13 +524,30 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     h.tiles.resize(tile_off[B]);
     h.segs.resize(seg_off[B]);
     h.seg_first.assign(seg_off.begin(), seg_off.end());
+    if (any_full) {
+        // the whole grid: orientation priors normalised by their mean over classes, directions and psi
+        // (src/ml_optimiser.cpp:7790-7803), the same for every particle (pdf_dir_mean, summed in grid order)
+#pragma omp parallel for schedule(static)
+        for (int q = 0; q < n_grid; ++q) {
+            const int k = q / (n_dir * n_psi), og = q - k * (n_dir * n_psi), d = og / n_psi;
+            const double po = e.pdf_direction[(size_t)k * n_dir + d];
+            const bool on = po > 0. && e.pdf_class[k] > 0.;
+            h.or_proj[q] = on ? k * e.n_or + og : -1;
+            h.or_prior[q] = po / e.pdf_dir_mean;
+            h.or_frame[q] = og * 8;
+            h.or_class[q] = k;
+            h.or_dir[q] = d;
+        }
+    }
     // pass 2: every particle fills its slices
 #pragma omp parallel for schedule(dynamic, 16)
     for (int b = 0; b < B; ++b) {
         const rln_particle_meta &m = meta[b];
         const int base = h.or_off[b];
         int q = base;
-        if (n_set(b) > 0) {
+        if (full_grid(b)) {
+            // entries in the shared block
+        } else if (n_set(b) > 0) {
             // explicit list: global-search priors, normalised by their mean over the whole grid so that weights
             // and sums stay those of the exhaustive search restricted to the list
             const int *og_list = set_og + (set_off[b] - set_off[0]);
@@ -619,58 +672,59 @@ struct FineLaunch {
     rlnk::FinePlan *plan;
     const unsigned char *mask;
     const long long *d2off;
-    const int *or_cnt, *or_off, *ocnt;
+    const int *or_cnt, *oc_off, *ocnt;
     int NT;
     rlnk::FineJob *jobs;
     int2 *items_s, *items_w;
-    rlnk::FineArgs fs, fw;
+    rlnk::FineArgs fa;
     rlnk::WeightArgs wf;
     rlnk::WsumArgs ws;
     double *shell, *part, *pdfc;
     long long n_shell, n_part, n_pdfc;
-    int grid_small, grid_wide, grid_wsum;
+    int grid_fine, grid_wsum;
 };
 
-// part 0: plan, job lists, work items, fine diff2; part 1: fine weights; part 2: weighted sums and
+// part 0: plan, job lists and work items; part 1: fine diff2; part 2: fine weights; part 3: weighted sums and
 // backprojection. Each part is captured into its own graph so that stage events sit between graph launches.
-static int launch_fine(Engine &e, const FineLaunch &L, int part) {
+static void launch_plan(Engine &e, const FineLaunch &L) {
     hipStream_t st = e.stream;
-    if (part == 1) {
-        hipLaunchKernelGGL(rlnk::k_weights, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.wf);
-        return launch_check("fine weights");
-    }
-    if (part == 2) {
-        hipLaunchKernelGGL(rlnk::k_zero_d, dim3(64), dim3(rlnk::kBlock), 0, st, L.shell, L.n_shell);
-        hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.part, L.n_part);
-        hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.pdfc, L.n_pdfc);
-        hipLaunchKernelGGL(rlnk::k_wsum_bp, dim3(L.grid_wsum), dim3(rlnk::kBlock), 0, st, L.ws);
-        return launch_check("weighted sums");
-    }
     hipLaunchKernelGGL(rlnk::k_fine_plan, dim3(1), dim3(rlnk::kWBlock), 0, st, L.np, L.nj, L.cp, L.cap_pairs,
                        L.cap_jobs, L.pair_off, L.job_off, L.fine_n, L.foff, L.plan);
     hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt,
-                       L.or_off, L.ocnt, L.NT,
+                       L.oc_off, L.ocnt, L.NT,
                        (const long long *)L.pair_off, (const int *)L.job_off, L.pairs, L.jobs,
                        (const rlnk::FinePlan *)L.plan);
-    hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs, 2,
-                       rlnk::kWideTC, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
-    hipLaunchKernelGGL(rlnk::k_fine_diff2<2>, dim3(L.grid_small), dim3(rlnk::kBlock), 0, st, L.fs);
-    hipLaunchKernelGGL(rlnk::k_fine_diff2_wide, dim3(L.grid_wide), dim3(rlnk::kBlock), 0, st, L.fw);
-    return launch_check("fine pass");
+    hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs,
+                       rlnk::kFineSmallTC, rlnk::kWideTC, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
 }
 
-// Persistent grid sizes of the two fine diff2 kernels: each candidate timed once on the first chunk (the
-// kernels only write fine diff2, so repeating them is harmless), the fastest kept for the engine's life.
+static int launch_fine(Engine &e, const FineLaunch &L, int part) {
+    hipStream_t st = e.stream;
+    switch (part) {
+        case 0:
+            launch_plan(e, L);
+            return launch_check("fine plan");
+        case 1:
+            hipLaunchKernelGGL(rlnk::k_fine_diff2<0>, dim3(L.grid_fine), dim3(rlnk::kBlock), 0, st, L.fa);
+            return launch_check("fine pass");
+        case 2:
+            hipLaunchKernelGGL(rlnk::k_weights, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.wf);
+            return launch_check("fine weights");
+        default:
+            hipLaunchKernelGGL(rlnk::k_zero_d, dim3(64), dim3(rlnk::kBlock), 0, st, L.shell, L.n_shell);
+            hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.part, L.n_part);
+            hipLaunchKernelGGL(rlnk::k_zero_d, dim3(16), dim3(rlnk::kBlock), 0, st, L.pdfc, L.n_pdfc);
+            hipLaunchKernelGGL(rlnk::k_wsum_bp, dim3(L.grid_wsum), dim3(rlnk::kBlock), 0, st, L.ws);
+            return launch_check("weighted sums");
+    }
+}
+
+// Persistent grid size of the fine diff2 kernel: each candidate (including the resident-block count) timed once
+// on the first chunk, the fastest kept for the engine's life. The kernel only writes fine diff2, so repeating it
+// is harmless; its item counter is reset before every candidate.
 static int tune_fine(Engine &e, FineLaunch L) {
     hipStream_t st = e.stream;
-    hipLaunchKernelGGL(rlnk::k_fine_plan, dim3(1), dim3(rlnk::kWBlock), 0, st, L.np, L.nj, L.cp, L.cap_pairs,
-                       L.cap_jobs, L.pair_off, L.job_off, L.fine_n, L.foff, L.plan);
-    hipLaunchKernelGGL(rlnk::k_write_jobs, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.mask, L.d2off, L.or_cnt,
-                       L.or_off, L.ocnt, L.NT,
-                       (const long long *)L.pair_off, (const int *)L.job_off, L.pairs, L.jobs,
-                       (const rlnk::FinePlan *)L.plan);
-    hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs, 2,
-                       rlnk::kWideTC, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
+    launch_plan(e, L);
     RLN_TRY(launch_check("fine plan"));
     rlnk::FinePlan plan{};
     HIP_TRY(hipMemcpyAsync(&plan, L.plan, sizeof(plan), hipMemcpyDeviceToHost, st));
@@ -679,57 +733,51 @@ static int tune_fine(Engine &e, FineLaunch L) {
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    const int cand[] = {256, 512, 1024, 2048, 4096};
-    float best_s = 1e30f, best_w = 1e30f;
+    std::vector<int> cand = {512, 1024, 2048, 4096};
+    if (e.resident_fine > 0) cand.push_back(e.resident_fine);
+    float best = 1e30f;
     for (int g : cand) {
         float ms = 0.f;
+        hipLaunchKernelGGL(rlnk::k_reset_items, dim3(1), dim3(64), 0, st, L.plan);
         HIP_TRY(hipEventRecord(a, st));
-        hipLaunchKernelGGL(rlnk::k_fine_diff2<2>, dim3(g), dim3(rlnk::kBlock), 0, st, L.fs);
+        hipLaunchKernelGGL(rlnk::k_fine_diff2<1>, dim3(g), dim3(rlnk::kBlock), 0, st, L.fa);
         HIP_TRY(hipEventRecord(b, st));
         HIP_TRY(hipEventSynchronize(b));
         HIP_TRY(hipEventElapsedTime(&ms, a, b));
-        if (plan.n_small && ms < best_s) {
-            best_s = ms;
-            e.grid_small = g;
-        }
-        HIP_TRY(hipEventRecord(a, st));
-        hipLaunchKernelGGL(rlnk::k_fine_diff2_wide, dim3(g), dim3(rlnk::kBlock), 0, st, L.fw);
-        HIP_TRY(hipEventRecord(b, st));
-        HIP_TRY(hipEventSynchronize(b));
-        HIP_TRY(hipEventElapsedTime(&ms, a, b));
-        if (plan.n_wide && ms < best_w) {
-            best_w = ms;
-            e.grid_wide = g;
+        if (ms < best) {
+            best = ms;
+            e.grid_fine = g;
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     RLN_TRY(launch_check("fine tuning"));
     e.tuned = true;
-    e.counters[14] = e.grid_small;
-    e.counters[15] = e.grid_wide;
     return RLN_OK;
+}
+
+static void drop_fine_graphs(Engine &e) {
+    for (int q = 0; q < kFineGraphs; ++q) {
+        if (e.fine_exec[q]) (void)hipGraphExecDestroy(e.fine_exec[q]);
+        if (e.fine_graph[q]) (void)hipGraphDestroy(e.fine_graph[q]);
+        e.fine_exec[q] = nullptr;
+        e.fine_graph[q] = nullptr;
+    }
 }
 
 static int run_fine_stage(Engine &e, const FineLaunch &L0) {
     FineLaunch L = L0;
     if (!e.tuned) {
         RLN_TRY(tune_fine(e, L));
-        L.grid_small = e.grid_small;
-        L.grid_wide = e.grid_wide;
+        L.grid_fine = e.grid_fine;
     }
     if (!e.graph_off) {
         const unsigned char *kb = reinterpret_cast<const unsigned char *>(&L);
         std::vector<unsigned char> key(kb, kb + sizeof(L));
         if (!e.fine_exec[0] || key != e.fine_key) {
-            for (int q = 0; q < 3; ++q) {
-                if (e.fine_exec[q]) (void)hipGraphExecDestroy(e.fine_exec[q]);
-                if (e.fine_graph[q]) (void)hipGraphDestroy(e.fine_graph[q]);
-                e.fine_exec[q] = nullptr;
-                e.fine_graph[q] = nullptr;
-            }
+            drop_fine_graphs(e);
             bool ok = true;
-            for (int q = 0; q < 3 && ok; ++q) {
+            for (int q = 0; q < kFineGraphs && ok; ++q) {
                 ok = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
                 if (!ok) break;
                 const int rc = launch_fine(e, L, q);
@@ -741,21 +789,16 @@ static int run_fine_stage(Engine &e, const FineLaunch &L0) {
             }
             (void)hipGetLastError();
             if (!ok) {
-                for (int q = 0; q < 3; ++q) {
-                    if (e.fine_exec[q]) (void)hipGraphExecDestroy(e.fine_exec[q]);
-                    if (e.fine_graph[q]) (void)hipGraphDestroy(e.fine_graph[q]);
-                    e.fine_exec[q] = nullptr;
-                    e.fine_graph[q] = nullptr;
-                }
+                drop_fine_graphs(e);
                 e.graph_off = true;  // capture unsupported here: launch directly from now on
             } else {
                 e.fine_key = key;
             }
         }
     }
-    for (int q = 0; q < 3; ++q) {
-        if (q == 1) HIP_TRY(hipEventRecord(e.ev[kStFineW], e.stream));
-        if (q == 2) HIP_TRY(hipEventRecord(e.ev[kStWsum], e.stream));
+    static const Stage before[kFineGraphs] = {kStJobs, kStFine, kStFineW, kStWsum};
+    for (int q = 0; q < kFineGraphs; ++q) {
+        HIP_TRY(hipEventRecord(e.ev[before[q]], e.stream));
         if (e.fine_exec[q]) {
             HIP_TRY(hipGraphLaunch(e.fine_exec[q], e.stream));
             e.counters[12] += 1;
@@ -833,6 +876,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         dC = e.d_fctf.as<const float>();
     }
     RLN_TRY(upload(e.d_or_off, h.or_off, st));
+    RLN_TRY(upload(e.d_oc_off, h.oc_off, st));
     RLN_TRY(upload(e.d_or_cnt, h.or_cnt, st));
     RLN_TRY(upload(e.d_or_proj, h.or_proj, st));
     RLN_TRY(upload(e.d_or_prior, h.or_prior, st));
@@ -912,6 +956,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     wa.n_entries = d_nent_tmp.as<const long long>();
     wa.or_prior = e.d_or_prior.as<const double>();
     wa.or_off = e.d_or_off.as<const int>();
+    wa.oc_off = e.d_oc_off.as<const int>();
     wa.tr_prior = e.d_tr_prior.as<const double>();
     wa.NT = NT;
     wa.fine = 0;
@@ -934,8 +979,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         HIP_TRY(hipMemsetAsync(e.d_hcnt.ptr, 0, sizeof(unsigned int) * 256 * B, st));
         // per-orientation significant counts and per-particle job counts of the fine pass
         RLN_TRY(ensure_fine_capacity(e));
-        RLN_TRY(e.d_ocnt.ensure(sizeof(int) * std::max<size_t>(1, h.or_proj.size())));
-        HIP_TRY(hipMemsetAsync(e.d_ocnt.ptr, 0, sizeof(int) * std::max<size_t>(1, h.or_proj.size()), st));
+        RLN_TRY(e.d_ocnt.ensure(sizeof(int) * std::max<long long>(1, h.total_oc)));
+        HIP_TRY(hipMemsetAsync(e.d_ocnt.ptr, 0, sizeof(int) * std::max<long long>(1, h.total_oc), st));
         HIP_TRY(hipMemsetAsync(e.d_npairs.as<int>() + cB, 0, sizeof(int) * B, st));
         HIP_TRY(hipMemsetAsync(e.d_hsum.ptr, 0, sizeof(double) * 256 * B, st));
         const rlnk::WSeg *sg = e.d_wsegs.as<const rlnk::WSeg>();
@@ -973,7 +1018,6 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.cp_host.fctf = dC;
     e.cp_host.B = B;
     HIP_TRY(hipMemcpyAsync(e.d_cp.ptr, &e.cp_host, sizeof(rlnk::ChunkPtrs), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipEventRecord(e.ev[kStFine], st));
     rlnk::FinePlan plan{};
     for (int attempt = 0;; ++attempt) {
         RLN_TRY(ensure_fine_capacity(e));
@@ -996,17 +1040,17 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         L.mask = e.d_mask.as<const unsigned char>();
         L.d2off = d_d2off.as<const long long>();
         L.or_cnt = e.d_or_cnt.as<const int>();
-        L.or_off = e.d_or_off.as<const int>();
+        L.oc_off = e.d_oc_off.as<const int>();
         L.ocnt = e.d_ocnt.as<const int>();
         L.NT = NT;
         L.jobs = e.d_jobs.as<rlnk::FineJob>();
         L.items_s = e.d_items_s.as<int2>();
         L.items_w = e.d_items_w.as<int2>();
-        rlnk::FineArgs &fa = L.fs;
+        rlnk::FineArgs &fa = L.fa;
         fa.jobs = e.d_jobs.as<const rlnk::FineJob>();
-        fa.items = e.d_items_s.as<const int2>();
-        fa.plan = e.d_plan.as<const rlnk::FinePlan>();
-        fa.which = 0;
+        fa.items_s = e.d_items_s.as<const int2>();
+        fa.items_w = e.d_items_w.as<const int2>();
+        fa.plan = e.d_plan.as<rlnk::FinePlan>();
         fa.cp = e.d_cp.as<const rlnk::ChunkPtrs>();
         fa.pairs = e.d_pairs.as<const int>();
         fa.pair_off = e.d_pair_off.as<const long long>();
@@ -1027,9 +1071,6 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         fa.NT = NT;
         fa.do_ctf = cfg.do_ctf_correction;
         fa.diff2 = e.d_fdiff2.as<float>();
-        L.fw = fa;
-        L.fw.items = e.d_items_w.as<const int2>();
-        L.fw.which = 1;
         rlnk::WeightArgs &wf = L.wf;
         wf = wa;
         wf.diff2 = e.d_fdiff2.as<const float>();
@@ -1062,7 +1103,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         ws.fine_frames = e.d_frames_f.as<const rlnk::Frame>();
         ws.ig = e.igf;
         ws.cp = e.d_cp.as<const rlnk::ChunkPtrs>();
-        ws.plan = e.d_plan.as<const rlnk::FinePlan>();
+        ws.plan = e.d_plan.as<rlnk::FinePlan>();
         ws.scale_w = d_scale_w;
         ws.minv_w = e.d_minv_w.as<const float>();
         ws.mresol = e.d_mresol.as<const int>();
@@ -1089,8 +1130,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         ws.pdf_class = e.d_pdfc_acc.as<double>();
         ws.pdf_direction = e.d_pdfd_acc.as<double>();
         ws.K = K;
-        L.grid_small = e.grid_small;
-        L.grid_wide = e.grid_wide;
+        L.grid_fine = e.grid_fine;
         L.grid_wsum = e.grid_wsum;
         RLN_TRY(run_fine_stage(e, L));
         HIP_TRY(hipMemcpyAsync(&plan, e.d_plan.ptr, sizeof(plan), hipMemcpyDeviceToHost, st));
@@ -1186,8 +1226,9 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[9] += (long long)h.tiles.size();
     e.counters[10] += n_jobs;
     e.counters[11] += 1;
-    e.counters[14] = e.grid_small;
-    e.counters[15] = e.grid_wide;
+    e.counters[14] = e.grid_fine;
+    e.counters[15] = e.grid_wsum;
+    e.counters[16] = e.pf_sampled;
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
         if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
@@ -1392,7 +1433,7 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
                            &e->d_mresol, &e->d_xa_shell, &e->d_ex_c, &e->d_ey_c, &e->d_ex_f, &e->d_ey_f,
                            &e->d_tr_ix, &e->d_tr_iy, &e->d_over_x, &e->d_over_y, &e->d_vols, &e->d_bp_ptrs,
                            &e->d_fimg, &e->d_fimgn, &e->d_fctf, &e->d_meta_f, &e->d_meta_d, &e->d_meta_ll,
-                           &e->d_or_off, &e->d_or_cnt, &e->d_or_proj, &e->d_or_prior, &e->d_or_frame,
+                           &e->d_or_off, &e->d_oc_off, &e->d_or_cnt, &e->d_or_proj, &e->d_or_prior, &e->d_or_frame,
                            &e->d_or_class, &e->d_or_dir, &e->d_tr_prior, &e->d_ctfq, &e->d_yq, &e->d_tiles,
                            &e->d_diff2, &e->d_weight, &e->d_mask, &e->d_stats_c, &e->d_stats_f, &e->d_npairs,
                            &e->d_njobs, &e->d_nchunks, &e->d_pair_off, &e->d_job_off, &e->d_chunk_off, &e->d_pairs,
@@ -1400,10 +1441,7 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
                            &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc,
                            &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w, &e->d_wsegs, &e->d_seg_first,
                            &e->d_wpart, &e->d_hcnt, &e->d_hsum, &e->d_wcand, &e->d_segbest, &e->d_ocnt};
-    for (int q = 0; q < 3; ++q) {
-        if (e->fine_exec[q]) (void)hipGraphExecDestroy(e->fine_exec[q]);
-        if (e->fine_graph[q]) (void)hipGraphDestroy(e->fine_graph[q]);
-    }
+    rln::drop_fine_graphs(*e);
     for (rln::DevBuf *b : bufs) b->release();
     for (int i = 0; i <= rln::kStCount; ++i)
         if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
@@ -1472,12 +1510,7 @@ extern "C" int rln_estep_use_graphs(rln_estep *e, int enable) {
     std::lock_guard<std::mutex> lk(e->mtx);
     e->graph_off = !enable;
     if (!enable) {
-        for (int q = 0; q < 3; ++q) {
-            if (e->fine_exec[q]) (void)hipGraphExecDestroy(e->fine_exec[q]);
-            if (e->fine_graph[q]) (void)hipGraphDestroy(e->fine_graph[q]);
-            e->fine_exec[q] = nullptr;
-            e->fine_graph[q] = nullptr;
-        }
+        rln::drop_fine_graphs(*e);
         e->fine_key.clear();
     }
     return RLN_OK;
@@ -1660,7 +1693,7 @@ extern "C" int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over,
 
 extern "C" int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out) {
     if (!e || !out) RLN_FAIL(RLN_ERR_ARG, "null argument");
-    const int n = std::min(n_max, 16);
+    const int n = std::min(n_max, 17);
     for (int i = 0; i < n; ++i) out[i] = e->counters[i];
     if (n_out) *n_out = n;
     return RLN_OK;

@@ -566,6 +566,8 @@ struct FinePlan {
     int n_small, n_wide; // work items of k_fine_diff2<2> and k_fine_diff2_wide
     int overflow;
     int B;               // particles in the chunk
+    int next_fine;       // next work item of k_fine_diff2 (dynamic fetch; zeroed by k_fine_plan)
+    int next_wsum;       // next work item of k_wsum_bp
     int pad_;
 };
 
@@ -587,6 +589,7 @@ struct WeightArgs {
     const long long *n_entries;  // per particle
     const double *or_prior;      // normalised pdf_orientation per orientation entry (or_off based)
     const int *or_off;
+    const int *oc_off;          // per-particle offsets of per-orientation state (ocnt)
     const double *tr_prior;      // normalised pdf_offset [b][NT]
     const int *pairs;            // fine mode: coarse index per pair, at pair_off[b]
     const long long *pair_off;
@@ -1266,7 +1269,7 @@ __global__ void __launch_bounds__(kBlock) k_wseg_mask(WeightArgs a, const WSeg *
         above += sig ? 1 : 0;
         if (sig) {
             const int o = (int)(e / a.NT);
-            if (atomicAdd(&ocnt[a.or_off[b] + o], 1) == 0) atomicAdd(&njobs[b], 1);
+            if (atomicAdd(&ocnt[a.oc_off[b] + o], 1) == 0) atomicAdd(&njobs[b], 1);
         }
         if (v > 0.) {
             const long long key = wsum_order_key(a, b, e);
@@ -1421,6 +1424,7 @@ __global__ void __launch_bounds__(kWBlock) k_fine_plan(const int *__restrict__ n
         plan->n_pairs = pbase;
         plan->n_jobs = jbase;
         plan->n_small = plan->n_wide = 0;
+        plan->next_fine = plan->next_wsum = 0;
         plan->B = B;
         plan->overflow = (pbase > cap_pairs || jbase > cap_jobs) ? 1 : 0;
     }
@@ -1430,7 +1434,7 @@ __global__ void __launch_bounds__(kWBlock) k_fine_plan(const int *__restrict__ n
 // (ocnt, counted by k_wseg_mask). grid (B), block 1024.
 __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__restrict__ mask,
                                                         const long long *__restrict__ d2_off,
-                                                        const int *__restrict__ or_cnt, const int *__restrict__ or_off,
+                                                        const int *__restrict__ or_cnt, const int *__restrict__ oc_off,
                                                         const int *__restrict__ ocnt, int NT,
                                                         const long long *__restrict__ pair_off,
                                                         const int *__restrict__ job_off, int *__restrict__ pairs,
@@ -1443,7 +1447,7 @@ __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__r
     const int n_o = or_cnt[b];
     for (int o0 = 0; o0 < n_o; o0 += kWBlock) {
         const int o = o0 + threadIdx.x;
-        const int c = (o < n_o) ? ocnt[or_off[b] + o] : 0;  // significant translations of orientation o
+        const int c = (o < n_o) ? ocnt[oc_off[b] + o] : 0;  // significant translations of orientation o
         int tot_p, tot_j;
         const int ps = block_exclusive_scan_i(c, tmp, tot_p);
         const int js = block_exclusive_scan_i(c > 0 ? 1 : 0, tmp, tot_j);
@@ -1469,7 +1473,7 @@ __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__r
 // --------------------------------------------------------------------------------------------------
 
 // Work items of the fine pass: jobs with at most small_max significant translations are one item of
-// k_fine_diff2<2>; the others one item of k_fine_diff2_wide per kWideTC translations. One block of 1024.
+// the small path of k_fine_diff2; the others one wide item per kWideTC translations. One block of 1024.
 __global__ void __launch_bounds__(kWBlock) k_fine_items(const FineJob *__restrict__ jobs, int small_max, int wide_tc,
                                                         int cap_small, int cap_wide, int2 *__restrict__ small_items,
                                                         int2 *__restrict__ wide_items, FinePlan *__restrict__ plan) {
@@ -1504,9 +1508,9 @@ __global__ void __launch_bounds__(kWBlock) k_fine_items(const FineJob *__restric
 
 struct FineArgs {
     const FineJob *jobs;
-    const int2 *items;             // (job, first job-local pair) per work item
-    const FinePlan *plan;
-    int which;                     // 0: items counted by plan->n_small, 1: by plan->n_wide
+    const int2 *items_w;           // (job, first job-local pair) per wide work item (plan->n_wide)
+    const int2 *items_s;           // per small work item (plan->n_small)
+    FinePlan *plan;                // header; next_fine is the dynamic item counter
     const ChunkPtrs *cp;           // this chunk's images
     const int *pairs;
     const long long *pair_off;
@@ -1529,31 +1533,32 @@ struct FineArgs {
     float *diff2;                  // out [pair_off[b]*32 + s*32 + r*4 + u]
 };
 
-template <int kFineTC>
-__global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
-    constexpr int kFineNF = 4 * kFineTC;  // fine translations per block
-    __shared__ float red[kBlock / 64][8 * kFineNF];
-    __shared__ int s_tix[kFineNF], s_tiy[kFineNF];
-    __shared__ int s_t[kFineTC];
-    if (a.plan->overflow) return;
-    const int n_items = a.which ? a.plan->n_wide : a.plan->n_small;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const int job_id = a.items[item].x;
-    const int first = a.items[item].y;
+// One work item: a job with at most kFineSmallTC significant coarse translations. Each thread owns pixels,
+// projects the 8 oversampled orientations once per pixel and accumulates the 8 x 4*TC squared differences.
+constexpr int kFineSmallTC = 2;
+
+struct FineSmallLds {
+    float red[kBlock / 64][8 * 4 * kFineSmallTC];
+    int tix[4 * kFineSmallTC], tiy[4 * kFineSmallTC];
+    int t[kFineSmallTC];
+};
+
+__device__ __forceinline__ void fine_small_item(const FineArgs &a, int job_id, int first, FineSmallLds &L) {
+    constexpr int kFineNF = 4 * kFineSmallTC;  // fine translations per block
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
-    const int nc = min(kFineTC, job.s_count - first);
+    const int nc = min(kFineSmallTC, job.s_count - first);
     const int tid = threadIdx.x;
-    if (tid < kFineTC) {
+    if (tid < kFineSmallTC) {
         const int s = job.s_begin + first + tid;
-        s_t[tid] = (tid < nc) ? a.pairs[a.pair_off[b] + s] % a.NT : 0;
+        L.t[tid] = (tid < nc) ? a.pairs[a.pair_off[b] + s] % a.NT : 0;
     }
     __syncthreads();
     if (tid < kFineNF) {
         const int c = tid >> 2, u = tid & 3;
-        const int t = s_t[c];
-        s_tix[tid] = a.tr_ix[t * 4 + u];
-        s_tiy[tid] = a.tr_iy[t * 4 + u];
+        const int t = L.t[c];
+        L.tix[tid] = a.tr_ix[t * 4 + u];
+        L.tiy[tid] = a.tr_iy[t * 4 + u];
     }
     __syncthreads();
     const int oe = a.or_off[b] + job.o;
@@ -1587,7 +1592,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
         const float2 Xq = make_float2(qp * Xp.x, qp * Xp.y);
 #pragma unroll
         for (int j = 0; j < kFineNF; ++j) {
-            const float2 ph = cmul(a.ex[(size_t)s_tix[j] * a.ig.xs + x], a.ey[(size_t)s_tiy[j] * a.ig.s + i]);
+            const float2 ph = cmul(a.ex[(size_t)L.tix[j] * a.ig.xs + x], a.ey[(size_t)L.tiy[j] * a.ig.s + i]);
             const float2 Y = cmul(Xq, ph);
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
@@ -1605,7 +1610,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
         for (int j = 0; j < kFineNF; ++j) {
             float v = acc[r][j];
             for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-            if (lane == 0) red[w][r * kFineNF + j] = v;
+            if (lane == 0) L.red[w][r * kFineNF + j] = v;
         }
     __syncthreads();
     if (tid < 8 * kFineNF) {
@@ -1613,35 +1618,30 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2(FineArgs a) {
         const int c = j >> 2, u = j & 3;
         if (c < nc) {
             float v = 0.f;
-            for (int ww = 0; ww < kBlock / 64; ++ww) v += red[ww][tid];
+            for (int ww = 0; ww < kBlock / 64; ++ww) v += L.red[ww][tid];
             const long long s = job.s_begin + first + c;
             a.diff2[(a.pair_off[b] + s) * 32 + r * 4 + u] = a.xi2half[b] + v;
         }
     }
-    __syncthreads();  // LDS of this item consumed before the next item's writes
-    }
 }
 
-// Wide variant for jobs with more than 2 significant translations: one block per (job, 4 * kWideCW coarse
-// translations). Per tile of 64 pixels the four waves first project the 8 fine orientations together (two per
+// One work item of a job with more than kFineSmallTC significant translations: up to 4 * kWideCW coarse
+// translations. Per tile of 64 pixels the four waves first project the 8 fine orientations together (two per
 // thread) into LDS; then wave w accumulates the squared differences of its kWideCW coarse translations (4 * kWideCW
 // fine translations) over the tile's pixels. A job with up to kWideTC significant translations reads each
 // trilinear sample once.
 constexpr int kWideCW = 2;             // coarse translations per wave
 constexpr int kWideTC = 4 * kWideCW;   // coarse translations per block
 
-__global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
+struct FineWideLds {
+    float2 A[8][64];
+    float2 X[64];
+    int p[64];
+    int tix[4][4 * kWideCW], tiy[4][4 * kWideCW];  // per wave
+};
+
+__device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, int first, FineWideLds &L) {
     constexpr int kNF = 4 * kWideCW;  // fine translations per wave
-    __shared__ float2 s_A[8][64];
-    __shared__ float2 s_X[64];
-    __shared__ int s_p[64];
-    __shared__ int s_tix[4][kNF], s_tiy[4][kNF];  // per wave
-    if (a.plan->overflow) return;
-    const int n_items = a.which ? a.plan->n_wide : a.plan->n_small;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-    __syncthreads();  // previous item's translation tables consumed
-    const int job_id = a.items[item].x;
-    const int first = a.items[item].y;
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
     const int nc = min(kWideTC, job.s_count - first);  // coarse translations of this block
@@ -1651,8 +1651,8 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
         const int c = kWideCW * w + (j >> 2), u = j & 3;
         int t = 0;
         if (c < nc) t = a.pairs[a.pair_off[b] + job.s_begin + first + c] % a.NT;
-        s_tix[w][j] = a.tr_ix[t * 4 + u];
-        s_tiy[w][j] = a.tr_iy[t * 4 + u];
+        L.tix[w][j] = a.tr_ix[t * 4 + u];
+        L.tiy[w][j] = a.tr_iy[t * 4 + u];
     }
     const int oe = a.or_off[b] + job.o;
     const int k = a.or_class[oe];
@@ -1686,31 +1686,32 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
                     }
                     if (wave == 0) {
                         const float2 Xp = X[p];
-                        s_X[lane] = make_float2(qp * Xp.x, qp * Xp.y);
-                        s_p[lane] = p;
+                        L.X[lane] = make_float2(qp * Xp.x, qp * Xp.y);
+                        L.p[lane] = p;
                     }
                 } else if (wave == 0) {
-                    s_p[lane] = -1;
+                    L.p[lane] = -1;
                 }
             } else if (wave == 0) {
-                s_p[lane] = -1;
+                L.p[lane] = -1;
             }
-            s_A[2 * wave][lane] = make_float2(c * v0.x, c * v0.y);
-            s_A[2 * wave + 1][lane] = make_float2(c * v1.x, c * v1.y);
+            L.A[2 * wave][lane] = make_float2(c * v0.x, c * v0.y);
+            L.A[2 * wave + 1][lane] = make_float2(c * v1.x, c * v1.y);
         }
         __syncthreads();
-        // phase B: this wave's 16 fine translations over the tile
+        // phase B: this wave's fine translations over the tile
         if (active) {
-            const int p = s_p[lane];
+            const int p = L.p[lane];
             if (p >= 0) {
                 const int i = p / a.ig.xs, x = p - i * a.ig.xs;
                 float2 A[8];
 #pragma unroll
-                for (int r = 0; r < 8; ++r) A[r] = s_A[r][lane];
-                const float2 Xq = s_X[lane];
+                for (int r = 0; r < 8; ++r) A[r] = L.A[r][lane];
+                const float2 Xq = L.X[lane];
 #pragma unroll
                 for (int j = 0; j < kNF; ++j) {
-                    const float2 ph = cmul(a.ex[(size_t)s_tix[wave][j] * a.ig.xs + x], a.ey[(size_t)s_tiy[wave][j] * a.ig.s + i]);
+                    const float2 ph = cmul(a.ex[(size_t)L.tix[wave][j] * a.ig.xs + x],
+                                           a.ey[(size_t)L.tiy[wave][j] * a.ig.s + i]);
                     const float2 Y = cmul(Xq, ph);
 #pragma unroll
                     for (int r = 0; r < 8; ++r) {
@@ -1723,35 +1724,74 @@ __global__ void __launch_bounds__(kBlock) k_fine_diff2_wide(FineArgs a) {
         }
     }
     if (active) {
-    // wave reduction; lanes 0 .. kWideCW-1 write the wave's translations
+        // wave reduction; lanes 0 .. kWideCW-1 write the wave's translations
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
+        for (int r = 0; r < 8; ++r)
 #pragma unroll
-        for (int j = 0; j < kNF; ++j) {
-            float v = acc[r][j];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-            acc[r][j] = v;
-        }
-    if (lane < kWideCW) {
-        const int c = kWideCW * wave + lane;  // one coarse translation per lane
-        if (c < nc) {
-            const long long s = job.s_begin + first + c;
-            float *out = a.diff2 + (a.pair_off[b] + s) * 32;
+            for (int j = 0; j < kNF; ++j) {
+                float v = acc[r][j];
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                acc[r][j] = v;
+            }
+        if (lane < kWideCW) {
+            const int c = kWideCW * wave + lane;  // one coarse translation per lane
+            if (c < nc) {
+                const long long s = job.s_begin + first + c;
+                float *out = a.diff2 + (a.pair_off[b] + s) * 32;
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
+                for (int r = 0; r < 8; ++r)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    // acc index j = 4 * lane + u, selected without dynamic register indexing
-                    float v = acc[r][u];
+                    for (int u = 0; u < 4; ++u) {
+                        // acc index j = 4 * lane + u, selected without dynamic register indexing
+                        float v = acc[r][u];
 #pragma unroll
-                    for (int q = 1; q < kWideCW; ++q)
-                        if (lane == q) v = acc[r][4 * q + u];
-                    out[r * 4 + u] = a.xi2half[b] + v;
-                }
+                        for (int q = 1; q < kWideCW; ++q)
+                            if (lane == q) v = acc[r][4 * q + u];
+                        out[r * 4 + u] = a.xi2half[b] + v;
+                    }
+            }
         }
     }
-    }  // active
-    }  // items
+}
+
+// Fine pass, persistent: blocks fetch work items from plan->next_fine (wide items first, the heaviest, then
+// small ones) until none is left, so uneven items spread over the GPU; every block exits once the counter passes
+// the last item. Three waves per SIMD, as the two item paths have when compiled alone (at most 168 VGPRs).
+// kTuning only separates the grid-size tuning launches from the production ones in kernel traces.
+template <int kTuning>
+__global__ void __launch_bounds__(kBlock, 3) k_fine_diff2(FineArgs a) {
+    __shared__ union {
+        FineSmallLds s;
+        FineWideLds w;
+    } lds;
+    __shared__ int s_item;
+    if (a.plan->overflow) return;
+    const int n_wide = a.plan->n_wide;
+    const int n_items = n_wide + a.plan->n_small;
+    for (;;) {
+        __syncthreads();  // previous item's LDS and s_item consumed
+        if (threadIdx.x == 0) s_item = atomicAdd(&a.plan->next_fine, 1);
+        __syncthreads();
+        // one value per block: readfirstlane makes the branches below provably uniform, so the two item paths
+        // are not linearised and do not hold their registers at the same time
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= n_items) break;
+        if (item < n_wide) {
+            const int2 it = a.items_w[item];
+            fine_wide_item(a, it.x, it.y, lds.w);
+        } else {
+            const int2 it = a.items_s[item - n_wide];
+            fine_small_item(a, it.x, it.y, lds.s);
+        }
+    }
+}
+
+// Resets the dynamic item counters (tuning launches of the fine pass reuse one plan).
+__global__ void k_reset_items(FinePlan *plan) {
+    if (threadIdx.x == 0) {
+        plan->next_fine = 0;
+        plan->next_wsum = 0;
+    }
 }
 
 // Zero fill used inside captured graphs instead of memset nodes.
@@ -1778,7 +1818,7 @@ struct WsumArgs {
     const Frame *fine_frames;
     ImgGeom ig;
     const ChunkPtrs *cp;            // this chunk's images
-    const FinePlan *plan;           // items: plan->n_jobs * 8
+    FinePlan *plan;                 // items: plan->n_jobs * 8, fetched through plan->next_wsum
     const float *scale_w;           // max(scale, 0.001) (src/ml_optimiser.cpp:8402-8416)
     const float *minv_w;            // Minvsigma2 with the origin restored
     const int *mresol;              // resolution shell per pixel, -1 excluded
@@ -1816,9 +1856,16 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     __shared__ long long sx_base[kBlock];
     __shared__ float sx_val[kBlock][3];
     __shared__ float sx_w[kBlock][8];
+    __shared__ int s_item;
     if (a.plan->overflow) return;
     const int n_items = 8 * a.plan->n_jobs;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    // persistent: items fetched dynamically, since only some (job, orientation) items have significant weights
+    for (;;) {
+    __syncthreads();  // previous item's LDS and s_item consumed
+    if (threadIdx.x == 0) s_item = atomicAdd(&a.plan->next_wsum, 1);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= n_items) break;
     const int job_id = item >> 3;
     const int r = item & 7;
     const FineJob job = a.jobs[job_id];
@@ -1867,10 +1914,7 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     }
     __syncthreads();
     const int nsig = s_n;
-    if (nsig == 0) {
-        __syncthreads();
-        continue;
-    }
+    if (nsig == 0) continue;
     const int oe = a.or_off[b] + job.o;
     const int k = a.or_class[oe];
     const Frame fr = a.fine_frames[a.or_frame[oe] + r];

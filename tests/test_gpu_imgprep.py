@@ -121,7 +121,8 @@ def test_imgprep_chunks_and_device_buffers():
     hipmem.synchronize()
     assert np.array_equal(d_f.to_host(), host["fimg"]) and np.array_equal(d_u.to_host(), host["fimg_nomask"])
     assert np.array_equal(d_c.to_host(), host["fctf"]) and np.array_equal(off, host["old_offset"])
-    assert np.array_equal(power, host["power"])
+    # spectra are summed with FP64 shared-memory atomics, whose order differs between runs
+    assert np.allclose(power, host["power"], rtol=1e-12, atol=0)
     pick = [0, 1, 4095, 4096, 4099]
     ref = orc.prepare(imgs[pick], [metas[i] for i in pick], prep.config_dict(), sigma2=sigma2)
     _compare({k: v[pick] for k, v in host.items()}, ref, "chunks")
