@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "../../include/rln_proposal.h"
@@ -47,6 +48,7 @@ struct rln_proposal {
     int cap = 0;
     rln::DevBuf x[rln::kLayers + 1], col[rln::kLayers], logits, dlog, dtmp, dz_a, dz_b, loss, labels, idx;
     rln::DevBuf src, src_ctf;  // particle images and CTFs staged from the host
+    rln::DevBuf top_bins, top_probs;  // predictions of one predict call
     // training set
     rln::DevBuf train_img;  // bf16 [n_train][W][W]
     std::vector<int> train_lab;
@@ -105,12 +107,42 @@ void conv_fwd_t(rln_proposal &p, const ConvLayer &L, int n) {
                        p.x[&L - p.conv + 1].as<rlnp::bf16>());
 }
 
+// LDS-resident weights (Cin % 8 == 0, Kp % 32 == 0, weights within 80 KB so that two blocks share a CU)
+bool conv_lds_ok(const ConvLayer &L) {
+    return L.cin % 8 == 0 && L.Kp % 32 == 0 && (size_t)L.cout * (L.Kp + rlnp::kConvLdsPad) * 2 <= 80 * 1024;
+}
+
+template <int NT>
+int conv_fwd_lds_t(rln_proposal &p, const ConvLayer &L, int n) {
+    const long long M = (long long)n * L.Ho * L.Ho;
+    const size_t lds = (size_t)16 * NT * (L.Kp + rlnp::kConvLdsPad) * sizeof(rlnp::bf16);
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_fwd_lds<NT>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long tiles = (M + 64 * rlnp::kConvMT - 1) / (64 * rlnp::kConvMT);
+    const unsigned grid = (unsigned)std::min<long long>(tiles, 1024);
+    hipLaunchKernelGGL((rlnp::k_conv_fwd_lds<NT>), dim3(grid), dim3(rlnp::kB), lds, p.stream,
+                       p.x[&L - p.conv].as<const rlnp::bf16>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp,
+                       p.wb.as<const rlnp::bf16>() + L.w_off, p.w.as<const float>() + L.b_off,
+                       p.x[&L - p.conv + 1].as<rlnp::bf16>());
+    HIP_TRY(hipGetLastError());
+    return RLN_OK;
+}
+
 // logits of the n images in x[0]; with keep_col the im2col buffers are written (the backward pass needs them),
 // otherwise the convolutions run as implicit GEMMs straight from the activations
 int forward(rln_proposal &p, int n, bool keep_col = true) {
     for (int l = 0; l < kLayers; ++l) {
         const ConvLayer &L = p.conv[l];
         const long long M = (long long)n * L.Ho * L.Ho;
+        if (!keep_col && L.cout % 16 == 0 && L.cout <= 64 && conv_lds_ok(L)) {
+            switch (L.cout / 16) {
+                case 1: RLN_TRY(conv_fwd_lds_t<1>(p, L, n)); break;
+                case 2: RLN_TRY(conv_fwd_lds_t<2>(p, L, n)); break;
+                case 3: RLN_TRY(conv_fwd_lds_t<3>(p, L, n)); break;
+                default: RLN_TRY(conv_fwd_lds_t<4>(p, L, n)); break;
+            }
+            continue;
+        }
         if (!keep_col && L.cout % 16 == 0 && L.cout <= 64) {
             switch (L.cout / 16) {
                 case 1: conv_fwd_t<1>(p, L, n); break;
@@ -547,11 +579,12 @@ extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, c
     if (topk < 1 || topk > rlnp::kMaxTopK || topk > p->NB) RLN_FAIL(RLN_ERR_ARG, "topk %d out of range", topk);
     std::lock_guard<std::mutex> lk(p->mtx);
     HIP_TRY(hipSetDevice(p->device));
+    // chunks run back to back on the stream (they share the activation buffers in stream order); the top-k
+    // of all particles collect on the device and come back in one copy
     const int chunk = 1024;
     RLN_TRY(ensure_batch(*p, std::min(n, chunk)));
-    DevBuf d_bins, d_probs;
-    RLN_TRY(d_bins.ensure(sizeof(int) * (size_t)chunk * topk));
-    RLN_TRY(d_probs.ensure(sizeof(float) * (size_t)chunk * topk));
+    RLN_TRY(p->top_bins.ensure(sizeof(int) * (size_t)std::max(n, 1) * topk));
+    RLN_TRY(p->top_probs.ensure(sizeof(float) * (size_t)std::max(n, 1) * topk));
     const size_t per = (size_t)current_size * (current_size / 2 + 1);
     for (int i0 = 0; i0 < n; i0 += chunk) {
         const int m = std::min(chunk, n - i0);
@@ -559,14 +592,17 @@ extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, c
                                 p->x[0].as<rlnp::bf16>(), nullptr));
         RLN_TRY(forward(*p, m, false));
         hipLaunchKernelGGL(rlnp::k_topk, dim3(m), dim3(rlnp::kB), sizeof(float) * p->NB, p->stream,
-                           p->logits.as<const float>(), p->NB, topk, d_bins.as<int>(), d_probs.as<float>());
+                           p->logits.as<const float>(), p->NB, topk, p->top_bins.as<int>() + (size_t)i0 * topk,
+                           p->top_probs.as<float>() + (size_t)i0 * topk);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(bins + (size_t)i0 * topk, d_bins.ptr, sizeof(int) * (size_t)m * topk,
-                               hipMemcpyDeviceToHost, p->stream));
-        HIP_TRY(hipMemcpyAsync(probs + (size_t)i0 * topk, d_probs.ptr, sizeof(float) * (size_t)m * topk,
-                               hipMemcpyDeviceToHost, p->stream));
-        HIP_TRY(hipStreamSynchronize(p->stream));
     }
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(bins, p->top_bins.ptr, sizeof(int) * (size_t)n * topk, hipMemcpyDeviceToHost,
+                               p->stream));
+        HIP_TRY(hipMemcpyAsync(probs, p->top_probs.ptr, sizeof(float) * (size_t)n * topk, hipMemcpyDeviceToHost,
+                               p->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(p->stream));
     return RLN_OK;
 }
 
@@ -614,29 +650,44 @@ extern "C" int rln_proposal_orientation_sets(const rln_proposal *p, int n, const
         RLN_FAIL(RLN_ERR_ARG, "rln_proposal_orientation_sets: null argument");
     if (grid_order < p->cfg.healpix_order || grid_order > 12 || n_psi_grid < 1)
         RLN_FAIL(RLN_ERR_ARG, "rln_proposal_orientation_sets: invalid grid");
+    for (long long q = 0; q < (long long)n * topk; ++q)
+        if (bins[q] < 0 || bins[q] >= p->NB) RLN_FAIL(RLN_ERR_ARG, "bin %d out of range", bins[q]);
+    // orientations of every bin (the grid is fixed), then the particles' sorted unions in parallel; offsets and
+    // the copy into the caller's arrays follow in particle order
     std::vector<std::vector<int>> cache(p->NB);
-    std::vector<int> acc;
+    for (int b = 0; b < p->NB; ++b) bin_orients(*p, grid_order, n_psi_grid, b, cache[b]);
+    std::vector<std::vector<int>> lists(n);
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto build = [&](int t0) {
+        std::vector<int> acc;
+        for (int i = t0; i < n; i += nt) {
+            acc.clear();
+            if (probs[(size_t)i * topk] >= min_top1) {
+                for (int t = 0; t < topk; ++t) {
+                    const std::vector<int> &c = cache[bins[(size_t)i * topk + t]];
+                    acc.insert(acc.end(), c.begin(), c.end());
+                }
+                std::sort(acc.begin(), acc.end());
+                acc.erase(std::unique(acc.begin(), acc.end()), acc.end());
+            }
+            lists[i] = acc;
+        }
+    };
+    {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(build, t);
+        build(0);
+        for (auto &th : pool) th.join();
+    }
     long long total = 0;
     set_offsets[0] = 0;
     for (int i = 0; i < n; ++i) {
-        acc.clear();
-        if (probs[(size_t)i * topk] >= min_top1) {
-            for (int t = 0; t < topk; ++t) {
-                const int b = bins[(size_t)i * topk + t];
-                if (b < 0 || b >= p->NB) RLN_FAIL(RLN_ERR_ARG, "bin %d out of range", b);
-                if (cache[b].empty()) bin_orients(*p, grid_order, n_psi_grid, b, cache[b]);
-                acc.insert(acc.end(), cache[b].begin(), cache[b].end());
-            }
-            std::sort(acc.begin(), acc.end());
-            acc.erase(std::unique(acc.begin(), acc.end()), acc.end());
-        }
-        if (total + (long long)acc.size() > capacity)
-            RLN_FAIL(RLN_ERR_ARG, "orientation list capacity %lld exceeded", capacity);
-        std::copy(acc.begin(), acc.end(), set_orients + total);
-        total += (long long)acc.size();
+        total += (long long)lists[i].size();
+        if (total > capacity) RLN_FAIL(RLN_ERR_ARG, "orientation list capacity %lld exceeded", capacity);
         if (total > 0x7fffffffLL) RLN_FAIL(RLN_ERR_ARG, "orientation lists exceed 2^31 entries");
         set_offsets[i + 1] = (int)total;
     }
+    for (int i = 0; i < n; ++i) std::copy(lists[i].begin(), lists[i].end(), set_orients + set_offsets[i]);
     return RLN_OK;
 }
 

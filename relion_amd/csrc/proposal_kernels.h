@@ -141,12 +141,15 @@ __global__ void __launch_bounds__(kB) k_window_image(const float2 *__restrict__ 
     for (int idx = tid; idx < xw * W; idx += kB) {
         const int kx = idx / W, v = idx - kx * W;
         float2 acc = make_float2(0.f, 0.f);
+        // twiddle index (ky * v) mod W advances by v per row, also across the wrap to negative ky (-W v = 0 mod W)
+        int ti = 0;
         for (int r = 0; r < W; ++r) {
-            const int ky = (r < W / 2) ? r : r - W;
-            const float2 t = tw[((ky * v) % W + W) % W];
+            const float2 t = tw[ti];
             const float2 f = Fw[r][kx];
             acc.x += f.x * t.x - f.y * t.y;
             acc.y += f.x * t.y + f.y * t.x;
+            ti += v;
+            if (ti >= W) ti -= W;
         }
         G[kx][v] = acc;
     }
@@ -155,11 +158,14 @@ __global__ void __launch_bounds__(kB) k_window_image(const float2 *__restrict__ 
     for (int idx = tid; idx < W * W; idx += kB) {
         const int v = idx / W, u = idx - v * W;
         float acc = 0.f;
+        int ti = 0;  // (kx * u) mod W
         for (int kx = 0; kx < xw; ++kx) {
             const float h = (kx == 0 || kx == W / 2) ? 1.f : 2.f;
-            const float2 t = tw[(kx * u) % W];
+            const float2 t = tw[ti];
             const float2 g = G[kx][v];
             acc += h * (g.x * t.x - g.y * t.y);
+            ti += u;
+            if (ti >= W) ti -= W;
         }
         img[idx] = acc;
         s1 += acc;
@@ -437,6 +443,99 @@ __global__ void __launch_bounds__(kB) k_conv_fwd(const bf16 *__restrict__ X, int
         for (int r = 0; r < 4; ++r) {
             const long long mm = ((long long)blockIdx.x * 4 + wave) * 16 + 4 * gq + r;
             if (mm < M) Y[(size_t)mm * Cout + col] = to_bf16(fmaxf(acc[j][r] + bj, 0.f));
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
+// Forward 3x3 convolution as an implicit GEMM with the weights resident in LDS (inference, Cin % 8 == 0).
+// The block loads the whole weight matrix [Cout][Kp] into LDS once (rows padded by 8 elements against bank
+// conflicts) and then walks output-pixel tiles of 256 (4 waves x 4 row tiles of 16) with a grid stride. Per
+// K step of 32 a wave reads 4 A fragments (8 channels of one tap per lane) from the NHWC input and NT B
+// fragments from LDS and issues 4 x NT MFMAs; the A fragments of the next step are loaded before the MFMAs of
+// the current one.
+// --------------------------------------------------------------------------------------------------
+constexpr int kConvMT = 4;         // row tiles (16 output pixels each) per wave
+constexpr int kConvLdsPad = 8;     // bf16 elements of padding per weight row in LDS
+
+__device__ __forceinline__ uint4 conv_a_frag(const bf16 *__restrict__ X, int H, int Cin, int stride, int Ho,
+                                             long long M, long long m, int k) {
+    uint4 av = make_uint4(0u, 0u, 0u, 0u);
+    if (m < M) {
+        const int img = (int)(m / (Ho * Ho));
+        const int pix = (int)(m - (long long)img * Ho * Ho);
+        const int oy = pix / Ho, ox = pix - (pix / Ho) * Ho;
+        const int tap = k / Cin, c = k - tap * Cin;
+        const int iy = oy * stride + tap / 3 - 1, ix = ox * stride + tap % 3 - 1;
+        if (iy >= 0 && iy < H && ix >= 0 && ix < H)
+            av = *reinterpret_cast<const uint4 *>(X + (((size_t)img * H + iy) * H + ix) * Cin + c);
+    }
+    return av;
+}
+
+template <int NT>  // column tiles of 16 output channels
+__global__ void __launch_bounds__(kB) k_conv_fwd_lds(const bf16 *__restrict__ X, int n, int H, int Cin, int stride,
+                                                     int Ho, int Kp, const bf16 *__restrict__ Wt,
+                                                     const float *__restrict__ bias, bf16 *__restrict__ Y) {
+    extern __shared__ uint4 s_w4[];  // [16 * NT][Kp + kConvLdsPad] bf16
+    bf16 *s_w = reinterpret_cast<bf16 *>(s_w4);
+    const int ldw = Kp + kConvLdsPad;
+    const int Cout = 16 * NT;
+    // weights into LDS, 16 bytes at a time (Kp % 8 == 0)
+    for (int q = threadIdx.x; q < Cout * (Kp / 8); q += kB) {
+        const int row = q / (Kp / 8), c8 = q - row * (Kp / 8);
+        *reinterpret_cast<uint4 *>(s_w + (size_t)row * ldw + 8 * c8) =
+            *reinterpret_cast<const uint4 *>(Wt + (size_t)row * Kp + 8 * c8);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, gq = lane >> 4;
+    const long long M = (long long)n * Ho * Ho;
+    const long long n_tiles = (M + 64 * kConvMT - 1) / (64 * kConvMT);
+    for (long long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const long long mbase = (tile * 4 + wave) * (16 * kConvMT);  // this wave's first output pixel
+        floatx4_t acc[kConvMT][NT];
+#pragma unroll
+        for (int i = 0; i < kConvMT; ++i)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+        uint4 a_cur[kConvMT];
+#pragma unroll
+        for (int i = 0; i < kConvMT; ++i) a_cur[i] = conv_a_frag(X, H, Cin, stride, Ho, M, mbase + 16 * i + r16, 8 * gq);
+        for (int k0 = 0; k0 < Kp; k0 += 32) {
+            uint4 a_nxt[kConvMT];
+            const bool more = k0 + 32 < Kp;
+#pragma unroll
+            for (int i = 0; i < kConvMT; ++i)
+                a_nxt[i] = more ? conv_a_frag(X, H, Cin, stride, Ho, M, mbase + 16 * i + r16, k0 + 32 + 8 * gq)
+                                : make_uint4(0u, 0u, 0u, 0u);
+            bf16x8_t b[NT];
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                b[j] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(
+                                                        s_w + (size_t)(16 * j + r16) * ldw + k0 + 8 * gq));
+#pragma unroll
+            for (int i = 0; i < kConvMT; ++i) {
+                const bf16x8_t a = __builtin_bit_cast(bf16x8_t, a_cur[i]);
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < kConvMT; ++i) a_cur[i] = a_nxt[i];
+        }
+        // D rows 4*gq + r of each 16-pixel tile, column r16 of each channel tile: bias, ReLU, bf16 NHWC
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int col = 16 * j + r16;
+            const float bj = bias[col];
+#pragma unroll
+            for (int i = 0; i < kConvMT; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long long mm = mbase + 16 * i + 4 * gq + r;
+                    if (mm < M) Y[(size_t)mm * Cout + col] = to_bf16(fmaxf(acc[i][j][r] + bj, 0.f));
+                }
         }
     }
 }
