@@ -185,6 +185,20 @@ def uneven(args):
     return 0
 
 
+def committed_traffic(name, kernel):
+    """HBM bytes per launch of `kernel` from a committed counter summary (two rocprofv3 --pmc runs of the same
+    workload, FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or (None, None)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(float(parts[2])), f"profiles/{name}"
+    return None, None
+
+
 def prep_leg(args, box, px, diameter, d_img, nvar):
     """Image preparation on device-resident real-space particles: norm correction, re-centring, soft mask with a
     noise background, forward transforms, power spectra, CTF images. The real-space stack is the inverse
@@ -468,9 +482,13 @@ def main():
             work = (64.0 + 256.0) * counters["wsum_orientations"] * counters["fine_pixels_sampled"]
         bound, unit, peak = "hbm", "GB/s", PEAK_HBM_GBS
         achieved = work / (timings[stage] * 1e-3) / 1e9
+    traffic, traffic_src = None, None
+    if args.traffic_kib is not None:
+        traffic, traffic_src = round(args.traffic_kib * 1024.0), "--traffic-kib"
+    elif head == "proposal" and box == 256 and n == 10000 and stage == "fine_diff2":
+        traffic, traffic_src = committed_traffic("estep_proposal_box256_order3_10k_traffic.md", "rlnk::k_fine_diff2<0>")
     roofline = {"kernel": stage, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 1),
-                "unit": unit, "frac": round(achieved / peak, 4),
-                "traffic": None if args.traffic_kib is None else round(args.traffic_kib * 1024.0),
+                "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "stage_ms": round(timings[stage], 2), "launches": counters["chunks"],
                 "avg_launch_ms": round(timings[stage] / max(1, counters["chunks"]), 3)}
 

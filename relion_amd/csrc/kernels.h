@@ -372,8 +372,11 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
                                                         const unsigned int *__restrict__ zero_row,
                                                         float *__restrict__ diff2) {
     constexpr int kTW = (NTT + 3) / 4;  // translation tiles per wave
+    constexpr int kAStride = 40;        // bf16 per staged A row (32 + 8 padding against bank conflicts)
     __shared__ int og_s[kMfO];
     __shared__ float a_norm[kMfO];
+    // A operand of one K step (hi and lo planes of the block's 64 projection rows), double buffered
+    __shared__ __align__(16) unsigned short s_a[2][2][kMfO * kAStride];
     const CoarseTile tile = tiles[blockIdx.x];
     const int b = tile.b;
     const int n_o = or_cnt[b];
@@ -411,12 +414,13 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
 #pragma unroll
         for (int j = 0; j < kTW; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
     const int r16 = lane & 15, gq = lane >> 4;
-    const uint4 *arow_hi[4], *arow_lo[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int og = og_s[16 * i + r16];
-        arow_hi[i] = reinterpret_cast<const uint4 *>(og >= 0 ? phi + (size_t)og * Kc : zero_row);
-        arow_lo[i] = reinterpret_cast<const uint4 *>(og >= 0 ? plo + (size_t)og * Kc : zero_row);
+    // staging: thread tid loads 8 consecutive k (16 bytes) of row tid / 4, hi and lo, one K step ahead
+    const int srow = tid >> 2, sq = tid & 3;
+    const uint4 *st_hi, *st_lo;
+    {
+        const int og = og_s[srow];
+        st_hi = reinterpret_cast<const uint4 *>(og >= 0 ? phi + (size_t)og * Kc : zero_row);
+        st_lo = reinterpret_cast<const uint4 *>(og >= 0 ? plo + (size_t)og * Kc : zero_row);
     }
     int tt[kTW];
 #pragma unroll
@@ -425,13 +429,25 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
         tt[j] = ((wave + 4 * j) < NTT && t < NT) ? t : -1;
     }
     const float4 *wrow = reinterpret_cast<const float4 *>(w + (size_t)b * Kc);
+    uint4 pre_h = st_hi[sq], pre_l = st_lo[sq];  // K step 0: pixel groups 0..3
+    *reinterpret_cast<uint4 *>(&s_a[0][0][srow * kAStride + 8 * sq]) = pre_h;
+    *reinterpret_cast<uint4 *>(&s_a[0][1][srow * kAStride + 8 * sq]) = pre_l;
+    __syncthreads();
+    int buf = 0;
     for (int k0 = 0; k0 < Kp; k0 += 32) {
+        const bool more = k0 + 32 < Kp;
+        if (more) {
+            const int gn = ((k0 + 32) >> 3) + sq;
+            pre_h = st_hi[gn];
+            pre_l = st_lo[gn];
+        }
         const int g = (k0 >> 3) + gq;  // this lane's pixel group
         bf16x8_t ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ah[i] = __builtin_bit_cast(bf16x8_t, arow_hi[i][g]);
-            al[i] = __builtin_bit_cast(bf16x8_t, arow_lo[i][g]);
+            const int row = 16 * i + r16;
+            ah[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(&s_a[buf][0][row * kAStride + 8 * gq]));
+            al[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(&s_a[buf][1][row * kAStride + 8 * gq]));
         }
         const float4 w01 = wrow[2 * g], w23 = wrow[2 * g + 1];
         const int2 gr = grp[g];
@@ -465,6 +481,13 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
             }
         }
+        if (more) {
+            // the other buffer was last read in the previous step, before the barrier that ended it
+            *reinterpret_cast<uint4 *>(&s_a[buf ^ 1][0][srow * kAStride + 8 * sq]) = pre_h;
+            *reinterpret_cast<uint4 *>(&s_a[buf ^ 1][1][srow * kAStride + 8 * sq]) = pre_l;
+        }
+        __syncthreads();
+        buf ^= 1;
     }
     const float base = xi2half[b] + (float)ynorm[b];
     float *out = diff2 + d2_off[b];

@@ -5,7 +5,8 @@ HBM traffic:   python relion_amd/profile_summary.py --fetch <fetch.db> --write <
                [--write-scale 1.0] [--command "..."] > traffic.md
 The traffic mode averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, multiplies them by the
 calibration scales (what the counters read for a device-to-device copy of known size) and reports GB per
-launch and the resulting bandwidth over the dispatch durations of the counter runs.
+launch and the resulting bandwidth over the dispatch durations of the counter runs, then one line
+``traffic_bytes_per_launch <kernel> <bytes>`` per kernel.
 """
 import argparse
 import sqlite3
@@ -30,7 +31,8 @@ def counter(db, name):
 
 
 def short(name):
-    return name.split("(")[0]
+    name = name.split("(")[0]
+    return name[5:] if name.startswith("void ") else name
 
 
 def kernel_table(a):
@@ -59,9 +61,14 @@ def traffic_table(a):
         fb = fk * 1024.0 * a.fetch_scale
         wb = wk * 1024.0 * a.write_scale
         rows.append((n * dur, name, n, dur, fb, wb))
-    for _, name, n, dur, fb, wb in sorted(rows, reverse=True)[:a.top]:
+    top = sorted(rows, reverse=True)[:a.top]
+    for _, name, n, dur, fb, wb in top:
         bw = (fb + wb) / (dur * 1e-9) / 1e9 if dur else 0.0
         print(f"| {short(name)} | {n} | {dur / 1e3:.1f} | {fb / 1e9:.3f} | {wb / 1e9:.3f} | {bw:.0f} |")
+    # machine-readable lines (bench.py reads the one of its dominant kernel)
+    print()
+    for _, name, n, dur, fb, wb in top:
+        print(f"traffic_bytes_per_launch {short(name)} {fb + wb:.0f}")
 
 
 def main():
