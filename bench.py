@@ -457,6 +457,11 @@ def main():
     # ---------------------------------------------------------------- image preparation (the stage before the E-step)
     if "prep" in paths:
         extra["image_preparation"] = prep_leg(args, box, px, diameter, d_img, nvar)
+    if not runs:  # image preparation alone
+        print(json.dumps({"metric": "image preparation particles/s, box %d" % box,
+                          "value": extra["image_preparation"]["particles_per_s"], "unit": "particles/s", "n_gpus": 1,
+                          "higher_is_better": True, "data": "synthetic", **extra}), flush=True)
+        return 0
     head = "proposal" if "proposal" in runs else ("local_split" if "local_split" in runs else next(iter(runs)))
     R = runs[head]
     timings, counters = R["timings"], R["counters"]

@@ -41,8 +41,26 @@ __host__ __device__ __forceinline__ void pixel_xy(int p, int s, int xs, int &x, 
 //   [V(z0,y0,x0), V(z0,y0,x0+1)], [V(z0,y0+1,x0), V(z0,y0+1,x0+1)],
 //   [V(z0+1,y0,x0), V(z0+1,y0,x0+1)], [V(z0+1,y0+1,x0), V(z0+1,y0+1,x0+1)]
 // so that one trilinear sample is one aligned 64-byte read instead of four scattered cache lines.
-__device__ __forceinline__ size_t packed_index(const VolGeom &g, int x0, int y0, int z0) {
-    return ((size_t)z0 * (size_t)(g.ydim - 1) + (size_t)y0) * (size_t)(g.xdim - 1) + (size_t)x0;
+// Base voxels are grouped in bricks of 4 x 4 x 4 (4 KB of the reference, 8 KB of backprojection slots), bricks
+// and the voxels inside them in x-fastest order: the samples of a projection that neighbour each other in any
+// direction of the volume then fall into few bricks, which keeps their random 64-byte reads on few DRAM pages.
+constexpr int kBrickLog = 2;
+
+__host__ __device__ __forceinline__ int brick_count(int dim) {
+    return (dim - 1 + (1 << kBrickLog) - 1) >> kBrickLog;  // bricks covering the dim - 1 base positions
+}
+
+__host__ __device__ __forceinline__ size_t packed_index(const VolGeom &g, int x0, int y0, int z0) {
+    constexpr int m = (1 << kBrickLog) - 1;
+    const size_t brick = ((size_t)(z0 >> kBrickLog) * (size_t)brick_count(g.ydim) + (size_t)(y0 >> kBrickLog)) *
+                             (size_t)brick_count(g.xdim) +
+                         (size_t)(x0 >> kBrickLog);
+    return (brick << (3 * kBrickLog)) + (size_t)((((z0 & m) << kBrickLog) + (y0 & m)) << kBrickLog) + (size_t)(x0 & m);
+}
+
+// Base-voxel slots to allocate (whole bricks).
+__host__ __device__ __forceinline__ size_t packed_bases(int xdim, int ydim, int zdim) {
+    return ((size_t)brick_count(xdim) * (size_t)brick_count(ydim) * (size_t)brick_count(zdim)) << (3 * kBrickLog);
 }
 
 // Trilinear gather of the padded reference at (xp, yp, zp), Hermitian flip for xp < 0

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 
+#include "device_common.h"
 #include "rln_common.h"
 
 #define HIP_TRY(expr)                                                                                     \
@@ -80,15 +81,15 @@ inline int check_dims(const MdlDims &d) {
 struct rln_projector {
     int device = 0;
     rln::MdlDims dims;
-    rln::DevBuf data;  // corner-packed volume: 4 float4 per trilinear base voxel
+    rln::DevBuf data;  // corner-packed volume: 4 float4 per trilinear base voxel, in bricks
     unsigned long long version = 0;
-    size_t bases() const { return (size_t)(dims.xdim - 1) * (dims.ydim - 1) * (dims.zdim - 1); }
+    size_t bases() const { return rlnk::packed_bases(dims.xdim, dims.ydim, dims.zdim); }
 };
 
 struct rln_backprojector {
     int device = 0;
     rln::MdlDims dims;
-    rln::DevBuf slots;  // corner-slot accumulators: kSlot floats per trilinear base voxel
-    size_t bases() const { return (size_t)(dims.xdim - 1) * (dims.ydim - 1) * (dims.zdim - 1); }
+    rln::DevBuf slots;  // corner-slot accumulators: kSlot floats per trilinear base voxel, in bricks
+    size_t bases() const { return rlnk::packed_bases(dims.xdim, dims.ydim, dims.zdim); }
 };
 

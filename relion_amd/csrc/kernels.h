@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(kBlock) k_pack_volume(const float2 *__restrict
     const float2 *b = plain + (size_t)z0 * slab + (size_t)y0 * row + x0;
     const float2 d000 = b[0], d001 = b[1], d010 = b[row], d011 = b[row + 1];
     const float2 d100 = b[slab], d101 = b[slab + 1], d110 = b[slab + row], d111 = b[slab + row + 1];
-    float4 *o = packed + 4 * idx;
+    float4 *o = packed + 4 * packed_index(g, x0, y0, z0);
     o[0] = make_float4(d000.x, d000.y, d001.x, d001.y);
     o[1] = make_float4(d010.x, d010.y, d011.x, d011.y);
     o[2] = make_float4(d100.x, d100.y, d101.x, d101.y);

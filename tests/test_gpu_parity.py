@@ -128,8 +128,9 @@ def test_projector_roundtrip():
     pr.initMdl(p.volumes[0])
     bp = ra.AccBackprojector(0)
     nbytes = bp.setMdlDim(*dims, 2.0)
-    # corner-slot accumulators: 32 floats per trilinear base voxel
-    assert nbytes == 4 * 32 * (dims[0] - 1) * (dims[1] - 1) * (dims[2] - 1)
+    # corner-slot accumulators: 32 floats per trilinear base voxel, allocated in whole bricks of 4 x 4 x 4
+    bricks = [-(-(d - 1) // 4) for d in dims[:3]]
+    assert nbytes == 4 * 32 * 64 * bricks[0] * bricks[1] * bricks[2]
     bp.initMdl()
     re, im, w = bp.getMdlData()
     assert not re.any() and not im.any() and not w.any()
