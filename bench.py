@@ -66,9 +66,9 @@ def parse():
                          "(the E-step workload of the run that saved them; for counter-collection runs)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the fine stage directly instead of through captured graphs (same kernels)")
-    ap.add_argument("--paths", default="proposal,plain,local,prep",
+    ap.add_argument("--paths", default="proposal,plain,local,prep,refprep",
                     help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant), "
-                         "prep (image preparation before the E-step)")
+                         "prep (image preparation before the E-step), refprep (reference preparation)")
     ap.add_argument("--workload", default="refine256", choices=["refine256", "uneven192"],
                     help="refine256: the headline (box 256); uneven192: box 192, 20,000 particles of log-uniform "
                          "SNR in [0.01, 0.3], global order-3 coarse search, order-4 fine pass (batched fine pass)")
@@ -197,6 +197,43 @@ def committed_traffic(name, kernel):
             if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
                 return int(float(parts[2])), f"profiles/{name}"
     return None, None
+
+
+def refprep_leg(args, spec, box):
+    """Reference preparation (Projector::computeFourierTransformMap) of the box^3 blob map at padding 2 with
+    gridding correction, timed over --steps calls after one warm-up; the FP64 oracle on the host beside it when
+    --cpu-seconds > 0. HBM bytes counted per call: the map read (4 box^3), the padded map written and read by the
+    FFT (8 P^3), the transform written and read (16 P^2 (P/2+1)), the plain window written and read (16 S^2 (S/2+1))
+    and the corner-packed volume written (64 per base voxel); the FFT's internal passes are not counted."""
+    import relion_amd as ra
+    from relion_amd import hipmem
+
+    vol = ra.synth_real_map(spec, 0, box)
+    pr = ra.AccProjector(0)
+    pr.computeFourierTransformMap(vol, box, 2.0)
+    hipmem.synchronize()
+    d_vol = hipmem.DeviceArray(vol)
+    steps = max(1, args.steps)
+    t0 = time.time()
+    for _ in range(steps):
+        pr.computeFourierTransformMap(d_vol.address, box, 2.0, ori_size=box, on_device=True)
+    hipmem.synchronize()
+    ms = 1000.0 * (time.time() - t0) / steps
+    P, S = 2 * box, 2 * (box + 1) + 1
+    nbytes = 4 * box ** 3 + 8 * P ** 3 + 16 * P * P * (P // 2 + 1) + 16 * S * S * (S // 2 + 1) + 64 * (S // 2) * (S - 1) ** 2
+    out = {"box": box, "padded_box": P, "ms_per_call": round(ms, 2), "calls": steps,
+           "roofline": {"bound": "hbm", "bytes_per_call": nbytes, "achieved": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}}
+    log(f"[refprep] box {box}: {ms:.1f} ms per call", args)
+    if args.cpu_seconds > 0:
+        from oracle import refprep as orc
+        t = time.time()
+        orc.compute_fourier_map(vol, box, 2.0)
+        out["cpu_oracle_s"] = round(time.time() - t, 2)
+        out["cpu_oracle_kind"] = "port (numpy FP64, one process)"
+        log(f"[refprep] oracle {out['cpu_oracle_s']} s", args)
+    pr.clear()
+    return out
 
 
 def prep_leg(args, box, px, diameter, d_img, nvar):
@@ -454,13 +491,15 @@ def main():
             runs[name] = timed(name, local_step, n, args.steps, args.warmup, es_l)
             es_l.close()
 
+    # ---------------------------------------------------------------- reference preparation (the producer of the projector)
+    if "refprep" in paths:
+        extra["reference_preparation"] = refprep_leg(args, spec, box)
     # ---------------------------------------------------------------- image preparation (the stage before the E-step)
     if "prep" in paths:
         extra["image_preparation"] = prep_leg(args, box, px, diameter, d_img, nvar)
-    if not runs:  # image preparation alone
-        print(json.dumps({"metric": "image preparation particles/s, box %d" % box,
-                          "value": extra["image_preparation"]["particles_per_s"], "unit": "particles/s", "n_gpus": 1,
-                          "higher_is_better": True, "data": "synthetic", **extra}), flush=True)
+    if not runs:  # preparation stages alone
+        print(json.dumps({"metric": "preparation stages, box %d" % box, "n_gpus": 1, "data": "synthetic", **extra}),
+              flush=True)
         return 0
     head = "proposal" if "proposal" in runs else ("local_split" if "local_split" in runs else next(iter(runs)))
     R = runs[head]

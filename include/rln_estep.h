@@ -143,6 +143,18 @@ int rln_projector_set_mdl_dim(rln_projector *p, int xdim, int ydim, int zdim, in
 int rln_projector_init_mdl(rln_projector *p, const float *real, const float *imag);
 /* AccProjector::initMdl(Complex *data) (acc_projector.h:88-95): host interleaved double complex. */
 int rln_projector_init_mdl_complex(rln_projector *p, const double *data);
+/* Projector::computeFourierTransformMap (src/projector.cpp:116-593) for a 3-D reference and 2-D data, trilinear
+ * interpolation: from a real-space map vol[z][y][x] (ori_size^3 floats, origin at ori_size/2; host, or device when
+ * on_device) to the projector's padded Fourier volume, without a host round trip. Dimensions the projector as the
+ * reference does: padoridim = ROUND(padding_factor * ori_size) made even, pad = padoridim / ori_size,
+ * r_max = min(current_size, ori_size) / 2 (current_size <= 0: ori_size), pad_size = 2 (ROUND(pad r_max) + 1) + 1.
+ * do_gridding: divide the map by sinc^2(r / padoridim) first (Projector::griddingCorrect, :595-628). min_ires > 0:
+ * zero the data within ROUND(min_ires * pad) after the spectrum. Optional host outputs: power_spectrum
+ * [ori_size/2 + 1] (radial average of |data|^2 / 2 per shell ROUND(r / pad)); data_out: the plain array as
+ * interleaved double complex [pad_size][pad_size][pad_size/2 + 1] (PPref.data: z and y from -(pad_size/2), x from 0). */
+int rln_projector_compute_fourier_map(rln_projector *p, const float *vol, int on_device, int ori_size,
+                                      int current_size, double padding_factor, int do_gridding, int min_ires,
+                                      double *power_spectrum, double *data_out);
 /* AccProjector::clear() (acc_projector.h:97) + destructor. */
 int rln_projector_destroy(rln_projector *p);
 
@@ -276,6 +288,9 @@ typedef struct rln_synth_spec {
  * double complex, zdim*ydim*xdim as in the projector layout for (current_size, padding_factor). */
 int rln_synth_reference(const rln_synth_spec *spec, int iclass, double share_fraction, int current_size,
                         double padding_factor, double *out);
+/* The same blob map sampled in real space: out[z][y][x] (ori_size^3 floats, origin at ori_size/2) =
+ * sum of a exp(-|r - c|^2 / (2 s^2)) over the class's blobs. */
+int rln_synth_real_map(const rln_synth_spec *spec, int iclass, double share_fraction, float *out);
 /* Particles [first, first+n): images at current_size (interleaved float complex), CTFs, metadata,
  * true poses (rot, tilt, psi, xoff, yoff, class) and the noise variance per Fourier component. */
 int rln_synth_particles(const rln_synth_spec *spec, int nr_classes, double share_fraction, int current_size,

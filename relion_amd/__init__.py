@@ -135,6 +135,8 @@ _SIGNATURES = {
     "rln_projector_init_mdl": (C.c_int, [_vp, _fp, _fp]),
     "rln_projector_init_mdl_complex": (C.c_int, [_vp, _dp]),
     "rln_projector_destroy": (C.c_int, [_vp]),
+    "rln_projector_compute_fourier_map": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                                    _dp, _dp]),
     "rln_backprojector_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "rln_backprojector_set_mdl_dim": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                                 C.POINTER(C.c_size_t)]),
@@ -161,6 +163,7 @@ _SIGNATURES = {
     "rln_sampling_local": (C.c_int, [C.POINTER(EstepConfig), C.c_double, C.c_double, C.c_double, _ip, _dp, C.c_int,
                                      _ip, _ip, _dp, C.c_int, _ip]),
     "rln_synth_reference": (C.c_int, [C.POINTER(SynthSpec), C.c_int, C.c_double, C.c_int, C.c_double, _dp]),
+    "rln_synth_real_map": (C.c_int, [C.POINTER(SynthSpec), C.c_int, C.c_double, _fp]),
     "rln_synth_particles": (C.c_int, [C.POINTER(SynthSpec), C.c_int, C.c_double, C.c_int, C.c_longlong, C.c_int,
                                       _fp, _fp, C.POINTER(ParticleMeta), _dp, _dp]),
     "rln_ctf_value": (C.c_double, [C.c_double] * 11),
@@ -358,6 +361,15 @@ def synth_reference(spec, iclass, current_size, padding_factor=2.0, share_fracti
     return out
 
 
+def synth_real_map(spec, iclass, ori_size, share_fraction=0.0):
+    """The blob map of `spec` sampled in real space, float32 [ori][ori][ori] (z, y, x), origin at ori/2."""
+    if ori_size != spec.ori_size:
+        raise ValueError("ori_size must equal spec.ori_size")
+    out = np.zeros((ori_size,) * 3, np.float32)
+    _check(lib.rln_synth_real_map(C.byref(spec), iclass, share_fraction, _ptr(out, C.c_float)))
+    return out
+
+
 def synth_particles(spec, n, current_size, nr_classes=1, first=0, share_fraction=0.0):
     xs = current_size // 2 + 1
     fimg = np.zeros((n, current_size, xs, 2), np.float32)
@@ -406,6 +418,32 @@ class AccProjector:
             a = np.stack([a.real, a.imag], axis=-1)
         a = np.ascontiguousarray(a, np.float64)
         _check(lib.rln_projector_init_mdl_complex(self._h, _ptr(a, C.c_double)))
+
+    def computeFourierTransformMap(self, vol, current_size, padding_factor=2.0, do_gridding=True, min_ires=-1,
+                                   return_data=False, ori_size=None, on_device=False):
+        """Projector::computeFourierTransformMap on the device: a real-space map (float32 [ori]^3, or a device
+        address with on_device and ori_size) becomes this projector's padded Fourier volume. Returns the power
+        spectrum [ori/2+1] (and the plain data array, complex128 [S][S][S/2+1], with return_data)."""
+        if on_device:
+            ori, ptr, keep = int(ori_size), _vp(int(vol)), None
+        else:
+            keep = np.ascontiguousarray(vol, np.float32)
+            ori, ptr = keep.shape[0], _vp(keep.ctypes.data)
+        ps = np.zeros(ori // 2 + 1)
+        data = None
+        if return_data:
+            P = int(np.floor(padding_factor * ori + 0.5))
+            P += P % 2
+            r_max = min(current_size if current_size > 0 else ori, ori) // 2
+            S = 2 * (int(np.floor(P / ori * r_max + 0.5)) + 1) + 1
+            data = np.zeros((S, S, S // 2 + 1, 2))
+        _check(lib.rln_projector_compute_fourier_map(self._h, ptr, int(on_device), ori, current_size, padding_factor,
+                                                     int(do_gridding), min_ires, _ptr(ps, C.c_double),
+                                                     None if data is None else _ptr(data, C.c_double)))
+        del keep
+        if data is None:
+            return ps
+        return ps, data[..., 0] + 1j * data[..., 1]
 
     def clear(self):
         if self._h:

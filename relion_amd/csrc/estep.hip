@@ -38,6 +38,18 @@ const char *last_error() { return g_last_error.c_str(); }
 #include "host_common.h"
 
 namespace rln {
+// Packs a plain [z][y][x] float2 volume already on the device into the projector's corner layout.
+int pack_device_into_projector(rln_projector *p, const float2 *dev_plain, hipStream_t st) {
+    const rln::MdlDims &d = p->dims;
+    const rlnk::VolGeom g{d.xdim, d.ydim, d.zdim, d.inity, d.initz};
+    const size_t n = p->bases();
+    hipLaunchKernelGGL(rlnk::k_pack_volume, dim3((unsigned)((n + rlnk::kBlock - 1) / rlnk::kBlock)),
+                       dim3(rlnk::kBlock), 0, st, dev_plain, g, p->data.as<float4>());
+    HIP_TRY(hipGetLastError());
+    p->version++;
+    return RLN_OK;
+}
+
 // Uploads a plain [z][y][x] float2 volume and packs it into the projector's corner layout.
 static int pack_into_projector(rln_projector *p, const float2 *host_plain) {
     const rln::MdlDims &d = p->dims;

@@ -93,3 +93,9 @@ struct rln_backprojector {
     size_t bases() const { return rlnk::packed_bases(dims.xdim, dims.ydim, dims.zdim); }
 };
 
+namespace rln {
+// Packs a plain [z][y][x] float2 volume on the device (dims of p) into p's corner layout, on stream st.
+int pack_device_into_projector(rln_projector *p, const float2 *dev_plain, hipStream_t st);
+}  // namespace rln
+
+

@@ -114,6 +114,26 @@ extern "C" double rln_ctf_value(double defU, double defV, double defAngle, doubl
     return rln::ctf_value(defU, defV, defAngle, voltage, cs, q0, bfactor, scale, phase_shift, x, y);
 }
 
+extern "C" int rln_synth_real_map(const rln_synth_spec *spec, int iclass, double share_fraction, float *out) {
+    using namespace rln;
+    if (!spec || !out || spec->ori_size <= 0) RLN_FAIL(RLN_ERR_ARG, "rln_synth_real_map: invalid argument");
+    const int n = spec->ori_size, h = n / 2;
+    const std::vector<Blob> blobs = blobs_for_class(*spec, iclass, share_fraction);
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < n; ++z) {
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) {
+                double v = 0.;
+                for (const Blob &B : blobs) {
+                    const double dx = x - h - B.x, dy = y - h - B.y, dz = z - h - B.z;
+                    v += B.amp * exp(-(dx * dx + dy * dy + dz * dz) / (2. * B.sigma * B.sigma));
+                }
+                out[((size_t)z * n + y) * n + x] = (float)v;
+            }
+    }
+    return RLN_OK;
+}
+
 extern "C" int rln_synth_reference(const rln_synth_spec *spec, int iclass, double share_fraction, int current_size,
                                    double padding_factor, double *out) {
     using namespace rln;
