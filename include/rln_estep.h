@@ -172,6 +172,20 @@ int rln_backprojector_init_mdl(rln_backprojector *b);
 /* AccBackprojector::getMdlData(real, imag, weight) (acc_backprojector_impl.h:109-138):
  * synchronise and copy to host float arrays of xdim*ydim*zdim elements. */
 int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real, float *imag, float *weight);
+/* BackProjector::reconstruct (src/backprojector.cpp:1379-1780) for a 3-D reference, 2-D data and trilinear
+ * interpolation, the default path (window to the original size): from the data and weight accumulated in b, on the
+ * device. do_map: the weight gains 1 / (pad^3 * tau2_fudge * tau2[ires]) (1 / (0.001 * weight) where tau2 is 0) for
+ * shells ires = ROUND(r / pad) >= minres_map. skip_gridding (relion_refine's default) or max_iter_preweight <= 0:
+ * data / max(weight, 1/1000 of the weight's average over its shell FLOOR(r / pad)). Otherwise data and weight are
+ * divided by normalise (the class's pdf weight sum) and max_iter_preweight gridding iterations with the tabulated
+ * Kaiser-Bessel blob (radius 3.8, alpha 15, order 0) give the weights;
+ * then the inverse transform, the real-space window, the soft mask (radius ori_size/2, width 3, average background)
+ * and the sinc^2 gridding correction. tau2: n_tau2 shells (ori_size/2 + 1). vol_out: ori_size^3 floats [z][y][x],
+ * origin at ori_size/2, host or device (out_on_device). padding_factor * ori_size must be an even integer. */
+int rln_backprojector_reconstruct(rln_backprojector *b, int ori_size, int skip_gridding, int max_iter_preweight,
+                                  int do_map,
+                                  const double *tau2, int n_tau2, double tau2_fudge, double normalise, int minres_map,
+                                  float *vol_out, int out_on_device);
 int rln_backprojector_destroy(rln_backprojector *b);
 
 /* ---------------------------------------------------------------------------------------------

@@ -143,6 +143,8 @@ _SIGNATURES = {
     "rln_backprojector_init_mdl": (C.c_int, [_vp]),
     "rln_backprojector_get_mdl_data": (C.c_int, [_vp, _fp, _fp, _fp]),
     "rln_backprojector_destroy": (C.c_int, [_vp]),
+    "rln_backprojector_reconstruct": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_double, C.c_double,
+                                                C.c_int, _fp, C.c_int]),
     "rln_estep_create": (C.c_int, [C.c_int, C.POINTER(EstepConfig), C.POINTER(_vp)]),
     "rln_estep_destroy": (C.c_int, [_vp]),
     "rln_estep_set_model": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _dp, _dp, _dp, _dp]),
@@ -488,6 +490,20 @@ class AccBackprojector:
         if self._h:
             lib.rln_backprojector_destroy(self._h)
             self._h = _vp()
+
+    def reconstruct(self, ori_size, max_iter_preweight=10, do_map=True, tau2=None, tau2_fudge=1.0, normalise=1.0,
+                    minres_map=3, skip_gridding=True, out_address=None):
+        """BackProjector::reconstruct on the device (src/backprojector.cpp:1379-1780): float32 map [ori]^3.
+
+        skip_gridding defaults to relion_refine's setting (src/ml_optimiser.cpp:586). With out_address (a device
+        pointer to ori^3 floats) the map stays on the device and None is returned."""
+        t = np.ascontiguousarray(tau2 if tau2 is not None else np.zeros(0), np.float64)
+        out = None if out_address is not None else np.zeros((ori_size,) * 3, np.float32)
+        dst = C.cast(C.c_void_p(out_address), _fp) if out is None else _ptr(out, C.c_float)
+        _check(lib.rln_backprojector_reconstruct(self._h, ori_size, int(skip_gridding), max_iter_preweight, int(do_map),
+                                                 _ptr(t, C.c_double) if len(t) else None, len(t), tau2_fudge,
+                                                 normalise, minres_map, dst, int(out is None)))
+        return out
 
     def __del__(self):
         try:

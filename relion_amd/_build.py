@@ -40,7 +40,7 @@ def build(verbose=True):
         o = os.path.join(OBJ, src.replace(".cpp", ".o"))
         _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    for src in ("estep.hip", "proposal.hip", "imgprep.hip", "refprep.hip"):
+    for src in ("estep.hip", "proposal.hip", "imgprep.hip", "refprep.hip", "reconstruct.hip"):
         o = os.path.join(OBJ, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

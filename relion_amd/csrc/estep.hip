@@ -50,6 +50,16 @@ int pack_device_into_projector(rln_projector *p, const float2 *dev_plain, hipStr
     return RLN_OK;
 }
 
+// Sums the backprojector's corner slots into plain [z][y][x] data (re, im) and weight volumes on the device.
+int fold_backprojector_device(rln_backprojector *b, float *re, float *im, float *w, hipStream_t st) {
+    const size_t nv = b->dims.voxels();
+    const rlnk::VolGeom g{b->dims.xdim, b->dims.ydim, b->dims.zdim, b->dims.inity, b->dims.initz};
+    hipLaunchKernelGGL(rlnk::k_fold_slots, dim3((unsigned)((nv + rlnk::kBlock - 1) / rlnk::kBlock)), dim3(rlnk::kBlock),
+                       0, st, b->slots.as<const float>(), g, re, im, w);
+    HIP_TRY(hipGetLastError());
+    return RLN_OK;
+}
+
 // Uploads a plain [z][y][x] float2 volume and packs it into the projector's corner layout.
 static int pack_into_projector(rln_projector *p, const float2 *host_plain) {
     const rln::MdlDims &d = p->dims;

@@ -96,6 +96,8 @@ struct rln_backprojector {
 namespace rln {
 // Packs a plain [z][y][x] float2 volume on the device (dims of p) into p's corner layout, on stream st.
 int pack_device_into_projector(rln_projector *p, const float2 *dev_plain, hipStream_t st);
+// Sums b's corner slots into plain [z][y][x] data (re, im) and weight volumes (dims of b) on the device, on stream st.
+int fold_backprojector_device(rln_backprojector *b, float *re, float *im, float *w, hipStream_t st);
 }  // namespace rln
 
 
