@@ -66,9 +66,10 @@ def parse():
                          "(the E-step workload of the run that saved them; for counter-collection runs)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the fine stage directly instead of through captured graphs (same kernels)")
-    ap.add_argument("--paths", default="proposal,plain,local,prep,refprep",
+    ap.add_argument("--paths", default="proposal,plain,local,prep,refprep,recon",
                     help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant), "
-                         "prep (image preparation before the E-step), refprep (reference preparation)")
+                         "prep (image preparation before the E-step), refprep (reference preparation), recon "
+                         "(reconstruction from the filled backprojector)")
     ap.add_argument("--workload", default="refine256", choices=["refine256", "uneven192"],
                     help="refine256: the headline (box 256); uneven192: box 192, 20,000 particles of log-uniform "
                          "SNR in [0.01, 0.3], global order-3 coarse search, order-4 fine pass (batched fine pass)")
@@ -233,6 +234,53 @@ def refprep_leg(args, spec, box):
         out["cpu_oracle_kind"] = "port (numpy FP64, one process)"
         log(f"[refprep] oracle {out['cpu_oracle_s']} s", args)
     pr.clear()
+    return out
+
+
+def recon_leg(args, bp, box, tau2):
+    """Reconstruction (BackProjector::reconstruct) of the box^3 map from the backprojector the last E-step filled,
+    output left on the device: relion_refine's default path without gridding iterations, and the path with 10
+    gridding iterations, each timed over --steps calls after one warm-up; the FP64 oracle of the default path on the
+    host beside it when --cpu-seconds > 0. HBM bytes counted per call of the default path: the slots folded into the
+    plain arrays (24 per plain voxel), the decentred weight, gridding weight and data written and read by the
+    radial average, the division and the window (112 per half-complex S^3 element), the padded transform written and
+    read by the inverse FFT (32 P^2 (P/2+1)), its real output written and read (8 P^3 + 8 box^3) and the map written
+    (4 box^3); the FFT's internal passes are not counted."""
+    import relion_amd as ra  # noqa: F401
+    from relion_amd import hipmem
+
+    d_out = hipmem.DeviceArray(np.zeros((box,) * 3, np.float32))
+    steps = max(1, args.steps)
+    out = {"box": box}
+    for name, skip, iters in (("skip_gridding", True, 10), ("gridding_10_iterations", False, 10)):
+        kw = dict(max_iter_preweight=iters, do_map=True, tau2=tau2, tau2_fudge=1.0, normalise=1.0, minres_map=3,
+                  skip_gridding=skip, out_address=d_out.address)
+        bp.reconstruct(box, **kw)
+        hipmem.synchronize()
+        t0 = time.time()
+        for _ in range(steps):
+            bp.reconstruct(box, **kw)
+        hipmem.synchronize()
+        ms = 1000.0 * (time.time() - t0) / steps
+        out[name] = {"ms_per_call": round(ms, 2), "calls": steps}
+        log(f"[recon] box {box} {name}: {ms:.1f} ms per call", args)
+    P, S = 2 * box, 2 * (box + 1) + 1
+    nv, nS = S * S * (S // 2 + 1), S * S * (S // 2 + 1)
+    nbytes = 24 * nv + 112 * nS + 32 * P * P * (P // 2 + 1) + 8 * P ** 3 + 12 * box ** 3
+    ms = out["skip_gridding"]["ms_per_call"]
+    out["skip_gridding"]["roofline"] = {"bound": "hbm", "bytes_per_call": nbytes,
+                                        "achieved": round(nbytes / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS,
+                                        "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    if args.cpu_seconds > 0:
+        from oracle import reconstruct as orc
+        re, im, w = bp.getMdlData()
+        t = time.time()
+        orc.reconstruct(re.astype(np.float64) + 1j * im, w.astype(np.float64), box, 2.0, tau2=tau2, minres_map=3,
+                        do_map=True, skip_gridding=True)
+        out["cpu_oracle_skip_gridding_s"] = round(time.time() - t, 2)
+        out["cpu_oracle_kind"] = "port (numpy FP64, one process)"
+        log(f"[recon] oracle {out['cpu_oracle_skip_gridding_s']} s", args)
+    d_out.free()
     return out
 
 
@@ -491,6 +539,9 @@ def main():
             runs[name] = timed(name, local_step, n, args.steps, args.warmup, es_l)
             es_l.close()
 
+    # ---------------------------------------------------------------- reconstruction (the consumer of the backprojector)
+    if "recon" in paths and runs:
+        extra["reconstruction"] = recon_leg(args, bp, box, 1e3 * np.exp(-np.arange(box // 2 + 1) / (box / 16.0)))
     # ---------------------------------------------------------------- reference preparation (the producer of the projector)
     if "refprep" in paths:
         extra["reference_preparation"] = refprep_leg(args, spec, box)
