@@ -188,7 +188,7 @@ def uneven(args):
 
 def committed_traffic(name, kernel):
     """HBM bytes per launch of `kernel` from a committed counter summary (two rocprofv3 --pmc runs of the same
-    workload, FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or (None, None)."""
+    workload, FETCH_SIZE and WRITE_SIZE scaled per kernel by the request-size calibrations), or (None, None)."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None

@@ -2,11 +2,13 @@
 
 Kernel trace:  python relion_amd/profile_summary.py <kt_results.db> [--command "..."] > summary.md
 HBM traffic:   python relion_amd/profile_summary.py --fetch <fetch.db> --write <write.db> [--fetch-scale 2.0]
-               [--write-scale 1.0] [--command "..."] > traffic.md
+               [--write-scale 1.0] [--kernel-fetch-scale NAME=SCALE ...] [--command "..."] > traffic.md
 The traffic mode averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, multiplies them by the
 calibration scales (what the counters read for a device-to-device copy of known size) and reports GB per
 launch and the resulting bandwidth over the dispatch durations of the counter runs, then one line
-``traffic_bytes_per_launch <kernel> <bytes>`` per kernel.
+``traffic_bytes_per_launch <kernel> <bytes>`` per kernel. FETCH_SIZE counts a 128-byte read request as 64 bytes
+and a 64-byte one as 64 bytes, so the copy calibration (x2) holds for streaming kernels; --kernel-fetch-scale sets
+the scale of a kernel whose reads are 64-byte gathers to 1.
 """
 import argparse
 import sqlite3
@@ -52,13 +54,16 @@ def traffic_table(a):
     w = counter(a.write, "WRITE_SIZE")
     if a.command:
         print(f"command: `{a.command}`\n")
-    print(f"FETCH_SIZE x {a.fetch_scale:.3f}, WRITE_SIZE x {a.write_scale:.3f} (copy calibration)\n")
+    print(f"FETCH_SIZE x {a.fetch_scale:.3f}, WRITE_SIZE x {a.write_scale:.3f} (copy calibration)")
+    for k, v in a.kernel_scale.items():
+        print(f"FETCH_SIZE x {v:.3f} for {k} (64-byte gathers)")
+    print()
     print("| kernel | launches | avg us | fetch GB/launch | write GB/launch | GB/s |")
     print("|---|---|---|---|---|---|")
     rows = []
     for name, (n, fk, dur) in f.items():
         wk = w.get(name, (0, 0.0, dur))[1]
-        fb = fk * 1024.0 * a.fetch_scale
+        fb = fk * 1024.0 * a.kernel_scale.get(short(name), a.fetch_scale)
         wb = wk * 1024.0 * a.write_scale
         rows.append((n * dur, name, n, dur, fb, wb))
     top = sorted(rows, reverse=True)[:a.top]
@@ -78,9 +83,11 @@ def main():
     ap.add_argument("--write")
     ap.add_argument("--fetch-scale", type=float, default=2.0)
     ap.add_argument("--write-scale", type=float, default=1.0)
+    ap.add_argument("--kernel-fetch-scale", action="append", default=[], metavar="NAME=SCALE")
     ap.add_argument("--command", default="")
     ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args()
+    a.kernel_scale = {k: float(v) for k, v in (x.rsplit("=", 1) for x in a.kernel_fetch_scale)}
     if a.fetch and a.write:
         traffic_table(a)
     else:
