@@ -344,8 +344,8 @@ def prep_leg(args, box, px, diameter, d_img, nvar):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: run with --gpus 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "this benchmark measures one GPU: run with --gpus 1 "
+                                   "(sharding over several GPUs is designed, not run)"}))
         return 2
     if args.workload == "uneven192":
         return uneven(args)

@@ -137,7 +137,8 @@ def test_projector_roundtrip():
 
 
 def test_global_search_box64():
-    """configs[0]-like: box 64, HEALPix order 2, offsets 5 px step 1, global search, noisy particles."""
+    """The reference's CPU-runnable case in shape: box 64, HEALPix order 2, offsets 5 px step 1, global search,
+    noisy particles."""
     p = make_problem(box=64, px=2.0, order=2, offset_range_px=5, offset_step_px=1, n=8, snr=0.1)
     rg, ro = check_run(p)
     assert (rg["nr_fine_evaluated"] > 0).all()
