@@ -151,7 +151,9 @@ int rln_projector_init_mdl_complex(rln_projector *p, const double *data);
  * do_gridding: divide the map by sinc^2(r / padoridim) first (Projector::griddingCorrect, :595-628). min_ires > 0:
  * zero the data within ROUND(min_ires * pad) after the spectrum. Optional host outputs: power_spectrum
  * [ori_size/2 + 1] (radial average of |data|^2 / 2 per shell ROUND(r / pad)); data_out: the plain array as
- * interleaved double complex [pad_size][pad_size][pad_size/2 + 1] (PPref.data: z and y from -(pad_size/2), x from 0). */
+ * interleaved double complex [pad_size][pad_size][pad_size/2 + 1] (PPref.data: z and y from -(pad_size/2), x from 0).
+ * Work buffers and FFT plans are kept per device for the next call (with the reconstruction's; about 2 GB at
+ * box 256) until the process ends; calls on one device are serialised. */
 int rln_projector_compute_fourier_map(rln_projector *p, const float *vol, int on_device, int ori_size,
                                       int current_size, double padding_factor, int do_gridding, int min_ires,
                                       double *power_spectrum, double *data_out);
@@ -181,10 +183,11 @@ int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real, float *ima
  * Kaiser-Bessel blob (radius 3.8, alpha 15, order 0) give the weights;
  * then the inverse transform, the real-space window, the soft mask (radius ori_size/2, width 3, average background)
  * and the sinc^2 gridding correction. tau2: n_tau2 shells (ori_size/2 + 1). vol_out: ori_size^3 floats [z][y][x],
- * origin at ori_size/2, host or device (out_on_device). padding_factor * ori_size must be an even integer. */
+ * origin at ori_size/2, host or device (out_on_device). padding_factor * ori_size must be an even integer. The
+ * accumulators are left unchanged. Work buffers and FFT plans are kept per device for the next call (about 3.5 GB
+ * at box 256, FP64) until the process ends; calls on one device are serialised. */
 int rln_backprojector_reconstruct(rln_backprojector *b, int ori_size, int skip_gridding, int max_iter_preweight,
-                                  int do_map,
-                                  const double *tau2, int n_tau2, double tau2_fudge, double normalise, int minres_map,
+                                  int do_map, const double *tau2, int n_tau2, double tau2_fudge, double normalise, int minres_map,
                                   float *vol_out, int out_on_device);
 int rln_backprojector_destroy(rln_backprojector *b);
 

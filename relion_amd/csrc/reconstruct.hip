@@ -16,11 +16,13 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "../../include/rln_estep.h"
 #include "host_common.h"
+#include "map_workspace.h"
 
 using namespace rln;
 
@@ -77,7 +79,8 @@ __global__ void __launch_bounds__(kThreads) k_rc_init(const float *__restrict__ 
 }
 
 // Radial sums of the regularised weight for the path without gridding iterations (src/backprojector.cpp:1513-1532):
-// shells FLOOR(r / pad) over r^2 < ROUND((r_max pad)^2), FP64 shared-memory bins.
+// shells FLOOR(r / pad) over r^2 < ROUND((r_max pad)^2), FP64 shared-memory bins; grid-stride over a bounded
+// grid so that each global bin takes few atomics.
 __global__ void __launch_bounds__(kThreads) k_rc_radavg(const double *__restrict__ fweight, Geo g, long long round_max_r2,
                                                         int r_max, double *__restrict__ acc) {
     __shared__ double s_sum[kMaxShells], s_cnt[kMaxShells];
@@ -85,8 +88,7 @@ __global__ void __launch_bounds__(kThreads) k_rc_radavg(const double *__restrict
     __syncthreads();
     const int S = g.S, xs = g.h + 1;
     const size_t n = (size_t)S * S * xs;
-    const size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (e < n) {
+    for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < n; e += (size_t)gridDim.x * kThreads) {
         const int j = (int)(e % xs);
         const size_t zy = e / xs;
         const int i = (int)(zy % S), k = (int)(zy / S);
@@ -204,14 +206,14 @@ __global__ void __launch_bounds__(kThreads) k_rc_window(const double2 *__restric
     out[e] = v;
 }
 
-// Average background of the soft mask over the original-size window (src/mask.cpp:53-74): sum of w and w * v.
+// Average background of the soft mask over the original-size window (src/mask.cpp:53-74): sum of w and w * v,
+// grid-stride over a bounded grid (two FP64 atomics per block).
 __global__ void __launch_bounds__(kThreads) k_rc_bg(const double *__restrict__ m, int P, int ori, double scale,
                                                     double radius, double width, double *__restrict__ acc) {
     __shared__ double red[2][kThreads / 64];
     const size_t n = (size_t)ori * ori * ori;
-    const size_t v = (size_t)blockIdx.x * kThreads + threadIdx.x;
     double sw = 0., swv = 0.;
-    if (v < n) {
+    for (size_t v = (size_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (size_t)gridDim.x * kThreads) {
         const int x = (int)(v % ori);
         const size_t zy = v / ori;
         const int y = (int)(zy % ori), z = (int)(zy / ori);
@@ -220,8 +222,8 @@ __global__ void __launch_bounds__(kThreads) k_rc_bg(const double *__restrict__ m
         if (r >= radius) {
             const double val = m[((size_t)(z + o) * P + (y + o)) * P + (x + o)] * scale;
             const double w = r > radius + width ? 1. : 0.5 + 0.5 * cos(3.14159265358979323846 * (radius + width - r) / width);
-            sw = w;
-            swv = w * val;
+            sw += w;
+            swv += w * val;
         }
     }
     for (int s = 32; s > 0; s >>= 1) {
@@ -334,13 +336,19 @@ extern "C" int rln_backprojector_reconstruct(rln_backprojector *b, int ori_size,
                  nPr = (size_t)P * P * P, nv = d.voxels(), nvol = (size_t)ori_size * ori_size * ori_size;
     HIP_TRY(hipSetDevice(b->device));
     HIP_TRY(hipDeviceSynchronize());  // the E-step's scatter into the slots has finished
+    rln::MapWorkspace &ws = rln::map_workspace(b->device);
+    std::lock_guard<std::mutex> lk(ws.mtx);
     hipStream_t st;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    struct StreamGuard {
+    struct StreamGuard {  // drains the stream before the workspace is released to the next call
         hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
+        ~StreamGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
     } guard{st};
-    DevBuf d_plain, d_fw, d_fn, d_fd, d_fc, d_m, d_table, d_tau, d_acc, d_out;
+    DevBuf &d_plain = ws.buf[0], &d_fw = ws.buf[1], &d_fn = ws.buf[2], &d_fd = ws.buf[3], &d_fc = ws.buf[4],
+           &d_m = ws.buf[5], &d_table = ws.buf[6], &d_tau = ws.buf[7], &d_acc = ws.buf[8], &d_out = ws.buf[9];
     RLN_TRY(d_plain.ensure(3 * sizeof(float) * nv));
     RLN_TRY(d_fw.ensure(sizeof(double) * nS));
     RLN_TRY(d_fn.ensure(sizeof(double) * nS));
@@ -359,6 +367,10 @@ extern "C" int rln_backprojector_reconstruct(rln_backprojector *b, int ori_size,
     if (do_map) HIP_TRY(hipMemcpyAsync(d_tau.ptr, tau2, sizeof(double) * n_tau2, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemsetAsync(d_acc.ptr, 0, sizeof(double) * 2 * r_max, st));
     auto grid = [](size_t n) { return dim3((unsigned)((n + rlnc::kThreads - 1) / rlnc::kThreads)); };
+    // reductions: at most 2048 blocks (8 per CU), each looping over its share
+    auto reduce_grid = [](size_t n) {
+        return dim3((unsigned)std::min<size_t>(2048, (n + rlnc::kThreads - 1) / rlnc::kThreads));
+    };
     const double ovs = pad * pad * pad;
     hipLaunchKernelGGL(rlnc::k_rc_init, grid(nS), dim3(rlnc::kThreads), 0, st, re, im, wt, g, do_map ? 1 : 0,
                        d_tau.as<const double>(), std::max(1, n_tau2), 1. / (ovs * tau2_fudge), minres_map,
@@ -366,7 +378,7 @@ extern "C" int rln_backprojector_reconstruct(rln_backprojector *b, int ori_size,
     HIP_TRY(hipGetLastError());
     if (!gridding) {
         const double rm = (double)r_max * pad;
-        hipLaunchKernelGGL(rlnc::k_rc_radavg, grid(nS), dim3(rlnc::kThreads), 0, st, d_fw.as<const double>(), g,
+        hipLaunchKernelGGL(rlnc::k_rc_radavg, reduce_grid(nS), dim3(rlnc::kThreads), 0, st, d_fw.as<const double>(), g,
                            (long long)ROUND(rm * rm), r_max, d_acc.as<double>());
         hipLaunchKernelGGL(rlnc::k_rc_skip, grid(nS), dim3(rlnc::kThreads), 0, st, d_fw.as<const double>(), g, r_max,
                            d_acc.as<const double>(), d_fd.as<double2>(), d_fn.as<double>());
@@ -374,19 +386,9 @@ extern "C" int rln_backprojector_reconstruct(rln_backprojector *b, int ori_size,
         HIP_TRY(hipMemsetAsync(d_acc.ptr, 0, sizeof(double) * 2, st));  // reused by the soft mask's background sums
     }
     hipfftHandle c2r_s = 0, r2c_s = 0, c2r_p = 0;
-    struct PlanGuard {
-        hipfftHandle *h[3];
-        ~PlanGuard() {
-            for (auto p : h)
-                if (*p) hipfftDestroy(*p);
-        }
-    } pg{{&c2r_s, &r2c_s, &c2r_p}};
     if (gridding) {
-        if (hipfftPlan3d(&c2r_s, S, S, S, HIPFFT_Z2D) != HIPFFT_SUCCESS ||
-            hipfftPlan3d(&r2c_s, S, S, S, HIPFFT_D2Z) != HIPFFT_SUCCESS)
-            RLN_FAIL(RLN_ERR_HIP, "hipfftPlan3d of the %d^3 box failed", S);
-        hipfftSetStream(c2r_s, st);
-        hipfftSetStream(r2c_s, st);
+        RLN_TRY(ws.plan3d(S, HIPFFT_Z2D, st, &c2r_s));
+        RLN_TRY(ws.plan3d(S, HIPFFT_D2Z, st, &r2c_s));
     }
     for (int it = 0; gridding && it < max_iter_preweight; ++it) {
         hipLaunchKernelGGL(rlnc::k_rc_product, grid(nS), dim3(rlnc::kThreads), 0, st, d_fn.as<const double>(),
@@ -404,12 +406,11 @@ extern "C" int rln_backprojector_reconstruct(rln_backprojector *b, int ori_size,
     hipLaunchKernelGGL(rlnc::k_rc_window, grid(nP), dim3(rlnc::kThreads), 0, st, d_fd.as<const double2>(),
                        d_fn.as<const double>(), g, P, d_fc.as<double2>());
     HIP_TRY(hipGetLastError());
-    if (hipfftPlan3d(&c2r_p, P, P, P, HIPFFT_Z2D) != HIPFFT_SUCCESS) RLN_FAIL(RLN_ERR_HIP, "hipfftPlan3d of %d^3 failed", P);
-    hipfftSetStream(c2r_p, st);
+    RLN_TRY(ws.plan3d(P, HIPFFT_Z2D, st, &c2r_p));
     if (hipfftExecZ2D(c2r_p, d_fc.as<hipfftDoubleComplex>(), d_m.as<hipfftDoubleReal>()) != HIPFFT_SUCCESS)
         RLN_FAIL(RLN_ERR_HIP, "Z2D of the padded box failed");
     const double scale = 1. / (ovs * ori_size), radius = ori_size / 2., width = 3.;  // softMaskOutsideMap defaults
-    hipLaunchKernelGGL(rlnc::k_rc_bg, grid(nvol), dim3(rlnc::kThreads), 0, st, d_m.as<const double>(), P, ori_size, scale,
+    hipLaunchKernelGGL(rlnc::k_rc_bg, reduce_grid(nvol), dim3(rlnc::kThreads), 0, st, d_m.as<const double>(), P, ori_size, scale,
                        radius, width, d_acc.as<double>());
     float *dst = vol_out;
     if (!out_on_device) {

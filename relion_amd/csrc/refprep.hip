@@ -10,11 +10,13 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "../../include/rln_estep.h"
 #include "host_common.h"
+#include "map_workspace.h"
 
 using namespace rln;
 
@@ -53,6 +55,7 @@ __global__ void __launch_bounds__(kThreads) k_grid_pad(const float *__restrict__
 
 // Window of the centred transform into the projector's plain array [S][S][S/2+1] (S = pad_size, z and y from -S/2,
 // x from 0) and the per-shell power (src/projector.cpp:506-530). F: unnormalised R2C output [P][P][P/2+1].
+// Grid-stride over a bounded grid, so that each global shell bin takes few FP64 atomics.
 __global__ void __launch_bounds__(kThreads) k_fourier_window(const float2 *__restrict__ F, int P, int S, long long max_r2,
                                                              long long min_r2, double scale, double pad, int nshell,
                                                              float2 *__restrict__ data, double *__restrict__ power,
@@ -62,8 +65,7 @@ __global__ void __launch_bounds__(kThreads) k_fourier_window(const float2 *__res
     __syncthreads();
     const int h = S / 2, xs = h + 1;
     const size_t n = (size_t)S * S * xs;
-    const size_t q = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (q < n) {
+    for (size_t q = (size_t)blockIdx.x * kThreads + threadIdx.x; q < n; q += (size_t)gridDim.x * kThreads) {
         const int ix = (int)(q % xs);
         const size_t zy = q / xs;
         const int iy = (int)(zy % S), iz = (int)(zy / S);
@@ -116,15 +118,20 @@ extern "C" int rln_projector_compute_fourier_map(rln_projector *p, const float *
     const double normfft = pad * pad * pad * ori_size;  // 3-D reference, 2-D data
     RLN_TRY(rln_projector_set_mdl_dim(p, h + 1, S, S, -h, -h, r_max, pad, nullptr));
     HIP_TRY(hipSetDevice(p->device));
+    rln::MapWorkspace &ws = rln::map_workspace(p->device);
+    std::lock_guard<std::mutex> lk(ws.mtx);
     hipStream_t st;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    struct StreamGuard {
+    struct StreamGuard {  // drains the stream before the workspace is released to the next call
         hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
+        ~StreamGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
     } guard{st};
     const size_t nvol = (size_t)ori_size * ori_size * ori_size, npad = (size_t)P * P * P;
     const size_t nF = (size_t)P * P * (P / 2 + 1), nd = (size_t)S * S * (h + 1);
-    DevBuf d_vol, d_pad, d_F, d_data, d_pow;
+    DevBuf &d_vol = ws.buf[0], &d_pad = ws.buf[1], &d_F = ws.buf[2], &d_data = ws.buf[3], &d_pow = ws.buf[4];
     const float *src = vol;
     if (!on_device) {
         RLN_TRY(d_vol.ensure(sizeof(float) * nvol));
@@ -139,16 +146,12 @@ extern "C" int rln_projector_compute_fourier_map(rln_projector *p, const float *
     hipLaunchKernelGGL(rlnr::k_grid_pad, dim3((unsigned)((npad + rlnr::kThreads - 1) / rlnr::kThreads)),
                        dim3(rlnr::kThreads), 0, st, src, ori_size, P, do_gridding ? 1 : 0, d_pad.as<float>());
     HIP_TRY(hipGetLastError());
-    hipfftHandle plan;
-    if (hipfftPlan3d(&plan, P, P, P, HIPFFT_R2C) != HIPFFT_SUCCESS) RLN_FAIL(RLN_ERR_HIP, "hipfftPlan3d failed");
-    struct PlanGuard {
-        hipfftHandle h;
-        ~PlanGuard() { hipfftDestroy(h); }
-    } pguard{plan};
-    if (hipfftSetStream(plan, st) != HIPFFT_SUCCESS) RLN_FAIL(RLN_ERR_HIP, "hipfftSetStream failed");
+    hipfftHandle plan = 0;
+    RLN_TRY(ws.plan3d(P, HIPFFT_R2C, st, &plan));
     if (hipfftExecR2C(plan, d_pad.as<hipfftReal>(), d_F.as<hipfftComplex>()) != HIPFFT_SUCCESS)
         RLN_FAIL(RLN_ERR_HIP, "hipfftExecR2C failed");
-    hipLaunchKernelGGL(rlnr::k_fourier_window, dim3((unsigned)((nd + rlnr::kThreads - 1) / rlnr::kThreads)),
+    hipLaunchKernelGGL(rlnr::k_fourier_window,
+                       dim3((unsigned)std::min<size_t>(2048, (nd + rlnr::kThreads - 1) / rlnr::kThreads)),
                        dim3(rlnr::kThreads), 0, st, d_F.as<const float2>(), P, S, max_r2, min_r2,
                        normfft / (double)npad, pad, nshell, d_data.as<float2>(), d_pow.as<double>(),
                        d_pow.as<double>() + nshell);
