@@ -58,6 +58,8 @@ def parse():
                     help="CPU oracle budget: batches of one particle per thread until this much time (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--format-particles", type=int, default=4096,
+                    help="particles written to and read from MRC stacks and a STAR file in the formats leg")
     ap.add_argument("--prep-particles", type=int, default=8192)
     ap.add_argument("--lists-out", default=None,
                     help="save the proposal's orientation lists of the last timed step (npz), for profiling runs")
@@ -66,10 +68,10 @@ def parse():
                          "(the E-step workload of the run that saved them; for counter-collection runs)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the fine stage directly instead of through captured graphs (same kernels)")
-    ap.add_argument("--paths", default="proposal,plain,local,prep,refprep,recon",
+    ap.add_argument("--paths", default="proposal,plain,local,prep,refprep,recon,formats",
                     help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant), "
                          "prep (image preparation before the E-step), refprep (reference preparation), recon "
-                         "(reconstruction from the filled backprojector)")
+                         "(reconstruction from the filled backprojector), formats (particles from STAR and MRC files)")
     ap.add_argument("--workload", default="refine256", choices=["refine256", "uneven192"],
                     help="refine256: the headline (box 256); uneven192: box 192, 20,000 particles of log-uniform "
                          "SNR in [0.01, 0.3], global order-3 coarse search, order-4 fine pass (batched fine pass)")
@@ -282,6 +284,63 @@ def recon_leg(args, bp, box, tau2):
         log(f"[recon] oracle {out['cpu_oracle_skip_gridding_s']} s", args)
     d_out.free()
     return out
+
+
+def formats_leg(args, box, px):
+    """Particle input from disk: --format-particles images of box^2 float32 in MRC stacks of 1,024 with a particles
+    STAR file (optics + particles tables) written to a temporary directory, then read back by read_particles (STAR
+    parse, per-particle metadata, images by 16 threads, stacks in the page cache as just written). CPU leg beside
+    it: one process reading every image with numpy (np.fromfile at the image's offset), the reference's
+    per-particle Image::read pattern."""
+    import shutil
+    import tempfile
+
+    from relion_amd import formats as F
+
+    n = args.format_particles
+    tmp = tempfile.mkdtemp(prefix="rln_formats_")
+    try:
+        rng = np.random.default_rng(args.seed)
+        block = rng.normal(size=(64, box, box)).astype(np.float32)
+        names, metas = [], []
+        for k0 in range(0, n, 1024):
+            k = min(1024, n - k0)
+            stack = np.tile(block, (-(-k // 64), 1, 1))[:k]
+            fn = f"stack{k0 // 1024:03d}.mrcs"
+            F.write_mrc(os.path.join(tmp, fn), stack, pixel_size=px)
+            names += [f"{i + 1:06d}@{fn}" for i in range(k)]
+        for i in range(n):
+            metas.append(dict(norm_correction=1.0, old_offset_x=float(i % 5 - 2), old_offset_y=float(i % 3 - 1),
+                              defocus_u=15000.0 + i, defocus_v=14800.0 + i, defocus_angle=30.0, bfactor=0.0,
+                              kfactor=1.0, phase_shift=0.0))
+        star = os.path.join(tmp, "particles.star")
+        F.write_particles(star, names, metas, [dict(group=1, pixel_size=px, image_size=box, voltage=300.0, cs=2.7,
+                                                    q0=0.1)])
+        t = time.time()
+        F.read_particles(star, base_dir=tmp, load_images=False)
+        t_star = time.time() - t
+        t = time.time()
+        ps = F.read_particles(star, base_dir=tmp, threads=16)
+        t_all = time.time() - t
+        gb = ps.images.nbytes / 1e9
+        del ps
+        t = time.time()
+        m = 0
+        while m < n and time.time() - t < max(1.0, args.cpu_seconds):
+            idx, fn = names[m].split("@")
+            np.fromfile(os.path.join(tmp, fn), np.float32, count=box * box, offset=1024 + (int(idx) - 1) * box * box * 4)
+            m += 1
+        t_np = time.time() - t
+        out = {"particles": n, "box": box, "stacks": -(-n // 1024), "star_parse_ms": round(1000 * t_star, 1),
+               "read_particles_ms": round(1000 * t_all, 1), "particles_per_s": round(n / t_all, 1),
+               "image_GB_per_s": round(gb / t_all, 2),
+               "cpu_per_image_numpy": {"particles_per_s": round(m / t_np, 1), "sample": f"first {m} images",
+                                       "cores": 1}}
+        log(f"[formats] {n} particles: STAR {1000 * t_star:.1f} ms, with images {1000 * t_all:.1f} ms "
+            f"({gb / t_all:.2f} GB/s); numpy per image {m / t_np:.0f} particles/s", args)
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def prep_leg(args, box, px, diameter, d_img, nvar):
@@ -545,6 +604,9 @@ def main():
     # ---------------------------------------------------------------- reference preparation (the producer of the projector)
     if "refprep" in paths:
         extra["reference_preparation"] = refprep_leg(args, spec, box)
+    # ---------------------------------------------------------------- particle input from STAR / MRC files
+    if "formats" in paths:
+        extra["particle_input"] = formats_leg(args, box, px)
     # ---------------------------------------------------------------- image preparation (the stage before the E-step)
     if "prep" in paths:
         extra["image_preparation"] = prep_leg(args, box, px, diameter, d_img, nvar)

@@ -36,7 +36,7 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     objs = []
-    for src in ("sampling.cpp", "synth.cpp"):
+    for src in ("sampling.cpp", "synth.cpp", "formats.cpp"):
         o = os.path.join(OBJ, src.replace(".cpp", ".o"))
         _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
