@@ -536,6 +536,52 @@ def main():
 
         runs["proposal"] = timed("proposal", proposal_step, n, args.steps, args.warmup, es_p)
         extra["proposal_predict_ms"] = round(float(np.mean(pred_ms[-args.steps:])), 2)
+
+        def pipelined_steps(steps):
+            """`steps` proposal steps over the batch, the inference and orientation lists of step k + 1 running
+            (GPU: the proposal's stream; host: this thread) while step k's E-step runs (its own stream, called from
+            a second thread). Every step does the same work as proposal_step; only the order overlaps."""
+            import threading
+
+            def lists_of():
+                pp.predict_launch(d_img.address, d_ctf.address, n, box, topk=args.topk, on_device=True)
+                bins, probs = pp.predict_finish()
+                return pp.orientation_lists(bins, probs, order, n_psi, min_top1=args.min_top1)
+
+            lists = lists_of()
+            res = None
+            for k in range(steps):
+                bp.initMdl()
+                es_p.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
+                es_p.set_orientation_lists(*lists)
+                box_ = {}
+
+                def run():
+                    try:
+                        box_["res"] = es_p.run(d_img.address, d_ctf.address, meta, on_device=True)[0]
+                    except BaseException as exc:  # re-raised in the caller
+                        box_["exc"] = exc
+
+                th = threading.Thread(target=run)
+                th.start()
+                nxt = lists_of() if k + 1 < steps else None
+                th.join()
+                if "exc" in box_:
+                    raise box_["exc"]
+                res, cur = box_["res"], lists
+                if nxt is not None:
+                    lists = nxt
+            return res, cur[0]
+
+        hipmem.synchronize()
+        t0 = time.time()
+        pl_out = pipelined_steps(args.steps)
+        hipmem.synchronize()
+        el = time.time() - t0
+        runs["proposal_pipelined"] = dict(out=pl_out, elapsed=el, timings=es_p.timings(), counters=es_p.counters(),
+                                          value=n * args.steps / el, ms=1000.0 * el / args.steps, steps=args.steps,
+                                          warmup=args.warmup, particles=n)
+        log(f"[proposal_pipelined] timed {args.steps} steps: {el:.2f} s", args)
         off = runs["proposal"]["out"][1]
         if args.lists_out:
             bins, probs = pp.predict(d_img.address, d_ctf.address, n, box, topk=args.topk, on_device=True)
@@ -614,16 +660,18 @@ def main():
         print(json.dumps({"metric": "preparation stages, box %d" % box, "n_gpus": 1, "data": "synthetic", **extra}),
               flush=True)
         return 0
-    head = "proposal" if "proposal" in runs else ("local_split" if "local_split" in runs else next(iter(runs)))
+    head = ("proposal_pipelined" if "proposal_pipelined" in runs else "proposal" if "proposal" in runs else
+            ("local_split" if "local_split" in runs else next(iter(runs))))
     R = runs[head]
     timings, counters = R["timings"], R["counters"]
-    res = R["out"][0] if head == "proposal" else R["out"]
+    is_prop = head in ("proposal", "proposal_pipelined")
+    res = R["out"][0] if is_prop else R["out"]
     errs = np.array([angle_between(pose[i, :3], (res[i]["rot"], res[i]["tilt"], res[i]["psi"]))
                      for i in range(0, len(res), max(1, len(res) // 500))])
 
     # roofline of the dominant kernel of the reported path
     stage = max(("fine_diff2", "coarse_diff2", "wsum_backproject"), key=lambda k: timings[k])
-    split = head in ("proposal", "local_split")
+    split = is_prop or head == "local_split"
     if stage == "coarse_diff2":
         work = 4.0 * counters["coarse_poses"] * counters["coarse_pixels"]      # FLOP, cross-term count
         bound, unit = "mfma", "TFLOP/s"
@@ -642,7 +690,7 @@ def main():
     traffic, traffic_src = None, None
     if args.traffic_kib is not None:
         traffic, traffic_src = round(args.traffic_kib * 1024.0), "--traffic-kib"
-    elif head == "proposal" and box == 256 and n == 10000 and stage == "fine_diff2":
+    elif is_prop and box == 256 and n == 10000 and stage == "fine_diff2":
         traffic, traffic_src = committed_traffic("estep_proposal_box256_order3_10k_traffic.md", "rlnk::k_fine_diff2<0>")
     roofline = {"kernel": stage, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 1),
                 "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
@@ -653,7 +701,7 @@ def main():
     if args.cpu_seconds > 0:
         import oracle
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        ccfg = cfg_g if head == "proposal" else cfg_l
+        ccfg = cfg_g if is_prop else cfg_l
         m, dt = 0, 0.0
         while dt < args.cpu_seconds and m < n:
             k = min(threads, n - m)
@@ -663,7 +711,7 @@ def main():
                          pdf_direction=None if ccfg.local_search else pdf_dir[None, :], threads=threads)
             dt += time.time() - t
             m += k
-        search = "exhaustive global" if head == "proposal" else "local"
+        search = "exhaustive global" if is_prop else "local"
         cpu = {"value": round(m / dt, 4), "unit": "particles/s", "cores": threads, "kind": "port",
                "sample": f"first {m} particles of the same data, {search} search ({dt:.1f} s), FP64 oracle, one "
                          "particle per OpenMP thread"}
@@ -678,6 +726,12 @@ def main():
     speed = {}
     if "proposal" in runs and "plain" in runs:
         speed["proposal_vs_exhaustive_plain"] = round(runs["proposal"]["value"] / runs["plain"]["value"], 2)
+    if "proposal_pipelined" in runs and "plain" in runs:
+        speed["proposal_pipelined_vs_exhaustive_plain"] = round(runs["proposal_pipelined"]["value"] /
+                                                                runs["plain"]["value"], 2)
+    if "proposal_pipelined" in runs:
+        speed["pipelined_vs_sequential_proposal"] = round(runs["proposal_pipelined"]["value"] /
+                                                          runs["proposal"]["value"], 3)
     if "local_split" in runs and "local_plain" in runs:
         speed["local_split_vs_local_plain"] = round(runs["local_split"]["value"] / runs["local_plain"]["value"], 3)
     out = {
@@ -686,9 +740,11 @@ def main():
         "warmup": R["warmup"], "ms_per_step": round(R["ms"], 2), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16x3+f32" if split else "f32", "data": "synthetic",
         "config": {"workload": (f"refine box{box} order{order} "
-                                f"{'global search seeded by the pose proposal' if head == 'proposal' else 'local search'}"
+                                f"{'global search seeded by the pose proposal' if is_prop else 'local search'}"
                                 f" {n} particles"),
                    "path": {"proposal": "proposal top-16 + split-bf16 coarse + fp32 fine",
+                            "proposal_pipelined": "proposal top-16 + split-bf16 coarse + fp32 fine; the next batch's "
+                                                  "inference and orientation lists overlap this batch's E-step",
                             "local_split": "split-bf16 coarse + fp32 fine"}.get(head, head),
                    "global_batch": n, "coarse_size": coarse, "particle_diameter_A": diameter, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu,

@@ -79,6 +79,14 @@ int rln_proposal_train(rln_proposal *p, int epochs, int batch, double lr, double
 /* Top-k bins and their softmax probabilities per particle (bins[n][topk], probs[n][topk], descending). */
 int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
                          int on_device, int topk, int *bins, float *probs);
+/* The same in two halves, so that a caller can overlap the network with other GPU or host work (the next batch's
+ * inference with the current batch's E-step): launch enqueues the inference and the copy of the top-k into pinned
+ * host memory on the proposal's stream and returns; finish waits for them and fills bins and probs. One prediction
+ * may be in flight per proposal (a second launch before the finish is RLN_ERR_STATE); host inputs (on_device = 0)
+ * must stay valid until the finish. Other calls on the proposal stay ordered after the prediction on its stream. */
+int rln_proposal_predict_launch(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
+                                int on_device, int topk);
+int rln_proposal_predict_finish(rln_proposal *p, int *bins, float *probs);
 
 /* The bin of a pose, and the coarse grid orientations (idir * n_psi_grid + ipsi at grid_order >=
  * healpix_order, grid psi step 360 / n_psi_grid) that a bin covers: the NEST children of its direction times

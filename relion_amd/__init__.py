@@ -184,6 +184,8 @@ _SIGNATURES = {
     "rln_proposal_training_size": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
     "rln_proposal_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, _dp]),
     "rln_proposal_predict": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _ip, _fp]),
+    "rln_proposal_predict_launch": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "rln_proposal_predict_finish": (C.c_int, [_vp, _ip, _fp]),
     "rln_proposal_pose_bin": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _ip]),
     "rln_proposal_bin_orientations": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip]),
     "rln_proposal_orientation_sets": (C.c_int, [_vp, C.c_int, _ip, _fp, C.c_int, C.c_int, C.c_int, C.c_double, _ip,

@@ -49,6 +49,11 @@ struct rln_proposal {
     rln::DevBuf x[rln::kLayers + 1], col[rln::kLayers], logits, dlog, dtmp, dz_a, dz_b, loss, labels, idx;
     rln::DevBuf src, src_ctf;  // particle images and CTFs staged from the host
     rln::DevBuf top_bins, top_probs;  // predictions of one predict call
+    // a prediction launched and not yet finished: its size, the pinned host copies of its top-k, its end event
+    int pend_n = -1, pend_topk = 0;
+    void *h_top = nullptr;  // pinned: int bins[n][topk] then float probs[n][topk]
+    size_t h_top_bytes = 0;
+    hipEvent_t pend_ev = nullptr;
     // training set
     rln::DevBuf train_img;  // bf16 [n_train][W][W]
     std::vector<int> train_lab;
@@ -370,6 +375,8 @@ extern "C" int rln_proposal_destroy(rln_proposal *p) {
         (void)hipStreamSynchronize(p->stream);
         (void)hipStreamDestroy(p->stream);
     }
+    if (p->pend_ev) (void)hipEventDestroy(p->pend_ev);
+    if (p->h_top) (void)hipHostFree(p->h_top);
     delete p;
     return RLN_OK;
 }
@@ -573,14 +580,24 @@ extern "C" int rln_proposal_train(rln_proposal *p, int epochs, int batch, double
     return RLN_OK;
 }
 
-extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
-                                    int on_device, int topk, int *bins, float *probs) {
-    if (!p || n < 0 || (n > 0 && (!fimg || !bins || !probs))) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_predict: null argument");
+extern "C" int rln_proposal_predict_launch(rln_proposal *p, int n, const float *fimg, const float *fctf,
+                                           int current_size, int on_device, int topk) {
+    if (!p || n < 0 || (n > 0 && !fimg)) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_predict_launch: null argument");
     if (topk < 1 || topk > rlnp::kMaxTopK || topk > p->NB) RLN_FAIL(RLN_ERR_ARG, "topk %d out of range", topk);
     std::lock_guard<std::mutex> lk(p->mtx);
+    if (p->pend_n >= 0) RLN_FAIL(RLN_ERR_STATE, "a prediction is in flight: finish it first");
     HIP_TRY(hipSetDevice(p->device));
+    if (!p->pend_ev) HIP_TRY(hipEventCreateWithFlags(&p->pend_ev, hipEventDisableTiming));
+    const size_t need = (sizeof(int) + sizeof(float)) * (size_t)std::max(n, 1) * topk;
+    if (need > p->h_top_bytes) {
+        if (p->h_top) HIP_TRY(hipHostFree(p->h_top));
+        p->h_top = nullptr;
+        p->h_top_bytes = 0;
+        HIP_TRY(hipHostMalloc(&p->h_top, need, hipHostMallocDefault));
+        p->h_top_bytes = need;
+    }
     // chunks run back to back on the stream (they share the activation buffers in stream order); the top-k
-    // of all particles collect on the device and come back in one copy
+    // of all particles collect on the device and come back in one copy into pinned memory
     const int chunk = 1024;
     RLN_TRY(ensure_batch(*p, std::min(n, chunk)));
     RLN_TRY(p->top_bins.ensure(sizeof(int) * (size_t)std::max(n, 1) * topk));
@@ -597,13 +614,39 @@ extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, c
         HIP_TRY(hipGetLastError());
     }
     if (n > 0) {
-        HIP_TRY(hipMemcpyAsync(bins, p->top_bins.ptr, sizeof(int) * (size_t)n * topk, hipMemcpyDeviceToHost,
-                               p->stream));
-        HIP_TRY(hipMemcpyAsync(probs, p->top_probs.ptr, sizeof(float) * (size_t)n * topk, hipMemcpyDeviceToHost,
-                               p->stream));
+        char *h = static_cast<char *>(p->h_top);
+        HIP_TRY(hipMemcpyAsync(h, p->top_bins.ptr, sizeof(int) * (size_t)n * topk, hipMemcpyDeviceToHost, p->stream));
+        HIP_TRY(hipMemcpyAsync(h + sizeof(int) * (size_t)n * topk, p->top_probs.ptr, sizeof(float) * (size_t)n * topk,
+                               hipMemcpyDeviceToHost, p->stream));
     }
-    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipEventRecord(p->pend_ev, p->stream));
+    p->pend_n = n;
+    p->pend_topk = topk;
     return RLN_OK;
+}
+
+extern "C" int rln_proposal_predict_finish(rln_proposal *p, int *bins, float *probs) {
+    if (!p) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_predict_finish: null proposal");
+    std::lock_guard<std::mutex> lk(p->mtx);
+    if (p->pend_n < 0) RLN_FAIL(RLN_ERR_STATE, "no prediction in flight");
+    const size_t k = (size_t)p->pend_n * p->pend_topk;
+    if (k > 0 && (!bins || !probs)) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_predict_finish: null output");
+    HIP_TRY(hipSetDevice(p->device));
+    p->pend_n = -1;  // the in-flight state ends here whatever the outcome
+    HIP_TRY(hipEventSynchronize(p->pend_ev));
+    if (k > 0) {
+        const char *h = static_cast<const char *>(p->h_top);
+        std::memcpy(bins, h, sizeof(int) * k);
+        std::memcpy(probs, h + sizeof(int) * k, sizeof(float) * k);
+    }
+    return RLN_OK;
+}
+
+extern "C" int rln_proposal_predict(rln_proposal *p, int n, const float *fimg, const float *fctf, int current_size,
+                                    int on_device, int topk, int *bins, float *probs) {
+    if (!p || n < 0 || (n > 0 && (!fimg || !bins || !probs))) RLN_FAIL(RLN_ERR_ARG, "rln_proposal_predict: null argument");
+    RLN_TRY(rln_proposal_predict_launch(p, n, fimg, fctf, current_size, on_device, topk));
+    return rln_proposal_predict_finish(p, bins, probs);
 }
 
 extern "C" int rln_proposal_pose_bin(const rln_proposal *p, double rot, double tilt, double psi, int *bin) {

@@ -15,6 +15,7 @@ class PoseProposal:
 
     def __init__(self, window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=1, device=0):
         self.cfg = ProposalConfig(window, channels, healpix_order, n_psi_bins, seed)
+        self._pending = None
         self._h = _vp()
         _check(lib.rln_proposal_create(device, C.byref(self.cfg), C.byref(self._h)))
         n = C.c_longlong()
@@ -89,6 +90,25 @@ class PoseProposal:
         _check(lib.rln_proposal_predict(self._h, n, a, c, current_size, 1 if on_device else 0, topk,
                                         _ptr(bins, C.c_int), _ptr(probs, C.c_float)))
         del keep, keep_c
+        return bins, probs
+
+    def predict_launch(self, fimg, fctf, n, current_size, topk=16, on_device=False):
+        """Enqueues a prediction on the proposal's stream (rln_proposal_predict_launch); predict_finish returns it."""
+        a, keep = self._images_arg(fimg, on_device)
+        c, keep_c = self._images_arg(fctf, on_device)
+        _check(lib.rln_proposal_predict_launch(self._h, n, a, c, current_size, 1 if on_device else 0, topk))
+        self._pending = (n, topk, keep, keep_c)  # host inputs stay alive until the finish
+
+    def predict_finish(self):
+        if self._pending is None:
+            _check(lib.rln_proposal_predict_finish(self._h, None, None))  # raises: no prediction in flight
+        n, topk, _, _ = self._pending
+        bins = np.zeros((n, topk), np.int32)
+        probs = np.zeros((n, topk), np.float32)
+        try:
+            _check(lib.rln_proposal_predict_finish(self._h, _ptr(bins, C.c_int), _ptr(probs, C.c_float)))
+        finally:
+            self._pending = None
         return bins, probs
 
     def pose_bin(self, rot, tilt, psi):

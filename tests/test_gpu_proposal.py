@@ -18,6 +18,7 @@ import oracle
 import relion_amd as ra
 from oracle import pose_cnn as pc
 from problems import make_problem
+from relion_amd import hipmem
 from relion_amd.proposal import PoseProposal
 
 pytestmark = pytest.mark.gpu
@@ -183,4 +184,47 @@ def test_proposal_predict_matches_training_forward():
         ref = np.argsort(-z[i], kind="stable")[:16]
         assert np.array_equal(bins[i], ref), f"particle {i}: top-k bins differ"
         assert np.allclose(probs[i], pr[i][ref], rtol=1e-5, atol=1e-7)
+    pp.close()
+
+
+def test_proposal_predict_in_two_halves():
+    """predict_launch + predict_finish return what predict returns, host and device inputs; one prediction in
+    flight at a time; the launch overlaps an E-step on its own stream without changing either result."""
+    import threading
+    p = make_problem(box=64, px=2.0, order=2, n=24, snr=0.1, seed=95)
+    print("[proposal] predict in two halves, 24 particles, box 64", flush=True)
+    pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=13)
+    rng = np.random.default_rng(14)
+    w = pp.get_params()
+    w += rng.normal(0, 0.01, w.size).astype(np.float32)
+    pp.set_params(w)
+    ref_b, ref_p = pp.predict(p.fimg, p.fctf, 24, 64, topk=8)
+    pp.predict_launch(p.fimg, p.fctf, 24, 64, topk=8)
+    with pytest.raises(ra.RelionError):
+        pp.predict_launch(p.fimg, p.fctf, 24, 64, topk=8)  # one in flight
+    b, pr = pp.predict_finish()
+    assert np.array_equal(b, ref_b) and np.array_equal(pr, ref_p)
+    with pytest.raises(ra.RelionError):
+        pp.predict_finish()  # nothing in flight
+    d_img, d_ctf = hipmem.DeviceArray(p.fimg), hipmem.DeviceArray(p.fctf)
+    # an E-step in a second thread while the prediction runs
+    dims = ra.volume_dims(64)
+    proj = ra.AccProjector(0)
+    proj.setMdlDim(*dims, 2.0)
+    proj.initMdl(p.volumes[0])
+    bp = ra.AccBackprojector(0)
+    bp.setMdlDim(*dims, 2.0)
+    bp.initMdl()
+    es = ra.ExpectationStep(p.cfg, 0)
+    es.set_model([proj], [bp], p.pdf_class, p.sigma2_noise, pdf_direction=p.pdf_direction)
+    alone, _ = es.run(p.fimg, p.fctf, p.meta)
+    out = {}
+    th = threading.Thread(target=lambda: out.setdefault("res", es.run(p.fimg, p.fctf, p.meta)[0]))
+    pp.predict_launch(d_img.address, d_ctf.address, 24, 64, topk=8, on_device=True)
+    th.start()
+    b2, pr2 = pp.predict_finish()
+    th.join()
+    assert np.array_equal(b2, ref_b) and np.array_equal(pr2, ref_p)
+    assert np.array_equal(out["res"]["best_ihidden_over"], alone["best_ihidden_over"])
+    es.close()
     pp.close()
