@@ -105,7 +105,8 @@ struct Engine {
     bool coarse_valid = false;
     // constant tables
     DevBuf d_frames_c, d_frames_f, d_proj_c;
-    DevBuf d_phi, d_plo, d_pn2, d_zero_row;         // split-BF16 coarse projections
+    DevBuf d_pf, d_zero_row;                        // split-BF16 coarse projections (FP32 complex), a zero row
+    DevBuf d_ehi, d_elo;                            // split-BF16 translation phases [NT][Kp/2]
     DevBuf d_wc, d_ctf2, d_ynorm;                   // split-BF16 particle rows per chunk (group layout)
     DevBuf d_grp, d_exg;                            // pixel groups (row, x0); x phases with rows padded to xs4
     int Kp = 0;                                     // contraction length: 8 * groups, multiple of 32
@@ -383,8 +384,16 @@ static int engine_model_tables(Engine &e) {
         grp.resize(std::max(ng, 4), make_int2(0, 0));
         e.Kp = 8 * (int)grp.size();
         RLN_TRY(upload(e.d_grp, grp, e.stream));
-        std::vector<unsigned int> z(e.Kp / 2, 0u);
+        std::vector<float2> z(e.Kp / 2, make_float2(0.f, 0.f));
         RLN_TRY(upload(e.d_zero_row, z, e.stream));
+        const long long ne = (long long)e.NT * (e.Kp / 2);
+        RLN_TRY(e.d_ehi.ensure(sizeof(unsigned int) * (size_t)ne));
+        RLN_TRY(e.d_elo.ensure(sizeof(unsigned int) * (size_t)ne));
+        hipLaunchKernelGGL(rlnk::k_phase_split, dim3((unsigned)((ne + rlnk::kBlock - 1) / rlnk::kBlock)),
+                           dim3(rlnk::kBlock), 0, e.stream, e.d_exg.as<const float2>(), e.xs4,
+                           e.d_ey_c.as<const float2>(), e.igc.s, e.d_grp.as<const int2>(), e.Kp / 2, e.NT,
+                           e.d_ehi.as<unsigned int>(), e.d_elo.as<unsigned int>());
+        HIP_TRY(hipGetLastError());
     }
     RLN_TRY(upload(e.d_q_c, qc, e.stream));
     RLN_TRY(upload(e.d_q_f, qf, e.stream));
@@ -424,14 +433,11 @@ static int engine_coarse_projections(Engine &e) {
     if (!stale) return RLN_OK;
     if (e.cfg.precision == RLN_PREC_SPLIT_BF16) {
         const size_t rows = (size_t)e.K * e.n_or;
-        RLN_TRY(e.d_phi.ensure(rows * e.Kp * sizeof(unsigned short)));
-        RLN_TRY(e.d_plo.ensure(rows * e.Kp * sizeof(unsigned short)));
-        RLN_TRY(e.d_pn2.ensure(rows * (e.Kp / 2) * sizeof(float)));
+        RLN_TRY(e.d_pf.ensure(rows * (e.Kp / 2) * sizeof(float2)));
         dim3 grid(e.K * e.n_or, (e.Kp / 2 + rlnk::kBlock - 1) / rlnk::kBlock);
         hipLaunchKernelGGL(rlnk::k_project_grid_split, grid, dim3(rlnk::kBlock), 0, e.stream,
                            e.d_vols.as<const float4 *const>(), e.vg, e.d_frames_c.as<const rlnk::Frame>(), e.n_or,
-                           e.igc, e.d_grp.as<const int2>(), e.G, e.Kp, e.d_phi.as<unsigned int>(),
-                           e.d_plo.as<unsigned int>(), e.d_pn2.as<float>());
+                           e.igc, e.d_grp.as<const int2>(), e.G, e.Kp, e.d_pf.as<float2>());
         RLN_TRY(launch_check("k_project_grid_split"));
     } else {
         const size_t n = (size_t)e.K * e.n_or * e.igc.P;
@@ -833,13 +839,12 @@ static int run_fine_stage(Engine &e, const FineLaunch &L0) {
 
 template <int NTT>
 static void launch_mfma_t(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
-    hipLaunchKernelGGL(rlnk::k_coarse_mfma<NTT>, dim3(n_tiles), dim3(rlnk::kBlock), 0, e.stream,
-                       e.d_tiles.as<const rlnk::CoarseTile>(), e.d_or_off.as<const int>(), e.d_or_cnt.as<const int>(),
-                       e.d_or_proj.as<const int>(), e.d_phi.as<const unsigned int>(),
-                       e.d_plo.as<const unsigned int>(), e.d_pn2.as<const float>(), e.d_wc.as<const float2>(),
-                       e.d_ctf2.as<const float>(), e.d_ynorm.as<const double>(), e.d_grp.as<const int2>(),
-                       e.d_exg.as<const float2>(), e.xs4, e.d_ey_c.as<const float2>(), e.igc.s, xi2, d2off, e.NT,
-                       e.Kp, e.d_zero_row.as<const unsigned int>(), e.d_diff2.as<float>());
+    hipLaunchKernelGGL(rlnk::k_coarse_mfma<NTT>, dim3((n_tiles + rlnk::kMfG - 1) / rlnk::kMfG),
+                       dim3(rlnk::kMfBlock), 0, e.stream, e.d_tiles.as<const rlnk::CoarseTile>(), (int)n_tiles,
+                       e.d_or_off.as<const int>(), e.d_or_cnt.as<const int>(),
+                       e.d_or_proj.as<const int>(), e.d_pf.as<const float2>(), e.d_wc.as<const float2>(), e.d_ctf2.as<const float>(), e.d_ynorm.as<const double>(),
+                       e.d_ehi.as<const unsigned int>(), e.d_elo.as<const unsigned int>(), xi2, d2off, e.NT, e.Kp,
+                       e.d_zero_row.as<const float2>(), e.d_diff2.as<float>());
 }
 
 static int launch_coarse_mfma(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
@@ -1450,7 +1455,7 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
     if (!e) return RLN_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    rln::DevBuf *bufs[] = {&e->d_frames_c, &e->d_frames_f, &e->d_proj_c, &e->d_phi, &e->d_plo, &e->d_pn2,
+    rln::DevBuf *bufs[] = {&e->d_frames_c, &e->d_frames_f, &e->d_proj_c, &e->d_pf, &e->d_ehi, &e->d_elo,
                            &e->d_zero_row, &e->d_wc, &e->d_grp, &e->d_exg, &e->d_ctf2, &e->d_ynorm, &e->d_q_c, &e->d_q_f, &e->d_minv_w,
                            &e->d_mresol, &e->d_xa_shell, &e->d_ex_c, &e->d_ey_c, &e->d_ex_f, &e->d_ey_f,
                            &e->d_tr_ix, &e->d_tr_iy, &e->d_over_x, &e->d_over_y, &e->d_vols, &e->d_bp_ptrs,

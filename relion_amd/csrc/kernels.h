@@ -266,14 +266,12 @@ __device__ __forceinline__ void split2_bf16(float a, float b, unsigned int &hi, 
     lo = __builtin_bit_cast(unsigned int, __builtin_convertvector((floatx2_t){ra, rb}, bf16x2_t));
 }
 
-// Split projection tables of the coarse grid in the group layout: hi/lo [K*n_or][Kp] and the FP32 power
-// |P|^2 [K*n_or][Kp/2] (one per complex slot). grid (K*n_or, ceil(Kp/2/256)).
+// Projection tables of the split pass in the group layout: FP32 complex pf [K*n_or][Kp/2] (the coarse kernel
+// multiplies them by the particle and splits them in BF16). grid (K*n_or, ceil(Kp/2/256)).
 __global__ void __launch_bounds__(kBlock) k_project_grid_split(const float4 *const *__restrict__ vols, VolGeom vg,
                                                                const Frame *__restrict__ frames, int n_or, ImgGeom ig,
                                                                const int2 *__restrict__ grp, int G, int Kp,
-                                                               unsigned int *__restrict__ phi,
-                                                               unsigned int *__restrict__ plo,
-                                                               float *__restrict__ pn2) {
+                                                               float2 *__restrict__ pf) {
     const int idx = blockIdx.y * kBlock + threadIdx.x;  // complex slot 4g + j
     const int ko = blockIdx.x;
     if (2 * idx >= Kp) return;
@@ -288,15 +286,11 @@ __global__ void __launch_bounds__(kBlock) k_project_grid_split(const float4 *con
             v = project_pixel(vols[k], vg, frames[o], x, y);
         }
     }
-    unsigned int h, l;
-    split2_bf16(v.x, v.y, h, l);
-    phi[(size_t)ko * (Kp / 2) + idx] = h;
-    plo[(size_t)ko * (Kp / 2) + idx] = l;
-    pn2[(size_t)ko * (Kp / 2) + idx] = v.x * v.x + v.y * v.y;
+    pf[(size_t)ko * (Kp / 2) + idx] = v;
 }
 
 // Particle side of the split pass, in the group layout: w[b][4g+j] = c_p q_p X(p) with c_p = q_p ctf_p s (the
-// translation phase is applied inside k_coarse_mfma), ctf2[b][4g+j] = c_p^2, and ynorm[b] = sum_p |q_p X(p)|^2
+// translation phase is the coarse kernel's B operand), ctf2[b][4g+j] = c_p^2, and ynorm[b] = sum_p |q_p X(p)|^2
 // over all pixels (FP64). grid (ceil(max(Kp/2, P)/256), B).
 __global__ void __launch_bounds__(kBlock) k_prep_coarse_split(const float2 *__restrict__ fimg,
                                                               const float *__restrict__ fctf,
@@ -350,63 +344,66 @@ __global__ void __launch_bounds__(kBlock) k_prep_coarse_split(const float2 *__re
     }
 }
 
-constexpr int kMfO = 64;  // orientations per block (4 MFMA row tiles)
+// Translation operand of the split pass, the same for every particle: the phase E[t][4g + m] = ex[t][x0 + m] *
+// ey[t][row] of every complex slot of pixel group g = (row, x0) (rows of exg padded with zeros past xs), split
+// into BF16 hi and lo (re, im) pairs. grid (ceil(NT * Kc / 256)).
+__global__ void __launch_bounds__(kBlock) k_phase_split(const float2 *__restrict__ exg, int xs4,
+                                                        const float2 *__restrict__ ey, int s,
+                                                        const int2 *__restrict__ grp, int Kc, int NT,
+                                                        unsigned int *__restrict__ ehi, unsigned int *__restrict__ elo) {
+    const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= (long long)NT * Kc) return;
+    const int t = (int)(idx / Kc), c = (int)(idx - (long long)t * Kc);
+    const int2 gr = grp[c >> 2];
+    const float2 v = cmul(exg[(size_t)t * xs4 + gr.y + (c & 3)], ey[(size_t)t * s + gr.x]);
+    unsigned int h, l;
+    split2_bf16(v.x, v.y, h, l);
+    ehi[idx] = h;
+    elo[idx] = l;
+}
 
-// One block (4 waves) per (particle, 64 orientations) and all translations. Wave w owns translation tiles
-// w, w+4, w+8, ... and all four orientation tiles. The translation operand B[t][k] = w(p) e^{-2 pi i (x tx + y
-// ty)/N} is formed in registers from the particle's w row and the phase tables exg [NT][xs4] (x phases, rows
-// padded to a multiple of 4) and ey [NT][s] (y phases), then split in BF16. grid (n_tiles).
+constexpr int kMfO = 64;    // orientations per tile (4 MFMA row tiles)
+constexpr int kMfG = 1;     // tiles per block, each with its own 4 waves
+constexpr int kMfBlock = 256 * kMfG;
+
+// One block per two coarse tiles (64 orientations of one particle each, possibly different particles) and all
+// translations. The cross term sum_p Re(conj(P_o(p)) w(p) E_t(p)) is a real dot product of (re, im) pairs
+// between A[o][p] = P_o(p) conj(w(p)), formed while a group stages its tile's projection rows (64 rows a K step)
+// and split in BF16, and B[t][p] = E_t(p), the particle-independent phase table split once per model
+// (k_phase_split) and read straight into the MFMA operands: the two groups read the same B fragments, the second
+// from the cache. Wave w of a group owns translation tiles w, w+4, w+8, ... and the group's four orientation
+// tiles; row tiles past the tile's orientations are skipped. grid (ceil(n_tiles / 2)).
 template <int NTT>  // translation tiles of 16 in the block (16*NTT >= NT)
-__global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__restrict__ tiles,
-                                                        const int *__restrict__ or_off, const int *__restrict__ or_cnt,
-                                                        const int *__restrict__ or_proj,
-                                                        const unsigned int *__restrict__ phi,
-                                                        const unsigned int *__restrict__ plo,
-                                                        const float *__restrict__ pn2,
-                                                        const float2 *__restrict__ w,
-                                                        const float *__restrict__ ctf2, const double *__restrict__ ynorm,
-                                                        const int2 *__restrict__ grp, const float2 *__restrict__ exg,
-                                                        int xs4, const float2 *__restrict__ ey, int s,
-                                                        const float *__restrict__ xi2half,
-                                                        const long long *__restrict__ d2_off, int NT, int Kp,
-                                                        const unsigned int *__restrict__ zero_row,
-                                                        float *__restrict__ diff2) {
+__global__ void __launch_bounds__(kMfBlock, 3) k_coarse_mfma(const CoarseTile *__restrict__ tiles, int n_tiles,
+                                                          const int *__restrict__ or_off,
+                                                          const int *__restrict__ or_cnt,
+                                                          const int *__restrict__ or_proj,
+                                                          const float2 *__restrict__ pf,
+                                                          const float2 *__restrict__ w,
+                                                          const float *__restrict__ ctf2,
+                                                          const double *__restrict__ ynorm,
+                                                          const unsigned int *__restrict__ ehi,
+                                                          const unsigned int *__restrict__ elo,
+                                                          const float *__restrict__ xi2half,
+                                                          const long long *__restrict__ d2_off, int NT, int Kp,
+                                                          const float2 *__restrict__ zero_row,
+                                                          float *__restrict__ diff2) {
     constexpr int kTW = (NTT + 3) / 4;  // translation tiles per wave
     constexpr int kAStride = 40;        // bf16 per staged A row (32 + 8 padding against bank conflicts)
-    __shared__ int og_s[kMfO];
-    __shared__ float a_norm[kMfO];
-    // A operand of one K step (hi and lo planes of the block's 64 projection rows), double buffered
-    __shared__ __align__(16) unsigned short s_a[2][2][kMfO * kAStride];
-    const CoarseTile tile = tiles[blockIdx.x];
-    const int b = tile.b;
-    const int n_o = or_cnt[b];
+    __shared__ int og_s[kMfG][kMfO];
+    __shared__ float a_norm[kMfG][kMfO];
+    // A operand of one K step (hi and lo planes of the block's 128 rows), double buffered
+    __shared__ __align__(16) unsigned short s_a[2][2][kMfG * kMfO * kAStride];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int grp = wave >> 2, wv = wave & 3, gtid = tid & 255;
+    const int ti = kMfG * blockIdx.x + grp;
+    const bool has = ti < n_tiles;
+    const CoarseTile tile = has ? tiles[ti] : CoarseTile{0, 0};
+    const int b = tile.b;
+    const int n_o = has ? or_cnt[b] : 0;
+    const int n_rows = has ? min(kMfO, n_o - tile.o_begin) : 0;  // valid orientations of the tile
     const int Kc = Kp / 2;  // complex slots per row
-    if (tid < kMfO) {
-        const int o = tile.o_begin + tid;
-        og_s[tid] = (o < n_o) ? or_proj[or_off[b] + o] : -1;
-    }
-    __syncthreads();
-    // sum_p c_p^2 |P_o(p)|^2 for the block's orientations: 4 lanes per orientation
-    {
-        const int oo = tid >> 2, part = tid & 3;
-        const int og = og_s[oo];
-        float acc_n = 0.f;
-        if (og >= 0) {
-            const float4 *pn = reinterpret_cast<const float4 *>(pn2 + (size_t)og * Kc);
-            const float4 *c2 = reinterpret_cast<const float4 *>(ctf2 + (size_t)b * Kc);
-            for (int g = part; g < Kc / 4; g += 4) {
-                const float4 a = pn[g], c = c2[g];
-                acc_n = fmaf(a.x, c.x, acc_n);
-                acc_n = fmaf(a.y, c.y, acc_n);
-                acc_n = fmaf(a.z, c.z, acc_n);
-                acc_n = fmaf(a.w, c.w, acc_n);
-            }
-        }
-        acc_n += __shfl_xor(acc_n, 1);
-        acc_n += __shfl_xor(acc_n, 2);
-        if (part == 0) a_norm[oo] = acc_n;
-    }
+    if (gtid < kMfO) og_s[grp][gtid] = gtid < n_rows ? or_proj[or_off[b] + tile.o_begin + gtid] : -1;
     __syncthreads();
     floatx4_t acc[4][kTW];
 #pragma unroll
@@ -414,81 +411,106 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
 #pragma unroll
         for (int j = 0; j < kTW; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
     const int r16 = lane & 15, gq = lane >> 4;
-    // staging: thread tid loads 8 consecutive k (16 bytes) of row tid / 4, hi and lo, one K step ahead
-    const int srow = tid >> 2, sq = tid & 3;
-    const uint4 *st_hi, *st_lo;
+    const int row_tiles = (n_rows + 15) >> 4;  // row tiles of this wave's group with orientations (uniform)
+    // staging: thread gtid of a group forms the 4 complex slots 16 ks + 4 sq of its tile's row gtid / 4 in K step
+    // ks (8 bf16 of hi and of lo), its projection values loaded one K step ahead
+    const int srow = gtid >> 2, sq = gtid & 3, arow = grp * kMfO + srow;
+    const float4 *st_p;
     {
-        const int og = og_s[srow];
-        st_hi = reinterpret_cast<const uint4 *>(og >= 0 ? phi + (size_t)og * Kc : zero_row);
-        st_lo = reinterpret_cast<const uint4 *>(og >= 0 ? plo + (size_t)og * Kc : zero_row);
+        const int og = og_s[grp][srow];
+        st_p = reinterpret_cast<const float4 *>(og >= 0 ? pf + (size_t)og * Kc : zero_row);
     }
+    const float4 *wrow = reinterpret_cast<const float4 *>(w + (size_t)b * Kc);
+    const float4 *c2row = reinterpret_cast<const float4 *>(ctf2 + (size_t)b * Kc);
+    float an = 0.f;  // this thread's part of sum_p c_p^2 |P_o(p)|^2 over its row's slots
+    auto stage = [&](float4 p0, float4 p1, int ks, int to) {
+        const float4 w0 = wrow[8 * ks + 2 * sq], w1 = wrow[8 * ks + 2 * sq + 1];
+        const float4 c2 = c2row[4 * ks + sq];
+        an = fmaf(p0.x * p0.x + p0.y * p0.y, c2.x, an);
+        an = fmaf(p0.z * p0.z + p0.w * p0.w, c2.y, an);
+        an = fmaf(p1.x * p1.x + p1.y * p1.y, c2.z, an);
+        an = fmaf(p1.z * p1.z + p1.w * p1.w, c2.w, an);
+        const float2 pv[4] = {make_float2(p0.x, p0.y), make_float2(p0.z, p0.w), make_float2(p1.x, p1.y),
+                              make_float2(p1.z, p1.w)};
+        const float2 wq[4] = {make_float2(w0.x, w0.y), make_float2(w0.z, w0.w), make_float2(w1.x, w1.y),
+                              make_float2(w1.z, w1.w)};
+        unsigned int hh[4], ll[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {  // P conj(w)
+            const float re = pv[m].x * wq[m].x + pv[m].y * wq[m].y;
+            const float im = pv[m].y * wq[m].x - pv[m].x * wq[m].y;
+            split2_bf16(re, im, hh[m], ll[m]);
+        }
+        *reinterpret_cast<uint4 *>(&s_a[to][0][arow * kAStride + 8 * sq]) = make_uint4(hh[0], hh[1], hh[2], hh[3]);
+        *reinterpret_cast<uint4 *>(&s_a[to][1][arow * kAStride + 8 * sq]) = make_uint4(ll[0], ll[1], ll[2], ll[3]);
+    };
     int tt[kTW];
 #pragma unroll
     for (int j = 0; j < kTW; ++j) {
-        const int t = 16 * (wave + 4 * j) + r16;
-        tt[j] = ((wave + 4 * j) < NTT && t < NT) ? t : -1;
+        const int t = 16 * (wv + 4 * j) + r16;
+        tt[j] = ((wv + 4 * j) < NTT && t < NT) ? t : -1;
     }
-    const float4 *wrow = reinterpret_cast<const float4 *>(w + (size_t)b * Kc);
-    uint4 pre_h = st_hi[sq], pre_l = st_lo[sq];  // K step 0: pixel groups 0..3
-    *reinterpret_cast<uint4 *>(&s_a[0][0][srow * kAStride + 8 * sq]) = pre_h;
-    *reinterpret_cast<uint4 *>(&s_a[0][1][srow * kAStride + 8 * sq]) = pre_l;
+    // projection values of kPre K steps in flight: register set d holds step ks with ks % kPre == d
+    constexpr int kPre = 3;
+    const int nK = Kp / 32;
+    float4 pa[kPre][2];
+#pragma unroll
+    for (int d = 0; d < kPre; ++d) {
+        pa[d][0] = d < nK ? st_p[8 * d + 2 * sq] : make_float4(0.f, 0.f, 0.f, 0.f);
+        pa[d][1] = d < nK ? st_p[8 * d + 2 * sq + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    stage(pa[0][0], pa[0][1], 0, 0);
     __syncthreads();
-    int buf = 0;
-    for (int k0 = 0; k0 < Kp; k0 += 32) {
-        const bool more = k0 + 32 < Kp;
-        if (more) {
-            const int gn = ((k0 + 32) >> 3) + sq;
-            pre_h = st_hi[gn];
-            pre_l = st_lo[gn];
-        }
-        const int g = (k0 >> 3) + gq;  // this lane's pixel group
-        bf16x8_t ah[4], al[4];
+    for (int ks0 = 0; ks0 < nK; ks0 += kPre) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 16 * i + r16;
-            ah[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(&s_a[buf][0][row * kAStride + 8 * gq]));
-            al[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(&s_a[buf][1][row * kAStride + 8 * gq]));
-        }
-        const float4 w01 = wrow[2 * g], w23 = wrow[2 * g + 1];
-        const int2 gr = grp[g];
+        for (int d = 0; d < kPre; ++d) {
+            const int ks = ks0 + d, buf = ks & 1;
+            if (ks >= nK) break;    // uniform over the block
+            if (ks + kPre < nK) {  // set d was staged at the end of step ks - 1: reload it for step ks + kPre
+                pa[d][0] = st_p[8 * (ks + kPre) + 2 * sq];
+                pa[d][1] = st_p[8 * (ks + kPre) + 2 * sq + 1];
+            }
+
+            if (row_tiles > 0) {
+                bf16x8_t ah[4], al[4];
 #pragma unroll
-        for (int j = 0; j < kTW; ++j) {
-            if (wave + 4 * j >= NTT) continue;
-            unsigned int hh[4], ll[4];
-            if (tt[j] >= 0) {
-                const float4 *exr = reinterpret_cast<const float4 *>(exg + (size_t)tt[j] * xs4 + gr.y);
-                const float4 e01 = exr[0], e23 = exr[1];
-                const float2 fy = ey[(size_t)tt[j] * s + gr.x];
-                const float2 wv[4] = {make_float2(w01.x, w01.y), make_float2(w01.z, w01.w),
-                                      make_float2(w23.x, w23.y), make_float2(w23.z, w23.w)};
-                const float2 ev[4] = {make_float2(e01.x, e01.y), make_float2(e01.z, e01.w),
-                                      make_float2(e23.x, e23.y), make_float2(e23.z, e23.w)};
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const float2 v = cmul(wv[m], cmul(ev[m], fy));
-                    split2_bf16(v.x, v.y, hh[m], ll[m]);
+                for (int i = 0; i < 4; ++i) {
+                    const int row = grp * kMfO + 16 * i + r16;
+                    ah[i] = __builtin_bit_cast(
+                        bf16x8_t, *reinterpret_cast<const uint4 *>(&s_a[buf][0][row * kAStride + 8 * gq]));
+                    al[i] = __builtin_bit_cast(
+                        bf16x8_t, *reinterpret_cast<const uint4 *>(&s_a[buf][1][row * kAStride + 8 * gq]));
                 }
-            } else {
+                const int slot = 16 * ks + 4 * gq;  // this lane's 4 complex slots of the B fragment
 #pragma unroll
-                for (int m = 0; m < 4; ++m) hh[m] = ll[m] = 0u;
-            }
-            const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, make_uint4(hh[0], hh[1], hh[2], hh[3]));
-            const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, make_uint4(ll[0], ll[1], ll[2], ll[3]));
+                for (int j = 0; j < kTW; ++j) {
+                    if (wv + 4 * j >= NTT) continue;
+                    uint4 hv = make_uint4(0u, 0u, 0u, 0u), lv = hv;
+                    if (tt[j] >= 0) {
+                        hv = *reinterpret_cast<const uint4 *>(ehi + (size_t)tt[j] * Kc + slot);
+                        lv = *reinterpret_cast<const uint4 *>(elo + (size_t)tt[j] * Kc + slot);
+                    }
+                    const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, hv);
+                    const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, lv);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+                    for (int i = 0; i < 4; ++i) {
+                        if (i >= row_tiles) continue;
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+                    }
+                }
             }
-        }
-        if (more) {
             // the other buffer was last read in the previous step, before the barrier that ended it
-            *reinterpret_cast<uint4 *>(&s_a[buf ^ 1][0][srow * kAStride + 8 * sq]) = pre_h;
-            *reinterpret_cast<uint4 *>(&s_a[buf ^ 1][1][srow * kAStride + 8 * sq]) = pre_l;
+            if (ks + 1 < nK) stage(pa[(d + 1) % kPre][0], pa[(d + 1) % kPre][1], ks + 1, buf ^ 1);
+            __syncthreads();
         }
-        __syncthreads();
-        buf ^= 1;
     }
+    an += __shfl_xor(an, 1);  // the 4 threads of a row are consecutive lanes
+    an += __shfl_xor(an, 2);
+    if (sq == 0) a_norm[grp][srow] = an;
+    __syncthreads();
+    if (!has) return;
     const float base = xi2half[b] + (float)ynorm[b];
     float *out = diff2 + d2_off[b];
 #pragma unroll
@@ -499,10 +521,10 @@ __global__ void __launch_bounds__(kBlock) k_coarse_mfma(const CoarseTile *__rest
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int oo = 16 * i + gq * 4 + r;  // D row
-                const int o = tile.o_begin + oo;
-                if (o >= n_o) continue;
-                out[(size_t)o * NT + t] = og_s[oo] < 0 ? -999.f : base + a_norm[oo] - 2.f * acc[i][j][r];
+                const int oo = 16 * i + gq * 4 + r;  // D row within the tile
+                if (oo >= n_rows) continue;
+                out[(size_t)(tile.o_begin + oo) * NT + t] =
+                    og_s[grp][oo] < 0 ? -999.f : base + a_norm[grp][oo] - 2.f * acc[i][j][r];
             }
     }
 }
