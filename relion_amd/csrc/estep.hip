@@ -52,10 +52,9 @@ int pack_device_into_projector(rln_projector *p, const float2 *dev_plain, hipStr
 
 // Sums the backprojector's corner slots into plain [z][y][x] data (re, im) and weight volumes on the device.
 int fold_backprojector_device(rln_backprojector *b, float *re, float *im, float *w, hipStream_t st) {
-    const size_t nv = b->dims.voxels();
     const rlnk::VolGeom g{b->dims.xdim, b->dims.ydim, b->dims.zdim, b->dims.inity, b->dims.initz};
-    hipLaunchKernelGGL(rlnk::k_fold_slots, dim3((unsigned)((nv + rlnk::kBlock - 1) / rlnk::kBlock)), dim3(rlnk::kBlock),
-                       0, st, b->slots.as<const float>(), g, re, im, w);
+    hipLaunchKernelGGL(rlnk::k_fold_slots, dim3(rlnk::fold_blocks(g)), dim3(rlnk::kBlock), 0, st,
+                       b->slots.as<const float>(), g, re, im, w);
     HIP_TRY(hipGetLastError());
     return RLN_OK;
 }
@@ -1415,8 +1414,8 @@ extern "C" int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real,
     RLN_TRY(plain.ensure(3 * sizeof(float) * nv));
     float *d_re = plain.as<float>(), *d_im = d_re + nv, *d_w = d_im + nv;
     const rlnk::VolGeom g{b->dims.xdim, b->dims.ydim, b->dims.zdim, b->dims.inity, b->dims.initz};
-    hipLaunchKernelGGL(rlnk::k_fold_slots, dim3((unsigned)((nv + rlnk::kBlock - 1) / rlnk::kBlock)), dim3(rlnk::kBlock),
-                       0, 0, b->slots.as<const float>(), g, d_re, d_im, d_w);
+    hipLaunchKernelGGL(rlnk::k_fold_slots, dim3(rlnk::fold_blocks(g)), dim3(rlnk::kBlock), 0, 0,
+                       b->slots.as<const float>(), g, d_re, d_im, d_w);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     if (real) HIP_TRY(hipMemcpy(real, d_re, sizeof(float) * nv, hipMemcpyDeviceToHost));

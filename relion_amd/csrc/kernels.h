@@ -39,29 +39,42 @@ __global__ void __launch_bounds__(kBlock) k_pack_volume(const float2 *__restrict
 // k_fold_slots sums the slots into plain volumes.
 constexpr int kSlot = 32;
 
-// Plain data/weight volumes from the slots: v(z,y,x) = sum over corners c of slot[base(z,y,x) - c][c].
+// Plain data/weight volumes from the slots: v(z,y,x) = sum over corners c (in order) of slot[base(z,y,x) - c][c].
+// One block per 4 x 4 x 4 brick of output voxels: the slots of the 5 x 5 x 5 bases it needs (the brick's own and
+// the lower neighbours' faces) are read into LDS, 24 floats per base, then 192 threads form the 64 voxels' re, im
+// and weight. grid (ceil(xdim/4) * ceil(ydim/4) * ceil(zdim/4)).
 __global__ void __launch_bounds__(kBlock) k_fold_slots(const float *__restrict__ slots, VolGeom g,
                                                        float *__restrict__ re, float *__restrict__ im,
                                                        float *__restrict__ w) {
-    const size_t n = (size_t)g.xdim * g.ydim * g.zdim;
-    const size_t v = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (v >= n) return;
-    const int x = (int)(v % g.xdim);
-    const size_t zy = v / g.xdim;
-    const int y = (int)(zy % g.ydim), z = (int)(zy / g.ydim);
-    float sr = 0.f, si = 0.f, sw = 0.f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int x0 = x - (c & 1), y0 = y - ((c >> 1) & 1), z0 = z - (c >> 2);
-        if (x0 < 0 || y0 < 0 || z0 < 0 || x0 >= g.xdim - 1 || y0 >= g.ydim - 1 || z0 >= g.zdim - 1) continue;
-        const float *s = slots + kSlot * packed_index(g, x0, y0, z0);
-        sr += s[c];
-        si += s[8 + c];
-        sw += s[16 + c];
+    __shared__ float s[125][25];  // [base][re c0..7 | im c0..7 | weight c0..7], padded against bank conflicts
+    const int obx = (g.xdim + 3) >> 2, oby = (g.ydim + 3) >> 2;
+    const int bx = blockIdx.x % obx, by = (blockIdx.x / obx) % oby, bz = blockIdx.x / (obx * oby);
+    for (int q = threadIdx.x; q < 125 * 24; q += kBlock) {
+        const int bi = q / 24, f = q - bi * 24;
+        const int x0 = 4 * bx - 1 + bi % 5, y0 = 4 * by - 1 + (bi / 5) % 5, z0 = 4 * bz - 1 + bi / 25;
+        float v = 0.f;
+        if (x0 >= 0 && y0 >= 0 && z0 >= 0 && x0 < g.xdim - 1 && y0 < g.ydim - 1 && z0 < g.zdim - 1)
+            v = slots[kSlot * packed_index(g, x0, y0, z0) + f];
+        s[bi][f] = v;
     }
-    re[v] = sr;
-    im[v] = si;
-    w[v] = sw;
+    __syncthreads();
+    if (threadIdx.x >= 192) return;
+    const int comp = threadIdx.x >> 6, vi = threadIdx.x & 63;  // comp: 0 re, 1 im, 2 weight
+    const int lx = vi & 3, ly = (vi >> 2) & 3, lz = vi >> 4;
+    const int x = 4 * bx + lx, y = 4 * by + ly, z = 4 * bz + lz;
+    if (x >= g.xdim || y >= g.ydim || z >= g.zdim) return;
+    float sum = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {  // base (x - dx, y - dy, z - dz) is local (lx + 1 - dx, ...)
+        const int bi = (lx + 1 - (c & 1)) + 5 * ((ly + 1 - ((c >> 1) & 1)) + 5 * (lz + 1 - (c >> 2)));
+        sum += s[bi][8 * comp + c];
+    }
+    float *out = comp == 0 ? re : (comp == 1 ? im : w);
+    out[((size_t)z * g.ydim + y) * g.xdim + x] = sum;
+}
+
+inline unsigned fold_blocks(const VolGeom &g) {
+    return (unsigned)(((g.xdim + 3) / 4) * ((g.ydim + 3) / 4) * ((g.zdim + 3) / 4));
 }
 
 
