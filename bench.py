@@ -551,9 +551,11 @@ def main():
             lists = lists_of()
             res = None
             for k in range(steps):
+                t_a = time.time()
                 bp.initMdl()
                 es_p.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
                 es_p.set_orientation_lists(*lists)
+                phase_ms["setup"] += 1000.0 * (time.time() - t_a)
                 box_ = {}
 
                 def run():
@@ -562,10 +564,13 @@ def main():
                     except BaseException as exc:  # re-raised in the caller
                         box_["exc"] = exc
 
+                t_a = time.time()
                 th = threading.Thread(target=run)
                 th.start()
                 nxt = lists_of() if k + 1 < steps else None
+                phase_ms["next_lists"] += 1000.0 * (time.time() - t_a)
                 th.join()
+                phase_ms["estep_run"] += 1000.0 * (time.time() - t_a)
                 if "exc" in box_:
                     raise box_["exc"]
                 res, cur = box_["res"], lists
@@ -573,6 +578,7 @@ def main():
                     lists = nxt
             return res, cur[0]
 
+        phase_ms = {"setup": 0.0, "next_lists": 0.0, "estep_run": 0.0}
         hipmem.synchronize()
         t0 = time.time()
         pl_out = pipelined_steps(args.steps)
@@ -581,7 +587,9 @@ def main():
         runs["proposal_pipelined"] = dict(out=pl_out, elapsed=el, timings=es_p.timings(), counters=es_p.counters(),
                                           value=n * args.steps / el, ms=1000.0 * el / args.steps, steps=args.steps,
                                           warmup=args.warmup, particles=n)
-        log(f"[proposal_pipelined] timed {args.steps} steps: {el:.2f} s", args)
+        extra["proposal_pipelined_host_ms_per_step"] = {k: round(v / args.steps, 2) for k, v in phase_ms.items()}
+        log(f"[proposal_pipelined] timed {args.steps} steps: {el:.2f} s; per step "
+            f"{json.dumps(extra['proposal_pipelined_host_ms_per_step'])}", args)
         off = runs["proposal"]["out"][1]
         if args.lists_out:
             bins, probs = pp.predict(d_img.address, d_ctf.address, n, box, topk=args.topk, on_device=True)
