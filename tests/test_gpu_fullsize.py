@@ -3,9 +3,11 @@ at 0.5 px steps: 197 translations, particle diameter 200 A: coarse size 48) agai
 tolerances of the parity suite (tests/test_gpu_parity.py): per-pose coarse and fine diff2 within 1e-5, identical
 best poses except near-ties, dLL within 1e-5; backprojected volumes (68.4 M voxels) within 3e-4 norm-wise. The
 backprojection's error grows with radius (data 1e-5 at the centre to 1.6e-4 near r = 250 voxels, weight ten times
-smaller: 1.2e-4 and 1.1e-5 norm-wise measured on the local case): FP32 interpolation coordinates of magnitude up to
-256 voxels (ulp 3e-5) and the partial cancellation of noisy high-frequency terms in the data sums, the FP32
-arithmetic of the reference's GPU path.
+smaller: 1.2e-4 and 1.1e-5 norm-wise measured on the local case). FP32 diff2 (errors near 1e-7 of diff2 values
+near 1e4) moves the relative weights of the significant translations slightly, and translations half a pixel
+apart differ by phases that grow with frequency, so the data's error grows with radius; the weight volume does not
+depend on the translation and keeps the smaller error of FP32 interpolation coordinates up to 256 voxels. This is
+the FP32 arithmetic of the reference's GPU path.
 A local search in plain FP32 and an exhaustive global search in split BF16 (7.26 M coarse poses per particle), on
 a few particles each so that the oracle finishes in seconds."""
 import pytest
