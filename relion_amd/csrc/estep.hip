@@ -1015,11 +1015,12 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         double *hs = e.d_hsum.as<double>();
         hipLaunchKernelGGL(rlnk::k_wpart_init, dim3(gB), dim3(rlnk::kBlock), 0, st, wp, B);
         hipLaunchKernelGGL(rlnk::k_wseg_min, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp);
-        hipLaunchKernelGGL(rlnk::k_wseg_exp, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp);
+        hipLaunchKernelGGL(rlnk::k_wseg_exp, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp, hc, hs);
         hipLaunchKernelGGL(rlnk::k_wsel_start, dim3(gB), dim3(rlnk::kBlock), 0, st, wa, wp, B);
         for (int shift = 56; shift >= 0; shift -= 8) {
-            hipLaunchKernelGGL(rlnk::k_wseg_hist, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg,
-                               (const rlnk::WPart *)wp, shift, hc, hs);
+            if (shift < 56)  // the top byte's histogram came with the weights
+                hipLaunchKernelGGL(rlnk::k_wseg_hist, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg,
+                                   (const rlnk::WPart *)wp, shift, hc, hs);
             hipLaunchKernelGGL(rlnk::k_wsel_pick, dim3(gB), dim3(rlnk::kBlock), 0, st, wp, B, shift, hc, hs);
         }
         hipLaunchKernelGGL(rlnk::k_wseg_gather, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp,

@@ -1050,14 +1050,22 @@ __global__ void __launch_bounds__(kBlock) k_wseg_min(WeightArgs a, const WSeg *_
     }
 }
 
+// Weights of one segment, their sum, and the first radix pass of the selection (the top byte of every positive
+// weight: the pass before any prefix is fixed), accumulated here instead of in a separate read of the weights.
 __global__ void __launch_bounds__(kBlock) k_wseg_exp(WeightArgs a, const WSeg *__restrict__ segs,
-                                                     WPart *__restrict__ part) {
+                                                     WPart *__restrict__ part, unsigned int *__restrict__ hcnt,
+                                                     double *__restrict__ hsum) {
     __shared__ double red[kBlock / 64];
+    __shared__ unsigned int c[256];
+    __shared__ double sm[256];
+    c[threadIdx.x] = 0;
+    sm[threadIdx.x] = 0.;
     const WSeg sg = segs[blockIdx.x];
     const int b = sg.b;
     const float *d2 = a.diff2 + a.d2_off[b];
     double *w = a.weight + a.d2_off[b];
     const double mind = (double)__uint_as_float(part[b].mn_bits);
+    __syncthreads();
     double sum = 0.;
     for (long long e = sg.begin + threadIdx.x; e < sg.end; e += kBlock) {
         const float v = d2[e];
@@ -1069,6 +1077,11 @@ __global__ void __launch_bounds__(kBlock) k_wseg_exp(WeightArgs a, const WSeg *_
         }
         w[e] = wt;
         sum += wt;
+        if (wt > 0.) {
+            const int bin = (int)((unsigned long long)__double_as_longlong(wt) >> 56);
+            atomicAdd(&c[bin], 1u);
+            atomicAdd(&sm[bin], wt);
+        }
     }
     for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
@@ -1076,6 +1089,10 @@ __global__ void __launch_bounds__(kBlock) k_wseg_exp(WeightArgs a, const WSeg *_
     if (threadIdx.x == 0) {
         for (int i = 1; i < kBlock / 64; ++i) sum += red[i];
         atomicAdd(&part[b].sum, sum);
+    }
+    if (c[threadIdx.x]) {
+        atomicAdd(&hcnt[(size_t)b * 256 + threadIdx.x], c[threadIdx.x]);
+        atomicAdd(&hsum[(size_t)b * 256 + threadIdx.x], sm[threadIdx.x]);
     }
 }
 
