@@ -843,7 +843,7 @@ static void launch_mfma_t(Engine &e, unsigned n_tiles, const float *xi2, const l
                        e.d_or_off.as<const int>(), e.d_or_cnt.as<const int>(),
                        e.d_or_proj.as<const int>(), e.d_pf.as<const float2>(), e.d_wc.as<const float2>(), e.d_ctf2.as<const float>(), e.d_ynorm.as<const double>(),
                        e.d_ehi.as<const unsigned int>(), e.d_elo.as<const unsigned int>(), xi2, d2off, e.NT, e.Kp,
-                       e.d_zero_row.as<const float2>(), e.d_diff2.as<float>());
+                       e.d_zero_row.as<const float2>(), e.d_diff2.as<float>(), e.d_wpart.as<rlnk::WPart>());
 }
 
 static int launch_coarse_mfma(Engine &e, unsigned n_tiles, const float *xi2, const long long *d2off) {
@@ -964,6 +964,10 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     RLN_TRY(e.d_diff2.ensure(sizeof(float) * std::max<long long>(1, h.total_entries)));
     RLN_TRY(e.d_weight.ensure(sizeof(double) * std::max<long long>(1, h.total_entries)));
     RLN_TRY(e.d_mask.ensure(std::max<long long>(1, h.total_entries)));
+    // per-particle selection state; the split coarse kernel folds the minimum diff2 into it as it writes
+    const unsigned gB = (unsigned)((B + rlnk::kBlock - 1) / rlnk::kBlock);
+    RLN_TRY(e.d_wpart.ensure(sizeof(rlnk::WPart) * B));
+    hipLaunchKernelGGL(rlnk::k_wpart_init, dim3(gB), dim3(rlnk::kBlock), 0, st, e.d_wpart.as<rlnk::WPart>(), B);
     if (split && !h.tiles.empty()) {
         RLN_TRY(launch_coarse_mfma(e, (unsigned)h.tiles.size(), d_xi2, d_d2off.as<const long long>()));
     } else if (!h.tiles.empty()) {
@@ -994,10 +998,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     // coarse weights over segments of kSegEntries poses (multi-block per particle)
     {
         const unsigned nseg = (unsigned)h.segs.size();
-        const unsigned gB = (unsigned)((B + rlnk::kBlock - 1) / rlnk::kBlock);
         RLN_TRY(upload(e.d_wsegs, h.segs, st));
         RLN_TRY(upload(e.d_seg_first, h.seg_first, st));
-        RLN_TRY(e.d_wpart.ensure(sizeof(rlnk::WPart) * B));
         RLN_TRY(e.d_hcnt.ensure(sizeof(unsigned int) * 256 * B));
         RLN_TRY(e.d_hsum.ensure(sizeof(double) * 256 * B));
         RLN_TRY(e.d_wcand.ensure(sizeof(double) * rlnk::kGather * B));
@@ -1013,8 +1015,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         rlnk::WPart *wp = e.d_wpart.as<rlnk::WPart>();
         unsigned int *hc = e.d_hcnt.as<unsigned int>();
         double *hs = e.d_hsum.as<double>();
-        hipLaunchKernelGGL(rlnk::k_wpart_init, dim3(gB), dim3(rlnk::kBlock), 0, st, wp, B);
-        hipLaunchKernelGGL(rlnk::k_wseg_min, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp);
+        if (!split) hipLaunchKernelGGL(rlnk::k_wseg_min, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp);
         hipLaunchKernelGGL(rlnk::k_wseg_exp, dim3(nseg), dim3(rlnk::kBlock), 0, st, wa, sg, wp, hc, hs);
         hipLaunchKernelGGL(rlnk::k_wsel_start, dim3(gB), dim3(rlnk::kBlock), 0, st, wa, wp, B);
         for (int shift = 56; shift >= 0; shift -= 8) {

@@ -375,6 +375,23 @@ __global__ void __launch_bounds__(kBlock) k_phase_split(const float2 *__restrict
     elo[idx] = l;
 }
 
+// Per-particle state of the coarse weights and significance selection (k_wpart_init .. k_wsel_stats); the
+// split coarse kernel already folds its minimum diff2 into mn_bits.
+struct WPart {
+    unsigned int mn_bits;   // min diff2 (non-negative float bits)
+    int status;
+    double sum;             // sum of weights
+    double target;          // adaptive_fraction * sum
+    unsigned long long prefix, mask;  // radix selection state
+    double above_sum;       // weights in digit bins above the crossing bin
+    long long above_cnt;
+    int gather;             // 1: crossing bin small enough, elements gathered
+    int ncand;
+    double thr;             // significance threshold
+    long long nr_sign;
+    long long n_above;      // entries >= thr
+};
+
 constexpr int kMfO = 64;    // orientations per tile (4 MFMA row tiles)
 constexpr int kMfG = 1;     // tiles per block, each with its own 4 waves
 constexpr int kMfBlock = 256 * kMfG;
@@ -400,7 +417,7 @@ __global__ void __launch_bounds__(kMfBlock, 3) k_coarse_mfma(const CoarseTile *_
                                                           const float *__restrict__ xi2half,
                                                           const long long *__restrict__ d2_off, int NT, int Kp,
                                                           const float2 *__restrict__ zero_row,
-                                                          float *__restrict__ diff2) {
+                                                          float *__restrict__ diff2, WPart *__restrict__ part) {
     constexpr int kTW = (NTT + 3) / 4;  // translation tiles per wave
     constexpr int kAStride = 40;        // bf16 per staged A row (32 + 8 padding against bank conflicts)
     __shared__ int og_s[kMfG][kMfO];
@@ -526,6 +543,7 @@ __global__ void __launch_bounds__(kMfBlock, 3) k_coarse_mfma(const CoarseTile *_
     if (!has) return;
     const float base = xi2half[b] + (float)ynorm[b];
     float *out = diff2 + d2_off[b];
+    float mn = 3.0e38f;  // minimum of the non-negative values written, for the weights' exponent offset
 #pragma unroll
     for (int j = 0; j < kTW; ++j) {
         if (tt[j] < 0) continue;
@@ -536,10 +554,14 @@ __global__ void __launch_bounds__(kMfBlock, 3) k_coarse_mfma(const CoarseTile *_
             for (int r = 0; r < 4; ++r) {
                 const int oo = 16 * i + gq * 4 + r;  // D row within the tile
                 if (oo >= n_rows) continue;
-                out[(size_t)(tile.o_begin + oo) * NT + t] =
-                    og_s[grp][oo] < 0 ? -999.f : base + a_norm[grp][oo] - 2.f * acc[i][j][r];
+                const float v = og_s[grp][oo] < 0 ? -999.f : base + a_norm[grp][oo] - 2.f * acc[i][j][r];
+                out[(size_t)(tile.o_begin + oo) * NT + t] = v;
+                if (v >= 0.f) mn = fminf(mn, v);
             }
     }
+    for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
+    // sign bit cleared: -0 orders as +0 among the unsigned bit patterns of non-negative floats
+    if (lane == 0 && mn < 3.0e38f) atomicMin(&part[b].mn_bits, __float_as_uint(mn) & 0x7fffffffu);
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -1003,20 +1025,6 @@ struct WSeg {
     long long end;
 };
 
-struct WPart {
-    unsigned int mn_bits;   // min diff2 (non-negative float bits)
-    int status;
-    double sum;             // sum of weights
-    double target;          // adaptive_fraction * sum
-    unsigned long long prefix, mask;  // radix selection state
-    double above_sum;       // weights in digit bins above the crossing bin
-    long long above_cnt;
-    int gather;             // 1: crossing bin small enough, elements gathered
-    int ncand;
-    double thr;             // significance threshold
-    long long nr_sign;
-    long long n_above;      // entries >= thr
-};
 
 struct WSegBest {
     double w;
@@ -1046,7 +1054,7 @@ __global__ void __launch_bounds__(kBlock) k_wseg_min(WeightArgs a, const WSeg *_
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < kBlock / 64; ++w) mn = fminf(mn, red[w]);
-        if (mn < 3.0e38f) atomicMin(&part[sg.b].mn_bits, __float_as_uint(mn));
+        if (mn < 3.0e38f) atomicMin(&part[sg.b].mn_bits, __float_as_uint(mn) & 0x7fffffffu);
     }
 }
 
