@@ -247,7 +247,8 @@ int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *dif
 /* Device time of the last run per stage, in milliseconds, from HIP events on the engine's stream:
  * prep, coarse_diff2, coarse_weights, jobs (fine-pass plan and work items), fine_diff2 (the fine diff2 kernel),
  * fine_weights, wsum_backproject; then host wall
- * time of chunk preparation, job-list building, the epilogue, and the whole run. */
+ * time of chunk preparation (list building not overlapped with the device), of it the first chunk's lists (built
+ * before any device work of the run), the epilogue, and the whole run. */
 int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
 /* Work counters of the last run: particles, coarse poses, coarse pixels per image, fine poses, fine pixels
  * per image, fine diff2 work items, backprojected fine orientations, significant (orientation, translation)

@@ -202,7 +202,7 @@ _SIGNATURES = {
 }
 
 STAGES = ("prep", "coarse_diff2", "coarse_weights", "jobs", "fine_diff2", "fine_weights", "wsum_backproject",
-          "host_chunk_prep", "host_job_lists", "host_epilogue", "run_wall")
+          "host_chunk_prep", "host_first_chunk", "host_epilogue", "run_wall")
 
 
 def _load():

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -77,6 +78,31 @@ static int pack_into_projector(rln_projector *p, const float2 *host_plain) {
 
 namespace rln {
 
+struct ChunkHost;
+
+// Allocator whose value-less construct leaves trivial elements uninitialised: resize() of a host list that is
+// written in full afterwards costs no zero fill.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U> &) {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using HostList = std::vector<T, UninitAlloc<T>>;
+
 // Stage timers of one run (HIP events on the engine's stream).
 enum Stage { kStPrep, kStCoarse, kStCoarseW, kStJobs, kStFine, kStFineW, kStWsum, kStCount };
 // fine stage graphs: plan and work items; fine diff2; fine weights; weighted sums and backprojection
@@ -134,6 +160,8 @@ struct Engine {
     int need_jobs = 0, need_small = 0, need_wide = 0;
     DevBuf d_plan, d_cp, d_items_s, d_items_w;
     DevBuf d_wsegs, d_seg_first, d_wpart, d_hcnt, d_hsum, d_wcand, d_segbest, d_ocnt;  // segmented coarse weights
+    PinnedBuf h_stage[2];  // a chunk's lists packed for upload, alternating between consecutive chunks
+    std::unique_ptr<ChunkHost> chunk_host[2];  // host lists of consecutive chunks, kept to reuse their memory
     rlnk::ChunkPtrs cp_host{};
     bool tuned = false;
     int grid_fine = 1024, grid_wsum = 2048;  // persistent grids (fine: tuned; weighted sums: resident blocks)
@@ -158,7 +186,7 @@ struct Engine {
     // timings
     hipEvent_t ev[kStCount + 1] = {};
     double stage_ms[kStCount] = {};
-    double host_ms[4] = {};  // host phases: chunk preparation, job lists, epilogue, whole run (wall)
+    double host_ms[4] = {};  // host phases: chunk preparation, of it the first chunk's lists, epilogue, whole run
     long long counters[17] = {};
     long long pf_sampled = 0;  // fine pixels gathered per fine orientation
     bool have_timing = false;
@@ -458,18 +486,33 @@ static int engine_coarse_projections(Engine &e) {
 // ------------------------------------------------------------------------------------------------
 struct ChunkHost {
     int B = 0;
-    std::vector<int> or_off, or_cnt, or_proj, or_frame, or_class, or_dir;
+    std::vector<int> or_off, or_cnt;
+    HostList<int> or_proj, or_frame, or_class, or_dir;  // per orientation entry, every element written
     std::vector<int> oc_off;       // per particle: offset of its per-orientation state (significant counts)
     long long total_oc = 0;
     std::vector<int> nd, np;  // per particle: directions, psi (an explicit list counts as nd = its length)
-    std::vector<double> or_prior, tr_prior;
+    HostList<double> or_prior, tr_prior;
     std::vector<long long> d2_off, n_entries;
     std::vector<float> xi2half, scale, scale_w;
     std::vector<double> old_off, prior_off;
-    std::vector<rlnk::CoarseTile> tiles;
-    std::vector<rlnk::WSeg> segs;  // coarse-weight segments (kSegEntries entries each)
+    HostList<rlnk::CoarseTile> tiles;
+    HostList<rlnk::WSeg> segs;  // coarse-weight segments (kSegEntries entries each)
     std::vector<int> seg_first;    // [B+1] first segment of each particle
     long long total_entries = 0;
+    // per-particle scalars as uploaded, with the chunk capacity cap_B as stride so that their device addresses do
+    // not depend on the chunk's size: xi2half, scale, scale_w; old offsets (x, y), prior offsets (x, y)
+    std::vector<float> meta_f;
+    std::vector<double> meta_d;
+    // the uploaded arrays packed in page-locked memory (pack_chunk_host), offset and size of each; null: uploaded
+    // from the vectors
+    const unsigned char *pk = nullptr;
+    size_t pk_off[16] = {}, pk_bytes[16] = {};
+};
+
+// Arrays of a chunk's upload, in packing order.
+enum PackedArray {
+    kPkOrOff, kPkOcOff, kPkOrCnt, kPkOrProj, kPkOrPrior, kPkOrFrame, kPkOrClass, kPkOrDir, kPkTrPrior, kPkTiles,
+    kPkD2Off, kPkNent, kPkMetaF, kPkMetaD, kPkSegs, kPkSegFirst, kPkCount
 };
 
 // set_off/set_og (optional): explicit coarse orientation lists of the chunk's particles (global grid indices
@@ -479,7 +522,10 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
                              const int *set_off = nullptr, const int *set_og = nullptr) {
     const rln_estep_config &cfg = e.cfg;
     const int K = e.K, NT = e.NT;
-    h = ChunkHost();
+    // the lists of an earlier chunk are overwritten in place (no new allocation); every element of every list
+    // below is written
+    h.pk = nullptr;
+    h.total_oc = h.total_entries = 0;
     h.B = B;
     h.or_off.resize(B);
     h.oc_off.resize(B);
@@ -644,6 +690,18 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         h.prior_off[2 * b] = m.prior_offset_x;
         h.prior_off[2 * b + 1] = m.prior_offset_y;
     }
+    const int cB = e.cap_B;
+    h.meta_f.assign(3 * (size_t)cB, 0.f);
+    for (int b = 0; b < B; ++b) {
+        h.meta_f[b] = h.xi2half[b];
+        h.meta_f[cB + b] = h.scale[b];
+        h.meta_f[2 * cB + b] = h.scale_w[b];
+    }
+    h.meta_d.assign(4 * (size_t)cB, 0.);
+    for (int b = 0; b < 2 * B; ++b) {
+        h.meta_d[b] = h.old_off[b];
+        h.meta_d[2 * cB + b] = h.prior_off[b];
+    }
     // Global searches enumerate the same grid for every particle: order the coarse tiles by orientation block
     // first, so that blocks running together read the same projection rows (L2 hits) instead of each particle
     // streaming the whole grid from HBM. Tiles of particles searched over the whole grid (all of them, or the
@@ -657,6 +715,62 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
             return fa && a.o_begin < b.o_begin;
         });
     }
+}
+
+// Host array `which` (PackedArray) of a chunk's upload: address and bytes.
+static std::pair<const void *, size_t> chunk_array(const ChunkHost &h, int which) {
+    auto src = [](const auto &v) { return std::make_pair((const void *)v.data(), v.size() * sizeof(v[0])); };
+    switch (which) {
+        case kPkOrOff: return src(h.or_off);
+        case kPkOcOff: return src(h.oc_off);
+        case kPkOrCnt: return src(h.or_cnt);
+        case kPkOrProj: return src(h.or_proj);
+        case kPkOrPrior: return src(h.or_prior);
+        case kPkOrFrame: return src(h.or_frame);
+        case kPkOrClass: return src(h.or_class);
+        case kPkOrDir: return src(h.or_dir);
+        case kPkTrPrior: return src(h.tr_prior);
+        case kPkTiles: return src(h.tiles);
+        case kPkD2Off: return src(h.d2_off);
+        case kPkNent: return src(h.n_entries);
+        case kPkMetaF: return src(h.meta_f);
+        case kPkMetaD: return src(h.meta_d);
+        case kPkSegs: return src(h.segs);
+        default: return src(h.seg_first);
+    }
+}
+
+// Packs a chunk's upload into page-locked memory, each array 256-byte aligned, so that run_chunk uploads it with
+// copies asynchronous to the host. For chunks built on the second host thread while the device runs the previous
+// chunk; the first chunk of a run is uploaded from its vectors, as packing it would delay the device's start.
+static int pack_chunk_host(Engine &e, ChunkHost &h, PinnedBuf &pb) {
+    size_t tot = 0;
+    for (int i = 0; i < kPkCount; ++i) {
+        h.pk_off[i] = tot;
+        h.pk_bytes[i] = chunk_array(h, i).second;
+        tot += (h.pk_bytes[i] + 255) & ~(size_t)255;
+    }
+    HIP_TRY(hipSetDevice(e.device));
+    RLN_TRY(pb.ensure(tot));
+    unsigned char *dst = static_cast<unsigned char *>(pb.ptr);
+    for (int i = 0; i < kPkCount; ++i) {
+        const std::pair<const void *, size_t> a = chunk_array(h, i);
+        if (a.second) std::memcpy(dst + h.pk_off[i], a.first, a.second);
+    }
+    h.pk = dst;
+    return RLN_OK;
+}
+
+// One array of a chunk's upload to its device buffer (grown to at least 256 bytes), from the page-locked
+// package when there is one, else from the host vector.
+static int upload_packed(DevBuf &buf, const ChunkHost &h, int which, hipStream_t st) {
+    const std::pair<const void *, size_t> a = chunk_array(h, which);
+    RLN_TRY(buf.ensure(std::max<size_t>(256, a.second)));
+    if (a.second) {
+        const void *src = h.pk ? (const void *)(h.pk + h.pk_off[which]) : a.first;
+        HIP_TRY(hipMemcpyAsync(buf.ptr, src, a.second, hipMemcpyHostToDevice, st));
+    }
+    return RLN_OK;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -901,36 +1015,25 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         dX = e.d_fimg.as<const float2>();
         dC = e.d_fctf.as<const float>();
     }
-    RLN_TRY(upload(e.d_or_off, h.or_off, st));
-    RLN_TRY(upload(e.d_oc_off, h.oc_off, st));
-    RLN_TRY(upload(e.d_or_cnt, h.or_cnt, st));
-    RLN_TRY(upload(e.d_or_proj, h.or_proj, st));
-    RLN_TRY(upload(e.d_or_prior, h.or_prior, st));
-    RLN_TRY(upload(e.d_or_frame, h.or_frame, st));
-    RLN_TRY(upload(e.d_or_class, h.or_class, st));
-    RLN_TRY(upload(e.d_or_dir, h.or_dir, st));
-    RLN_TRY(upload(e.d_tr_prior, h.tr_prior, st));
-    RLN_TRY(upload(e.d_tiles, h.tiles, st));
-    std::vector<long long> d2off = h.d2_off;
-    RLN_TRY(upload(e.d_meta_ll, d2off, st));
+    // the chunk's lists, packed in page-locked memory when they were built: asynchronous copies
+    RLN_TRY(upload_packed(e.d_or_off, h, kPkOrOff, st));
+    RLN_TRY(upload_packed(e.d_oc_off, h, kPkOcOff, st));
+    RLN_TRY(upload_packed(e.d_or_cnt, h, kPkOrCnt, st));
+    RLN_TRY(upload_packed(e.d_or_proj, h, kPkOrProj, st));
+    RLN_TRY(upload_packed(e.d_or_prior, h, kPkOrPrior, st));
+    RLN_TRY(upload_packed(e.d_or_frame, h, kPkOrFrame, st));
+    RLN_TRY(upload_packed(e.d_or_class, h, kPkOrClass, st));
+    RLN_TRY(upload_packed(e.d_or_dir, h, kPkOrDir, st));
+    RLN_TRY(upload_packed(e.d_tr_prior, h, kPkTrPrior, st));
+    RLN_TRY(upload_packed(e.d_tiles, h, kPkTiles, st));
+    RLN_TRY(upload_packed(e.d_meta_ll, h, kPkD2Off, st));
     DevBuf &d_d2off = e.d_meta_ll;
     DevBuf &d_nent_tmp = e.d_nent;  // entries per particle
-    RLN_TRY(upload(d_nent_tmp, h.n_entries, st));
-    // per-particle scalars with the chunk capacity as stride, so that their addresses do not depend on B
+    RLN_TRY(upload_packed(d_nent_tmp, h, kPkNent, st));
+    // per-particle scalars with the chunk capacity as stride
     const int cB = e.cap_B;
-    std::vector<float> mf(3 * (size_t)cB, 0.f);
-    for (int b = 0; b < B; ++b) {
-        mf[b] = h.xi2half[b];
-        mf[cB + b] = h.scale[b];
-        mf[2 * cB + b] = h.scale_w[b];
-    }
-    RLN_TRY(upload(e.d_meta_f, mf, st));
-    std::vector<double> md(4 * (size_t)cB, 0.);
-    for (int b = 0; b < 2 * B; ++b) {
-        md[b] = h.old_off[b];
-        md[2 * cB + b] = h.prior_off[b];
-    }
-    RLN_TRY(upload(e.d_meta_d, md, st));
+    RLN_TRY(upload_packed(e.d_meta_f, h, kPkMetaF, st));
+    RLN_TRY(upload_packed(e.d_meta_d, h, kPkMetaD, st));
     const float *d_xi2 = e.d_meta_f.as<const float>();
     const float *d_scale = d_xi2 + cB;
     const float *d_scale_w = d_xi2 + 2 * cB;
@@ -998,8 +1101,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     // coarse weights over segments of kSegEntries poses (multi-block per particle)
     {
         const unsigned nseg = (unsigned)h.segs.size();
-        RLN_TRY(upload(e.d_wsegs, h.segs, st));
-        RLN_TRY(upload(e.d_seg_first, h.seg_first, st));
+        RLN_TRY(upload_packed(e.d_wsegs, h, kPkSegs, st));
+        RLN_TRY(upload_packed(e.d_seg_first, h, kPkSegFirst, st));
         RLN_TRY(e.d_hcnt.ensure(sizeof(unsigned int) * 256 * B));
         RLN_TRY(e.d_hsum.ensure(sizeof(double) * 256 * B));
         RLN_TRY(e.d_wcand.ensure(sizeof(double) * rlnk::kGather * B));
@@ -1612,14 +1715,17 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     for (size_t c = 0; c + 1 < starts.size(); ++c) e->cap_B = std::max(e->cap_B, starts[c + 1] - starts[c]);
     // chunk c runs on the device while a second host thread builds the lists of chunk c + 1
     using clk = std::chrono::steady_clock;
-    rln::ChunkHost cur, nxt;
+    for (auto &ch : e->chunk_host)
+        if (!ch) ch.reset(new rln::ChunkHost());
+    rln::ChunkHost *pcur = e->chunk_host[0].get(), *pnxt = e->chunk_host[1].get();
     auto so = [&](int i) { return have_sets ? set_off.data() + i : nullptr; };
     auto sg = [&](int i) { return have_sets ? set_og.data() + set_off[i] : nullptr; };
     const int n_chunks = (int)starts.size() - 1;
     if (n_chunks > 0) {
         const clk::time_point t0 = clk::now();
-        rln::build_chunk_host(*e, starts[1] - starts[0], meta, cur, so(0), sg(0));
-        e->host_ms[0] += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        rln::build_chunk_host(*e, starts[1] - starts[0], meta, *pcur, so(0), sg(0));
+        e->host_ms[1] = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        e->host_ms[0] += e->host_ms[1];
     }
     for (int c = 0; c < n_chunks; ++c) {
         const int i0 = starts[c], B = starts[c + 1] - starts[c];
@@ -1627,9 +1733,11 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
         std::thread builder;
         if (c + 1 < n_chunks) {
             const int i1 = starts[c + 1], B1 = starts[c + 2] - starts[c + 1];
-            builder = std::thread([&, i1, B1] {
+            // chunk c + 1 packs into the staging buffer of chunk c - 1, whose copies ended with that chunk
+            builder = std::thread([&, i1, B1, c] {
                 try {
-                    rln::build_chunk_host(*e, B1, meta + i1, nxt, so(i1), sg(i1));
+                    rln::build_chunk_host(*e, B1, meta + i1, *pnxt, so(i1), sg(i1));
+                    if (rln::pack_chunk_host(*e, *pnxt, e->h_stage[(c + 1) & 1]) != RLN_OK) next_ok = false;
                 } catch (...) {
                     next_ok = false;
                 }
@@ -1637,7 +1745,7 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
         }
         const size_t off_c = (size_t)i0 * Pf;
         const int rc = rln::run_chunk(*e, B, fimg + 2 * off_c, fimg_nomask + 2 * off_c, fctf + off_c,
-                                      inputs_on_device, meta + i0, cur, results + i0, wsum);
+                                      inputs_on_device, meta + i0, *pcur, results + i0, wsum);
         if (builder.joinable()) {
             const clk::time_point t0 = clk::now();
             builder.join();
@@ -1645,7 +1753,7 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
         }
         if (rc != RLN_OK) return rc;
         if (!next_ok) RLN_FAIL(RLN_ERR_MEMORY, "rln_estep_run: building the lists of a chunk failed");
-        std::swap(cur, nxt);
+        std::swap(pcur, pnxt);
     }
     e->have_timing = true;
     if (wsum && wsum->pdf_direction) {

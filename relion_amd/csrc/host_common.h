@@ -59,6 +59,33 @@ struct DevBuf {
     }
 };
 
+// Grow-only page-locked host buffer (hipHostMalloc): copies from it run asynchronously to the host.
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+    int ensure(size_t need) {
+        if (need <= bytes) return RLN_OK;
+        release();
+        const size_t want = std::max<size_t>(need + need / 4, 4096);
+        if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess) {
+            ptr = nullptr;
+            set_error("page-locked host allocation of %zu bytes failed", want);
+            return RLN_ERR_MEMORY;
+        }
+        bytes = want;
+        return RLN_OK;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
 struct MdlDims {
     int xdim = 0, ydim = 0, zdim = 0, inity = 0, initz = 0, maxr = 0;
     double pad = 0.;
