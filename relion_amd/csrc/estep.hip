@@ -589,6 +589,26 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     }
     h.total_entries = d2;
     h.total_oc = tot_oc;
+    // Tile placement. Global searches enumerate the same grid for every particle: the coarse tiles are ordered by
+    // orientation block first, so that blocks running together read the same projection rows (L2 hits) instead
+    // of each particle streaming the whole grid from HBM. Tiles of particles searched over the whole grid (all of
+    // them, or the exhaustive fallbacks among explicit lists) come first, tile j of the r-th such particle at
+    // j * n_full + r; tiles of explicit lists after them in particle order. Local searches keep particle order.
+    std::vector<int> tile_base(B), full_rank(B, -1);
+    int n_full = 0;
+    if (!cfg.local_search) {
+        for (int b = 0; b < B; ++b)
+            if (full_grid(b)) full_rank[b] = n_full++;
+        const int t_full = (n_grid + rlnk::kCoTO - 1) / rlnk::kCoTO;
+        int t = n_full * t_full;
+        for (int b = 0; b < B; ++b)
+            if (full_rank[b] < 0) {
+                tile_base[b] = t;
+                t += tile_off[b + 1] - tile_off[b];
+            }
+    } else {
+        for (int b = 0; b < B; ++b) tile_base[b] = tile_off[b];
+    }
     h.or_proj.resize(tot_or);
     h.or_prior.resize(tot_or);
     h.or_frame.resize(tot_or);
@@ -674,8 +694,13 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         tmean = tmean * K / (double)(K * NT);  // mean over classes x translations (same per class)
         if (tmean > 0.)
             for (int t = 0; t < NT; ++t) h.tr_prior[(size_t)b * NT + t] /= tmean;
-        int ti = tile_off[b];
-        for (int o = 0; o < h.or_cnt[b]; o += rlnk::kCoTO) h.tiles[ti++] = rlnk::CoarseTile{b, o};
+        if (full_rank[b] >= 0) {
+            for (int o = 0, j = 0; o < h.or_cnt[b]; o += rlnk::kCoTO, ++j)
+                h.tiles[(size_t)j * n_full + full_rank[b]] = rlnk::CoarseTile{b, o};
+        } else {
+            int ti = tile_base[b];
+            for (int o = 0; o < h.or_cnt[b]; o += rlnk::kCoTO) h.tiles[ti++] = rlnk::CoarseTile{b, o};
+        }
         const long long n = h.n_entries[b];
         for (int sI = seg_off[b]; sI < seg_off[b + 1]; ++sI) {
             const long long qs = sI - seg_off[b];
@@ -701,19 +726,6 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
     for (int b = 0; b < 2 * B; ++b) {
         h.meta_d[b] = h.old_off[b];
         h.meta_d[2 * cB + b] = h.prior_off[b];
-    }
-    // Global searches enumerate the same grid for every particle: order the coarse tiles by orientation block
-    // first, so that blocks running together read the same projection rows (L2 hits) instead of each particle
-    // streaming the whole grid from HBM. Tiles of particles searched over the whole grid (all of them, or the
-    // exhaustive fallbacks among explicit lists) come first, by orientation block; tiles of explicit lists after
-    // them in particle order.
-    if (!cfg.local_search) {
-        auto full = [&](const rlnk::CoarseTile &t) { return !set_off || set_off[t.b + 1] == set_off[t.b]; };
-        std::stable_sort(h.tiles.begin(), h.tiles.end(), [&](const rlnk::CoarseTile &a, const rlnk::CoarseTile &b) {
-            const bool fa = full(a), fb = full(b);
-            if (fa != fb) return fa;
-            return fa && a.o_begin < b.o_begin;
-        });
     }
 }
 
