@@ -140,6 +140,9 @@ struct Engine {
     DevBuf d_ex_c, d_ey_c, d_ex_f, d_ey_f, d_tr_ix, d_tr_iy, d_over_x, d_over_y;
     DevBuf d_vols, d_bp_ptrs;
     std::vector<double> over_x, over_y;  // [NT*4] pixels
+    // coarse translations on their lattice: distinct x and y values (Angstrom) and each translation's indices
+    std::vector<double> tr_ux, tr_uy;
+    std::vector<int> tr_cix, tr_ciy;
     std::vector<int> mresol_f;
     double logsigma2 = 0.;
     // per-chunk buffers
@@ -327,6 +330,29 @@ static int engine_init(Engine &e, const rln_estep_config &cfg) {
     for (int t = 0; t < e.NT; ++t) {
         cx[t] = e.grid.tx[t] / cfg.pixel_size;
         cy[t] = e.grid.ty[t] / cfg.pixel_size;
+    }
+    {
+        std::map<double, int> mx, my;
+        for (int t = 0; t < e.NT; ++t) {
+            mx.emplace(e.grid.tx[t], 0);
+            my.emplace(e.grid.ty[t], 0);
+        }
+        e.tr_ux.clear();
+        e.tr_uy.clear();
+        for (auto &kv : mx) {
+            kv.second = (int)e.tr_ux.size();
+            e.tr_ux.push_back(kv.first);
+        }
+        for (auto &kv : my) {
+            kv.second = (int)e.tr_uy.size();
+            e.tr_uy.push_back(kv.first);
+        }
+        e.tr_cix.resize(e.NT);
+        e.tr_ciy.resize(e.NT);
+        for (int t = 0; t < e.NT; ++t) {
+            e.tr_cix[t] = mx[e.grid.tx[t]];
+            e.tr_ciy[t] = my[e.grid.ty[t]];
+        }
     }
     std::vector<float2> rows;
     phase_rows(cx, e.sc, false, cfg.ori_size, rows);
@@ -680,16 +706,40 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
             if (mean != 0.)
                 for (int i = 0; i < K * ND * NP; ++i) h.or_prior[base + i] /= mean;
         }
-        // offset priors (src/ml_optimiser.cpp:7805-7872): offsets in pixels plus translations in Angstrom
+        // offset priors (src/ml_optimiser.cpp:7805-7872): offsets in pixels plus translations in Angstrom. The
+        // Gaussian exp(td) factors into exp(x part) * exp(y part), evaluated once per distinct x and y value of the
+        // translation lattice (within a few ulp of exp(td), against one exp per translation)
         double tmean = 0.;
-        for (int t = 0; t < NT; ++t) {
-            const double ox = m.old_offset_x + e.grid.tx[t], oy = m.old_offset_y + e.grid.ty[t];
-            double td = (ox - m.prior_offset_x) * (ox - m.prior_offset_x) / (-2. * s2);
-            td += (oy - m.prior_offset_y) * (oy - m.prior_offset_y) / (-2. * s2);
-            td *= px * px;
-            const double pf = (s2 < 0.0001) ? ((td > 0.) ? 0. : 1.) : exp(td) / offset_norm;
-            h.tr_prior[(size_t)b * NT + t] = pf;
-            tmean += pf;
+        if (s2 < 0.0001) {
+            for (int t = 0; t < NT; ++t) {
+                const double ox = m.old_offset_x + e.grid.tx[t], oy = m.old_offset_y + e.grid.ty[t];
+                double td = (ox - m.prior_offset_x) * (ox - m.prior_offset_x) / (-2. * s2);
+                td += (oy - m.prior_offset_y) * (oy - m.prior_offset_y) / (-2. * s2);
+                td *= px * px;
+                const double pf = (td > 0.) ? 0. : 1.;
+                h.tr_prior[(size_t)b * NT + t] = pf;
+                tmean += pf;
+            }
+        } else {
+            const size_t nux = e.tr_ux.size(), nuy = e.tr_uy.size();
+            double fx_s[64], fy_s[64];
+            std::vector<double> fx_v, fy_v;
+            double *fx = fx_s, *fy = fy_s;
+            if (nux > 64) fx = (fx_v.resize(nux), fx_v.data());
+            if (nuy > 64) fy = (fy_v.resize(nuy), fy_v.data());
+            for (size_t i = 0; i < nux; ++i) {
+                const double d = m.old_offset_x + e.tr_ux[i] - m.prior_offset_x;
+                fx[i] = exp(d * d / (-2. * s2) * px * px);
+            }
+            for (size_t i = 0; i < nuy; ++i) {
+                const double d = m.old_offset_y + e.tr_uy[i] - m.prior_offset_y;
+                fy[i] = exp(d * d / (-2. * s2) * px * px);
+            }
+            for (int t = 0; t < NT; ++t) {
+                const double pf = fx[e.tr_cix[t]] * fy[e.tr_ciy[t]] / offset_norm;
+                h.tr_prior[(size_t)b * NT + t] = pf;
+                tmean += pf;
+            }
         }
         tmean = tmean * K / (double)(K * NT);  // mean over classes x translations (same per class)
         if (tmean > 0.)
