@@ -153,6 +153,7 @@ _SIGNATURES = {
     "rln_estep_keep_intermediates": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_orientation_sets": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "rln_estep_use_graphs": (C.c_int, [_vp, C.c_int]),
+    "rln_estep_set_pool": (C.c_int, [_vp, C.c_int]),
     "rln_estep_coarse_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _ip, _ip, _ip]),
     "rln_estep_get_coarse": (C.c_int, [_vp, C.c_int, _fp, _dp, C.POINTER(C.c_ubyte)]),
     "rln_estep_fine_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong)]),
@@ -543,6 +544,10 @@ class ExpectationStep:
 
     def use_graphs(self, enable=True):
         _check(lib.rln_estep_use_graphs(self._h, 1 if enable else 0))
+
+    def set_pool(self, max_particles):
+        """Particles per chunk at most (the reference's --pool); results do not depend on it."""
+        _check(lib.rln_estep_set_pool(self._h, int(max_particles)))
 
     def set_orientation_lists(self, offsets, orients):
         """set_orientation_sets in the flat form: offsets[n+1], orients[offsets[n]] (int32)."""

@@ -175,6 +175,7 @@ struct Engine {
     bool graph_off = false;               // capture unsupported: launch directly
     // intermediates kept for tests
     bool keep = false;
+    int max_pool = 4096;  // particles per chunk at most (rln_estep_set_pool)
     struct Kept {
         long long n_coarse = 0;
         int n_dir = 0, n_psi = 0;
@@ -1698,6 +1699,14 @@ extern "C" int rln_estep_set_model(rln_estep *e, rln_projector *const *projector
     return RLN_OK;
 }
 
+extern "C" int rln_estep_set_pool(rln_estep *e, int max_particles) {
+    if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_pool: null engine");
+    if (max_particles < 1) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_pool: %d particles per chunk", max_particles);
+    std::lock_guard<std::mutex> lk(e->mtx);
+    e->max_pool = max_particles;
+    return RLN_OK;
+}
+
 extern "C" int rln_estep_use_graphs(rln_estep *e, int enable) {
     if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_use_graphs: null engine");
     std::lock_guard<std::mutex> lk(e->mtx);
@@ -1764,7 +1773,7 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
             long long orients = e->cfg.local_search ? std::max(1, e->n_or / 8) : e->n_or;
             if (have_sets && set_off[i + 1] > set_off[i]) orients = set_off[i + 1] - set_off[i];
             const double need = 13.0 * (double)e->K * (double)orients * e->NT + fixed;
-            if (cnt == 0 || (!e->keep && (used + need > budget || cnt >= 4096))) {
+            if (cnt == 0 || (!e->keep && (used + need > budget || cnt >= e->max_pool))) {
                 starts.push_back(i);
                 used = 0.;
                 cnt = 0;

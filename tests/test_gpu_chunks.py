@@ -1,0 +1,81 @@
+"""A batch split into many chunks gives the results of the same batch in one chunk.
+
+The E-step processes particles in chunks (at most `set_pool` particles each). From the third chunk on, a chunk's
+lists are built and packed into page-locked memory on a second host thread while the device runs the chunk before,
+alternating between two staging buffers; these tests run 13 particles as one chunk and as five chunks (3, 3, 3, 3,
+1) and require the same per-particle results. Per-particle values do not depend on chunking except through the
+order of floating-point atomic sums (FP64 sums of weights over segments), hence the tolerances; the best poses and
+the minimum diff2 are equal. Sums over particles (weighted sums, backprojection) are compared at the tolerance of
+their FP32 / FP64 atomic accumulation order, which varies from run to run as well. The split case mixes exhaustive
+fallbacks (empty lists) with explicit lists, so that the coarse tiles of both kinds are placed in every chunk.
+"""
+import numpy as np
+import pytest
+
+import relion_amd as ra
+from problems import make_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def engine(p, precision):
+    cfg = p.cfg
+    cfg.precision = precision
+    dims = ra.volume_dims(cfg.current_size, cfg.padding_factor)
+    pr = ra.AccProjector(0)
+    pr.setMdlDim(*dims, cfg.padding_factor)
+    pr.initMdl(p.volumes[0])
+    bp = ra.AccBackprojector(0)
+    bp.setMdlDim(*dims, cfg.padding_factor)
+    es = ra.ExpectationStep(cfg, 0)
+    return es, pr, bp
+
+
+def run(es, pr, bp, p, sets):
+    bp.initMdl()
+    es.set_model([pr], [bp], p.pdf_class, p.sigma2_noise, pdf_direction=p.pdf_direction)
+    if sets is not None:
+        es.set_orientation_sets(sets)
+    res, wsum = es.run(p.fimg, p.fctf, p.meta)
+    return res, wsum, bp.getMdlData(), es.counters()["chunks"]
+
+
+@pytest.mark.parametrize("precision", [ra.RLN_PREC_PLAIN_FP32, ra.RLN_PREC_SPLIT_BF16], ids=["plain", "split"])
+def test_chunks_match_one_batch(precision):
+    n = 13
+    p = make_problem(box=64, px=2.0, order=2, n=n, snr=0.1, seed=31)
+    print(f"[chunks] box 64 order 2 global, {n} particles, precision {precision}", flush=True)
+    es, pr, bp = engine(p, precision)
+    sets = None
+    if precision == ra.RLN_PREC_SPLIT_BF16:
+        n_or = es.n_dir * es.n_psi
+        rng = np.random.default_rng(3)
+        sets = [np.zeros(0, np.int32) if i % 4 == 1 else np.sort(rng.choice(n_or, 150, replace=False)).astype(np.int32)
+                for i in range(n)]
+    es.set_pool(4096)
+    r1, w1, v1, c1 = run(es, pr, bp, p, sets)
+    es.set_pool(3)
+    r5, w5, v5, c5 = run(es, pr, bp, p, sets)
+    print(f"[chunks] {c1} chunk(s) vs {c5} chunks", flush=True)
+    assert c1 == 1 and c5 == 5
+    for f in ("best_ihidden_over", "iclass", "rot", "tilt", "psi", "xoff", "yoff", "min_diff2", "coarse_min_diff2",
+              "nr_fine_evaluated"):
+        assert np.array_equal(r1[f], r5[f]), f
+    for f in ("sum_weight", "coarse_sum_weight", "pmax", "dLL", "significant_weight", "coarse_significant_weight",
+              "signal_product", "reference_power", "wsum_norm_correction", "nr_significant"):
+        assert np.allclose(r1[f], r5[f], rtol=1e-9, atol=0), f
+    # sigma2_noise: per work item, shell sums are FP32 atomics in LDS (their order varies from run to run)
+    for f, rtol in (("sigma2_noise", 1e-5), ("pdf_class", 1e-9), ("pdf_direction", 1e-9)):
+        a, b = np.asarray(w1[f]), np.asarray(w5[f])
+        rel = np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-300))
+        print(f"[chunks] {f}: max relative difference {rel:.3g}", flush=True)
+        assert np.allclose(a, b, rtol=rtol, atol=1e-300), f
+    for f in ("sigma2_offset", "sumw_group", "signal_product", "reference_power", "LL", "ave_Pmax"):
+        assert np.isclose(w1[f], w5[f], rtol=1e-9, atol=0), f
+    # backprojection: FP32 atomics in another order
+    for a, b, name in zip(v1, v5, ("data re", "data im", "weight")):
+        err = np.linalg.norm(a - b) / np.linalg.norm(a)
+        assert err <= 1e-5, f"{name}: relative norm difference {err:.3g}"
+    with pytest.raises(ra.RelionError):
+        es.set_pool(0)
+    es.close()
