@@ -560,7 +560,9 @@ def main():
 
                 def run():
                     try:
+                        box_["t0"] = time.time()
                         box_["res"] = es_p.run(d_img.address, d_ctf.address, meta, on_device=True)[0]
+                        box_["t1"] = time.time()
                     except BaseException as exc:  # re-raised in the caller
                         box_["exc"] = exc
 
@@ -570,15 +572,23 @@ def main():
                 nxt = lists_of() if k + 1 < steps else None
                 phase_ms["next_lists"] += 1000.0 * (time.time() - t_a)
                 th.join()
-                phase_ms["estep_run"] += 1000.0 * (time.time() - t_a)
+                t_b = time.time()
+                phase_ms["estep_run"] += 1000.0 * (t_b - t_a)
                 if "exc" in box_:
                     raise box_["exc"]
+                phase_ms["estep_call"] += 1000.0 * (box_["t1"] - box_["t0"])
+                phase_ms["library_call"] += es_p.last_call_ms
+                phase_ms["thread_start"] += 1000.0 * (box_["t0"] - t_a)
+                phase_ms["thread_end"] += 1000.0 * (t_b - box_["t1"])
                 res, cur = box_["res"], lists
                 if nxt is not None:
                     lists = nxt
             return res, cur[0]
 
-        phase_ms = {"setup": 0.0, "next_lists": 0.0, "estep_run": 0.0}
+        # host wall time per step: set-up, the next batch's inference and lists (this thread), the E-step from
+        # thread start to join, of it the E-step wrapper call and the library call, and the thread's start / end
+        phase_ms = {"setup": 0.0, "next_lists": 0.0, "estep_run": 0.0, "estep_call": 0.0, "library_call": 0.0,
+                    "thread_start": 0.0, "thread_end": 0.0}
         hipmem.synchronize()
         t0 = time.time()
         pl_out = pipelined_steps(args.steps)

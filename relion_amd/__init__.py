@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 
 import numpy as np
 
@@ -592,7 +593,9 @@ class ExpectationStep:
             for i, m in enumerate(meta):
                 arr[i] = m
             meta = arr
+        t_call = time.perf_counter()
         _check(lib.rln_estep_run(self._h, n, a, an, c, 1 if on_device else 0, meta, res, C.byref(ws)))
+        self.last_call_ms = 1000.0 * (time.perf_counter() - t_call)  # the library call alone (wall)
         del keep
         results = np.frombuffer(res, dtype=RESULT_DTYPE, count=n).copy()
         wsum = dict(sigma2_noise=ws_s2, pdf_class=ws_pc, pdf_direction=ws_pd.reshape(K, self.n_dir),
