@@ -1767,13 +1767,17 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     const double budget = 24.0e9;
     std::vector<int> starts;
     {
+        // the first chunk holds at most a quarter of the pool: its lists are built before any device work of the
+        // run, the later chunks' while the device runs the chunk before
+        const int first_pool = std::max(1, e->max_pool / 4);
         double used = 0.;
         int cnt = 0;
         for (int i = 0; i < n; ++i) {
             long long orients = e->cfg.local_search ? std::max(1, e->n_or / 8) : e->n_or;
             if (have_sets && set_off[i + 1] > set_off[i]) orients = set_off[i + 1] - set_off[i];
             const double need = 13.0 * (double)e->K * (double)orients * e->NT + fixed;
-            if (cnt == 0 || (!e->keep && (used + need > budget || cnt >= e->max_pool))) {
+            const int pool = starts.size() == 1 ? first_pool : e->max_pool;
+            if (cnt == 0 || (!e->keep && (used + need > budget || cnt >= pool))) {
                 starts.push_back(i);
                 used = 0.;
                 cnt = 0;

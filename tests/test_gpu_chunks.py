@@ -2,8 +2,8 @@
 
 The E-step processes particles in chunks (at most `set_pool` particles each). From the third chunk on, a chunk's
 lists are built and packed into page-locked memory on a second host thread while the device runs the chunk before,
-alternating between two staging buffers; these tests run 13 particles as one chunk and as five chunks (3, 3, 3, 3,
-1) and require the same per-particle results. Per-particle values do not depend on chunking except through the
+alternating between two staging buffers; these tests run 13 particles as one chunk and as five chunks (1, 3, 3, 3,
+3: the first chunk of a run holds at most a quarter of the pool) and require the same per-particle results. Per-particle values do not depend on chunking except through the
 order of floating-point atomic sums (FP64 sums of weights over segments), hence the tolerances; the best poses and
 the minimum diff2 are equal. Sums over particles (weighted sums, backprojection) are compared at the tolerance of
 their FP32 / FP64 atomic accumulation order, which varies from run to run as well. The split case mixes exhaustive
