@@ -791,13 +791,16 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
         __syncthreads();
         const int nc = s_ncand;
         if (nc <= kCand) {
-            for (int i = tid; i < kCand; i += kWBlock)
+            // sort size: the next power of two holding the candidates (uniform), padded with -1 below every weight
+            int n2 = 2;
+            while (n2 < nc) n2 <<= 1;
+            for (int i = tid; i < n2; i += kWBlock)
                 if (i >= nc) s_cand[i] = -1.;
             __syncthreads();
             // bitonic sort, ascending
-            for (int ks = 2; ks <= kCand; ks <<= 1)
+            for (int ks = 2; ks <= n2; ks <<= 1)
                 for (int j = ks >> 1; j > 0; j >>= 1) {
-                    for (int i = tid; i < kCand; i += kWBlock) {
+                    for (int i = tid; i < n2; i += kWBlock) {
                         const int l = i ^ j;
                         if (l > i) {
                             const double x = s_cand[i], y = s_cand[l];
@@ -816,7 +819,7 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
                 long long rank = -1;
                 for (int r0 = 0; r0 < nc; r0 += 64) {
                     const int r = r0 + tid;
-                    const double v = (r < nc) ? s_cand[kCand - 1 - r] : 0.;
+                    const double v = (r < nc) ? s_cand[n2 - 1 - r] : 0.;
                     double x = v;
                     for (int o = 1; o < 64; o <<= 1) {
                         const double y = __shfl_up(x, o);
@@ -836,7 +839,7 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
                     if (crossed && a.maxsig > 0 && !a.fine && nsig > a.maxsig) nsig = a.maxsig;
                     s_cand_ok = crossed;
                     s_cand_nsig = nsig;
-                    s_cand_thr = crossed ? s_cand[kCand - nsig] : 0.;
+                    s_cand_thr = crossed ? s_cand[n2 - nsig] : 0.;
                 }
             }
             __syncthreads();
@@ -1215,11 +1218,15 @@ __global__ void __launch_bounds__(kWBlock) k_wsel_final(WeightArgs a, WPart *__r
     WPart &p = part[b];
     if (p.gather == 2) return;
     const int nc = min(p.ncand, kGather);
-    for (int i = tid; i < kGather; i += kWBlock) s_c[i] = (i < nc) ? cand[(size_t)b * kGather + i] : -1.;
+    // sort size: the next power of two holding the candidates (uniform over the block), padded with -1 below
+    // every weight
+    int n2 = 2;
+    while (n2 < nc) n2 <<= 1;
+    for (int i = tid; i < n2; i += kWBlock) s_c[i] = (i < nc) ? cand[(size_t)b * kGather + i] : -1.;
     __syncthreads();
-    for (int ks = 2; ks <= kGather; ks <<= 1)  // bitonic sort, ascending
+    for (int ks = 2; ks <= n2; ks <<= 1)  // bitonic sort, ascending
         for (int j = ks >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < kGather; i += kWBlock) {
+            for (int i = tid; i < n2; i += kWBlock) {
                 const int l = i ^ j;
                 if (l > i) {
                     const double x = s_c[i], y = s_c[l];
@@ -1238,7 +1245,7 @@ __global__ void __launch_bounds__(kWBlock) k_wsel_final(WeightArgs a, WPart *__r
         long long rank = nc - 1;
         for (int r0 = 0; r0 < nc; r0 += 64) {
             const int r = r0 + tid;
-            double x = (r < nc) ? s_c[kGather - 1 - r] : 0.;
+            double x = (r < nc) ? s_c[n2 - 1 - r] : 0.;
             for (int o = 1; o < 64; o <<= 1) {
                 const double y = __shfl_up(x, o);
                 if (tid >= o) x += y;
@@ -1264,7 +1271,7 @@ __global__ void __launch_bounds__(kWBlock) k_wsel_final(WeightArgs a, WPart *__r
     if (p.ncand <= kGather && nc > 0) {
         // rank within the bin; without a crossing (rounding) the smallest element, as the reference's loop ends
         const long long rank = s_ok ? s_rank : (long long)nc - 1;
-        thr = s_c[kGather - 1 - rank];
+        thr = s_c[n2 - 1 - rank];
         nr_sign = p.above_cnt + rank + 1;
     } else {
         // all 64 bits fixed and more than kGather equal elements: the bin is one value
