@@ -51,7 +51,7 @@ def parse():
                     help="particles of the exhaustive plain-FP32 baseline (and of the miss-rate / FSC check)")
     ap.add_argument("--box", type=int, default=256)
     ap.add_argument("--train-samples", type=int, default=200000)
-    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--train-epochs", type=int, default=5)
     ap.add_argument("--topk", type=int, default=16)
     ap.add_argument("--min-top1", type=float, default=0.2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
