@@ -2,7 +2,7 @@
 oracle/estep_oracle.cpp by oracle/Makefile) and of the reference's own HEALPix (oracle/_ref, built from
 /root/reference/src/Healpix_2.15a where the reference is present).
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, as the checker.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, to check the product's results.
 The product (relion_amd) never imports, links or runs anything under oracle/.
 """
 from __future__ import annotations

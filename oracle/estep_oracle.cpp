@@ -1,7 +1,7 @@
 /*
  * TEST INFRASTRUCTURE ONLY -- the parity oracle. Never linked into, loaded by or called from the
  * product library (relion_amd/); only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg use it, as the checker.
+ * leg use it, to check the product's results.
  *
  * An FP64 restatement, written from the reference's behaviour, of RELION's CPU expectation step:
  *   MlOptimiser::getAllSquaredDifferences            src/ml_optimiser.cpp:7098-7701

@@ -1,6 +1,6 @@
 """CPU restatement (FP64 numpy) of the image part of MlOptimiser::getFourierTransformsAndCtfs.
 
-Test infrastructure only: tests, the smoke check and the CPU leg of the benchmark use it as the checker of
+Test infrastructure only: tests, the smoke check and the CPU leg of the benchmark use it to check
 ``rln_imgprep_run``; nothing in the product imports it.
 
 Follows src/ml_optimiser.cpp:5840-6531 for 2-D particles with one image each:

@@ -1,7 +1,7 @@
 """CPU restatement (FP64 numpy) of BackProjector::reconstruct for a 3-D reference, 2-D data, trilinear
 interpolation, the default (windowing) path.
 
-Test infrastructure only: tests and the CPU leg of the benchmark use it as the checker of
+Test infrastructure only: tests and the CPU leg of the benchmark use it to check
 ``rln_backprojector_reconstruct``; nothing in the product imports it.
 
 Follows src/backprojector.cpp:1379-1780: decenter of weight and data (Projector::decenter, src/projector.h:249-259);

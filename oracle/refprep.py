@@ -1,6 +1,6 @@
 """CPU restatement (FP64 numpy) of Projector::computeFourierTransformMap for a 3-D reference and 2-D data.
 
-Test infrastructure only: tests and the CPU leg of the benchmark use it as the checker of
+Test infrastructure only: tests and the CPU leg of the benchmark use it to check
 ``rln_projector_compute_fourier_map``; nothing in the product imports it.
 
 Follows src/projector.cpp:116-593 (the CPU branch) and Projector::griddingCorrect (:595-628), trilinear
