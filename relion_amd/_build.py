@@ -12,7 +12,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librln_estep.so")
-CALIB = os.path.join(HERE, "gather_calibration")
 ARCH = os.environ.get("RLN_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -48,8 +47,6 @@ def build(verbose=True):
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lgomp",
           f"-L{ROCM_LIB}", "-lhipfft", f"-Wl,-rpath,{ROCM_LIB}"])
     os.replace(tmp, LIB)
-    # reference kernels of known traffic for calibrating HBM counters (a separate executable)
-    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "gather_calibration.hip"), "-o", CALIB])
     return LIB
 
 
