@@ -47,6 +47,7 @@ struct rln_proposal {
     // activations for a batch of up to cap images
     int cap = 0;
     rln::DevBuf x[rln::kLayers + 1], col[rln::kLayers], logits, dlog, dtmp, dz_a, dz_b, loss, labels, idx;
+    rln::DevBuf wt;  // one layer's weights rearranged for its data gradient (k_wt_bwd)
     rln::DevBuf src, src_ctf;  // particle images and CTFs staged from the host
     rln::DevBuf top_bins, top_probs;  // predictions of one predict call
     // a prediction launched and not yet finished: its size, the pinned host copies of its top-k, its end event
@@ -63,6 +64,12 @@ struct rln_proposal {
 namespace rln {
 namespace {
 
+// The data gradient of a layer runs as the implicit GEMM k_conv_bwd_data when its channel counts fit it.
+bool bwd_data_direct(const ConvLayer &L) { return L.cin % 16 == 0 && L.cin <= 64 && L.cout % 8 == 0; }
+
+// Bias gradients: out[j] += column sums of Z [M][N] (bf16)
+void col_sum(rln_proposal &p, const rlnp::bf16 *Z, int M, int N, float *out);
+
 int ensure_batch(rln_proposal &p, int n) {
     if (n <= p.cap) return RLN_OK;
     const int cap = std::max(n, 64);
@@ -75,7 +82,12 @@ int ensure_batch(rln_proposal &p, int n) {
         RLN_TRY(p.x[l + 1].ensure(sizeof(rlnp::bf16) * M * C));
     }
     size_t dmax = (size_t)cap * std::max(p.NB, p.F);
-    for (int l = 0; l < kLayers; ++l) dmax = std::max(dmax, (size_t)cap * p.conv[l].Ho * p.conv[l].Ho * p.conv[l].Kp);
+    for (int l = 1; l < kLayers; ++l)  // dcol of the layers whose data gradient takes the col2im path
+        if (!bwd_data_direct(p.conv[l]))
+            dmax = std::max(dmax, (size_t)cap * p.conv[l].Ho * p.conv[l].Ho * p.conv[l].Kp);
+    size_t wtmax = 1;
+    for (int l = 1; l < kLayers; ++l) wtmax = std::max(wtmax, (size_t)9 * p.conv[l].cin * p.conv[l].cout);
+    RLN_TRY(p.wt.ensure(sizeof(rlnp::bf16) * wtmax));
     RLN_TRY(p.logits.ensure(sizeof(float) * (size_t)cap * p.NB));
     RLN_TRY(p.dlog.ensure(sizeof(rlnp::bf16) * (size_t)cap * p.NB));
     RLN_TRY(p.dtmp.ensure(sizeof(float) * dmax));
@@ -184,8 +196,7 @@ int backward(rln_proposal &p, int n) {
     // fully connected layer
     RLN_TRY((gemm<false, false, rlnp::kEpiAtomicF32>(p, dl, 1, p.NB, x6, p.F, 1, p.NB, p.F, n, nullptr,
                                                       gw + p.fcw_off, nullptr, p.F)));
-    hipLaunchKernelGGL(rlnp::k_col_sum, dim3(grid1(p.NB), (n + 255) / 256), dim3(rlnp::kB), 0, p.stream, dl, n, p.NB,
-                       256, gw + p.fcb_off);
+    col_sum(p, dl, n, p.NB, gw + p.fcb_off);
     HIP_TRY(hipGetLastError());
     RLN_TRY((gemm<true, false, rlnp::kEpiF32>(p, dl, p.NB, 1, wbf(p, p.fcw_off), p.F, 1, n, p.F, p.NB, nullptr,
                                                p.dtmp.as<float>(), nullptr, p.F)));
@@ -201,19 +212,46 @@ int backward(rln_proposal &p, int n) {
         RLN_TRY((gemm<false, false, rlnp::kEpiAtomicF32>(p, dz, 1, L.cout, p.col[l].as<const rlnp::bf16>(), L.Kp, 1,
                                                           L.cout, L.Kp, M, nullptr, gw + L.w_off, nullptr, L.Kp,
                                                           kchunk)));
-        hipLaunchKernelGGL(rlnp::k_col_sum, dim3(grid1(L.cout), (M + 1023) / 1024), dim3(rlnp::kB), 0, p.stream, dz, M,
-                           L.cout, 1024, gw + L.b_off);
+        col_sum(p, dz, M, L.cout, gw + L.b_off);
         HIP_TRY(hipGetLastError());
         if (l == 0) break;
-        RLN_TRY((gemm<true, false, rlnp::kEpiF32>(p, dz, L.cout, 1, wbf(p, L.w_off), L.Kp, 1, M, L.Kp, L.cout,
-                                                   nullptr, p.dtmp.as<float>(), nullptr, L.Kp)));
-        const long long nx = (long long)n * L.H * L.H * L.cin;
-        hipLaunchKernelGGL(rlnp::k_col2im_mask, dim3(grid1(nx)), dim3(rlnp::kB), 0, p.stream, p.dtmp.as<const float>(),
-                           n, L.H, L.cin, L.stride, L.Ho, L.Kp, p.x[l].as<const rlnp::bf16>(), dz_next);
-        HIP_TRY(hipGetLastError());
+        if (bwd_data_direct(L)) {
+            const int nwt = 9 * L.cin * L.cout;
+            hipLaunchKernelGGL(rlnp::k_wt_bwd, dim3(grid1(nwt)), dim3(rlnp::kB), 0, p.stream, wbf(p, L.w_off), L.cin,
+                               L.cout, L.Kp, p.wt.as<rlnp::bf16>());
+            const long long rows = (long long)n * L.H * L.H;
+            const dim3 grid((unsigned)((rows + 63) / 64));
+            const rlnp::bf16 *wd = p.wt.as<const rlnp::bf16>(), *y = p.x[l].as<const rlnp::bf16>();
+            switch (L.cin / 16) {
+                case 1: hipLaunchKernelGGL(rlnp::k_conv_bwd_data<1>, grid, dim3(rlnp::kB), 0, p.stream, dz, n, L.H, L.cout, L.stride, L.Ho, wd, y, dz_next); break;
+                case 2: hipLaunchKernelGGL(rlnp::k_conv_bwd_data<2>, grid, dim3(rlnp::kB), 0, p.stream, dz, n, L.H, L.cout, L.stride, L.Ho, wd, y, dz_next); break;
+                case 3: hipLaunchKernelGGL(rlnp::k_conv_bwd_data<3>, grid, dim3(rlnp::kB), 0, p.stream, dz, n, L.H, L.cout, L.stride, L.Ho, wd, y, dz_next); break;
+                default: hipLaunchKernelGGL(rlnp::k_conv_bwd_data<4>, grid, dim3(rlnp::kB), 0, p.stream, dz, n, L.H, L.cout, L.stride, L.Ho, wd, y, dz_next); break;
+            }
+            HIP_TRY(hipGetLastError());
+        } else {
+            RLN_TRY((gemm<true, false, rlnp::kEpiF32>(p, dz, L.cout, 1, wbf(p, L.w_off), L.Kp, 1, M, L.Kp, L.cout,
+                                                       nullptr, p.dtmp.as<float>(), nullptr, L.Kp)));
+            const long long nx = (long long)n * L.H * L.H * L.cin;
+            hipLaunchKernelGGL(rlnp::k_col2im_mask, dim3(grid1(nx)), dim3(rlnp::kB), 0, p.stream,
+                               p.dtmp.as<const float>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp,
+                               p.x[l].as<const rlnp::bf16>(), dz_next);
+            HIP_TRY(hipGetLastError());
+        }
         std::swap(dz, dz_next);
     }
     return RLN_OK;
+}
+
+void col_sum(rln_proposal &p, const rlnp::bf16 *Z, int M, int N, float *out) {
+    if (N % 8 == 0 && N <= 8 * rlnp::kB) {
+        const int rows = std::max(256, (M + 2047) / 2048);
+        hipLaunchKernelGGL(rlnp::k_col_sum8, dim3(1, (unsigned)((M + rows - 1) / rows)), dim3(rlnp::kB), 0, p.stream,
+                           Z, M, N, rows, out);
+    } else {
+        hipLaunchKernelGGL(rlnp::k_col_sum, dim3(grid1(N), (M + 255) / 256), dim3(rlnp::kB), 0, p.stream, Z, M, N,
+                           256, out);
+    }
 }
 
 int adam_step(rln_proposal &p, double lr) {

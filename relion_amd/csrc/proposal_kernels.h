@@ -7,7 +7,8 @@
 //
 // Convolution as a product: col[m][k] (m = (image, oy, ox), k = tap * Cin + c with tap = 3*ky + kx, zero
 // padded to Kp) times W[Cout][Kp]^T gives the NHWC output. Backward: dW = dZ^T col (split over m with FP32
-// atomics), dcol = dZ W, then a gather col2im fused with the previous layer's ReLU mask.
+// atomics); the data gradient is an implicit GEMM over the taps fused with the previous layer's ReLU mask
+// (k_conv_bwd_data), or, for channel counts it does not take, dcol = dZ W and a gather col2im.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -276,6 +277,111 @@ __global__ void __launch_bounds__(kB) k_col_sum(const bf16 *__restrict__ Z, int 
     float s = 0.f;
     for (int i = r0; i < r1; ++i) s += from_bf16(Z[(size_t)i * N + j]);
     atomicAdd(&out[j], s);
+}
+
+// out[j] += sum_i Z[i][j] over rows [blockIdx.y*rows, +rows), for N % 8 == 0 and N <= 8 * kB: N/8 threads per row
+// read 16 bytes each (a row per N/8 lanes, kB/(N/8) rows per pass), FP32 partials per thread, then a block sum per
+// column through LDS and one atomic per column and block. grid (1, ceil(M/rows)).
+__global__ void __launch_bounds__(kB) k_col_sum8(const bf16 *__restrict__ Z, int M, int N, int rows,
+                                                 float *__restrict__ out) {
+    __shared__ float part[8 * kB];
+    const int tpr = N >> 3, rpp = kB / tpr;  // threads per row, rows per pass
+    const int tid = threadIdx.x, rr = tid / tpr, cg = tid - rr * tpr;
+    float s[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] = 0.f;
+    const int r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
+    if (rr < rpp) {
+        for (int i = r0 + rr; i < r1; i += rpp) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(Z + (size_t)i * N + 8 * cg);
+            const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                s[2 * q] += __uint_as_float(w[q] << 16);
+                s[2 * q + 1] += __uint_as_float(w[q] & 0xffff0000u);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) part[rr * N + 8 * cg + q] = s[q];
+    }
+    __syncthreads();
+    for (int j = tid; j < N; j += kB) {
+        float t = 0.f;
+        for (int q = 0; q < rpp; ++q) t += part[q * N + j];
+        atomicAdd(&out[j], t);
+    }
+}
+
+// Weights of a layer rearranged for the data gradient: Wd[c][tap*Cout + co] = W[co][tap*Cin + c].
+__global__ void __launch_bounds__(kB) k_wt_bwd(const bf16 *__restrict__ W, int Cin, int Cout, int Kp,
+                                               bf16 *__restrict__ Wd) {
+    const int idx = blockIdx.x * kB + threadIdx.x;
+    const int Kd = 9 * Cout;
+    if (idx >= Cin * Kd) return;
+    const int c = idx / Kd, k = idx - c * Kd, tap = k / Cout, co = k - tap * Cout;
+    Wd[idx] = W[(size_t)co * Kp + tap * Cin + c];
+}
+
+// Data gradient of a 3x3 convolution (pad 1), fused with the ReLU mask of the layer below:
+// dX[img][y][x][c] = (Y > 0) * sum over taps (ky, kx) and co of dZ[img][oy][ox][co] W[co][tap*Cin + c], with
+// oy = (y + 1 - ky) / stride where that divides and lies inside (likewise ox), as bf16. An implicit GEMM: rows
+// (img, y, x), k = tap*Cout + co, columns c; each lane's A fragment (8 channels of one tap) is read straight from
+// the NHWC dZ, the B fragments from Wd (k_wt_bwd). Block = 4 waves x 16 rows, NT column tiles (Cin = 16 NT),
+// Cout % 8 == 0.
+template <int NT>
+__global__ void __launch_bounds__(kB) k_conv_bwd_data(const bf16 *__restrict__ dZ, int n, int H, int Cout,
+                                                      int stride, int Ho, const bf16 *__restrict__ Wd,
+                                                      const bf16 *__restrict__ Y, bf16 *__restrict__ dX) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, gq = lane >> 4;
+    const long long M = (long long)n * H * H;
+    const long long m = ((long long)blockIdx.x * 4 + wave) * 16 + r16;  // this lane's A row: input pixel
+    int img = 0, y = 0, x = 0;
+    const bool mok = m < M;
+    if (mok) {
+        img = (int)(m / ((long long)H * H));
+        const int pix = (int)(m - (long long)img * H * H);
+        y = pix / H;
+        x = pix - y * H;
+    }
+    const int Kd = 9 * Cout;
+    floatx4_t acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < Kd; k0 += 32) {
+        const int k = k0 + 8 * gq;
+        uint4 av = make_uint4(0u, 0u, 0u, 0u);
+        if (mok && k < Kd) {
+            const int tap = k / Cout, co = k - tap * Cout;
+            const int ty = y + 1 - tap / 3, tx = x + 1 - tap % 3;
+            if (ty >= 0 && tx >= 0 && ty % stride == 0 && tx % stride == 0) {
+                const int oy = ty / stride, ox = tx / stride;
+                if (oy < Ho && ox < Ho)
+                    av = *reinterpret_cast<const uint4 *>(dZ + (((size_t)img * Ho + oy) * Ho + ox) * Cout + co);
+            }
+        }
+        const bf16x8_t a = __builtin_bit_cast(bf16x8_t, av);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int col = 16 * j + r16;  // B column: input channel
+            uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+            if (k < Kd) bv = *reinterpret_cast<const uint4 *>(Wd + (size_t)col * Kd + k);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8_t, bv), acc[j], 0, 0, 0);
+        }
+    }
+    const int Cin = 16 * NT;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + r16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long long mm = ((long long)blockIdx.x * 4 + wave) * 16 + 4 * gq + r;
+            if (mm < M) {
+                const size_t idx = (size_t)mm * Cin + col;
+                dX[idx] = to_bf16(from_bf16(Y[idx]) > 0.f ? acc[j][r] : 0.f);
+            }
+        }
+    }
 }
 
 // --------------------------------------------------------------------------------------------------
