@@ -231,6 +231,19 @@ int rln_estep_set_pool(rln_estep *e, int max_particles);
  * is planned on the host per particle (src/acc/acc_ml_optimiser_impl.h:1420-1890). */
 int rln_estep_use_graphs(rln_estep *e, int enable);
 
+/* Work-unit shape of the fine diff2 kernel. The kernel splits every work item's sampled pixels into n_bands radial
+ * shells of equal pixel count and runs all items' band 0, then band 1, ..., so that the gathers in flight touch one
+ * shell of the reference at a time. By default (n_bands = 0) the engine times the band counts 1, 2, 4, 8, 16, 32
+ * and the persistent grid sizes on its first chunk with enough work and keeps the fastest; n_bands in that set fixes
+ * the band count (grid: the resident-block count) and turns the tuning off. Results do not depend on it, up to the
+ * order of the FP64 sums of the bands. No counterpart in the reference, whose launch geometry is fixed
+ * (src/acc/cuda/cuda_settings.h:86-141). RLN_ERR_ARG for other values. */
+int rln_estep_set_fine_bands(rln_estep *e, int n_bands);
+/* The fine pass's tuning: the time in ms of every candidate tried ([shape][band choice][grid], 4 x 6 x 5 in that
+ * order; shapes (TC, CW) = (2, 2), (1, 2), (2, 1), (1, 1); band counts 1 .. 32; grids 512, 1024, 2048, 4096 and
+ * the resident-block count), -1 for those not tried; *n_out = 0 before any tuning. Copies min(n, capacity). */
+int rln_estep_fine_tuning(rln_estep *e, double *ms, int capacity, int *n_out);
+
 /* Explicit coarse orientation lists for the next rln_estep_run (global searches only), as produced by the
  * learned pose proposal: particle i is searched over orientations set_orients[set_offsets[i] ..
  * set_offsets[i+1]), each a global grid index idir * N_psi + ipsi of the coarse HEALPix grid, for all classes
@@ -244,12 +257,13 @@ int rln_estep_set_orientation_sets(rln_estep *e, int n, const int *set_offsets, 
 /* Stage hooks for parity tests. After rln_estep_run with keep_intermediates set, these copy the
  * last run's per-particle intermediates to host (particle index ip within that run):
  *   coarse: diff2 per coarse pose in ihidden order (K * N_dir_b * N_psi_b * N_tr), and weights;
- *   fine:   n_fine (ihidden_over, diff2, weight) triplets of the evaluated fine poses. */
+ *   fine:   n_fine (ihidden_over, diff2, weight) triplets of the evaluated fine poses (diff2 in FP64: the
+ *           pose-independent part of each particle's diff2 is summed in FP64, the pose-dependent part in FP32). */
 int rln_estep_keep_intermediates(rln_estep *e, int keep);
 int rln_estep_coarse_size(rln_estep *e, int ip, long long *n_coarse, int *n_dir, int *n_psi, int *n_trans);
 int rln_estep_get_coarse(rln_estep *e, int ip, float *diff2, double *weight, unsigned char *significant);
 int rln_estep_fine_size(rln_estep *e, int ip, long long *n_fine);
-int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *diff2, double *weight);
+int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, double *diff2, double *weight);
 
 /* Device time of the last run per stage, in milliseconds, from HIP events on the engine's stream:
  * prep, coarse_diff2, coarse_weights, jobs (fine-pass plan and work items), fine_diff2 (the fine diff2 kernel),
@@ -262,8 +276,9 @@ int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
  * pairs summed in the weighted sums, coarse grid projections computed, coarse tiles, fine jobs, particle chunks
  * (one launch of each per-chunk kernel per chunk), launches of the captured fine-pass graph, fine-pass
  * capacity retries, the persistent grid sizes of the fine diff2 kernel (tuned) and of the weighted sums (resident
- * blocks), and the fine-image pixels that are sampled per fine orientation (non-zero Minvsigma2 inside the
- * projection radius). */
+ * blocks), the fine-image pixels that are sampled per fine orientation (non-zero Minvsigma2 inside the
+ * projection radius), the number of radial pixel bands the fine pass splits each work item into, and its work-item
+ * shape: coarse translations per small item and per wave of a wide item (all three tuned). */
 int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out);
 
 /* ---------------------------------------------------------------------------------------------

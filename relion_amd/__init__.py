@@ -155,10 +155,12 @@ _SIGNATURES = {
     "rln_estep_set_orientation_sets": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "rln_estep_use_graphs": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_pool": (C.c_int, [_vp, C.c_int]),
+    "rln_estep_set_fine_bands": (C.c_int, [_vp, C.c_int]),
+    "rln_estep_fine_tuning": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(C.c_int)]),
     "rln_estep_coarse_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _ip, _ip, _ip]),
     "rln_estep_get_coarse": (C.c_int, [_vp, C.c_int, _fp, _dp, C.POINTER(C.c_ubyte)]),
     "rln_estep_fine_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong)]),
-    "rln_estep_get_fine": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _fp, _dp]),
+    "rln_estep_get_fine": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _dp, _dp]),
     "rln_estep_last_timings": (C.c_int, [_vp, _dp, C.c_int, _ip]),
     "rln_estep_last_counters": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.c_int, _ip]),
     "rln_coarse_image_size": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int]),
@@ -550,6 +552,21 @@ class ExpectationStep:
         """Particles per chunk at most (the reference's --pool); results do not depend on it."""
         _check(lib.rln_estep_set_pool(self._h, int(max_particles)))
 
+    def set_fine_bands(self, n_bands):
+        """Radial pixel bands per fine work item (1, 2, 4, 8, 16, 32), or 0 for the engine's own tuning."""
+        _check(lib.rln_estep_set_fine_bands(self._h, int(n_bands)))
+
+    def fine_tuning(self):
+        """Times (ms) of the fine pass's tuning candidates as an array [shape][band choice][grid] (-1: not tried),
+        or None before any tuning."""
+        n = C.c_int()
+        _check(lib.rln_estep_fine_tuning(self._h, None, 0, C.byref(n)))
+        if n.value == 0:
+            return None
+        ms = np.zeros(n.value)
+        _check(lib.rln_estep_fine_tuning(self._h, _ptr(ms, C.c_double), n.value, C.byref(n)))
+        return ms.reshape(4, 6, 5)
+
     def set_orientation_lists(self, offsets, orients):
         """set_orientation_sets in the flat form: offsets[n+1], orients[offsets[n]] (int32)."""
         off = np.ascontiguousarray(offsets, np.int32)
@@ -617,9 +634,9 @@ class ExpectationStep:
         n = C.c_longlong()
         _check(lib.rln_estep_fine_size(self._h, ip, C.byref(n)))
         idx = np.zeros(n.value, np.int64)
-        d2 = np.zeros(n.value, np.float32)
+        d2 = np.zeros(n.value)
         w = np.zeros(n.value)
-        _check(lib.rln_estep_get_fine(self._h, ip, _ptr(idx, C.c_longlong), _ptr(d2, C.c_float), _ptr(w, C.c_double)))
+        _check(lib.rln_estep_get_fine(self._h, ip, _ptr(idx, C.c_longlong), _ptr(d2, C.c_double), _ptr(w, C.c_double)))
         return idx, d2, w
 
     def timings(self):
@@ -631,7 +648,7 @@ class ExpectationStep:
     COUNTERS = ("particles", "coarse_poses", "coarse_pixels", "fine_poses", "fine_pixels", "fine_items",
                 "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks",
                 "fine_graph_launches", "fine_capacity_retries", "fine_grid", "wsum_grid",
-                "fine_pixels_sampled")
+                "fine_pixels_sampled", "fine_bands", "fine_small_tc", "fine_wide_cw")
 
     def counters(self):
         out = np.zeros(len(self.COUNTERS), np.int64)

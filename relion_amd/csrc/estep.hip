@@ -103,6 +103,21 @@ struct UninitAlloc : std::allocator<T> {
 template <class T>
 using HostList = std::vector<T, UninitAlloc<T>>;
 
+// The production fine diff2 kernel of a work-item shape (index into rlnk::kFineShapes).
+inline const void *fine_kernel_ptr(int shape) {
+    switch (shape) {
+        case 1: return reinterpret_cast<const void *>(&rlnk::k_fine_diff2<0, 1, 2>);
+        case 2: return reinterpret_cast<const void *>(&rlnk::k_fine_diff2<0, 2, 1>);
+        case 3: return reinterpret_cast<const void *>(&rlnk::k_fine_diff2<0, 1, 1>);
+        default: return reinterpret_cast<const void *>(&rlnk::k_fine_diff2<0, 2, 2>);
+    }
+}
+
+// Band counts the fine pass can split its sampled pixels into (radial shells of equal pixel count; the tuner
+// picks one per engine).
+constexpr int kBandChoices[] = {1, 2, 4, 8, 16, 32};
+constexpr int kNumBandChoices = (int)(sizeof(kBandChoices) / sizeof(kBandChoices[0]));
+
 // Stage timers of one run (HIP events on the engine's stream).
 enum Stage { kStPrep, kStCoarse, kStCoarseW, kStJobs, kStFine, kStFineW, kStWsum, kStCount };
 // fine stage graphs: plan and work items; fine diff2; fine weights; weighted sums and backprojection
@@ -155,6 +170,7 @@ struct Engine {
     DevBuf d_npairs, d_njobs, d_nchunks, d_pair_off, d_job_off, d_chunk_off, d_pairs, d_jobs, d_chunk_job,
         d_chunk_first;
     DevBuf d_fdiff2, d_fweight, d_fine_n, d_nent, d_foff;
+    DevBuf d_fbase;  // pose-independent part of each particle's fine diff2 (FP64)
     DevBuf d_shell_acc, d_part_acc, d_pdfc_acc, d_pdfd_acc;
     // fine pass planned on the device: capacities (elements) the buffers were sized for, the chunk capacity B,
     // the plan header, per-chunk image pointers, work items, the tuned persistent grids and the captured graph
@@ -168,7 +184,13 @@ struct Engine {
     rlnk::ChunkPtrs cp_host{};
     bool tuned = false;
     int grid_fine = 1024, grid_wsum = 2048;  // persistent grids (fine: tuned; weighted sums: resident blocks)
-    int resident_fine = 0;                   // resident blocks of k_fine_diff2 on the device
+    int band_choice = 0;                     // index into kBandChoices of the fine pass's band count (tuned)
+    int fine_shape = 0;                      // index into rlnk::kFineShapes of the work-item shape (tuned)
+    int resident_shape[rlnk::kNumFineShapes] = {};  // resident blocks of each shape's kernel on the device
+    DevBuf d_band_pix;                       // sampled fine pixels, one banded list per band choice
+    std::vector<int> band_src;               // the sampled pixels the lists were built from (pixel order)
+    int resident_fine = 0;                   // resident blocks of the default shape's k_fine_diff2
+    int n_cu = 0;                            // compute units of the device
     hipGraph_t fine_graph[kFineGraphs] = {};
     hipGraphExec_t fine_exec[kFineGraphs] = {};
     std::vector<unsigned char> fine_key;  // kernel arguments the graph was captured with
@@ -183,7 +205,7 @@ struct Engine {
         std::vector<double> cweight;
         std::vector<unsigned char> cmask;
         std::vector<long long> f_index;
-        std::vector<float> fdiff2;
+        std::vector<double> fdiff2;
         std::vector<double> fweight;
     };
     std::vector<Kept> kept;
@@ -191,8 +213,9 @@ struct Engine {
     hipEvent_t ev[kStCount + 1] = {};
     double stage_ms[kStCount] = {};
     double host_ms[4] = {};  // host phases: chunk preparation, of it the first chunk's lists, epilogue, whole run
-    long long counters[17] = {};
+    long long counters[20] = {};
     long long pf_sampled = 0;  // fine pixels gathered per fine orientation
+    std::vector<double> tune_ms;  // fine-pass tuning: time of each (band choice, grid) candidate, -1 untried
     bool have_timing = false;
     std::mutex mtx;
 };
@@ -309,9 +332,15 @@ static int engine_init(Engine &e, const rln_estep_config &cfg) {
     {
         int cus = 0, nb_f = 0, nb_w = 0;
         HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e.device));
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_f, rlnk::k_fine_diff2<0>, rlnk::kBlock, 0));
+        for (int sh = 0; sh < rlnk::kNumFineShapes; ++sh) {
+            int nb = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fine_kernel_ptr(sh), rlnk::kBlock, 0));
+            e.resident_shape[sh] = nb;
+        }
+        nb_f = e.resident_shape[0];
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_w, rlnk::k_wsum_bp, rlnk::kBlock, 0));
         e.resident_fine = cus * nb_f;
+        e.n_cu = cus;
         if (cus * nb_w > 0) e.grid_wsum = cus * nb_w;
     }
     // orientation frames of the coarse grid and of its order-1 oversampling
@@ -452,11 +481,42 @@ static int engine_model_tables(Engine &e) {
     RLN_TRY(upload(e.d_q_c, qc, e.stream));
     RLN_TRY(upload(e.d_q_f, qf, e.stream));
     // fine pixels gathered per fine orientation: non-zero Minvsigma2 inside the projection radius
-    e.pf_sampled = 0;
+    std::vector<int> samp;
     for (int n = 0; n < e.igf.P; ++n) {
         int x, y;
         rlnk::pixel_xy(n, e.igf.s, e.igf.xs, x, y);
-        if (qf[n] != 0.f && x * x + y * y <= e.igf.proj_r2_max) ++e.pf_sampled;
+        if (qf[n] != 0.f && x * x + y * y <= e.igf.proj_r2_max) samp.push_back(n);
+    }
+    e.pf_sampled = (long long)samp.size();
+    if (samp != e.band_src) {
+        // banded lists: rank the sampled pixels by radius (ties in pixel order), cut the ranks into NB equal
+        // parts, and keep each band in pixel order (rows stay contiguous for the image reads)
+        const int ns = (int)samp.size();
+        std::vector<int> byr(samp);
+        std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) {
+            int xa, ya, xb, yb;
+            rlnk::pixel_xy(a, e.igf.s, e.igf.xs, xa, ya);
+            rlnk::pixel_xy(b, e.igf.s, e.igf.xs, xb, yb);
+            return xa * xa + ya * ya < xb * xb + yb * yb;
+        });
+        std::vector<int> band_p(ns);
+        std::vector<int2> lists((size_t)kNumBandChoices * std::max(ns, 1));
+        for (int c = 0; c < kNumBandChoices; ++c) {
+            const int nb = kBandChoices[c];
+            int2 *dst = lists.data() + (size_t)c * ns;
+            for (int band = 0; band < nb; ++band) {
+                const int i0 = band * ns / nb, i1 = (band + 1) * ns / nb;
+                std::copy(byr.begin() + i0, byr.begin() + i1, band_p.begin() + i0);
+                std::sort(band_p.begin() + i0, band_p.begin() + i1);
+            }
+            // (p, x | row << 16): the kernels read the pixel's coordinates instead of dividing by the row length
+            for (int k = 0; k < ns; ++k) {
+                const int p = band_p[k], row = p / e.igf.xs;
+                dst[k] = make_int2(p, (p - row * e.igf.xs) | (row << 16));
+            }
+        }
+        RLN_TRY(upload(e.d_band_pix, lists, e.stream));
+        e.band_src = samp;
     }
     RLN_TRY(upload(e.d_minv_w, mw, e.stream));
     // logsigma2 of the dLL (src/ml_optimiser.cpp:9024-9033)
@@ -773,11 +833,12 @@ static void build_chunk_host(Engine &e, int B, const rln_particle_meta *meta, Ch
         h.meta_f[cB + b] = h.scale[b];
         h.meta_f[2 * cB + b] = h.scale_w[b];
     }
-    h.meta_d.assign(4 * (size_t)cB, 0.);
+    h.meta_d.assign(5 * (size_t)cB, 0.);
     for (int b = 0; b < 2 * B; ++b) {
         h.meta_d[b] = h.old_off[b];
         h.meta_d[2 * cB + b] = h.prior_off[b];
     }
+    for (int b = 0; b < B; ++b) h.meta_d[4 * (size_t)cB + b] = meta[b].highres_Xi2 / 2.;
 }
 
 // Host array `which` (PackedArray) of a chunk's upload: address and bytes.
@@ -849,7 +910,7 @@ static int ensure_fine_capacity(Engine &e) {
     RLN_TRY(e.d_jobs.ensure(sizeof(rlnk::FineJob) * e.need_jobs));
     RLN_TRY(e.d_items_s.ensure(sizeof(int2) * e.need_small));
     RLN_TRY(e.d_items_w.ensure(sizeof(int2) * e.need_wide));
-    RLN_TRY(e.d_fdiff2.ensure(sizeof(float) * 32 * e.need_pairs));
+    RLN_TRY(e.d_fdiff2.ensure(sizeof(double) * 32 * e.need_pairs));
     RLN_TRY(e.d_fweight.ensure(sizeof(double) * 32 * e.need_pairs));
     RLN_TRY(e.d_npairs.ensure(sizeof(int) * 2 * cB));
     RLN_TRY(e.d_pair_off.ensure(sizeof(long long) * cB));
@@ -863,6 +924,16 @@ static int ensure_fine_capacity(Engine &e) {
     RLN_TRY(e.d_plan.ensure(sizeof(rlnk::FinePlan)));
     RLN_TRY(e.d_cp.ensure(sizeof(rlnk::ChunkPtrs)));
     return RLN_OK;
+}
+
+template <int kTuning>
+static void launch_fine_shape(int shape, dim3 grid, hipStream_t st, const rlnk::FineArgs &fa) {
+    switch (shape) {
+        case 1: hipLaunchKernelGGL((rlnk::k_fine_diff2<kTuning, 1, 2>), grid, dim3(rlnk::kBlock), 0, st, fa); break;
+        case 2: hipLaunchKernelGGL((rlnk::k_fine_diff2<kTuning, 2, 1>), grid, dim3(rlnk::kBlock), 0, st, fa); break;
+        case 3: hipLaunchKernelGGL((rlnk::k_fine_diff2<kTuning, 1, 1>), grid, dim3(rlnk::kBlock), 0, st, fa); break;
+        default: hipLaunchKernelGGL((rlnk::k_fine_diff2<kTuning, 2, 2>), grid, dim3(rlnk::kBlock), 0, st, fa); break;
+    }
 }
 
 // All kernel arguments of the fine pass of a chunk; the graph is reused while they are unchanged.
@@ -880,6 +951,8 @@ struct FineLaunch {
     int NT;
     rlnk::FineJob *jobs;
     int2 *items_s, *items_w;
+    const double *fbase;
+    int shape;  // work-item shape (index into rlnk::kFineShapes)
     rlnk::FineArgs fa;
     rlnk::WeightArgs wf;
     rlnk::WsumArgs ws;
@@ -898,8 +971,9 @@ static void launch_plan(Engine &e, const FineLaunch &L) {
                        L.oc_off, L.ocnt, L.NT,
                        (const long long *)L.pair_off, (const int *)L.job_off, L.pairs, L.jobs,
                        (const rlnk::FinePlan *)L.plan);
+    const rlnk::FineShape sh = rlnk::kFineShapes[L.shape];
     hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs,
-                       rlnk::kFineSmallTC, rlnk::kWideTC, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
+                       sh.tc, 4 * sh.cw, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
 }
 
 static int launch_fine(Engine &e, const FineLaunch &L, int part) {
@@ -909,7 +983,9 @@ static int launch_fine(Engine &e, const FineLaunch &L, int part) {
             launch_plan(e, L);
             return launch_check("fine plan");
         case 1:
-            hipLaunchKernelGGL(rlnk::k_fine_diff2<0>, dim3(L.grid_fine), dim3(rlnk::kBlock), 0, st, L.fa);
+            hipLaunchKernelGGL(rlnk::k_fine_init, dim3(L.grid_B), dim3(rlnk::kBlock), 0, st, L.np, L.pair_off, L.fbase,
+                               (const rlnk::FinePlan *)L.plan, L.fa.diff2);
+            launch_fine_shape<0>(L.shape, dim3(L.grid_fine), st, L.fa);
             return launch_check("fine pass");
         case 2:
             hipLaunchKernelGGL(rlnk::k_weights, dim3(L.grid_B), dim3(rlnk::kWBlock), 0, st, L.wf);
@@ -923,39 +999,100 @@ static int launch_fine(Engine &e, const FineLaunch &L, int part) {
     }
 }
 
-// Persistent grid size of the fine diff2 kernel: each candidate (including the resident-block count) timed once
-// on the first chunk, the fastest kept for the engine's life. The kernel only writes fine diff2, so repeating it
-// is harmless; its item counter is reset before every candidate.
+// Work-item shape, work-unit size and persistent grid of the fine diff2 kernel, timed on the first chunk with
+// enough work items, one parameter after the other at the best values found so far: the work-item shape (coarse
+// translations per small item and per wave of a wide item, which set the register tiles and the projections
+// shared per item; the plan's items are rebuilt for each shape), the band count (pixels per unit: all sampled
+// pixels, or the radial shells of 1/2 .. 1/32 of them) and the grid (512 .. 4096 blocks or the shape's
+// resident-block count). The fastest triple is kept for the engine's life. The kernel only adds into fine diff2,
+// which the production pass initialises again, so repeating it is harmless; its unit counter is reset before every
+// candidate. tune_ms keeps every candidate's time ([shape][band][grid], -1 untried) for rln_estep_tuning.
 static int tune_fine(Engine &e, FineLaunch L) {
     hipStream_t st = e.stream;
-    launch_plan(e, L);
-    RLN_TRY(launch_check("fine plan"));
-    rlnk::FinePlan plan{};
-    HIP_TRY(hipMemcpyAsync(&plan, L.plan, sizeof(plan), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (plan.overflow || plan.n_small + plan.n_wide < 512) return RLN_OK;  // retried with more room / too small
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    std::vector<int> cand = {512, 1024, 2048, 4096};
-    if (e.resident_fine > 0) cand.push_back(e.resident_fine);
-    float best = 1e30f;
-    for (int g : cand) {
-        float ms = 0.f;
+    const int n_grid = 5;
+    auto grid_of = [&](int sh, int gi) {
+        static const int g[4] = {512, 1024, 2048, 4096};
+        const int res = e.resident_shape[sh] > 0 ? e.resident_shape[sh] * e.n_cu : 1024;
+        return gi < 4 ? g[gi] : res;
+    };
+    e.tune_ms.assign((size_t)rlnk::kNumFineShapes * kNumBandChoices * n_grid, -1.);
+    // plan of a shape; false when it does not fit the capacities (the candidate is skipped) or is too small to tune
+    auto plan_for = [&](int sh, bool &ok) -> int {
+        FineLaunch P = L;
+        P.shape = sh;
+        launch_plan(e, P);
+        RLN_TRY(launch_check("fine plan"));
+        rlnk::FinePlan plan{};
+        HIP_TRY(hipMemcpyAsync(&plan, L.plan, sizeof(plan), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        ok = !plan.overflow && plan.n_small + plan.n_wide >= 512;
+        return RLN_OK;
+    };
+    auto time_one = [&](int sh, int ci, int gi, float &ms) -> int {
+        rlnk::FineArgs fa = L.fa;
+        fa.n_bands = kBandChoices[ci];
+        fa.pix = e.d_band_pix.as<const int2>() + (size_t)ci * e.pf_sampled;
         hipLaunchKernelGGL(rlnk::k_reset_items, dim3(1), dim3(64), 0, st, L.plan);
         HIP_TRY(hipEventRecord(a, st));
-        hipLaunchKernelGGL(rlnk::k_fine_diff2<1>, dim3(g), dim3(rlnk::kBlock), 0, st, L.fa);
+        launch_fine_shape<1>(sh, dim3(grid_of(sh, gi)), st, fa);
         HIP_TRY(hipEventRecord(b, st));
         HIP_TRY(hipEventSynchronize(b));
         HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        e.tune_ms[((size_t)sh * kNumBandChoices + ci) * n_grid + gi] = ms;
+        return RLN_OK;
+    };
+    bool ok = false;
+    RLN_TRY(plan_for(0, ok));
+    if (!ok) {  // retried with more room, or too small to be worth tuning
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return RLN_OK;
+    }
+    // the first band count with at least 8 bands of >= 256 pixels, or the largest below
+    int best_c = 0;
+    for (int ci = 0; ci < kNumBandChoices; ++ci)
+        if (e.pf_sampled / kBandChoices[ci] >= 256 && kBandChoices[ci] <= 8) best_c = ci;
+    int best_s = 0, best_g = n_grid - 1;
+    float best = 1e30f;
+    for (int sh = 0; sh < rlnk::kNumFineShapes; ++sh) {
+        if (sh > 0) {
+            RLN_TRY(plan_for(sh, ok));
+            if (!ok) continue;
+        }
+        float ms = 0.f;
+        RLN_TRY(time_one(sh, best_c, best_g, ms));
         if (ms < best) {
             best = ms;
-            e.grid_fine = g;
+            best_s = sh;
+        }
+    }
+    RLN_TRY(plan_for(best_s, ok));
+    for (int ci = 0; ci < kNumBandChoices; ++ci) {
+        if (ci == best_c || (kBandChoices[ci] > 1 && e.pf_sampled / kBandChoices[ci] < 256)) continue;
+        float ms = 0.f;
+        RLN_TRY(time_one(best_s, ci, best_g, ms));
+        if (ms < best) {
+            best = ms;
+            best_c = ci;
+        }
+    }
+    for (int gi = 0; gi + 1 < n_grid; ++gi) {
+        float ms = 0.f;
+        RLN_TRY(time_one(best_s, best_c, gi, ms));
+        if (ms < best) {
+            best = ms;
+            best_g = gi;
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     RLN_TRY(launch_check("fine tuning"));
+    e.fine_shape = best_s;
+    e.band_choice = best_c;
+    e.grid_fine = grid_of(best_s, best_g);
     e.tuned = true;
     return RLN_OK;
 }
@@ -974,6 +1111,9 @@ static int run_fine_stage(Engine &e, const FineLaunch &L0) {
     if (!e.tuned) {
         RLN_TRY(tune_fine(e, L));
         L.grid_fine = e.grid_fine;
+        L.shape = e.fine_shape;
+        L.fa.n_bands = kBandChoices[e.band_choice];
+        L.fa.pix = e.d_band_pix.as<const int2>() + (size_t)e.band_choice * e.pf_sampled;
     }
     if (!e.graph_off) {
         const unsigned char *kb = reinterpret_cast<const unsigned char *>(&L);
@@ -1102,6 +1242,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     const float *d_scale_w = d_xi2 + 2 * cB;
     const double *d_old = e.d_meta_d.as<const double>();
     const double *d_prior = d_old + 2 * cB;
+    const double *d_xi2d = d_old + 4 * cB;  // highres_Xi2 / 2 in FP64
 
     // coarse: scaled images, shifted images
     const bool split = cfg.precision == RLN_PREC_SPLIT_BF16;
@@ -1212,6 +1353,10 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.cp_host.fctf = dC;
     e.cp_host.B = B;
     HIP_TRY(hipMemcpyAsync(e.d_cp.ptr, &e.cp_host, sizeof(rlnk::ChunkPtrs), hipMemcpyHostToDevice, st));
+    RLN_TRY(e.d_fbase.ensure(sizeof(double) * cB));
+    hipLaunchKernelGGL(rlnk::k_fine_base, dim3(B), dim3(rlnk::kBlock), 0, st, e.d_cp.as<const rlnk::ChunkPtrs>(),
+                       e.d_q_f.as<const float>(), (int)Pf, d_xi2d, e.d_fbase.as<double>());
+    RLN_TRY(launch_check("k_fine_base"));
     rlnk::FinePlan plan{};
     for (int attempt = 0;; ++attempt) {
         RLN_TRY(ensure_fine_capacity(e));
@@ -1261,13 +1406,18 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         fa.ey = e.d_ey_f.as<const float2>();
         fa.tr_ix = e.d_tr_ix.as<const int>();
         fa.tr_iy = e.d_tr_iy.as<const int>();
-        fa.xi2half = d_xi2;
+        fa.pix = e.d_band_pix.as<const int2>() + (size_t)e.band_choice * e.pf_sampled;
+        fa.n_pix = (int)e.pf_sampled;
+        fa.n_bands = kBandChoices[e.band_choice];
+        L.fbase = e.d_fbase.as<const double>();
+        L.shape = e.fine_shape;
         fa.NT = NT;
         fa.do_ctf = cfg.do_ctf_correction;
-        fa.diff2 = e.d_fdiff2.as<float>();
+        fa.diff2 = e.d_fdiff2.as<double>();
         rlnk::WeightArgs &wf = L.wf;
         wf = wa;
-        wf.diff2 = e.d_fdiff2.as<const float>();
+        wf.diff2 = nullptr;
+        wf.diff2d = e.d_fdiff2.as<const double>();
         wf.d2_off = e.d_foff.as<const long long>();
         wf.n_entries = e.d_fine_n.as<const long long>();
         wf.pairs = e.d_pairs.as<const int>();
@@ -1390,7 +1540,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
             if (counts[b]) {
                 HIP_TRY(hipMemcpyAsync(pr.data(), e.d_pairs.as<int>() + pair_off[b], sizeof(int) * counts[b],
                                        hipMemcpyDeviceToHost, st));
-                HIP_TRY(hipMemcpyAsync(kp.fdiff2.data(), e.d_fdiff2.as<float>() + foff[b], sizeof(float) * nf,
+                HIP_TRY(hipMemcpyAsync(kp.fdiff2.data(), e.d_fdiff2.as<double>() + foff[b], sizeof(double) * nf,
                                        hipMemcpyDeviceToHost, st));
                 HIP_TRY(hipMemcpyAsync(kp.fweight.data(), e.d_fweight.as<double>() + foff[b], sizeof(double) * nf,
                                        hipMemcpyDeviceToHost, st));
@@ -1423,6 +1573,9 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[14] = e.grid_fine;
     e.counters[15] = e.grid_wsum;
     e.counters[16] = e.pf_sampled;
+    e.counters[17] = kBandChoices[e.band_choice];
+    e.counters[18] = rlnk::kFineShapes[e.fine_shape].tc;
+    e.counters[19] = rlnk::kFineShapes[e.fine_shape].cw;
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
         if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
@@ -1631,7 +1784,7 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
                            &e->d_or_class, &e->d_or_dir, &e->d_tr_prior, &e->d_ctfq, &e->d_yq, &e->d_tiles,
                            &e->d_diff2, &e->d_weight, &e->d_mask, &e->d_stats_c, &e->d_stats_f, &e->d_npairs,
                            &e->d_njobs, &e->d_nchunks, &e->d_pair_off, &e->d_job_off, &e->d_chunk_off, &e->d_pairs,
-                           &e->d_jobs, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fweight,
+                           &e->d_jobs, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fbase, &e->d_fweight,
                            &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc,
                            &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w, &e->d_wsegs, &e->d_seg_first,
                            &e->d_wpart, &e->d_hcnt, &e->d_hsum, &e->d_wcand, &e->d_segbest, &e->d_ocnt};
@@ -1716,6 +1869,33 @@ extern "C" int rln_estep_use_graphs(rln_estep *e, int enable) {
         e->fine_key.clear();
     }
     return RLN_OK;
+}
+
+extern "C" int rln_estep_fine_tuning(rln_estep *e, double *ms, int capacity, int *n_out) {
+    if (!e || !n_out) RLN_FAIL(RLN_ERR_ARG, "rln_estep_fine_tuning: null argument");
+    std::lock_guard<std::mutex> lk(e->mtx);
+    const int n = (int)e->tune_ms.size();
+    *n_out = n;
+    if (ms)
+        for (int i = 0; i < std::min(n, capacity); ++i) ms[i] = e->tune_ms[i];
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_set_fine_bands(rln_estep *e, int n_bands) {
+    if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_bands: null engine");
+    std::lock_guard<std::mutex> lk(e->mtx);
+    if (n_bands == 0) {
+        e->tuned = false;
+        return RLN_OK;
+    }
+    for (int c = 0; c < rln::kNumBandChoices; ++c)
+        if (rln::kBandChoices[c] == n_bands) {
+            e->band_choice = c;
+            e->grid_fine = e->resident_fine > 0 ? e->resident_fine : 1024;
+            e->tuned = true;
+            return RLN_OK;
+        }
+    RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_bands: %d bands (1, 2, 4, 8, 16, 32 or 0)", n_bands);
 }
 
 extern "C" int rln_estep_set_orientation_sets(rln_estep *e, int n, const int *set_offsets, const int *set_orients) {
@@ -1893,7 +2073,7 @@ extern "C" int rln_estep_fine_size(rln_estep *e, int ip, long long *n_fine) {
     return RLN_OK;
 }
 
-extern "C" int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, float *diff2, double *weight) {
+extern "C" int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over, double *diff2, double *weight) {
     if (!e || ip < 0 || ip >= (int)e->kept.size()) RLN_FAIL(RLN_ERR_STATE, "no kept intermediates for particle %d", ip);
     const rln::Engine::Kept &k = e->kept[ip];
     if (ihidden_over) std::copy(k.f_index.begin(), k.f_index.end(), ihidden_over);
@@ -1904,7 +2084,7 @@ extern "C" int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over,
 
 extern "C" int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out) {
     if (!e || !out) RLN_FAIL(RLN_ERR_ARG, "null argument");
-    const int n = std::min(n_max, 17);
+    const int n = std::min(n_max, 20);
     for (int i = 0; i < n; ++i) out[i] = e->counters[i];
     if (n_out) *n_out = n;
     return RLN_OK;

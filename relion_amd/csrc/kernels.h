@@ -619,6 +619,23 @@ __device__ __forceinline__ float block_min_f(float v, float *red) {
     return r;
 }
 
+__device__ __forceinline__ double block_min_d(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double t = (threadIdx.x < (blockDim.x >> 6)) ? red[threadIdx.x] : 1.0e300;
+        for (int o = 32; o > 0; o >>= 1) t = fmin(t, __shfl_xor(t, o));
+        if (threadIdx.x == 0) red[0] = t;
+    }
+    __syncthreads();
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
 __device__ __forceinline__ long long block_sum_ll(long long v, long long *red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -664,7 +681,8 @@ struct ChunkPtrs {
 // Mode of k_weights: coarse entries are e = o*NT + t; fine entries are e = s*32 + r*4 + u where pair s
 // holds the coarse index (o*NT + t) in pairs[].
 struct WeightArgs {
-    const float *diff2;          // per particle at d2_off[b]
+    const float *diff2;          // coarse mode: per particle at d2_off[b]; entries < 0 were not evaluated
+    const double *diff2d;        // fine mode: per particle at d2_off[b], every entry evaluated
     const long long *d2_off;
     const long long *n_entries;  // per particle
     const double *or_prior;      // normalised pdf_orientation per orientation entry (or_off based)
@@ -727,23 +745,31 @@ __global__ void __launch_bounds__(kWBlock) k_weights(WeightArgs a) {
     const int b = blockIdx.x;
     if (a.plan && (a.plan->overflow || b >= a.plan->B)) return;
     const long long n = a.n_entries[b];
-    const float *d2 = a.diff2 + a.d2_off[b];
     double *w = a.weight + a.d2_off[b];
     const int tid = threadIdx.x;
 
-    float mn = 3.0e38f;
-    for (long long e = tid; e < n; e += kWBlock) {
-        const float v = d2[e];
-        if (v >= 0.f) mn = fminf(mn, v);
+    // fine entries are FP64 and all evaluated; coarse entries FP32, < 0 where not evaluated
+    const float *d2 = a.fine ? nullptr : a.diff2 + a.d2_off[b];
+    const double *d2d = a.fine ? a.diff2d + a.d2_off[b] : nullptr;
+    double mind;
+    if (a.fine) {
+        double mnd = 1.0e300;
+        for (long long e = tid; e < n; e += kWBlock) mnd = fmin(mnd, d2d[e]);
+        mind = block_min_d(mnd, red_d);
+    } else {
+        float mn = 3.0e38f;
+        for (long long e = tid; e < n; e += kWBlock) {
+            const float v = d2[e];
+            if (v >= 0.f) mn = fminf(mn, v);
+        }
+        mind = (double)block_min_f(mn, red_f);
     }
-    mn = block_min_f(mn, red_f);
-    const double mind = (double)mn;
     double sum = 0.;
     for (long long e = tid; e < n; e += kWBlock) {
-        const float v = d2[e];
+        const double v = a.fine ? d2d[e] : (double)d2[e];
         double wt = 0.;
-        if (v >= 0.f) {
-            const double diff = (double)v - mind;
+        if (a.fine || v >= 0.) {
+            const double diff = v - mind;
             wt = entry_prior(a, b, e);
             wt = (diff > 700.) ? 0. : wt * exp(-diff);
         }
@@ -1563,7 +1589,7 @@ __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__r
 // --------------------------------------------------------------------------------------------------
 
 // Work items of the fine pass: jobs with at most small_max significant translations are one item of
-// the small path of k_fine_diff2; the others one wide item per kWideTC translations. One block of 1024.
+// the small path of k_fine_diff2; the others one wide item per wide_tc translations. One block of 1024.
 __global__ void __launch_bounds__(kWBlock) k_fine_items(const FineJob *__restrict__ jobs, int small_max, int wide_tc,
                                                         int cap_small, int cap_wide, int2 *__restrict__ small_items,
                                                         int2 *__restrict__ wide_items, FinePlan *__restrict__ plan) {
@@ -1617,23 +1643,87 @@ struct FineArgs {
     const float2 *ey;              // [n_ys][s]
     const int *tr_ix;              // [NT*4] x-shift index of fine translation t*4+u
     const int *tr_iy;
-    const float *xi2half;
+    const int2 *pix;               // sampled pixels (Minvsigma2 > 0, inside the projection radius) as (p, x | row
+                                   // << 16), in n_bands bands of equal count by radius, pixel order within a band
+    int n_pix;
+    int n_bands;                   // band b: pix[b n_pix / n_bands .. (b + 1) n_pix / n_bands)
     int NT;
     int do_ctf;
-    float *diff2;                  // out [pair_off[b]*32 + s*32 + r*4 + u]
+    double *diff2;                 // [pair_off[b]*32 + s*32 + r*4 + u]: k_fine_init's base, plus each band's sum
 };
 
-// One work item: a job with at most kFineSmallTC significant coarse translations. Each thread owns pixels,
-// projects the 8 oversampled orientations once per pixel and accumulates the 8 x 4*TC squared differences.
-constexpr int kFineSmallTC = 2;
+// Pose-independent part of a particle's fine diff2, in FP64: highres_Xi2 / 2 + sum over the pixels with
+// Minvsigma2 > 0 of |q X|^2 (q = sqrt(Minvsigma2 / 2)). The fine kernels accumulate only the pose-dependent
+// remainder |A|^2 - 2 Re(A conj(Y)) of |A - Y|^2 (A the scaled projection, Y = q X shifted; |Y| = |q X|), whose
+// magnitude is that of the signal, not of the noise power, so that FP32 partial sums keep the differences between
+// poses that the weights exponentiate (the reference CPU path sums diff2 in double, src/ml_optimiser.cpp:7436-7442).
+// grid (capacity of the chunk), block kBlock; blocks beyond the chunk's particle count return.
+__global__ void __launch_bounds__(kBlock) k_fine_base(const ChunkPtrs *__restrict__ cp, const float *__restrict__ q,
+                                                      int P, const double *__restrict__ xi2half,
+                                                      double *__restrict__ fbase) {
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x;
+    if (b >= cp->B) return;
+    const float2 *X = cp->fimg + (size_t)b * P;
+    double acc = 0.;
+    for (int p = threadIdx.x; p < P; p += kBlock) {
+        const float qp = q[p];
+        if (qp == 0.f) continue;
+        const float2 x = X[p];
+        const double xr = (double)qp * x.x, xi = (double)qp * x.y;
+        acc += xr * xr + xi * xi;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = xi2half[b];
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+        fbase[b] = t;
+    }
+}
 
+// Fine diff2 of the chunk's evaluated poses set to their particle's pose-independent part; the bands of the
+// fine pass add their pose-dependent sums to it. grid (capacity of the chunk), block kBlock.
+__global__ void __launch_bounds__(kBlock) k_fine_init(const int *__restrict__ n_pairs_b,
+                                                      const long long *__restrict__ pair_off,
+                                                      const double *__restrict__ fbase, const FinePlan *__restrict__ plan,
+                                                      double *__restrict__ diff2) {
+    const int b = blockIdx.x;
+    if (plan->overflow || b >= plan->B) return;
+    const long long n = 32LL * n_pairs_b[b];
+    double *d = diff2 + 32LL * pair_off[b];
+    const double v = fbase[b];
+    for (long long e = threadIdx.x; e < n; e += kBlock) d[e] = v;
+}
+
+// Pixel range [i0, i1) of band `band` in FineArgs::pix.
+__device__ __forceinline__ void band_range(const FineArgs &a, int band, int &i0, int &i1) {
+    i0 = band * a.n_pix / a.n_bands;
+    i1 = (band + 1) * a.n_pix / a.n_bands;
+}
+
+// Work-item shapes of the fine pass (chosen per engine by the tuner): a job with at most TC significant coarse
+// translations is one small item; a larger job is split into wide items of 4 * CW coarse translations, CW per wave.
+struct FineShape {
+    int tc, cw;
+};
+constexpr FineShape kFineShapes[] = {{2, 2}, {1, 2}, {2, 1}, {1, 1}};
+constexpr int kNumFineShapes = (int)(sizeof(kFineShapes) / sizeof(kFineShapes[0]));
+
+// One work item on one band of pixels: a job with at most TC significant coarse translations. Each thread owns
+// pixels, projects the 8 oversampled orientations once per pixel and accumulates the 8 x 4*TC squared differences.
+template <int TC>
 struct FineSmallLds {
-    float red[kBlock / 64][8 * 4 * kFineSmallTC];
-    int tix[4 * kFineSmallTC], tiy[4 * kFineSmallTC];
-    int t[kFineSmallTC];
+    float red[kBlock / 64][8 * 4 * TC];
+    int tix[4 * TC], tiy[4 * TC];
+    int t[TC];
 };
 
-__device__ __forceinline__ void fine_small_item(const FineArgs &a, int job_id, int first, FineSmallLds &L) {
+template <int TC>
+__device__ __forceinline__ void fine_small_item(const FineArgs &a, int job_id, int first, int band,
+                                                FineSmallLds<TC> &L) {
+    constexpr int kFineSmallTC = TC;
     constexpr int kFineNF = 4 * kFineSmallTC;  // fine translations per block
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
@@ -1663,14 +1753,18 @@ __device__ __forceinline__ void fine_small_item(const FineArgs &a, int job_id, i
     for (int r = 0; r < 8; ++r)
 #pragma unroll
         for (int j = 0; j < kFineNF; ++j) acc[r][j] = 0.f;
-    for (int p = tid; p < a.ig.P; p += kBlock) {
+    int i0, i1;
+    band_range(a, band, i0, i1);
+    // pixels with Minvsigma2 == 0 contribute nothing, those outside the projection radius only |Y|^2 (in fbase)
+    for (int ip = i0 + tid; ip < i1; ip += kBlock) {
+        const int2 pe = a.pix[ip];
+        const int p = pe.x, x = pe.y & 0xffff, i = pe.y >> 16;
+        const int y = (i < a.ig.xs) ? i : i - a.ig.s;
         const float qp = a.q[p];
-        if (qp == 0.f) continue;  // Minvsigma2 == 0: no contribution
-        int x, y;
-        pixel_xy(p, a.ig.s, a.ig.xs, x, y);
-        const int i = p / a.ig.xs;
         const float c = qp * (a.do_ctf ? C[p] : 1.f) * sc;
         float2 A[8];
+        // every listed pixel is inside the projection radius; testing it anyway keeps the compiler from hoisting
+        // all 32 corner-block loads of the 8 orientations at once (which spills registers at 3 waves per SIMD)
         const bool inside = x * x + y * y <= a.ig.proj_r2_max;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
@@ -1679,16 +1773,16 @@ __device__ __forceinline__ void fine_small_item(const FineArgs &a, int job_id, i
             A[r] = make_float2(c * v.x, c * v.y);
         }
         const float2 Xp = X[p];
-        const float2 Xq = make_float2(qp * Xp.x, qp * Xp.y);
+        const float2 Xq2 = make_float2(2.f * qp * Xp.x, 2.f * qp * Xp.y);
 #pragma unroll
         for (int j = 0; j < kFineNF; ++j) {
             const float2 ph = cmul(a.ex[(size_t)L.tix[j] * a.ig.xs + x], a.ey[(size_t)L.tiy[j] * a.ig.s + i]);
-            const float2 Y = cmul(Xq, ph);
+            const float2 Y2 = cmul(Xq2, ph);  // 2 Y
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const float dr = A[r].x - Y.x, di = A[r].y - Y.y;
-                acc[r][j] = fmaf(dr, dr, acc[r][j]);
-                acc[r][j] = fmaf(di, di, acc[r][j]);
+                // |A|^2 - 2 Re(A conj(Y)) = A.x (A.x - 2 Y.x) + A.y (A.y - 2 Y.y)
+                acc[r][j] = fmaf(A[r].x, A[r].x - Y2.x, acc[r][j]);
+                acc[r][j] = fmaf(A[r].y, A[r].y - Y2.y, acc[r][j]);
             }
         }
     }
@@ -1710,27 +1804,29 @@ __device__ __forceinline__ void fine_small_item(const FineArgs &a, int job_id, i
             float v = 0.f;
             for (int ww = 0; ww < kBlock / 64; ++ww) v += L.red[ww][tid];
             const long long s = job.s_begin + first + c;
-            a.diff2[(a.pair_off[b] + s) * 32 + r * 4 + u] = a.xi2half[b] + v;
+            atomicAdd(&a.diff2[(a.pair_off[b] + s) * 32 + r * 4 + u], (double)v);
         }
     }
 }
 
-// One work item of a job with more than kFineSmallTC significant translations: up to 4 * kWideCW coarse
+// One work item of a job with more than TC significant translations: up to 4 * kWideCW coarse
 // translations. Per tile of 64 pixels the four waves first project the 8 fine orientations together (two per
 // thread) into LDS; then wave w accumulates the squared differences of its kWideCW coarse translations (4 * kWideCW
 // fine translations) over the tile's pixels. A job with up to kWideTC significant translations reads each
 // trilinear sample once.
-constexpr int kWideCW = 2;             // coarse translations per wave
-constexpr int kWideTC = 4 * kWideCW;   // coarse translations per block
-
+template <int CW>
 struct FineWideLds {
     float2 A[8][64];
     float2 X[64];
     int p[64];
-    int tix[4][4 * kWideCW], tiy[4][4 * kWideCW];  // per wave
+    int tix[4][4 * CW], tiy[4][4 * CW];  // per wave
 };
 
-__device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, int first, FineWideLds &L) {
+template <int CW>
+__device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, int first, int band,
+                                               FineWideLds<CW> &L) {
+    constexpr int kWideCW = CW;            // coarse translations per wave
+    constexpr int kWideTC = 4 * kWideCW;   // coarse translations per block
     constexpr int kNF = 4 * kWideCW;  // fine translations per wave
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
@@ -1757,30 +1853,27 @@ __device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, in
     for (int r = 0; r < 8; ++r)
 #pragma unroll
         for (int j = 0; j < kNF; ++j) acc[r][j] = 0.f;
-    for (int p0 = 0; p0 < a.ig.P; p0 += 64) {
+    int i0, i1;
+    band_range(a, band, i0, i1);
+    for (int t0 = i0; t0 < i1; t0 += 64) {
         __syncthreads();  // previous tile consumed (and the translation tables written, first time)
-        // phase A: 8 projections of 64 pixels, 2 per thread
+        // phase A: 8 projections of 64 sampled pixels, 2 per thread
         {
-            const int p = p0 + lane;
+            const int ip = t0 + lane;
             float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
             float c = 0.f;
-            if (p < a.ig.P) {
+            if (ip < i1) {
+                const int2 pe = a.pix[ip];
+                const int p = pe.x, x = pe.y & 0xffff, i = pe.y >> 16;
+                const int y = (i < a.ig.xs) ? i : i - a.ig.s;
                 const float qp = a.q[p];
-                if (qp != 0.f) {
-                    int x, y;
-                    pixel_xy(p, a.ig.s, a.ig.xs, x, y);
-                    c = qp * (a.do_ctf ? C[p] : 1.f) * sc;
-                    if (x * x + y * y <= a.ig.proj_r2_max) {
-                        v0 = project_pixel(vol, a.vg, fr[2 * wave], x, y);
-                        v1 = project_pixel(vol, a.vg, fr[2 * wave + 1], x, y);
-                    }
-                    if (wave == 0) {
-                        const float2 Xp = X[p];
-                        L.X[lane] = make_float2(qp * Xp.x, qp * Xp.y);
-                        L.p[lane] = p;
-                    }
-                } else if (wave == 0) {
-                    L.p[lane] = -1;
+                c = qp * (a.do_ctf ? C[p] : 1.f) * sc;
+                v0 = project_pixel(vol, a.vg, fr[2 * wave], x, y);
+                v1 = project_pixel(vol, a.vg, fr[2 * wave + 1], x, y);
+                if (wave == 0) {
+                    const float2 Xp = X[p];
+                    L.X[lane] = make_float2(2.f * qp * Xp.x, 2.f * qp * Xp.y);  // 2 q X
+                    L.p[lane] = p;
                 }
             } else if (wave == 0) {
                 L.p[lane] = -1;
@@ -1797,17 +1890,17 @@ __device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, in
                 float2 A[8];
 #pragma unroll
                 for (int r = 0; r < 8; ++r) A[r] = L.A[r][lane];
-                const float2 Xq = L.X[lane];
+                const float2 Xq2 = L.X[lane];
 #pragma unroll
                 for (int j = 0; j < kNF; ++j) {
                     const float2 ph = cmul(a.ex[(size_t)L.tix[wave][j] * a.ig.xs + x],
                                            a.ey[(size_t)L.tiy[wave][j] * a.ig.s + i]);
-                    const float2 Y = cmul(Xq, ph);
+                    const float2 Y2 = cmul(Xq2, ph);  // 2 Y
 #pragma unroll
                     for (int r = 0; r < 8; ++r) {
-                        const float dr = A[r].x - Y.x, di = A[r].y - Y.y;
-                        acc[r][j] = fmaf(dr, dr, acc[r][j]);
-                        acc[r][j] = fmaf(di, di, acc[r][j]);
+                        // pose-dependent part of |A - Y|^2 (|Y|^2 is in fbase)
+                        acc[r][j] = fmaf(A[r].x, A[r].x - Y2.x, acc[r][j]);
+                        acc[r][j] = fmaf(A[r].y, A[r].y - Y2.y, acc[r][j]);
                     }
                 }
             }
@@ -1827,7 +1920,7 @@ __device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, in
             const int c = kWideCW * wave + lane;  // one coarse translation per lane
             if (c < nc) {
                 const long long s = job.s_begin + first + c;
-                float *out = a.diff2 + (a.pair_off[b] + s) * 32;
+                double *out = a.diff2 + (a.pair_off[b] + s) * 32;
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
 #pragma unroll
@@ -1837,41 +1930,47 @@ __device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, in
 #pragma unroll
                         for (int q = 1; q < kWideCW; ++q)
                             if (lane == q) v = acc[r][4 * q + u];
-                        out[r * 4 + u] = a.xi2half[b] + v;
+                        atomicAdd(&out[r * 4 + u], (double)v);
                     }
             }
         }
     }
 }
 
-// Fine pass, persistent: blocks fetch work items from plan->next_fine (wide items first, the heaviest, then
-// small ones) until none is left, so uneven items spread over the GPU; every block exits once the counter passes
-// the last item. Three waves per SIMD, as the two item paths have when compiled alone (at most 168 VGPRs).
-// kTuning only separates the grid-size tuning launches from the production ones in kernel traces.
-template <int kTuning>
+// Fine pass, persistent: blocks fetch work units from plan->next_fine until none is left, so uneven items spread
+// over the GPU; every block exits once the counter passes the last unit. A unit is one work item on one band of
+// pixels, band-major (all items of band 0, then band 1, ...; within a band the wide items first, the heaviest, then
+// the small ones): the units in flight at any time sample one or two radial shells of the reference, a fraction of
+// the packed volume that the memory-side cache can hold, instead of the whole volume. Each unit adds its partial
+// sums to the FP64 diff2 (k_fine_init wrote the pose-independent part). Three waves per SIMD, as the two item
+// paths have when compiled alone (at most 168 VGPRs). kTuning only separates the tuning launches from the
+// production ones in kernel traces.
+template <int kTuning, int TC, int CW>
 __global__ void __launch_bounds__(kBlock, 3) k_fine_diff2(FineArgs a) {
     __shared__ union {
-        FineSmallLds s;
-        FineWideLds w;
+        FineSmallLds<TC> s;
+        FineWideLds<CW> w;
     } lds;
     __shared__ int s_item;
     if (a.plan->overflow) return;
     const int n_wide = a.plan->n_wide;
     const int n_items = n_wide + a.plan->n_small;
+    const int n_units = n_items * a.n_bands;
     for (;;) {
-        __syncthreads();  // previous item's LDS and s_item consumed
+        __syncthreads();  // previous unit's LDS and s_item consumed
         if (threadIdx.x == 0) s_item = atomicAdd(&a.plan->next_fine, 1);
         __syncthreads();
         // one value per block: readfirstlane makes the branches below provably uniform, so the two item paths
         // are not linearised and do not hold their registers at the same time
-        const int item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item >= n_items) break;
+        const int unit = __builtin_amdgcn_readfirstlane(s_item);
+        if (unit >= n_units) break;
+        const int band = unit / n_items, item = unit - band * n_items;
         if (item < n_wide) {
             const int2 it = a.items_w[item];
-            fine_wide_item(a, it.x, it.y, lds.w);
+            fine_wide_item<CW>(a, it.x, it.y, band, lds.w);
         } else {
             const int2 it = a.items_s[item - n_wide];
-            fine_small_item(a, it.x, it.y, lds.s);
+            fine_small_item<TC>(a, it.x, it.y, band, lds.s);
         }
     }
 }
@@ -1961,6 +2060,9 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
     const int tid = threadIdx.x;
+    // a particle whose fine weights failed (zero or NaN sum) contributes nothing: its weights are not
+    // normalisable and would turn the shared accumulators into NaN (the run returns the particle's status)
+    if (a.stats[b].status != 0) continue;
     const double thr = a.stats[b].threshold;
     const double sumw = a.stats[b].sum_weight;
     if (tid == 0) s_n = 0;

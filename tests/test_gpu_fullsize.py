@@ -1,13 +1,11 @@
 """The E-step at the headline geometry (box 256, 1.0 A pixels, current size 256, HEALPix order 3, offsets 4 px
 at 0.5 px steps: 197 translations, particle diameter 200 A: coarse size 48) against the FP64 oracle, with the
 tolerances of the parity suite (tests/test_gpu_parity.py): per-pose coarse and fine diff2 within 1e-5, identical
-best poses except near-ties, dLL within 1e-5; backprojected volumes (68.4 M voxels) within 3e-4 norm-wise. The
-backprojection's error grows with radius (data 1e-5 at the centre to 1.6e-4 near r = 250 voxels, weight ten times
-smaller: 1.2e-4 and 1.1e-5 norm-wise measured on the local case). FP32 diff2 (errors near 1e-7 of diff2 values
-near 1e4) moves the relative weights of the significant translations slightly, and translations half a pixel
-apart differ by phases that grow with frequency, so the data's error grows with radius; the weight volume does not
-depend on the translation and keeps the smaller error of FP32 interpolation coordinates up to 256 voxels. This is
-the FP32 arithmetic of the reference's GPU path.
+best poses except near-ties, dLL within 1e-5, backprojected volumes (68.4 M voxels) within 1e-4 norm-wise.
+At this size the pose-independent part of a fine diff2 (the noise power sum(|q X|^2), about 1.6e4) is far larger
+than the differences between poses that the weights exponentiate; the fine kernels therefore sum only the
+pose-dependent part in FP32 and add the rest in FP64, so that the weights, and the backprojection they scale,
+keep the oracle's relative values.
 A local search in plain FP32 and an exhaustive global search in split BF16 (7.26 M coarse poses per particle), on
 a few particles each so that the oracle finishes in seconds."""
 import pytest
@@ -24,9 +22,9 @@ def _headline(n, **kw):
 
 
 def test_full_size_local_search_plain():
-    print("[fullsize] box 256 local search (sigma 7.5 deg), plain FP32, 6 particles", flush=True)
+    print("[fullsize] box 256 local search (sigma 7.5 deg), plain FP32, 6 particles, fine pass in 16 bands", flush=True)
     p = _headline(6, local=True, sigma_ang=7.5)
-    check_run(p, debug=(0,), bp_tol=3e-4)
+    check_run(p, debug=(0,), bp_tol=1e-4, bands=16)
 
 
 def test_full_size_global_search_split_bf16():
@@ -34,4 +32,4 @@ def test_full_size_global_search_split_bf16():
     print("[fullsize] box 256 exhaustive global search, split BF16, 3 particles", flush=True)
     p = _headline(3)
     p.cfg.precision = ra.RLN_PREC_SPLIT_BF16
-    check_run(p, debug=(0,), bp_tol=3e-4)
+    check_run(p, debug=(0,), bp_tol=1e-4)

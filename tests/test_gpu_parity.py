@@ -6,7 +6,9 @@ Tolerances (from the E-step's accuracy contract):
     two picks within 1e-4 relative);
   * backprojected data and weight volumes within 1e-4 relative (norm-wise, per class);
   * significance masks identical except entries whose oracle weight lies within 5% of the threshold (FP32
-    diff2 moves weights by exp(+-1e-5 * diff2)).
+    coarse diff2 moves weights by exp(+-1e-5 * diff2));
+  * sum_weight and pmax within 1e-3 relative, log(sum_weight) - min_diff2 within 1e-3 absolute: the fine diff2's
+    pose-independent part is summed in FP64, so the weights see only the FP32 error of the pose-dependent part.
 """
 import numpy as np
 import pytest
@@ -18,7 +20,7 @@ from problems import make_problem
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(p, keep=True):
+def run_gpu(p, keep=True, bands=None):
     cfg = p.cfg
     K = cfg.nr_classes
     dims = ra.volume_dims(cfg.current_size, cfg.padding_factor)
@@ -36,6 +38,8 @@ def run_gpu(p, keep=True):
     es.set_model(projs, bps, p.pdf_class, p.sigma2_noise, pdf_direction=p.pdf_direction,
                  data_vs_prior=p.data_vs_prior)
     es.keep_intermediates(keep)
+    if bands is not None:
+        es.set_fine_bands(bands)
     res, wsum = es.run(p.fimg, p.fctf, p.meta, fimg_nomask=p.fimg_nomask)
     bp_out = [b.getMdlData() for b in bps]
     return es, res, wsum, bp_out, (projs, bps)
@@ -81,8 +85,10 @@ def check_particle(es, ip, dbg, res_g, res_o):
         assert abs(w[bo] - w[bg]) <= 1e-4 * w[bo], f"best pose differs: gpu {bg} oracle {bo}"
 
 
-def check_run(p, debug=(0, 1), bp_tol=1e-4):
-    es, rg, wg, bpg, _ = run_gpu(p)
+def check_run(p, debug=(0, 1), bp_tol=1e-4, bands=None):
+    es, rg, wg, bpg, _ = run_gpu(p, bands=bands)
+    if bands is not None:
+        assert es.counters()["fine_bands"] == bands
     ro, wo, (bpd, bpw), dbg = run_oracle(p, debug)
     n = len(p.meta)
     for ip in debug:
@@ -94,9 +100,18 @@ def check_run(p, debug=(0, 1), bp_tol=1e-4):
     assert np.array_equal(rg["iclass"][same], ro["iclass"][same])
     for f in ("min_diff2", "coarse_min_diff2", "dLL"):
         assert rel(rg[f], ro[f]).max() < 1e-5, f"{f}: {rel(rg[f], ro[f]).max():.3e}"
-    for f in ("sum_weight", "coarse_sum_weight"):
-        assert rel(rg[f], ro[f]).max() < 5e-2, f
-    assert np.allclose(rg["pmax"], ro["pmax"], atol=2e-2, rtol=0)
+    # the weights' normalisation on its own: log(sum_weight) - min_diff2 is dLL without the pose-independent
+    # sum of log(2 pi sigma2), which dominates dLL's magnitude; FP32 diff2 differences move it by ~1e-5
+    lse_g = np.log(rg["sum_weight"]) - rg["min_diff2"]
+    lse_o = np.log(ro["sum_weight"]) - ro["min_diff2"]
+    err_lse = np.abs(lse_g - lse_o).max()
+    err_sw = max(rel(rg[f], ro[f]).max() for f in ("sum_weight", "coarse_sum_weight"))
+    err_pm = rel(rg["pmax"], ro["pmax"])[same].max()
+    print(f"[parity] log(sum_weight)-min_diff2 abs err {err_lse:.2e}, sum_weight rel err {err_sw:.2e}, "
+          f"pmax rel err {err_pm:.2e}", flush=True)
+    assert err_lse < 1e-3, f"log(sum_weight) - min_diff2: {err_lse:.3e}"
+    assert err_sw < 1e-3, f"sum_weight: {err_sw:.3e}"
+    assert err_pm < 1e-3, f"pmax: {err_pm:.3e}"
     assert np.array_equal(rg["nr_significant"] > 0, ro["nr_significant"] > 0)
     # weighted sums
     assert rel(wg["sigma2_noise"], wo["sigma2_noise"])[wo["sigma2_noise"] > 0].max() < 1e-3
@@ -114,8 +129,10 @@ def check_run(p, debug=(0, 1), bp_tol=1e-4):
         ref_d = bpd[k]
         num = np.sqrt(((re - ref_d[..., 0]) ** 2 + (im - ref_d[..., 1]) ** 2).sum())
         den = np.sqrt((ref_d ** 2).sum())
+        errw = np.abs(w - bpw[k]).sum() / np.abs(bpw[k]).sum()
+        print(f"[parity] class {k}: BP data rel err {num / den:.2e}, weight {errw:.2e}", flush=True)
         assert num <= bp_tol * den, f"class {k}: BP data rel err {num / den:.3e}"
-        assert np.abs(w - bpw[k]).sum() <= bp_tol * np.abs(bpw[k]).sum(), "BP weight"
+        assert errw <= bp_tol, f"class {k}: BP weight rel err {errw:.3e}"
     return rg, ro
 
 
@@ -136,11 +153,12 @@ def test_projector_roundtrip():
     assert not re.any() and not im.any() and not w.any()
 
 
-def test_global_search_box64():
+@pytest.mark.parametrize("bands", [None, 8])
+def test_global_search_box64(bands):
     """The reference's CPU-runnable case in shape: box 64, HEALPix order 2, offsets 5 px step 1, global search,
-    noisy particles."""
+    noisy particles; the fine pass as tuned (one band at this size) and split into 8 radial pixel bands."""
     p = make_problem(box=64, px=2.0, order=2, offset_range_px=5, offset_step_px=1, n=8, snr=0.1)
-    rg, ro = check_run(p)
+    rg, ro = check_run(p, bands=bands)
     assert (rg["nr_fine_evaluated"] > 0).all()
 
 

@@ -4,7 +4,8 @@ Tolerances for this path (it trades the coarse cross term's precision for MFMA t
   * coarse diff2 within 1e-4 relative of the oracle (the plain-FP32 path holds 1e-5); the measured maximum is
     printed;
   * the fine pass, weights and backprojection run in the same FP32/FP64 kernels as the plain path, so the
-    max-likelihood poses must agree with the oracle for >= 99% of particles (all but one in these small runs).
+    max-likelihood poses must agree with the oracle for >= 99% of particles (all but one in these small runs)
+    and the backprojected data within 1e-4 norm-wise.
 """
 import numpy as np
 import pytest
@@ -58,5 +59,6 @@ def test_split_bf16_estep(kind):
     assert rel(rg["min_diff2"], ro["min_diff2"]).max() < 1e-5  # fine pass is FP32 direct
     for k in range(p.cfg.nr_classes):
         re, im, w = bpg[k]
-        num = np.sqrt(((re - bpd[k][..., 0]) ** 2 + (im - bpd[k][..., 1]) ** 2).sum())
-        assert num <= 1e-3 * np.sqrt((bpd[k] ** 2).sum())
+        err = np.sqrt(((re - bpd[k][..., 0]) ** 2 + (im - bpd[k][..., 1]) ** 2).sum() / (bpd[k] ** 2).sum())
+        print(f"split-BF16 class {k}: BP data rel err {err:.2e}", flush=True)
+        assert err <= 1e-4
