@@ -9,7 +9,7 @@
   * bins cover the coarse grid: every orientation of an order-3 grid belongs to one or two bins (two on psi bin
     boundaries) and a grid orientation's own bin lists it;
   * the gain: trained on projections of the reference, the proposal seeds an order-3 search that evaluates a
-    fraction of the exhaustive coarse poses and keeps the exhaustive best pose.
+    fraction of the exhaustive coarse poses and keeps the exhaustive best pose for at least 99% of the particles.
 """
 import numpy as np
 import pytest
@@ -123,7 +123,7 @@ def angular_distance(r0, r1):
 
 
 def test_proposal_seeds_order3_search_and_keeps_best_pose():
-    n = 64
+    n = 128
     p = make_problem(box=64, px=2.0, order=3, n=n, snr=0.1, seed=93, offset_range_px=3, offset_step_px=1)
     cfg = p.cfg
     print(f"[proposal] gain test: box 64, order 3 global search, {n} particles", flush=True)
@@ -139,7 +139,7 @@ def test_proposal_seeds_order3_search_and_keeps_best_pose():
     res0, _ = es.run(p.fimg, p.fctf, p.meta)
     poses0 = es.counters()["coarse_poses"]
     pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=5)
-    for ep in range(3):  # fresh projections every epoch
+    for ep in range(5):  # fresh projections every epoch
         pp.clear_training()
         pp.generate(pr, 100000, ori_size=64, pixel_size=2.0, sigma2_noise=p.sigma2_noise, defocus=(1.0e4, 3.0e4),
                     max_astigmatism=300.0, offset_max=3.0, first=ep * 100000)
@@ -160,7 +160,7 @@ def test_proposal_seeds_order3_search_and_keeps_best_pose():
           f"coarse poses {poses1:,} vs exhaustive {poses0:,} ({poses0 / poses1:.1f}x fewer); "
           f"exhaustive best pose kept for {kept:.1%}")
     assert poses1 * 3 < poses0
-    assert kept >= 0.9
+    assert kept >= 0.99
     es.close()
     pp.close()
 
