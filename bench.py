@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--box", type=int, default=256)
     ap.add_argument("--train-samples", type=int, default=200000)
     ap.add_argument("--train-epochs", type=int, default=5)
+    ap.add_argument("--finetune-samples", type=int, default=100000,
+                    help="fresh projections of the warm-started training of the next iteration, which also sees "
+                         "the particles labelled with this iteration's max-likelihood poses (0: skip)")
     ap.add_argument("--topk", type=int, default=16)
     ap.add_argument("--min-top1", type=float, default=0.2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -68,7 +71,11 @@ def parse():
                          "(the E-step workload of the run that saved them; for counter-collection runs)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the fine stage directly instead of through captured graphs (same kernels)")
-    ap.add_argument("--paths", default="proposal,plain,local,prep,refprep,recon,formats",
+    ap.add_argument("--classify-particles", type=int, default=1000,
+                    help="particles of the 4-class box-128 classification leg (one step per precision)")
+    ap.add_argument("--uneven-particles", type=int, default=4096,
+                    help="particles of the box-192 uneven-SNR leg (one step)")
+    ap.add_argument("--paths", default="proposal,plain,local,classify,uneven,prep,refprep,recon,formats",
                     help="comma list: proposal (headline), plain (exhaustive baseline), local (precision variant), "
                          "prep (image preparation before the E-step), refprep (reference preparation), recon "
                          "(reconstruction from the filled backprojector), formats (particles from STAR and MRC files)")
@@ -101,26 +108,18 @@ def angle_between(r0, r1):
     return float(np.degrees(np.arccos(np.clip((np.trace(A @ B.T) - 1) / 2, -1, 1))))
 
 
-def fsc(v0, v1, pad, r_max):
-    """FSC per shell (original-box Fourier pixels) between two reconstructions data / (weight + tau), tau = the
-    mean weight of the voxel's shell (a Wiener-like regularisation: voxels with near-zero weight do not
-    dominate the shell sums)."""
-    (re0, im0, w0), (re1, im1, w1) = v0, v1
-    zd, yd, xd = re0.shape
-    z = np.arange(zd) - zd // 2
-    y = np.arange(yd) - yd // 2
-    x = np.arange(xd)
-    r = np.sqrt(z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2) / pad
+def fsc_maps(m0, m1, r_max):
+    """Fourier shell correlation of two real-space maps per shell ROUND(|k|) (original-box Fourier pixels),
+    shells 0 .. r_max."""
+    F0, F1 = np.fft.rfftn(m0.astype(np.float64)), np.fft.rfftn(m1.astype(np.float64))
+    n = m0.shape[0]
+    k = np.fft.fftfreq(n) * n
+    kx = np.arange(n // 2 + 1)
+    r = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kx[None, None, :] ** 2)
     shell = np.rint(r).astype(np.int64).ravel()
     keep = shell <= r_max
     sh = shell[keep]
-    cnt = np.maximum(np.bincount(sh, minlength=r_max + 1), 1)
-
-    def recon(re, im, w):
-        wv = w.ravel()[keep].astype(np.float64)
-        tau = (np.bincount(sh, wv, r_max + 1) / cnt)[sh]
-        return (re.ravel()[keep] + 1j * im.ravel()[keep]) / (wv + tau + 1e-30)
-    a, b = recon(re0, im0, w0), recon(re1, im1, w1)
+    a, b = F0.ravel()[keep], F1.ravel()[keep]
     num = np.bincount(sh, np.real(a * np.conj(b)), r_max + 1)
     d0 = np.bincount(sh, np.abs(a) ** 2, r_max + 1)
     d1 = np.bincount(sh, np.abs(b) ** 2, r_max + 1)
@@ -128,15 +127,23 @@ def fsc(v0, v1, pad, r_max):
 
 
 def uneven(args):
-    """Box 192, 1.2 A, 100 blobs, 20,000 particles with per-particle SNR log-uniform in [0.01, 0.3]: the
-    significant-pose counts of the fine pass spread over orders of magnitude. Global HEALPix order-3 coarse
-    search (split-BF16), order-4 fine pass (oversampling 1), offsets 6 px range 2 px step; the fine pass runs
-    from device-side plans as persistent kernels in captured graphs with autotuned grids."""
-    import relion_amd as ra
+    """--workload uneven192 on its own: the uneven leg with 20,000 particles (or --particles), one JSON line."""
     from relion_amd import hipmem
     hipmem.set_device(0)
-    box, px, order = 192, 1.2, 3
     n = args.particles if args.particles != 10000 else 20000
+    out = uneven_leg(args, n, args.steps, args.warmup)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def uneven_leg(args, n, steps, warmup):
+    """Box 192, 1.2 A, 100 blobs, n particles with per-particle SNR log-uniform in [0.01, 0.3]: the
+    significant-pose counts of the fine pass spread over orders of magnitude. Global HEALPix order-3 coarse
+    search (split-BF16), order-4 fine pass (oversampling 1), offsets 6 px range 2 px step; the fine pass runs
+    from device-side plans as persistent kernels in captured graphs with autotuned work units and grids."""
+    import relion_amd as ra
+    from relion_amd import hipmem
+    box, px, order = 192, 1.2, 3
     diameter = 0.8 * box * px
     coarse = ra.coarse_image_size(box, px, order, diameter)
     cfg = ra.make_config(box, px, order, offset_range=6 * px, offset_step=2 * px, coarse_size=coarse,
@@ -164,28 +171,88 @@ def uneven(args):
         es.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
         return es.run(d_img.address, d_ctf.address, meta, on_device=True)[0]
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     hipmem.synchronize()
     t = time.time()
-    for _ in range(args.steps):
+    for _ in range(steps):
         res = step()
     hipmem.synchronize()
     el = time.time() - t
     tm, cn = es.timings(), es.counters()
     fe = res["nr_fine_evaluated"] // 32
-    out = {"metric": "E-step particles/s, box 192 uneven-SNR batched fine pass", "value": round(n * args.steps / el, 2),
-           "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": round(1000 * el / args.steps, 2), "higher_is_better": True, "scaling": "weak",
+    out = {"metric": "E-step particles/s, box 192 uneven-SNR batched fine pass", "value": round(n * steps / el, 2),
+           "unit": "particles/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(1000 * el / steps, 2), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "bf16x3+f32", "data": "synthetic",
            "config": {"workload": f"box{box} order3 global, order4 fine, {n} particles, SNR log-U[0.01,0.3]",
                       "coarse_size": coarse, "global_batch": n, "parallelism": "1 GPU"},
            "significant_coarse_poses_per_particle": {"min": int(fe.min()), "median": int(np.median(fe)),
                                                      "p99": int(np.percentile(fe, 99)), "max": int(fe.max())},
            "stages_ms": {k: round(v, 2) for k, v in tm.items()}, "counters": cn}
-    print(json.dumps(out), flush=True)
+    log(f"[uneven192] {n} particles: {out['value']:.0f} particles/s, stages {json.dumps(out['stages_ms'])}", args)
     es.close()
-    return 0
+    return out
+
+
+def classify_leg(args, n):
+    """Classification: K = 4 classes of 128^3 blob maps (60 blobs each, 30% shared), 1.5 A pixels, n particles at
+    SNR 0.05 with uniform class labels, HEALPix order 3 global search over all classes (4 x 36,864 orientations),
+    offsets 6 px at 1 px steps (113 translations), particle diameter 180 A (coarse 40 px): 16.7 M coarse poses per
+    particle. One timed step each of the plain-FP32 and the split-BF16 coarse pass after one warm-up."""
+    import relion_amd as ra
+    from relion_amd import hipmem
+    box, px, order, K = 128, 1.5, 3, 4
+    coarse = ra.coarse_image_size(box, px, order, 180.0)
+    spec = ra.synth_spec(args.seed + 1, box, px, 60, snr=0.05, offset_max=3.0)
+    t0 = time.time()
+    vols = [ra.synth_reference(spec, k, box, share_fraction=0.3) for k in range(K)]
+    fimg, fctf, meta, pose, nvar = ra.synth_particles(spec, n, box, nr_classes=K, share_fraction=0.3)
+    log(f"[classify] generated {K} references + {n} particles in {time.time() - t0:.1f} s (coarse {coarse}px)", args)
+    d_img, d_ctf = hipmem.DeviceArray(fimg), hipmem.DeviceArray(fctf)
+    del fimg, fctf
+    sigma2 = np.full(box // 2 + 1, float(nvar.mean()))
+    dims = ra.volume_dims(box)
+    projs, bps = [], []
+    for k in range(K):
+        pr = ra.AccProjector(0)
+        pr.setMdlDim(*dims, 2.0)
+        pr.initMdl(vols[k])
+        bp = ra.AccBackprojector(0)
+        bp.setMdlDim(*dims, 2.0)
+        projs.append(pr)
+        bps.append(bp)
+    out = {"box": box, "classes": K, "particles": n, "coarse_size": coarse,
+           "coarse_poses_per_particle": K * 36864 * 113}
+    for name, prec in (("plain", ra.RLN_PREC_PLAIN_FP32), ("split", ra.RLN_PREC_SPLIT_BF16)):
+        cfg = ra.make_config(box, px, order, offset_range=6 * px, offset_step=px, coarse_size=coarse, nr_classes=K,
+                             precision=prec)
+        n_dir, _, _ = ra.sampling_counts(cfg)
+        es = ra.ExpectationStep(cfg, 0)
+        pdf_dir = np.full((K, n_dir), 1.0 / (K * n_dir))
+
+        def step():
+            for b in bps:
+                b.initMdl()
+            es.set_model(projs, bps, np.full(K, 1.0 / K), sigma2, pdf_direction=pdf_dir)
+            return es.run(d_img.address, d_ctf.address, meta, on_device=True)[0]
+
+        step()
+        hipmem.synchronize()
+        t = time.time()
+        res = step()
+        hipmem.synchronize()
+        el = time.time() - t
+        tm = es.timings()
+        acc = round(float(np.mean(res["iclass"] == pose[:, 5].astype(int))), 4)  # both 1-based
+        out[name] = {"particles_per_s": round(n / el, 1), "ms_per_step": round(1000 * el, 1),
+                     "stages_ms": {k: round(v, 1) for k, v in tm.items()}, "class_accuracy": acc}
+        log(f"[classify] {name}: {n / el:.0f} particles/s, stages {json.dumps(out[name]['stages_ms'])}", args)
+        es.close()
+    out["split_vs_plain"] = round(out["split"]["particles_per_s"] / out["plain"]["particles_per_s"], 2)
+    d_img.free()
+    d_ctf.free()
+    return out
 
 
 def committed_traffic(name, kernel):
@@ -622,27 +689,77 @@ def main():
             return res
 
         runs["plain"] = timed("plain", plain_step, n_plain, 1, 0, es_x)
-        vol_plain = bp.getMdlData()
+        map_plain = bp.reconstruct(box, do_map=False, skip_gridding=True)
         es_x.close()
-        if pp is not None:
-            # the proposal path on the same particles (untimed): misses against the exhaustive search, FSC
+        res_x = runs["plain"]["out"]
+        lim = int(0.8 * (box // 2))
+
+        def versus_exhaustive():
+            """The proposal path on the plain subset (untimed): best poses against the exhaustive search's, and the
+            FSC between the two paths' reconstructions (BackProjector::reconstruct on the device, no MAP term)."""
             res_p, off_p = proposal_step(n_plain, meta_plain)
-            vol_prop = bp.getMdlData()
-            res_x = runs["plain"]["out"]
+            map_prop = bp.reconstruct(box, do_map=False, skip_gridding=True)
             ang = np.array([angle_between((a["rot"], a["tilt"], a["psi"]), (b["rot"], b["tilt"], b["psi"]))
                             for a, b in zip(res_x, res_p)])
             same = (ang < 1e-3) & (np.abs(res_x["xoff"] - res_p["xoff"]) < 1e-6) & \
                    (np.abs(res_x["yoff"] - res_p["yoff"]) < 1e-6)
-            f = fsc(vol_plain, vol_prop, 2.0, box // 2)
-            lim = int(0.8 * (box // 2))
-            extra["proposal_vs_exhaustive"] = {
-                "particles": n_plain, "kept_best_pose": round(float(same.mean()), 4),
-                "miss_rate": round(float(1.0 - same.mean()), 4),
-                "misses_within_fine_step_deg": round(float(np.mean(ang[~same] < 3.75)) if (~same).any() else 1.0, 4),
-                "fallback_fraction": round(float(np.mean(np.diff(off_p) == 0)), 4),
-                "fsc_min_to_0.8_nyquist": round(float(f[1:lim + 1].min()), 5),
-                "fsc_mean_to_0.8_nyquist": round(float(f[1:lim + 1].mean()), 5)}
+            f = fsc_maps(map_plain, map_prop, box // 2)
+            return {"particles": n_plain, "kept_best_pose": round(float(same.mean()), 4),
+                    "miss_rate": round(float(1.0 - same.mean()), 4),
+                    "misses_within_fine_step_deg": round(float(np.mean(ang[~same] < 3.75)) if (~same).any() else 1.0, 4),
+                    "fallback_fraction": round(float(np.mean(np.diff(off_p) == 0)), 4),
+                    "fsc_min_to_0.8_nyquist": round(float(f[1:lim + 1].min()), 5),
+                    "fsc_mean_to_0.8_nyquist": round(float(f[1:lim + 1].mean()), 5),
+                    "fsc_of": "reconstructions (skip-gridding path, no MAP term) of the two paths' backprojectors"}
+
+        if pp is not None:
+            extra["proposal_vs_exhaustive"] = versus_exhaustive()
             log(f"proposal vs exhaustive: {json.dumps(extra['proposal_vs_exhaustive'])}", args)
+            if args.finetune_samples > 0:
+                # the next iteration: warm start from this network, fresh projections of the reference plus the
+                # particles labelled with this iteration's max-likelihood poses (the last timed step's results)
+                res_ml = runs["proposal"]["out"][0]
+                poses_ml = np.stack([res_ml["rot"], res_ml["tilt"], res_ml["psi"]], 1)
+                hipmem.synchronize()
+                t = time.time()
+                pp.clear_training()
+                pp.generate(proj, args.finetune_samples, ori_size=box, pixel_size=px, sigma2_noise=sigma2,
+                            defocus=(8000.0, 25000.0), max_astigmatism=300.0, offset_max=3.0,
+                            first=args.train_epochs * args.train_samples)
+                pp.add_images(d_img.address, d_ctf.address, box, poses_ml, on_device=True)
+                hipmem.synchronize()
+                t_g = time.time() - t
+                t = time.time()
+                loss2 = pp.train(epochs=1, batch=256, lr=3e-4)
+                hipmem.synchronize()
+                t_t = time.time() - t
+                log(f"proposal fine-tune: {args.finetune_samples} projections + {n} ML-labelled particles, "
+                    f"generation {t_g:.2f} s, training {t_t:.2f} s, loss {loss2:.3f}", args)
+                runs["proposal_finetuned"] = timed("proposal_finetuned", proposal_step, n, args.steps, args.warmup,
+                                                   es_p)
+                vs = versus_exhaustive()
+                log(f"fine-tuned proposal vs exhaustive: {json.dumps(vs)}", args)
+                it_s = runs["proposal_finetuned"]["ms"] / 1000.0 + t_g + t_t
+                v_plain, v_ft = runs["plain"]["value"], runs["proposal_finetuned"]["value"]
+
+                def break_even(mult):
+                    # particles per iteration at which n / (n / v_ft + t_ft) reaches mult * v_plain
+                    d = 1.0 / (mult * v_plain) - 1.0 / v_ft
+                    return int(np.ceil((t_g + t_t) / d)) if d > 0 else None
+
+                extra["proposal_finetune"] = {
+                    "samples": args.finetune_samples, "ml_labelled_particles": n, "epochs": 1, "lr": 3e-4,
+                    "generate_s": round(t_g, 3), "train_s": round(t_t, 3), "loss": round(loss2, 4),
+                    "vs_exhaustive": vs}
+                extra["amortised_per_iteration"] = {
+                    "particles": n, "estep_s": round(runs["proposal_finetuned"]["ms"] / 1000.0, 4),
+                    "training_s": round(t_g + t_t, 3), "particles_per_s": round(n / it_s, 1),
+                    "vs_exhaustive_plain": round(n / it_s / v_plain, 2),
+                    "break_even_particles_1x_plain": break_even(1.0),
+                    "break_even_particles_5x_plain": break_even(5.0),
+                    "note": "an iteration after the first: E-step with the warm-started network plus its training "
+                            "(fresh projections and this iteration's max-likelihood assignments); the first "
+                            "iteration's cold training is proposal_training"}
     if "proposal" in runs:
         es_p.close()
     # ---------------------------------------------------------------- precision variant: local search
@@ -662,6 +779,14 @@ def main():
             runs[name] = timed(name, local_step, n, args.steps, args.warmup, es_l)
             es_l.close()
 
+    # ---------------------------------------------------------------- classification and uneven-work workloads
+    if "classify" in paths:
+        extra["classification_box128_k4"] = classify_leg(args, args.classify_particles)
+    if "uneven" in paths:
+        u = uneven_leg(args, args.uneven_particles, 1, 1)
+        extra["uneven_box192"] = {k: u[k] for k in ("value", "ms_per_step", "config", "stages_ms",
+                                                    "significant_coarse_poses_per_particle")}
+        extra["uneven_box192"]["counters"] = {k: u["counters"][k] for k in ("fine_poses", "fine_grid", "fine_bands")}
     # ---------------------------------------------------------------- reconstruction (the consumer of the backprojector)
     if "recon" in paths and runs:
         extra["reconstruction"] = recon_leg(args, bp, box, 1e3 * np.exp(-np.arange(box // 2 + 1) / (box / 16.0)))
@@ -687,22 +812,38 @@ def main():
     errs = np.array([angle_between(pose[i, :3], (res[i]["rot"], res[i]["tilt"], res[i]["psi"]))
                      for i in range(0, len(res), max(1, len(res) // 500))])
 
-    # roofline of the dominant kernel of the reported path
+    # roofline of the dominant kernel of the reported path, on the algorithmic work of the E-step's cost model:
+    #   coarse diff2: F = 4 P_c N_pose FLOP (the cross term Re sum conj(X_t) CTF P, 2 FMA per complex pixel)
+    #     against FP32 peak (plain) or BF16 MFMA peak / 3 (split-BF16: hi*hi + hi*lo + lo*hi);
+    #   fine diff2: B_proj = 8 P N_or bytes, the complex-FP32 projections of the N_or fine orientations (8 per fine
+    #     job) at the fine image size P = s (s/2 + 1), against HBM peak;
+    #   weighted sums / backprojection: B_proj of the backprojected orientations plus the read-modify-write of
+    #     the three FP32 accumulators at the backprojected pixels (24 B per pixel and orientation).
+    # The fine pass actually gathers one 64-byte corner block of the packed reference per sampled pixel and
+    # orientation; that request rate is reported beside the roofline (gather_requests).
     stage = max(("fine_diff2", "coarse_diff2", "wsum_backproject"), key=lambda k: timings[k])
     split = is_prop or head == "local_split"
+    P_f = counters["fine_pixels"]
+    gather = None
     if stage == "coarse_diff2":
-        work = 4.0 * counters["coarse_poses"] * counters["coarse_pixels"]      # FLOP, cross-term count
+        work = 4.0 * counters["coarse_poses"] * counters["coarse_pixels"]
         bound, unit = "mfma", "TFLOP/s"
         peak = PEAK_BF16_TFLOPS / 3.0 if split else PEAK_FP32_TFLOPS
         achieved = work / (timings[stage] * 1e-3) / 1e12
+        alg = {"flop_per_step": work, "formula": "4 * P_c * N_pose"}
     else:
-        # gather-bound: 64 bytes of the corner-packed reference per (fine orientation, pixel) sample, plus the
-        # 256 bytes of slot line read and written per backprojected sample in the weighted sums
-        # pixels counted: those sampled (non-zero Minvsigma2 inside the projection radius)
         if stage == "fine_diff2":
-            work = 64.0 * 8 * counters["fine_jobs"] * counters["fine_pixels_sampled"]
+            n_or = 8 * counters["fine_jobs"]
+            work = 8.0 * P_f * n_or
+            alg = {"bytes_per_step": work, "formula": "8 * P * N_or", "P": P_f, "N_or": n_or}
+            req = 64.0 * n_or * counters["fine_pixels_sampled"]
+            gather = {"bytes_per_step": req, "GB_per_s": round(req / (timings[stage] * 1e-3) / 1e9, 1),
+                      "requests_per_s": round(req / 64.0 / (timings[stage] * 1e-3), 1),
+                      "note": "one 64-byte corner block per sampled (pixel, fine orientation)"}
         else:
-            work = (64.0 + 256.0) * counters["wsum_orientations"] * counters["fine_pixels_sampled"]
+            n_or = counters["wsum_orientations"]
+            work = (8.0 + 24.0) * P_f * n_or
+            alg = {"bytes_per_step": work, "formula": "(8 + 24) * P * N_or_backprojected", "P": P_f, "N_or": n_or}
         bound, unit, peak = "hbm", "GB/s", PEAK_HBM_GBS
         achieved = work / (timings[stage] * 1e-3) / 1e9
     traffic, traffic_src = None, None
@@ -712,8 +853,10 @@ def main():
         traffic, traffic_src = committed_traffic("estep_proposal_box256_order3_10k_traffic.md", "rlnk::k_fine_diff2<0>")
     roofline = {"kernel": stage, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 1),
                 "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
-                "stage_ms": round(timings[stage], 2), "launches": counters["chunks"],
+                "algorithmic": alg, "stage_ms": round(timings[stage], 2), "launches": counters["chunks"],
                 "avg_launch_ms": round(timings[stage] / max(1, counters["chunks"]), 3)}
+    if gather is not None:
+        roofline["gather_requests"] = gather
 
     cpu = None
     if args.cpu_seconds > 0:
