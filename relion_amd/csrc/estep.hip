@@ -103,6 +103,11 @@ struct UninitAlloc : std::allocator<T> {
 template <class T>
 using HostList = std::vector<T, UninitAlloc<T>>;
 
+// Pixel lists of the banded kernels: the pixels `pix` (pixel order) of an s x xs half-plane ranked by radius (ties
+// in pixel order), cut into nb equal parts for every band choice, each band in pixel order, stored as
+// (p, x | row << 16): kNumBandChoices lists of pix.size() entries.
+static std::vector<int2> banded_lists(const std::vector<int> &pix, int s, int xs);
+
 // The production fine diff2 kernel of a work-item shape (index into rlnk::kFineShapes).
 inline const void *fine_kernel_ptr(int shape) {
     switch (shape) {
@@ -117,6 +122,34 @@ inline const void *fine_kernel_ptr(int shape) {
 // picks one per engine).
 constexpr int kBandChoices[] = {1, 2, 4, 8, 16, 32};
 constexpr int kNumBandChoices = (int)(sizeof(kBandChoices) / sizeof(kBandChoices[0]));
+
+static std::vector<int2> banded_lists(const std::vector<int> &pix, int s, int xs) {
+    const int ns = (int)pix.size();
+    auto r2 = [&](int p) {
+        int x, y;
+        rlnk::pixel_xy(p, s, xs, x, y);
+        return x * x + y * y;
+    };
+    std::vector<int> byr(pix);
+    std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) { return r2(a) < r2(b); });
+    std::vector<int> band_p(ns);
+    std::vector<int2> lists((size_t)kNumBandChoices * std::max(ns, 1));
+    for (int c = 0; c < kNumBandChoices; ++c) {
+        const int nb = kBandChoices[c];
+        for (int band = 0; band < nb; ++band) {
+            const int i0 = band * ns / nb, i1 = (band + 1) * ns / nb;
+            std::copy(byr.begin() + i0, byr.begin() + i1, band_p.begin() + i0);
+            std::sort(band_p.begin() + i0, band_p.begin() + i1);
+        }
+        // the kernels read the pixel's coordinates instead of dividing by the row length
+        int2 *dst = lists.data() + (size_t)c * ns;
+        for (int k = 0; k < ns; ++k) {
+            const int p = band_p[k], row = p / xs;
+            dst[k] = make_int2(p, (p - row * xs) | (row << 16));
+        }
+    }
+    return lists;
+}
 
 // Stage timers of one run (HIP events on the engine's stream).
 enum Stage { kStPrep, kStCoarse, kStCoarseW, kStJobs, kStFine, kStFineW, kStWsum, kStCount };
@@ -188,6 +221,8 @@ struct Engine {
     int fine_shape = 0;                      // index into rlnk::kFineShapes of the work-item shape (tuned)
     int resident_shape[rlnk::kNumFineShapes] = {};  // resident blocks of each shape's kernel on the device
     DevBuf d_band_pix;                       // sampled fine pixels, one banded list per band choice
+    DevBuf d_wsum_pix;                       // pixels of the weighted sums, one banded list per band choice
+    int n_wsum_pix = 0;
     std::vector<int> band_src;               // the sampled pixels the lists were built from (pixel order)
     int resident_fine = 0;                   // resident blocks of the default shape's k_fine_diff2
     int n_cu = 0;                            // compute units of the device
@@ -419,6 +454,19 @@ static int engine_init(Engine &e, const rln_estep_config &cfg) {
     RLN_TRY(upload(e.d_over_y, e.over_y, e.stream));
     e.mresol_f = mresol_of(e.sf);
     RLN_TRY(upload(e.d_mresol, e.mresol_f, e.stream));
+    {
+        // pixels of the weighted sums: a resolution shell (noise spectrum) or inside the backprojection radius and
+        // not on the duplicated x = 0, y < 0 half column (backproject2Dto3D, src/backprojector.cpp:134-143)
+        std::vector<int> wp;
+        for (int n = 0; n < e.igf.P; ++n) {
+            int x, y;
+            rlnk::pixel_xy(n, e.igf.s, e.igf.xs, x, y);
+            const bool bp = x * x + y * y <= e.igf.bp_r2_max && !(n / e.igf.xs >= e.igf.xs && x == 0);
+            if (e.mresol_f[n] > -1 || bp) wp.push_back(n);
+        }
+        e.n_wsum_pix = (int)wp.size();
+        RLN_TRY(upload(e.d_wsum_pix, banded_lists(wp, e.igf.s, e.igf.xs), e.stream));
+    }
     if (cfg.precision == RLN_PREC_SPLIT_BF16 && e.NT > 256)
         RLN_FAIL(RLN_ERR_ARG, "split-BF16 coarse pass supports at most 256 translations (%d)", e.NT);
     if (cfg.precision == RLN_PREC_SPLIT_BF16) {
@@ -489,33 +537,7 @@ static int engine_model_tables(Engine &e) {
     }
     e.pf_sampled = (long long)samp.size();
     if (samp != e.band_src) {
-        // banded lists: rank the sampled pixels by radius (ties in pixel order), cut the ranks into NB equal
-        // parts, and keep each band in pixel order (rows stay contiguous for the image reads)
-        const int ns = (int)samp.size();
-        std::vector<int> byr(samp);
-        std::stable_sort(byr.begin(), byr.end(), [&](int a, int b) {
-            int xa, ya, xb, yb;
-            rlnk::pixel_xy(a, e.igf.s, e.igf.xs, xa, ya);
-            rlnk::pixel_xy(b, e.igf.s, e.igf.xs, xb, yb);
-            return xa * xa + ya * ya < xb * xb + yb * yb;
-        });
-        std::vector<int> band_p(ns);
-        std::vector<int2> lists((size_t)kNumBandChoices * std::max(ns, 1));
-        for (int c = 0; c < kNumBandChoices; ++c) {
-            const int nb = kBandChoices[c];
-            int2 *dst = lists.data() + (size_t)c * ns;
-            for (int band = 0; band < nb; ++band) {
-                const int i0 = band * ns / nb, i1 = (band + 1) * ns / nb;
-                std::copy(byr.begin() + i0, byr.begin() + i1, band_p.begin() + i0);
-                std::sort(band_p.begin() + i0, band_p.begin() + i1);
-            }
-            // (p, x | row << 16): the kernels read the pixel's coordinates instead of dividing by the row length
-            for (int k = 0; k < ns; ++k) {
-                const int p = band_p[k], row = p / e.igf.xs;
-                dst[k] = make_int2(p, (p - row * e.igf.xs) | (row << 16));
-            }
-        }
-        RLN_TRY(upload(e.d_band_pix, lists, e.stream));
+        RLN_TRY(upload(e.d_band_pix, banded_lists(samp, e.igf.s, e.igf.xs), e.stream));
         e.band_src = samp;
     }
     RLN_TRY(upload(e.d_minv_w, mw, e.stream));
@@ -1113,6 +1135,8 @@ static int run_fine_stage(Engine &e, const FineLaunch &L0) {
         L.grid_fine = e.grid_fine;
         L.shape = e.fine_shape;
         L.fa.n_bands = kBandChoices[e.band_choice];
+        L.ws.n_bands = kBandChoices[e.band_choice];
+        L.ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)e.band_choice * e.n_wsum_pix;
         L.fa.pix = e.d_band_pix.as<const int2>() + (size_t)e.band_choice * e.pf_sampled;
     }
     if (!e.graph_off) {
@@ -1451,6 +1475,9 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         ws.scale_w = d_scale_w;
         ws.minv_w = e.d_minv_w.as<const float>();
         ws.mresol = e.d_mresol.as<const int>();
+        ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)e.band_choice * e.n_wsum_pix;
+        ws.n_wpix = e.n_wsum_pix;
+        ws.n_bands = kBandChoices[e.band_choice];
         ws.xa_shell = (cfg.do_scale_correction && e.d_xa_shell.ptr) ? e.d_xa_shell.as<const unsigned char>() : nullptr;
         ws.ex = e.d_ex_f.as<const float2>();
         ws.ey = e.d_ey_f.as<const float2>();

@@ -2011,6 +2011,10 @@ struct WsumArgs {
     const float *scale_w;           // max(scale, 0.001) (src/ml_optimiser.cpp:8402-8416)
     const float *minv_w;            // Minvsigma2 with the origin restored
     const int *mresol;              // resolution shell per pixel, -1 excluded
+    const int2 *wpix;               // the pixels that contribute (a shell >= 0 or backprojected) as (p, x | row << 16),
+                                    // n_bands bands of equal count by radius, pixel order within a band
+    int n_wpix;
+    int n_bands;
     const unsigned char *xa_shell;  // [K][nshell] data_vs_prior > 3 (scale correction) or null
     const float2 *ex;
     const float2 *ey;
@@ -2048,13 +2052,17 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     __shared__ int s_item;
     if (a.plan->overflow) return;
     const int n_items = 8 * a.plan->n_jobs;
-    // persistent: items fetched dynamically, since only some (job, orientation) items have significant weights
+    const int n_units = n_items * a.n_bands;
+    // persistent: units fetched dynamically, since only some (job, orientation) items have significant weights; a
+    // unit is one item on one radial band of pixels, band-major, so that the units in flight read and update one
+    // shell of the reference and of the accumulators at a time
     for (;;) {
-    __syncthreads();  // previous item's LDS and s_item consumed
+    __syncthreads();  // previous unit's LDS and s_item consumed
     if (threadIdx.x == 0) s_item = atomicAdd(&a.plan->next_wsum, 1);
     __syncthreads();
-    const int item = s_item;
-    if (item >= n_items) break;
+    const int unit = s_item;
+    if (unit >= n_units) break;
+    const int band = unit / n_items, item = unit - band * n_items;
     const int job_id = item >> 3;
     const int r = item & 7;
     const FineJob job = a.jobs[job_id];
@@ -2093,7 +2101,7 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
             }
         }
         s_n = cnt;
-        if (cnt > 0) {
+        if (cnt > 0 && band == 0) {  // the item's scalar sums once, with its first band
             const int oe = a.or_off[b] + job.o;
             const int k = a.or_class[oe];
             atomicAdd(&a.part_acc[8 * b + 0], sw);
@@ -2119,21 +2127,22 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     const unsigned char *xash = a.xa_shell ? a.xa_shell + (size_t)k * a.nshell : nullptr;
     double xa = 0., aa = 0., nc = 0.;
     const int sh = a.ig.xs;
-    // start each block at a different pixel chunk so that concurrent blocks do not all scatter into the
-    // low-frequency voxels at the same time
-    const int n_chunks = (a.ig.P + kBlock - 1) / kBlock;
-    const int c0 = (int)(((unsigned)item * 2654435761u) % (unsigned)n_chunks);
+    // the band's pixels in chunks of kBlock; each block starts at a different chunk so that concurrent blocks do
+    // not all scatter into the same voxels at the same time
+    const int w0 = band * a.n_wpix / a.n_bands, w1 = (band + 1) * a.n_wpix / a.n_bands;
+    const int n_chunks = (w1 - w0 + kBlock - 1) / kBlock;
+    const int c0 = n_chunks > 0 ? (int)(((unsigned)item * 2654435761u) % (unsigned)n_chunks) : 0;
     for (int cc = 0; cc < n_chunks; ++cc) {
         int chunk = c0 + cc;
         if (chunk >= n_chunks) chunk -= n_chunks;
-        const int p = chunk * kBlock + tid;
+        const int ip = w0 + chunk * kBlock + tid;
         long long sc_base = -1;  // trilinear base voxel of this pixel's scatter (-1: none)
         float sc_val[3] = {0.f, 0.f, 0.f};
         float sc_w[8];
-        if (p < a.ig.P) {
-        int x, y;
-        pixel_xy(p, a.ig.s, a.ig.xs, x, y);
-        const int i = p / a.ig.xs;
+        if (ip < w1) {
+        const int2 pe = a.wpix[ip];
+        const int p = pe.x, x = pe.y & 0xffff, i = pe.y >> 16;
+        const int y = (i < a.ig.xs) ? i : i - a.ig.s;
         const float ctf = (a.do_ctf ? C[p] : 1.f) * scw;
         float2 ref = make_float2(0.f, 0.f);
         if (x * x + y * y <= a.ig.proj_r2_max) {
