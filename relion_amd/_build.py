@@ -35,7 +35,7 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     objs = []
-    for src in ("sampling.cpp", "synth.cpp", "formats.cpp"):
+    for src in ("sampling.cpp", "synth.cpp", "formats.cpp", "refine_args.cpp"):
         o = os.path.join(OBJ, src.replace(".cpp", ".o"))
         _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -252,7 +252,8 @@ _PARTICLE = (("rlnDefocusU", "defocus_u", 0.0), ("rlnDefocusV", "defocus_v", Non
              ("rlnNormCorrection", "norm_correction", 1.0), ("rlnOriginXAngst", "origin_x_A", 0.0),
              ("rlnOriginYAngst", "origin_y_A", 0.0), ("rlnAngleRot", "rot", 0.0), ("rlnAngleTilt", "tilt", 0.0),
              ("rlnAnglePsi", "psi", 0.0), ("rlnOpticsGroup", "optics_group", 1.0),
-             ("rlnRandomSubset", "random_subset", 0.0))
+             ("rlnRandomSubset", "random_subset", 0.0), ("rlnOriginXPriorAngst", "origin_x_prior_A", 0.0),
+             ("rlnOriginYPriorAngst", "origin_y_prior_A", 0.0))
 
 
 class ParticleSet:
