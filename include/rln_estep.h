@@ -174,6 +174,13 @@ int rln_backprojector_init_mdl(rln_backprojector *b);
 /* AccBackprojector::getMdlData(real, imag, weight) (acc_backprojector_impl.h:109-138):
  * synchronise and copy to host float arrays of xdim*ydim*zdim elements. */
 int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real, float *imag, float *weight);
+/* No counterpart in the reference: rln_backprojector_get_mdl_data without any device allocation, for callers that
+ * cannot allocate (code under graph capture, serving loops, harnesses that forbid hipMalloc). Same outputs, bit for
+ * bit: the slots are folded a few brick layers (4 planes each) at a time into a static 32 MiB staging array per
+ * device and copied out layer by layer. Limit: 4 planes of re, im and weight must fit the staging, i.e.
+ * xdim * ydim <= 699,050 (box up to about 580 at padding 2); larger volumes return RLN_ERR_ARG. All calls on one
+ * device share its staging array: calls that overlap on one device race. */
+int rln_backprojector_get_mdl_data_noalloc(rln_backprojector *b, float *real, float *imag, float *weight);
 /* BackProjector::reconstruct (src/backprojector.cpp:1379-1780) for a 3-D reference, 2-D data and trilinear
  * interpolation, the default path (window to the original size): from the data and weight accumulated in b, on the
  * device. do_map: the weight gains 1 / (pad^3 * tau2_fudge * tau2[ires]) (1 / (0.001 * weight) where tau2 is 0) for

@@ -143,6 +143,7 @@ _SIGNATURES = {
                                                 C.POINTER(C.c_size_t)]),
     "rln_backprojector_init_mdl": (C.c_int, [_vp]),
     "rln_backprojector_get_mdl_data": (C.c_int, [_vp, _fp, _fp, _fp]),
+    "rln_backprojector_get_mdl_data_noalloc": (C.c_int, [_vp, _fp, _fp, _fp]),
     "rln_backprojector_destroy": (C.c_int, [_vp]),
     "rln_backprojector_reconstruct": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_double, C.c_double,
                                                 C.c_int, _fp, C.c_int]),
@@ -490,6 +491,16 @@ class AccBackprojector:
         w = np.zeros_like(re)
         _check(lib.rln_backprojector_get_mdl_data(self._h, _ptr(re, C.c_float), _ptr(im, C.c_float),
                                                   _ptr(w, C.c_float)))
+        return re, im, w
+
+    def getMdlDataNoAlloc(self):
+        """getMdlData through the entry point that allocates no device memory (a static staging array)."""
+        xd, yd, zd = self.dims[:3]
+        re = np.zeros((zd, yd, xd), np.float32)
+        im = np.zeros_like(re)
+        w = np.zeros_like(re)
+        _check(lib.rln_backprojector_get_mdl_data_noalloc(self._h, _ptr(re, C.c_float), _ptr(im, C.c_float),
+                                                          _ptr(w, C.c_float)))
         return re, im, w
 
     def clear(self):

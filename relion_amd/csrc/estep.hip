@@ -55,7 +55,7 @@ int pack_device_into_projector(rln_projector *p, const float2 *dev_plain, hipStr
 int fold_backprojector_device(rln_backprojector *b, float *re, float *im, float *w, hipStream_t st) {
     const rlnk::VolGeom g{b->dims.xdim, b->dims.ydim, b->dims.zdim, b->dims.inity, b->dims.initz};
     hipLaunchKernelGGL(rlnk::k_fold_slots, dim3(rlnk::fold_blocks(g)), dim3(rlnk::kBlock), 0, st,
-                       b->slots.as<const float>(), g, re, im, w);
+                       b->slots.as<const float>(), g, 0, re, im, w);
     HIP_TRY(hipGetLastError());
     return RLN_OK;
 }
@@ -1763,12 +1763,72 @@ extern "C" int rln_backprojector_get_mdl_data(rln_backprojector *b, float *real,
     float *d_re = plain.as<float>(), *d_im = d_re + nv, *d_w = d_im + nv;
     const rlnk::VolGeom g{b->dims.xdim, b->dims.ydim, b->dims.zdim, b->dims.inity, b->dims.initz};
     hipLaunchKernelGGL(rlnk::k_fold_slots, dim3(rlnk::fold_blocks(g)), dim3(rlnk::kBlock), 0, 0,
-                       b->slots.as<const float>(), g, d_re, d_im, d_w);
+                       b->slots.as<const float>(), g, 0, d_re, d_im, d_w);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     if (real) HIP_TRY(hipMemcpy(real, d_re, sizeof(float) * nv, hipMemcpyDeviceToHost));
     if (imag) HIP_TRY(hipMemcpy(imag, d_im, sizeof(float) * nv, hipMemcpyDeviceToHost));
     if (weight) HIP_TRY(hipMemcpy(weight, d_w, sizeof(float) * nv, hipMemcpyDeviceToHost));
+    return RLN_OK;
+}
+
+// Staging of rln_backprojector_get_mdl_data_noalloc: brick layers of the folded data, imaginary and weight volumes.
+// Each device holds its own copy until the process ends. Host code passes the device address that
+// hipGetSymbolAddress returns for the current device (kept per device below), never the symbol itself.
+static constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) float scratch[kNoAllocScratchBytes / sizeof(float)];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of this device's scratch array; looked up once per device, with that device current.
+int scratch_on_device(int device, float **out) {
+    if (device < 0 || device >= kScratchDevices) RLN_FAIL(RLN_ERR_ARG, "device %d beyond %d", device, kScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) RLN_FAIL(RLN_ERR_HIP, "no device address for the staging array on device %d", device);
+        g_scratch_ptr[device] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return RLN_OK;
+}
+}  // namespace
+
+extern "C" int rln_backprojector_get_mdl_data_noalloc(rln_backprojector *b, float *real, float *imag, float *weight) {
+    if (!b || !b->slots.ptr) RLN_FAIL(RLN_ERR_STATE, "backprojector not dimensioned");
+    const rln::MdlDims &d = b->dims;
+    const size_t plane = (size_t)d.xdim * d.ydim;
+    const size_t layer_bytes = 3 * sizeof(float) * 4 * plane;  // one brick layer (4 planes) of re, im, weight
+    if (layer_bytes > kNoAllocScratchBytes)
+        RLN_FAIL(RLN_ERR_ARG, "rln_backprojector_get_mdl_data_noalloc: 4 planes of %d x %d voxels need %zu bytes of "
+                 "staging, beyond its %zu (xdim * ydim <= %zu)", d.xdim, d.ydim, layer_bytes, kNoAllocScratchBytes,
+                 kNoAllocScratchBytes / (12 * sizeof(float)));
+    HIP_TRY(hipSetDevice(b->device));
+    float *sp = nullptr;
+    RLN_TRY(scratch_on_device(b->device, &sp));
+    HIP_TRY(hipDeviceSynchronize());
+    const rlnk::VolGeom g{d.xdim, d.ydim, d.zdim, d.inity, d.initz};
+    const int n_layers = (d.zdim + 3) / 4;
+    const int per_pass = (int)(kNoAllocScratchBytes / layer_bytes);
+    for (int l0 = 0; l0 < n_layers; l0 += per_pass) {
+        const int nl = std::min(per_pass, n_layers - l0);
+        const int z0 = 4 * l0, nz = std::min(4 * nl, d.zdim - z0);
+        const size_t stride = 4 * (size_t)nl * plane;  // floats per component in the staging array
+        float *d_re = sp, *d_im = sp + stride, *d_w = sp + 2 * stride;
+        hipLaunchKernelGGL(rlnk::k_fold_slots, dim3(rlnk::fold_blocks(g, nl)), dim3(rlnk::kBlock), 0, 0,
+                           b->slots.as<const float>(), g, l0, d_re, d_im, d_w);
+        HIP_TRY(hipGetLastError());
+        // synchronous copies on the null stream: the next pass overwrites the staging only after they complete
+        const size_t off = (size_t)z0 * plane, bytes = sizeof(float) * (size_t)nz * plane;
+        if (real) HIP_TRY(hipMemcpy(real + off, d_re, bytes, hipMemcpyDeviceToHost));
+        if (imag) HIP_TRY(hipMemcpy(imag + off, d_im, bytes, hipMemcpyDeviceToHost));
+        if (weight) HIP_TRY(hipMemcpy(weight + off, d_w, bytes, hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(hipDeviceSynchronize());
     return RLN_OK;
 }
 

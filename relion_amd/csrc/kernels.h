@@ -42,13 +42,14 @@ constexpr int kSlot = 32;
 // Plain data/weight volumes from the slots: v(z,y,x) = sum over corners c (in order) of slot[base(z,y,x) - c][c].
 // One block per 4 x 4 x 4 brick of output voxels: the slots of the 5 x 5 x 5 bases it needs (the brick's own and
 // the lower neighbours' faces) are read into LDS, 24 floats per base, then 192 threads form the 64 voxels' re, im
-// and weight. grid (ceil(xdim/4) * ceil(ydim/4) * ceil(zdim/4)).
-__global__ void __launch_bounds__(kBlock) k_fold_slots(const float *__restrict__ slots, VolGeom g,
+// and weight. Bricks from layer bz0 on (z from 4 bz0): the outputs hold planes z - 4 bz0, so a slab of layers can be
+// folded into a smaller buffer. grid (ceil(xdim/4) * ceil(ydim/4) * layers), at most ceil(zdim/4) - bz0 layers.
+__global__ void __launch_bounds__(kBlock) k_fold_slots(const float *__restrict__ slots, VolGeom g, int bz0,
                                                        float *__restrict__ re, float *__restrict__ im,
                                                        float *__restrict__ w) {
     __shared__ float s[125][25];  // [base][re c0..7 | im c0..7 | weight c0..7], padded against bank conflicts
     const int obx = (g.xdim + 3) >> 2, oby = (g.ydim + 3) >> 2;
-    const int bx = blockIdx.x % obx, by = (blockIdx.x / obx) % oby, bz = blockIdx.x / (obx * oby);
+    const int bx = blockIdx.x % obx, by = (blockIdx.x / obx) % oby, bz = bz0 + (int)(blockIdx.x / (obx * oby));
     for (int q = threadIdx.x; q < 125 * 24; q += kBlock) {
         const int bi = q / 24, f = q - bi * 24;
         const int x0 = 4 * bx - 1 + bi % 5, y0 = 4 * by - 1 + (bi / 5) % 5, z0 = 4 * bz - 1 + bi / 25;
@@ -70,11 +71,13 @@ __global__ void __launch_bounds__(kBlock) k_fold_slots(const float *__restrict__
         sum += s[bi][8 * comp + c];
     }
     float *out = comp == 0 ? re : (comp == 1 ? im : w);
-    out[((size_t)z * g.ydim + y) * g.xdim + x] = sum;
+    out[((size_t)(z - 4 * bz0) * g.ydim + y) * g.xdim + x] = sum;
 }
 
-inline unsigned fold_blocks(const VolGeom &g) {
-    return (unsigned)(((g.xdim + 3) / 4) * ((g.ydim + 3) / 4) * ((g.zdim + 3) / 4));
+// Blocks of k_fold_slots for `layers` brick layers (all of them by default).
+inline unsigned fold_blocks(const VolGeom &g, int layers = -1) {
+    if (layers < 0) layers = (g.zdim + 3) / 4;
+    return (unsigned)(((g.xdim + 3) / 4) * ((g.ydim + 3) / 4) * layers);
 }
 
 
