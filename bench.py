@@ -69,6 +69,11 @@ def parse():
     ap.add_argument("--lists-in", default=None,
                     help="replay saved orientation lists instead of training and running the proposal network "
                          "(the E-step workload of the run that saved them; for counter-collection runs)")
+    ap.add_argument("--fine-order", type=int, default=-1,
+                    help="order of the fine pass's work units (0 particle order, 1 orientation order band-major, "
+                         "2 orientation order item-major; -1 tuned)")
+    ap.add_argument("--wsum-item-major", type=int, default=0,
+                    help="weighted sums' unit order: 0 band-major, 1 item-major")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the fine stage directly instead of through captured graphs (same kernels)")
     ap.add_argument("--classify-particles", type=int, default=1000,
@@ -93,6 +98,30 @@ def parse():
 def log(msg, args):
     if not args.quiet:
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def engine(cfg, args):
+    """An E-step engine on device 0 with the run's launch options (graphs, unit orders)."""
+    import relion_amd as ra
+    es = ra.ExpectationStep(cfg, 0)
+    if args.no_graphs:
+        es.use_graphs(False)
+    es.set_fine_order(args.fine_order, args.wsum_item_major)
+    return es
+
+
+def tuning_summary(es):
+    """The fine pass's tuning on the first chunk: the fastest time (ms) per unit order and the choice kept."""
+    t = es.fine_tuning()
+    if t is None:
+        return None
+    names = ("particle order, band-major", "orientation order, band-major", "orientation order, item-major")
+    best = {names[o]: (round(float(t[:, o][t[:, o] >= 0].min()), 3) if (t[:, o] >= 0).any() else None)
+            for o in range(t.shape[1])}
+    c = es.counters()
+    return {"fastest_ms_by_order": best, "kept": {"order": names[c["fine_order"]], "bands": c["fine_bands"],
+                                                   "grid": c["fine_grid"], "small_tc": c["fine_small_tc"],
+                                                   "wide_cw": c["fine_wide_cw"]}}
 
 
 def rotm(a, b, c):
@@ -163,7 +192,7 @@ def uneven_leg(args, n, steps, warmup):
     proj.initMdl(vol)
     bp = ra.AccBackprojector(0)
     bp.setMdlDim(*dims, 2.0)
-    es = ra.ExpectationStep(cfg, 0)
+    es = engine(cfg, args)
     pdf_dir = np.full(n_dir, 1.0 / n_dir)
 
     def step():
@@ -228,7 +257,7 @@ def classify_leg(args, n):
         cfg = ra.make_config(box, px, order, offset_range=6 * px, offset_step=px, coarse_size=coarse, nr_classes=K,
                              precision=prec)
         n_dir, _, _ = ra.sampling_counts(cfg)
-        es = ra.ExpectationStep(cfg, 0)
+        es = engine(cfg, args)
         pdf_dir = np.full((K, n_dir), 1.0 / (K * n_dir))
 
         def step():
@@ -550,9 +579,7 @@ def main():
             raise SystemExit(f"{args.lists_in} holds lists for {len(replay[0]) - 1} particles, not {n}")
     if "proposal" in paths and replay is not None:
         cfg_g.precision = ra.RLN_PREC_SPLIT_BF16
-        es_p = ra.ExpectationStep(cfg_g, 0)
-        if args.no_graphs:
-            es_p.use_graphs(False)
+        es_p = engine(cfg_g, args)
 
         def proposal_step(count=n, mt=meta):
             bp.initMdl()
@@ -563,6 +590,7 @@ def main():
             return res, off[:count + 1]
 
         runs["proposal"] = timed("proposal (replayed lists)", proposal_step, n, args.steps, args.warmup, es_p)
+        extra["fine_tuning"] = tuning_summary(es_p)
         extra["proposal_lists_replayed_from"] = args.lists_in
     elif "proposal" in paths:
         pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=args.seed)
@@ -585,9 +613,7 @@ def main():
                                       "batch": 256, "generate_s": round(t_gen, 2), "train_s": round(t_train, 2),
                                       "last_epoch_loss": round(loss, 4)}
         cfg_g.precision = ra.RLN_PREC_SPLIT_BF16
-        es_p = ra.ExpectationStep(cfg_g, 0)
-        if args.no_graphs:
-            es_p.use_graphs(False)
+        es_p = engine(cfg_g, args)
         pred_ms = []
 
         def proposal_step(count=n, mt=meta):
@@ -603,6 +629,7 @@ def main():
 
         runs["proposal"] = timed("proposal", proposal_step, n, args.steps, args.warmup, es_p)
         extra["proposal_predict_ms"] = round(float(np.mean(pred_ms[-args.steps:])), 2)
+        extra["fine_tuning"] = tuning_summary(es_p)
 
         def pipelined_steps(steps):
             """`steps` proposal steps over the batch, the inference and orientation lists of step k + 1 running
@@ -678,9 +705,7 @@ def main():
     # ---------------------------------------------------------------- exhaustive plain FP32 baseline
     if "plain" in paths:
         cfg_g.precision = ra.RLN_PREC_PLAIN_FP32
-        es_x = ra.ExpectationStep(cfg_g, 0)
-        if args.no_graphs:
-            es_x.use_graphs(False)
+        es_x = engine(cfg_g, args)
 
         def plain_step():
             bp.initMdl()
@@ -766,9 +791,7 @@ def main():
     if "local" in paths:
         for name, prec in (("local_plain", ra.RLN_PREC_PLAIN_FP32), ("local_split", ra.RLN_PREC_SPLIT_BF16)):
             cfg_l.precision = prec
-            es_l = ra.ExpectationStep(cfg_l, 0)
-            if args.no_graphs:
-                es_l.use_graphs(False)
+            es_l = engine(cfg_l, args)
 
             def local_step():
                 bp.initMdl()

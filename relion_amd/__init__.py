@@ -157,6 +157,7 @@ _SIGNATURES = {
     "rln_estep_use_graphs": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_pool": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_fine_bands": (C.c_int, [_vp, C.c_int]),
+    "rln_estep_set_fine_order": (C.c_int, [_vp, C.c_int, C.c_int]),
     "rln_estep_fine_tuning": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(C.c_int)]),
     "rln_estep_coarse_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _ip, _ip, _ip]),
     "rln_estep_get_coarse": (C.c_int, [_vp, C.c_int, _fp, _dp, C.POINTER(C.c_ubyte)]),
@@ -567,16 +568,22 @@ class ExpectationStep:
         """Radial pixel bands per fine work item (1, 2, 4, 8, 16, 32), or 0 for the engine's own tuning."""
         _check(lib.rln_estep_set_fine_bands(self._h, int(n_bands)))
 
+    def set_fine_order(self, order=-1, wsum_item_major=0):
+        """Order of the fine pass's work units (0: particle order, band-major; 1: orientation order, band-major;
+        2: orientation order, item-major; -1: the engine's tuning) and of the weighted sums' units (0 band-major,
+        1 item-major)."""
+        _check(lib.rln_estep_set_fine_order(self._h, int(order), int(wsum_item_major)))
+
     def fine_tuning(self):
-        """Times (ms) of the fine pass's tuning candidates as an array [shape][band choice][grid] (-1: not tried),
-        or None before any tuning."""
+        """Times (ms) of the fine pass's tuning candidates as an array [shape][order][band choice][grid] (-1: not
+        tried), or None before any tuning."""
         n = C.c_int()
         _check(lib.rln_estep_fine_tuning(self._h, None, 0, C.byref(n)))
         if n.value == 0:
             return None
         ms = np.zeros(n.value)
         _check(lib.rln_estep_fine_tuning(self._h, _ptr(ms, C.c_double), n.value, C.byref(n)))
-        return ms.reshape(4, 6, 5)
+        return ms.reshape(4, 3, 6, 5)
 
     def set_orientation_lists(self, offsets, orients):
         """set_orientation_sets in the flat form: offsets[n+1], orients[offsets[n]] (int32)."""
@@ -659,7 +666,8 @@ class ExpectationStep:
     COUNTERS = ("particles", "coarse_poses", "coarse_pixels", "fine_poses", "fine_pixels", "fine_items",
                 "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks",
                 "fine_graph_launches", "fine_capacity_retries", "fine_grid", "wsum_grid",
-                "fine_pixels_sampled", "fine_bands", "fine_small_tc", "fine_wide_cw")
+                "fine_pixels_sampled", "fine_bands", "fine_small_tc", "fine_wide_cw", "fine_order",
+                "wsum_item_major")
 
     def counters(self):
         out = np.zeros(len(self.COUNTERS), np.int64)

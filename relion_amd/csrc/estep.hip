@@ -123,6 +123,14 @@ inline const void *fine_kernel_ptr(int shape) {
 constexpr int kBandChoices[] = {1, 2, 4, 8, 16, 32};
 constexpr int kNumBandChoices = (int)(sizeof(kBandChoices) / sizeof(kBandChoices[0]));
 
+// Orders of the fine pass's work units: jobs in particle order or sorted by class and coarse orientation
+// (k_fine_items), units band-major or item-major (k_fine_diff2).
+struct FineOrder {
+    int sorted, item_major;
+};
+constexpr FineOrder kFineOrders[] = {{0, 0}, {1, 0}, {1, 1}};
+constexpr int kNumFineOrders = (int)(sizeof(kFineOrders) / sizeof(kFineOrders[0]));
+
 static std::vector<int2> banded_lists(const std::vector<int> &pix, int s, int xs) {
     const int ns = (int)pix.size();
     auto r2 = [&](int p) {
@@ -219,6 +227,10 @@ struct Engine {
     int grid_fine = 1024, grid_wsum = 2048;  // persistent grids (fine: tuned; weighted sums: resident blocks)
     int band_choice = 0;                     // index into kBandChoices of the fine pass's band count (tuned)
     int fine_shape = 0;                      // index into rlnk::kFineShapes of the work-item shape (tuned)
+    int fine_order = 0;                      // order of the fine units (kFineOrders; tuned)
+    int order_fixed = -1;                    // fine order set by rln_estep_set_fine_order (-1: tuned)
+    int wsum_item_major = 0;                 // unit order of the weighted sums: 0 band-major, 1 item-major
+    DevBuf d_job_order;                      // jobs in the fine pass's order (k_fine_items)
     int resident_shape[rlnk::kNumFineShapes] = {};  // resident blocks of each shape's kernel on the device
     DevBuf d_band_pix;                       // sampled fine pixels, one banded list per band choice
     DevBuf d_wsum_pix;                       // pixels of the weighted sums, one banded list per band choice
@@ -248,7 +260,7 @@ struct Engine {
     hipEvent_t ev[kStCount + 1] = {};
     double stage_ms[kStCount] = {};
     double host_ms[4] = {};  // host phases: chunk preparation, of it the first chunk's lists, epilogue, whole run
-    long long counters[20] = {};
+    long long counters[22] = {};
     long long pf_sampled = 0;  // fine pixels gathered per fine orientation
     std::vector<double> tune_ms;  // fine-pass tuning: time of each (band choice, grid) candidate, -1 untried
     bool have_timing = false;
@@ -930,6 +942,7 @@ static int ensure_fine_capacity(Engine &e) {
     if (e.need_wide == 0) e.need_wide = 16 * cB;
     RLN_TRY(e.d_pairs.ensure(sizeof(int) * e.need_pairs));
     RLN_TRY(e.d_jobs.ensure(sizeof(rlnk::FineJob) * e.need_jobs));
+    RLN_TRY(e.d_job_order.ensure(sizeof(int) * e.need_jobs));
     RLN_TRY(e.d_items_s.ensure(sizeof(int2) * e.need_small));
     RLN_TRY(e.d_items_w.ensure(sizeof(int2) * e.need_wide));
     RLN_TRY(e.d_fdiff2.ensure(sizeof(double) * 32 * e.need_pairs));
@@ -972,6 +985,9 @@ struct FineLaunch {
     const int *or_cnt, *oc_off, *ocnt;
     int NT;
     rlnk::FineJob *jobs;
+    int *job_order;
+    const int *or_off, *or_frame, *or_class;
+    int n_or_grid, key_shift, sorted;  // job keys (class, coarse orientation) >> key_shift; sorted: orientation order
     int2 *items_s, *items_w;
     const double *fbase;
     int shape;  // work-item shape (index into rlnk::kFineShapes)
@@ -995,7 +1011,8 @@ static void launch_plan(Engine &e, const FineLaunch &L) {
                        (const rlnk::FinePlan *)L.plan);
     const rlnk::FineShape sh = rlnk::kFineShapes[L.shape];
     hipLaunchKernelGGL(rlnk::k_fine_items, dim3(1), dim3(rlnk::kWBlock), 0, st, (const rlnk::FineJob *)L.jobs,
-                       sh.tc, 4 * sh.cw, L.cap_small, L.cap_wide, L.items_s, L.items_w, L.plan);
+                       L.or_off, L.or_frame, L.or_class, L.n_or_grid, L.key_shift, L.sorted, sh.tc, 4 * sh.cw,
+                       L.cap_small, L.cap_wide, L.job_order, L.items_s, L.items_w, L.plan);
 }
 
 static int launch_fine(Engine &e, const FineLaunch &L, int part) {
@@ -1021,14 +1038,16 @@ static int launch_fine(Engine &e, const FineLaunch &L, int part) {
     }
 }
 
-// Work-item shape, work-unit size and persistent grid of the fine diff2 kernel, timed on the first chunk with
-// enough work items, one parameter after the other at the best values found so far: the work-item shape (coarse
-// translations per small item and per wave of a wide item, which set the register tiles and the projections
-// shared per item; the plan's items are rebuilt for each shape), the band count (pixels per unit: all sampled
-// pixels, or the radial shells of 1/2 .. 1/32 of them) and the grid (512 .. 4096 blocks or the shape's
-// resident-block count). The fastest triple is kept for the engine's life. The kernel only adds into fine diff2,
-// which the production pass initialises again, so repeating it is harmless; its unit counter is reset before every
-// candidate. tune_ms keeps every candidate's time ([shape][band][grid], -1 untried) for rln_estep_tuning.
+// Work-item shape, unit order, work-unit size and persistent grid of the fine diff2 kernel, timed on the first chunk
+// with enough work items, one parameter after the other at the best values found so far: the work-item shape
+// (coarse translations per small item and per wave of a wide item, which set the register tiles and the projections
+// shared per item), the order of the units (jobs in particle order or in orientation order, units band-major or
+// item-major; kFineOrders), the band count (pixels per unit: all sampled pixels, or the radial shells of
+// 1/2 .. 1/32 of them) and the grid (512 .. 4096 blocks or the shape's resident-block count). The plan's items are
+// rebuilt for each shape and order. The fastest combination is kept for the engine's life. The kernel only adds into
+// fine diff2, which the production pass initialises again, so repeating it is harmless; its unit counter is reset
+// before every candidate. tune_ms keeps every candidate's time ([shape][order][band][grid], -1 untried) for
+// rln_estep_fine_tuning.
 static int tune_fine(Engine &e, FineLaunch L) {
     hipStream_t st = e.stream;
     hipEvent_t a, b;
@@ -1040,11 +1059,13 @@ static int tune_fine(Engine &e, FineLaunch L) {
         const int res = e.resident_shape[sh] > 0 ? e.resident_shape[sh] * e.n_cu : 1024;
         return gi < 4 ? g[gi] : res;
     };
-    e.tune_ms.assign((size_t)rlnk::kNumFineShapes * kNumBandChoices * n_grid, -1.);
-    // plan of a shape; false when it does not fit the capacities (the candidate is skipped) or is too small to tune
-    auto plan_for = [&](int sh, bool &ok) -> int {
+    e.tune_ms.assign((size_t)rlnk::kNumFineShapes * kNumFineOrders * kNumBandChoices * n_grid, -1.);
+    // plan of a shape and order; ok false when it does not fit the capacities (the candidate is skipped) or is too
+    // small to tune
+    auto plan_for = [&](int sh, int oi, bool &ok) -> int {
         FineLaunch P = L;
         P.shape = sh;
+        P.sorted = kFineOrders[oi].sorted;
         launch_plan(e, P);
         RLN_TRY(launch_check("fine plan"));
         rlnk::FinePlan plan{};
@@ -1053,9 +1074,10 @@ static int tune_fine(Engine &e, FineLaunch L) {
         ok = !plan.overflow && plan.n_small + plan.n_wide >= 512;
         return RLN_OK;
     };
-    auto time_one = [&](int sh, int ci, int gi, float &ms) -> int {
+    auto time_one = [&](int sh, int oi, int ci, int gi, float &ms) -> int {
         rlnk::FineArgs fa = L.fa;
         fa.n_bands = kBandChoices[ci];
+        fa.item_major = kFineOrders[oi].item_major;
         fa.pix = e.d_band_pix.as<const int2>() + (size_t)ci * e.pf_sampled;
         hipLaunchKernelGGL(rlnk::k_reset_items, dim3(1), dim3(64), 0, st, L.plan);
         HIP_TRY(hipEventRecord(a, st));
@@ -1063,11 +1085,12 @@ static int tune_fine(Engine &e, FineLaunch L) {
         HIP_TRY(hipEventRecord(b, st));
         HIP_TRY(hipEventSynchronize(b));
         HIP_TRY(hipEventElapsedTime(&ms, a, b));
-        e.tune_ms[((size_t)sh * kNumBandChoices + ci) * n_grid + gi] = ms;
+        e.tune_ms[(((size_t)sh * kNumFineOrders + oi) * kNumBandChoices + ci) * n_grid + gi] = ms;
         return RLN_OK;
     };
+    int best_o = e.order_fixed >= 0 ? e.order_fixed : 0;
     bool ok = false;
-    RLN_TRY(plan_for(0, ok));
+    RLN_TRY(plan_for(0, best_o, ok));
     if (!ok) {  // retried with more room, or too small to be worth tuning
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
@@ -1081,21 +1104,35 @@ static int tune_fine(Engine &e, FineLaunch L) {
     float best = 1e30f;
     for (int sh = 0; sh < rlnk::kNumFineShapes; ++sh) {
         if (sh > 0) {
-            RLN_TRY(plan_for(sh, ok));
+            RLN_TRY(plan_for(sh, best_o, ok));
             if (!ok) continue;
         }
         float ms = 0.f;
-        RLN_TRY(time_one(sh, best_c, best_g, ms));
+        RLN_TRY(time_one(sh, best_o, best_c, best_g, ms));
         if (ms < best) {
             best = ms;
             best_s = sh;
         }
     }
-    RLN_TRY(plan_for(best_s, ok));
+    if (e.order_fixed < 0) {
+        const int o0 = best_o;
+        for (int oi = 0; oi < kNumFineOrders; ++oi) {
+            if (oi == o0) continue;
+            RLN_TRY(plan_for(best_s, oi, ok));
+            if (!ok) continue;
+            float ms = 0.f;
+            RLN_TRY(time_one(best_s, oi, best_c, best_g, ms));
+            if (ms < best) {
+                best = ms;
+                best_o = oi;
+            }
+        }
+    }
+    RLN_TRY(plan_for(best_s, best_o, ok));
     for (int ci = 0; ci < kNumBandChoices; ++ci) {
         if (ci == best_c || (kBandChoices[ci] > 1 && e.pf_sampled / kBandChoices[ci] < 256)) continue;
         float ms = 0.f;
-        RLN_TRY(time_one(best_s, ci, best_g, ms));
+        RLN_TRY(time_one(best_s, best_o, ci, best_g, ms));
         if (ms < best) {
             best = ms;
             best_c = ci;
@@ -1103,7 +1140,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
     }
     for (int gi = 0; gi + 1 < n_grid; ++gi) {
         float ms = 0.f;
-        RLN_TRY(time_one(best_s, best_c, gi, ms));
+        RLN_TRY(time_one(best_s, best_o, best_c, gi, ms));
         if (ms < best) {
             best = ms;
             best_g = gi;
@@ -1113,6 +1150,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
     (void)hipEventDestroy(b);
     RLN_TRY(launch_check("fine tuning"));
     e.fine_shape = best_s;
+    e.fine_order = best_o;
     e.band_choice = best_c;
     e.grid_fine = grid_of(best_s, best_g);
     e.tuned = true;
@@ -1134,6 +1172,8 @@ static int run_fine_stage(Engine &e, const FineLaunch &L0) {
         RLN_TRY(tune_fine(e, L));
         L.grid_fine = e.grid_fine;
         L.shape = e.fine_shape;
+        L.sorted = kFineOrders[e.fine_order].sorted;
+        L.fa.item_major = kFineOrders[e.fine_order].item_major;
         L.fa.n_bands = kBandChoices[e.band_choice];
         L.ws.n_bands = kBandChoices[e.band_choice];
         L.ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)e.band_choice * e.n_wsum_pix;
@@ -1407,6 +1447,18 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         L.ocnt = e.d_ocnt.as<const int>();
         L.NT = NT;
         L.jobs = e.d_jobs.as<rlnk::FineJob>();
+        L.job_order = e.d_job_order.as<int>();
+        L.or_off = e.d_or_off.as<const int>();
+        L.or_frame = e.d_or_frame.as<const int>();
+        L.or_class = e.d_or_class.as<const int>();
+        L.n_or_grid = e.grid.n_dir * e.grid.n_psi;
+        {
+            const long long n_keys = (long long)K * L.n_or_grid;
+            int shift = 0;
+            while (((n_keys - 1) >> shift) >= rlnk::kJobKeys) ++shift;
+            L.key_shift = shift;
+        }
+        L.sorted = kFineOrders[e.fine_order].sorted;
         L.items_s = e.d_items_s.as<int2>();
         L.items_w = e.d_items_w.as<int2>();
         rlnk::FineArgs &fa = L.fa;
@@ -1433,6 +1485,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         fa.pix = e.d_band_pix.as<const int2>() + (size_t)e.band_choice * e.pf_sampled;
         fa.n_pix = (int)e.pf_sampled;
         fa.n_bands = kBandChoices[e.band_choice];
+        fa.item_major = kFineOrders[e.fine_order].item_major;
         L.fbase = e.d_fbase.as<const double>();
         L.shape = e.fine_shape;
         fa.NT = NT;
@@ -1478,6 +1531,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)e.band_choice * e.n_wsum_pix;
         ws.n_wpix = e.n_wsum_pix;
         ws.n_bands = kBandChoices[e.band_choice];
+        ws.item_major = e.wsum_item_major;
+        ws.job_order = e.d_job_order.as<const int>();
         ws.xa_shell = (cfg.do_scale_correction && e.d_xa_shell.ptr) ? e.d_xa_shell.as<const unsigned char>() : nullptr;
         ws.ex = e.d_ex_f.as<const float2>();
         ws.ey = e.d_ey_f.as<const float2>();
@@ -1603,6 +1658,8 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[17] = kBandChoices[e.band_choice];
     e.counters[18] = rlnk::kFineShapes[e.fine_shape].tc;
     e.counters[19] = rlnk::kFineShapes[e.fine_shape].cw;
+    e.counters[20] = e.fine_order;
+    e.counters[21] = e.wsum_item_major;
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
         if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
@@ -1871,7 +1928,7 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
                            &e->d_or_class, &e->d_or_dir, &e->d_tr_prior, &e->d_ctfq, &e->d_yq, &e->d_tiles,
                            &e->d_diff2, &e->d_weight, &e->d_mask, &e->d_stats_c, &e->d_stats_f, &e->d_npairs,
                            &e->d_njobs, &e->d_nchunks, &e->d_pair_off, &e->d_job_off, &e->d_chunk_off, &e->d_pairs,
-                           &e->d_jobs, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fbase, &e->d_fweight,
+                           &e->d_jobs, &e->d_job_order, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fbase, &e->d_fweight,
                            &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc,
                            &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w, &e->d_wsegs, &e->d_seg_first,
                            &e->d_wpart, &e->d_hcnt, &e->d_hsum, &e->d_wcand, &e->d_segbest, &e->d_ocnt};
@@ -1983,6 +2040,18 @@ extern "C" int rln_estep_set_fine_bands(rln_estep *e, int n_bands) {
             return RLN_OK;
         }
     RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_bands: %d bands (1, 2, 4, 8, 16, 32 or 0)", n_bands);
+}
+
+extern "C" int rln_estep_set_fine_order(rln_estep *e, int order, int wsum_item_major) {
+    if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_order: null engine");
+    if (order < -1 || order >= rln::kNumFineOrders || wsum_item_major < 0 || wsum_item_major > 1)
+        RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_order: order %d (-1 .. %d), weighted-sum order %d (0 or 1)", order,
+                 rln::kNumFineOrders - 1, wsum_item_major);
+    std::lock_guard<std::mutex> lk(e->mtx);
+    e->order_fixed = order;
+    if (order >= 0) e->fine_order = order;
+    e->wsum_item_major = wsum_item_major;
+    return RLN_OK;
 }
 
 extern "C" int rln_estep_set_orientation_sets(rln_estep *e, int n, const int *set_offsets, const int *set_orients) {
@@ -2171,7 +2240,7 @@ extern "C" int rln_estep_get_fine(rln_estep *e, int ip, long long *ihidden_over,
 
 extern "C" int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out) {
     if (!e || !out) RLN_FAIL(RLN_ERR_ARG, "null argument");
-    const int n = std::min(n_max, 20);
+    const int n = std::min(n_max, (int)(sizeof(e->counters) / sizeof(e->counters[0])));
     for (int i = 0; i < n; ++i) out[i] = e->counters[i];
     if (n_out) *n_out = n;
     return RLN_OK;

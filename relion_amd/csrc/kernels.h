@@ -1591,19 +1591,65 @@ __global__ void __launch_bounds__(kWBlock) k_write_jobs(const unsigned char *__r
 // 8 oversampled orientations once per pixel and accumulates the 8 x 4*TC squared differences.
 // --------------------------------------------------------------------------------------------------
 
+// Jobs are keyed for the fine pass's order by class and coarse orientation (direction-major), key >> key_shift so that
+// the keys fit the LDS histogram of k_fine_items.
+constexpr int kJobKeys = 4096;
+
 // Work items of the fine pass: jobs with at most small_max significant translations are one item of
-// the small path of k_fine_diff2; the others one wide item per wide_tc translations. One block of 1024.
-__global__ void __launch_bounds__(kWBlock) k_fine_items(const FineJob *__restrict__ jobs, int small_max, int wide_tc,
-                                                        int cap_small, int cap_wide, int2 *__restrict__ small_items,
+// the small path of k_fine_diff2; the others one wide item per wide_tc translations. With `sorted`, the jobs are first
+// put in the order of their class and coarse orientation (a counting sort over kJobKeys keys; the order within a key
+// is arbitrary), so that consecutive items are jobs of nearby orientations, often of different particles: jobs of one
+// coarse direction sample the same four fine-direction planes of the reference, and items processed together then
+// share its corner blocks in the caches. job_order receives the order (the weighted sums follow it too); without
+// `sorted` it is the identity (particle order). One block of 1024.
+__global__ void __launch_bounds__(kWBlock) k_fine_items(const FineJob *__restrict__ jobs, const int *__restrict__ or_off,
+                                                        const int *__restrict__ or_frame, const int *__restrict__ or_class,
+                                                        int n_or_grid, int key_shift, int sorted, int small_max,
+                                                        int wide_tc, int cap_small, int cap_wide,
+                                                        int *__restrict__ job_order, int2 *__restrict__ small_items,
                                                         int2 *__restrict__ wide_items, FinePlan *__restrict__ plan) {
     __shared__ int tmp[24];
+    __shared__ int hist[kJobKeys];
     if (plan->overflow) return;
     const int n_jobs = plan->n_jobs;
+    if (sorted) {
+        for (int q = threadIdx.x; q < kJobKeys; q += kWBlock) hist[q] = 0;
+        __syncthreads();
+        auto key_of = [&](int j) {
+            const FineJob jb = jobs[j];
+            const int oe = or_off[jb.b] + jb.o;
+            const long long g = (long long)or_class[oe] * n_or_grid + (or_frame[oe] >> 3);
+            return (int)min((long long)(kJobKeys - 1), g >> key_shift);
+        };
+        for (int j = threadIdx.x; j < n_jobs; j += kWBlock) atomicAdd(&hist[key_of(j)], 1);
+        __syncthreads();
+        // exclusive scan of the histogram, kJobKeys / kWBlock consecutive keys per thread
+        constexpr int kPer = kJobKeys / kWBlock;
+        int loc[kPer], sum = 0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            loc[q] = hist[threadIdx.x * kPer + q];
+            sum += loc[q];
+        }
+        int tot;
+        int run = block_exclusive_scan_i(sum, tmp, tot);
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            hist[threadIdx.x * kPer + q] = run;
+            run += loc[q];
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < n_jobs; j += kWBlock) job_order[atomicAdd(&hist[key_of(j)], 1)] = j;
+    } else {
+        for (int j = threadIdx.x; j < n_jobs; j += kWBlock) job_order[j] = j;
+    }
+    __syncthreads();
     int sbase = 0, wbase = 0;
     for (int j0 = 0; j0 < n_jobs; j0 += kWBlock) {
-        const int j = j0 + threadIdx.x;
-        int ns = 0, nw = 0, sc = 0;
-        if (j < n_jobs) {
+        const int jj = j0 + threadIdx.x;
+        int ns = 0, nw = 0, sc = 0, j = 0;
+        if (jj < n_jobs) {
+            j = job_order[jj];
             sc = jobs[j].s_count;
             if (sc <= small_max) ns = 1;
             else nw = (sc + wide_tc - 1) / wide_tc;
@@ -1650,6 +1696,7 @@ struct FineArgs {
                                    // << 16), in n_bands bands of equal count by radius, pixel order within a band
     int n_pix;
     int n_bands;                   // band b: pix[b n_pix / n_bands .. (b + 1) n_pix / n_bands)
+    int item_major;                // unit order: 0 band-major (all items of band 0, then band 1, ...), 1 item-major
     int NT;
     int do_ctf;
     double *diff2;                 // [pair_off[b]*32 + s*32 + r*4 + u]: k_fine_init's base, plus each band's sum
@@ -1942,9 +1989,10 @@ __device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, in
 
 // Fine pass, persistent: blocks fetch work units from plan->next_fine until none is left, so uneven items spread
 // over the GPU; every block exits once the counter passes the last unit. A unit is one work item on one band of
-// pixels, band-major (all items of band 0, then band 1, ...; within a band the wide items first, the heaviest, then
-// the small ones): the units in flight at any time sample one or two radial shells of the reference, a fraction of
-// the packed volume that the memory-side cache can hold, instead of the whole volume. Each unit adds its partial
+// pixels, either band-major (all items of band 0, then band 1, ...; within a band the wide items first, the heaviest,
+// then the small ones), so that the units in flight sample one or two radial shells of the reference, or item-major
+// (the bands of an item one after the other), so that with items in orientation order (k_fine_items) the units in
+// flight sample the planes of few coarse directions. Each unit adds its partial
 // sums to the FP64 diff2 (k_fine_init wrote the pose-independent part). Three waves per SIMD, as the two item
 // paths have when compiled alone (at most 168 VGPRs). kTuning only separates the tuning launches from the
 // production ones in kernel traces.
@@ -1967,7 +2015,14 @@ __global__ void __launch_bounds__(kBlock, 3) k_fine_diff2(FineArgs a) {
         // are not linearised and do not hold their registers at the same time
         const int unit = __builtin_amdgcn_readfirstlane(s_item);
         if (unit >= n_units) break;
-        const int band = unit / n_items, item = unit - band * n_items;
+        int band, item;
+        if (a.item_major) {
+            item = unit / a.n_bands;
+            band = unit - item * a.n_bands;
+        } else {
+            band = unit / n_items;
+            item = unit - band * n_items;
+        }
         if (item < n_wide) {
             const int2 it = a.items_w[item];
             fine_wide_item<CW>(a, it.x, it.y, band, lds.w);
@@ -1993,7 +2048,7 @@ __global__ void __launch_bounds__(kBlock) k_zero_d(double *__restrict__ p, long 
 
 // --------------------------------------------------------------------------------------------------
 // Weighted sums and backprojection (storeWeightedSums, src/ml_optimiser.cpp:8241-9156, with
-// BackProjector::backproject2Dto3D, src/backprojector.cpp:55-357). One block per (job, iover_rot).
+// BackProjector::backproject2Dto3D, src/backprojector.cpp:55-357). Persistent blocks over (job, pixel band) units.
 // --------------------------------------------------------------------------------------------------
 constexpr int kMaxSigT = 1024;  // fine translations of one orientation (4 per coarse translation); host checks 4*NT
 
@@ -2011,6 +2066,7 @@ struct WsumArgs {
     ImgGeom ig;
     const ChunkPtrs *cp;            // this chunk's images
     FinePlan *plan;                 // items: plan->n_jobs * 8, fetched through plan->next_wsum
+    const int *job_order;           // jobs in the fine pass's order (k_fine_items)
     const float *scale_w;           // max(scale, 0.001) (src/ml_optimiser.cpp:8402-8416)
     const float *minv_w;            // Minvsigma2 with the origin restored
     const int *mresol;              // resolution shell per pixel, -1 excluded
@@ -2018,6 +2074,7 @@ struct WsumArgs {
                                     // n_bands bands of equal count by radius, pixel order within a band
     int n_wpix;
     int n_bands;
+    int item_major;                 // unit order: 0 band-major, 1 item-major (the bands of an item in a row)
     const unsigned char *xa_shell;  // [K][nshell] data_vs_prior > 3 (scale correction) or null
     const float2 *ex;
     const float2 *ey;
@@ -2043,10 +2100,10 @@ struct WsumArgs {
     int K;
 };
 
-__global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
+__global__ void __launch_bounds__(kBlock, 5) k_wsum_bp(WsumArgs a) {
     __shared__ int s_ix[kMaxSigT], s_iy[kMaxSigT];
     __shared__ float s_w[kMaxSigT];
-    __shared__ int s_n;
+    __shared__ int s_tmp[24];
     __shared__ float shell[512];
     __shared__ double s_red[16];
     __shared__ long long sx_base[kBlock];
@@ -2054,73 +2111,38 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     __shared__ float sx_w[kBlock][8];
     __shared__ int s_item;
     if (a.plan->overflow) return;
-    const int n_items = 8 * a.plan->n_jobs;
+    const int n_items = a.plan->n_jobs;
     const int n_units = n_items * a.n_bands;
-    // persistent: units fetched dynamically, since only some (job, orientation) items have significant weights; a
-    // unit is one item on one radial band of pixels, band-major, so that the units in flight read and update one
-    // shell of the reference and of the accumulators at a time
+    const int tid = threadIdx.x;
+    // persistent: units fetched dynamically. A unit is one job (a coarse orientation of a particle with its
+    // significant coarse translations) on one radial band of pixels; it visits the job's 8 oversampled orientations
+    // and skips those without a significant fine pose (most of them). Band-major or item-major, in the fine pass's
+    // job order.
     for (;;) {
     __syncthreads();  // previous unit's LDS and s_item consumed
-    if (threadIdx.x == 0) s_item = atomicAdd(&a.plan->next_wsum, 1);
+    if (tid == 0) s_item = atomicAdd(&a.plan->next_wsum, 1);
     __syncthreads();
     const int unit = s_item;
     if (unit >= n_units) break;
-    const int band = unit / n_items, item = unit - band * n_items;
-    const int job_id = item >> 3;
-    const int r = item & 7;
+    int band, item;
+    if (a.item_major) {
+        item = unit / a.n_bands;
+        band = unit - item * a.n_bands;
+    } else {
+        band = unit / n_items;
+        item = unit - band * n_items;
+    }
+    const int job_id = a.job_order[item];
     const FineJob job = a.jobs[job_id];
     const int b = job.b;
-    const int tid = threadIdx.x;
     // a particle whose fine weights failed (zero or NaN sum) contributes nothing: its weights are not
     // normalisable and would turn the shared accumulators into NaN (the run returns the particle's status)
     if (a.stats[b].status != 0) continue;
     const double thr = a.stats[b].threshold;
     const double sumw = a.stats[b].sum_weight;
-    if (tid == 0) s_n = 0;
     for (int i = tid; i < a.nshell; i += kBlock) shell[i] = 0.f;
-    __syncthreads();
-    // significant fine translations of this orientation; scalar sums in FP64 by thread 0 in order
-    const long long base = (a.pair_off[b] + job.s_begin) * 32;
-    if (tid == 0) {
-        double sw = 0., so = 0.;
-        int cnt = 0;
-        const double px2 = a.pixel_size * a.pixel_size;
-        for (int sidx = 0; sidx < job.s_count; ++sidx) {
-            const int t = a.pairs[a.pair_off[b] + job.s_begin + sidx] % a.NT;
-            for (int u = 0; u < 4; ++u) {
-                const double wt = a.weight[base + (long long)sidx * 32 + r * 4 + u];
-                if (!(wt >= thr)) continue;
-                const double wn = wt / sumw;
-                if (cnt < kMaxSigT) {
-                    s_ix[cnt] = a.tr_ix[t * 4 + u];
-                    s_iy[cnt] = a.tr_iy[t * 4 + u];
-                    s_w[cnt] = (float)wn;
-                }
-                ++cnt;
-                sw += wn;
-                const double dx = a.prior_off[2 * b] - a.old_off[2 * b] - a.over_x[t * 4 + u];
-                const double dy = a.prior_off[2 * b + 1] - a.old_off[2 * b + 1] - a.over_y[t * 4 + u];
-                so += wn * px2 * (dx * dx + dy * dy);
-            }
-        }
-        s_n = cnt;
-        if (cnt > 0 && band == 0) {  // the item's scalar sums once, with its first band
-            const int oe = a.or_off[b] + job.o;
-            const int k = a.or_class[oe];
-            atomicAdd(&a.part_acc[8 * b + 0], sw);
-            atomicAdd(&a.part_acc[8 * b + 1], so);
-            atomicAdd(&a.part_acc[8 * b + 5], 1.0);
-            atomicAdd(&a.part_acc[8 * b + 6], (double)cnt);
-            atomicAdd(&a.pdf_class[(size_t)b * a.K + k], sw);
-            atomicAdd(&a.pdf_direction[(size_t)k * a.n_dir + a.or_dir[oe]], sw);
-        }
-    }
-    __syncthreads();
-    const int nsig = s_n;
-    if (nsig == 0) continue;
     const int oe = a.or_off[b] + job.o;
     const int k = a.or_class[oe];
-    const Frame fr = a.fine_frames[a.or_frame[oe] + r];
     const float4 *vol = a.vols[k];
     const float2 *X = a.cp->fimg + (size_t)b * a.ig.P;
     const float2 *Xn = a.cp->fimg_nomask + (size_t)b * a.ig.P;
@@ -2130,125 +2152,182 @@ __global__ void __launch_bounds__(kBlock) k_wsum_bp(WsumArgs a) {
     const unsigned char *xash = a.xa_shell ? a.xa_shell + (size_t)k * a.nshell : nullptr;
     double xa = 0., aa = 0., nc = 0.;
     const int sh = a.ig.xs;
-    // the band's pixels in chunks of kBlock; each block starts at a different chunk so that concurrent blocks do
-    // not all scatter into the same voxels at the same time
+    const long long base = (a.pair_off[b] + job.s_begin) * 32;
+    const int n4 = 4 * job.s_count;  // fine translations of one oversampled orientation
     const int w0 = band * a.n_wpix / a.n_bands, w1 = (band + 1) * a.n_wpix / a.n_bands;
     const int n_chunks = (w1 - w0 + kBlock - 1) / kBlock;
-    const int c0 = n_chunks > 0 ? (int)(((unsigned)item * 2654435761u) % (unsigned)n_chunks) : 0;
-    for (int cc = 0; cc < n_chunks; ++cc) {
-        int chunk = c0 + cc;
-        if (chunk >= n_chunks) chunk -= n_chunks;
-        const int ip = w0 + chunk * kBlock + tid;
-        long long sc_base = -1;  // trilinear base voxel of this pixel's scatter (-1: none)
-        float sc_val[3] = {0.f, 0.f, 0.f};
-        float sc_w[8];
-        if (ip < w1) {
-        const int2 pe = a.wpix[ip];
-        const int p = pe.x, x = pe.y & 0xffff, i = pe.y >> 16;
-        const int y = (i < a.ig.xs) ? i : i - a.ig.s;
-        const float ctf = (a.do_ctf ? C[p] : 1.f) * scw;
-        float2 ref = make_float2(0.f, 0.f);
-        if (x * x + y * y <= a.ig.proj_r2_max) {
-            const float2 v = project_pixel(vol, a.vg, fr, x, y);
-            ref = make_float2(ctf * v.x, ctf * v.y);
-        }
-        const int ires = a.mresol[p];
-        const float minv = a.minv_w[p];
-        const bool do_xa = a.do_scale && xash && ires > -1 && xash[ires];
-        float wd2 = 0.f, fre = 0.f, fim = 0.f, fw = 0.f, pxa = 0.f, paa = 0.f;
-        const float2 Xp = X[p], Xnp = Xn[p];
-        for (int j = 0; j < nsig; ++j) {
-            const float wn = s_w[j];
-            const float2 ph = cmul(a.ex[(size_t)s_ix[j] * a.ig.xs + x], a.ey[(size_t)s_iy[j] * a.ig.s + i]);
-            if (ires > -1) {
-                const float2 Y = cmul(Xp, ph);
-                const float dr = ref.x - Y.x, di = ref.y - Y.y;
-                wd2 = fmaf(wn, dr * dr + di * di, wd2);
-                if (do_xa) {
-                    pxa = fmaf(wn, ref.x * Y.x + ref.y * Y.y, pxa);
-                    paa = fmaf(wn, ref.x * ref.x + ref.y * ref.y, paa);
+    for (int r = 0; r < 8; ++r) {
+        // significant fine translations of orientation r, compacted in (coarse translation, sub-shift) order; their
+        // normalised weights' sums (pdf_class, pdf_direction, sigma2_offset) once per job and orientation, with band 0
+        int cnt = 0;
+        double sw = 0., so = 0.;
+        const double px2 = a.pixel_size * a.pixel_size;
+        for (int e0 = 0; e0 < n4; e0 += kBlock) {
+            const int e = e0 + tid;
+            bool sig = false;
+            double wn = 0.;
+            int t = 0, u = 0;
+            if (e < n4) {
+                const int sidx = e >> 2;
+                u = e & 3;
+                const double wt = a.weight[base + (long long)sidx * 32 + r * 4 + u];
+                sig = wt >= thr;
+                if (sig) {
+                    t = a.pairs[a.pair_off[b] + job.s_begin + sidx] % a.NT;
+                    wn = wt / sumw;
                 }
             }
-            const float2 Z = cmul(Xnp, ph);
-            const float wx = wn * ctf * minv;
-            fre = fmaf(Z.x, wx, fre);
-            fim = fmaf(Z.y, wx, fim);
-            fw = fmaf(wx, ctf, fw);
-        }
-        if (ires > -1) {
-            atomicAdd(&shell[ires], wd2);
-            nc += wd2;
-            xa += pxa;
-            aa += paa;
-        }
-        // backprojection of (fre, fim) with weight fw (backproject2Dto3D, src/backprojector.cpp:124-296)
-        if (fw > 0.f && x * x + y * y <= a.ig.bp_r2_max && !(i >= sh && x == 0)) {
-            const float fxp = (float)x, fyp = (float)y;
-            float xp = fr.e0x * fxp + fr.e1x * fyp;
-            float yp = fr.e0y * fxp + fr.e1y * fyp;
-            float zp = fr.e0z * fxp + fr.e1z * fyp;
-            float vim = fim;
-            if (xp < 0.f) {
-                xp = -xp;
-                yp = -yp;
-                zp = -zp;
-                vim = -vim;
+            int tot;
+            const int pos = cnt + block_exclusive_scan_i(sig ? 1 : 0, s_tmp, tot);
+            if (sig) {
+                if (pos < kMaxSigT) {
+                    s_ix[pos] = a.tr_ix[t * 4 + u];
+                    s_iy[pos] = a.tr_iy[t * 4 + u];
+                    s_w[pos] = (float)wn;
+                }
+                if (band == 0) {
+                    sw += wn;
+                    const double dx = a.prior_off[2 * b] - a.old_off[2 * b] - a.over_x[t * 4 + u];
+                    const double dy = a.prior_off[2 * b + 1] - a.old_off[2 * b + 1] - a.over_y[t * 4 + u];
+                    so += wn * px2 * (dx * dx + dy * dy);
+                }
             }
-            const float fx0 = floorf(xp), fy0 = floorf(yp), fz0 = floorf(zp);
-            const float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
-            const int x0 = (int)fx0, y0 = (int)fy0 - a.vg.starty, z0 = (int)fz0 - a.vg.startz;
-            if (!(x0 < 0 || x0 + 1 >= a.vg.xdim || y0 < 0 || y0 + 1 >= a.vg.ydim || z0 < 0 ||
-                  z0 + 1 >= a.vg.zdim)) {
-                const float mfx = 1.f - fx, mfy = 1.f - fy, mfz = 1.f - fz;
-                sc_base = (long long)packed_index(a.vg, x0, y0, z0);
-                sc_val[0] = fre;
-                sc_val[1] = vim;
-                sc_val[2] = fw;
-                sc_w[0] = mfz * mfy * mfx;
-                sc_w[1] = mfz * mfy * fx;
-                sc_w[2] = mfz * fy * mfx;
-                sc_w[3] = mfz * fy * fx;
-                sc_w[4] = fz * mfy * mfx;
-                sc_w[5] = fz * mfy * fx;
-                sc_w[6] = fz * fy * mfx;
-                sc_w[7] = fz * fy * fx;
+            cnt += tot;
+        }
+        if (cnt == 0) continue;  // uniform: cnt comes from the block scan
+        if (band == 0) {
+            sw = block_sum_d(sw, s_red);
+            so = block_sum_d(so, s_red);
+            if (tid == 0) {
+                atomicAdd(&a.part_acc[8 * b + 0], sw);
+                atomicAdd(&a.part_acc[8 * b + 1], so);
+                atomicAdd(&a.part_acc[8 * b + 5], 1.0);
+                atomicAdd(&a.part_acc[8 * b + 6], (double)cnt);
+                atomicAdd(&a.pdf_class[(size_t)b * a.K + k], sw);
+                atomicAdd(&a.pdf_direction[(size_t)k * a.n_dir + a.or_dir[oe]], sw);
             }
         }
-        }
-        // Scatter of the block's pixels into the corner slots. The 24 values of one pixel share one 128-byte
-        // slot line; lanes are regrouped through LDS so that each atomic instruction covers two pixels'
-        // lines (32 lanes each) instead of 64 lines, which is what bounds atomic throughput.
-        __syncthreads();
-        sx_base[tid] = sc_base;
+        __syncthreads();  // compacted lists visible
+        const int nsig = min(cnt, kMaxSigT);
+        const Frame fr = a.fine_frames[a.or_frame[oe] + r];
+        // the band's pixels in chunks of kBlock; each block starts at a different chunk so that concurrent blocks do
+        // not all scatter into the same voxels at the same time
+        const int c0 = n_chunks > 0 ? (int)(((unsigned)(8 * item + r) * 2654435761u) % (unsigned)n_chunks) : 0;
+        for (int cc = 0; cc < n_chunks; ++cc) {
+            int chunk = c0 + cc;
+            if (chunk >= n_chunks) chunk -= n_chunks;
+            const int ip = w0 + chunk * kBlock + tid;
+            long long sc_base = -1;  // trilinear base voxel of this pixel's scatter (-1: none)
+            float sc_val[3] = {0.f, 0.f, 0.f};
+            float sc_w[8];
+            if (ip < w1) {
+                const int2 pe = a.wpix[ip];
+                const int p = pe.x, x = pe.y & 0xffff, i = pe.y >> 16;
+                const int y = (i < a.ig.xs) ? i : i - a.ig.s;
+                const float ctf = (a.do_ctf ? C[p] : 1.f) * scw;
+                float2 ref = make_float2(0.f, 0.f);
+                if (x * x + y * y <= a.ig.proj_r2_max) {
+                    const float2 v = project_pixel(vol, a.vg, fr, x, y);
+                    ref = make_float2(ctf * v.x, ctf * v.y);
+                }
+                const int ires = a.mresol[p];
+                const float minv = a.minv_w[p];
+                const bool do_xa = a.do_scale && xash && ires > -1 && xash[ires];
+                float wd2 = 0.f, fre = 0.f, fim = 0.f, fw = 0.f, pxa = 0.f, paa = 0.f;
+                const float2 Xp = X[p], Xnp = Xn[p];
+                for (int j = 0; j < nsig; ++j) {
+                    const float wn = s_w[j];
+                    const float2 ph = cmul(a.ex[(size_t)s_ix[j] * a.ig.xs + x], a.ey[(size_t)s_iy[j] * a.ig.s + i]);
+                    if (ires > -1) {
+                        const float2 Y = cmul(Xp, ph);
+                        const float dr = ref.x - Y.x, di = ref.y - Y.y;
+                        wd2 = fmaf(wn, dr * dr + di * di, wd2);
+                        if (do_xa) {
+                            pxa = fmaf(wn, ref.x * Y.x + ref.y * Y.y, pxa);
+                            paa = fmaf(wn, ref.x * ref.x + ref.y * ref.y, paa);
+                        }
+                    }
+                    const float2 Z = cmul(Xnp, ph);
+                    const float wx = wn * ctf * minv;
+                    fre = fmaf(Z.x, wx, fre);
+                    fim = fmaf(Z.y, wx, fim);
+                    fw = fmaf(wx, ctf, fw);
+                }
+                if (ires > -1) {
+                    atomicAdd(&shell[ires], wd2);
+                    nc += wd2;
+                    xa += pxa;
+                    aa += paa;
+                }
+                // backprojection of (fre, fim) with weight fw (backproject2Dto3D, src/backprojector.cpp:124-296)
+                if (fw > 0.f && x * x + y * y <= a.ig.bp_r2_max && !(i >= sh && x == 0)) {
+                    const float fxp = (float)x, fyp = (float)y;
+                    float xp = fr.e0x * fxp + fr.e1x * fyp;
+                    float yp = fr.e0y * fxp + fr.e1y * fyp;
+                    float zp = fr.e0z * fxp + fr.e1z * fyp;
+                    float vim = fim;
+                    if (xp < 0.f) {
+                        xp = -xp;
+                        yp = -yp;
+                        zp = -zp;
+                        vim = -vim;
+                    }
+                    const float fx0 = floorf(xp), fy0 = floorf(yp), fz0 = floorf(zp);
+                    const float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
+                    const int x0 = (int)fx0, y0 = (int)fy0 - a.vg.starty, z0 = (int)fz0 - a.vg.startz;
+                    if (!(x0 < 0 || x0 + 1 >= a.vg.xdim || y0 < 0 || y0 + 1 >= a.vg.ydim || z0 < 0 ||
+                          z0 + 1 >= a.vg.zdim)) {
+                        const float mfx = 1.f - fx, mfy = 1.f - fy, mfz = 1.f - fz;
+                        sc_base = (long long)packed_index(a.vg, x0, y0, z0);
+                        sc_val[0] = fre;
+                        sc_val[1] = vim;
+                        sc_val[2] = fw;
+                        sc_w[0] = mfz * mfy * mfx;
+                        sc_w[1] = mfz * mfy * fx;
+                        sc_w[2] = mfz * fy * mfx;
+                        sc_w[3] = mfz * fy * fx;
+                        sc_w[4] = fz * mfy * mfx;
+                        sc_w[5] = fz * mfy * fx;
+                        sc_w[6] = fz * fy * mfx;
+                        sc_w[7] = fz * fy * fx;
+                    }
+                }
+            }
+            // Scatter of the block's pixels into the corner slots. The 24 values of one pixel share one 128-byte
+            // slot line; lanes are regrouped through LDS so that each atomic instruction covers two pixels'
+            // lines (32 lanes each) instead of 64 lines, which is what bounds atomic throughput.
+            __syncthreads();
+            sx_base[tid] = sc_base;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) sx_val[tid][c] = sc_val[c];
+            for (int c = 0; c < 3; ++c) sx_val[tid][c] = sc_val[c];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) sx_w[tid][c] = (sc_base >= 0) ? sc_w[c] : 0.f;
-        __syncthreads();
-        {
-            const int lane = tid & 63, wave0 = tid & ~63;
-            const int f = lane & 31;
-            for (int k2 = 0; k2 < 32; ++k2) {
-                const int src = wave0 + 2 * k2 + (lane >> 5);
-                const long long sb = sx_base[src];
-                if (sb >= 0 && f < 24) atomicAdd(slots + kSlot * sb + f, sx_w[src][f & 7] * sx_val[src][f >> 3]);
+            for (int c = 0; c < 8; ++c) sx_w[tid][c] = (sc_base >= 0) ? sc_w[c] : 0.f;
+            __syncthreads();
+            {
+                const int lane = tid & 63, wave0 = tid & ~63;
+                const int f = lane & 31;
+                for (int k2 = 0; k2 < 32; ++k2) {
+                    const int src = wave0 + 2 * k2 + (lane >> 5);
+                    const long long sb = sx_base[src];
+                    if (sb >= 0 && f < 24) atomicAdd(slots + kSlot * sb + f, sx_w[src][f & 7] * sx_val[src][f >> 3]);
+                }
             }
         }
-    }
+        __syncthreads();  // this orientation's lists consumed before the next one's are written
+    }  // r
     nc = block_sum_d(nc, s_red);
     xa = block_sum_d(xa, s_red);
     aa = block_sum_d(aa, s_red);
     __syncthreads();
     double *sa = a.shell_acc + (size_t)b * a.nshell;
-    for (int s = tid; s < a.nshell; s += kBlock)
-        if (shell[s] != 0.f) atomicAdd(&sa[s], (double)shell[s]);
-    if (tid == 0) {
+    for (int s2 = tid; s2 < a.nshell; s2 += kBlock)
+        if (shell[s2] != 0.f) atomicAdd(&sa[s2], (double)shell[s2]);
+    if (tid == 0 && (nc != 0. || xa != 0. || aa != 0.)) {
         atomicAdd(&a.part_acc[8 * b + 2], nc);
         atomicAdd(&a.part_acc[8 * b + 3], xa);
         atomicAdd(&a.part_acc[8 * b + 4], aa);
     }
-    __syncthreads();  // LDS of this item consumed before the next item
-    }  // items
+    }  // units
 }
 
 }  // namespace rlnk
