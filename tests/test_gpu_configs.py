@@ -4,8 +4,9 @@
     order 3 global search (36,864 orientations per class), offsets 6 px at 1 px steps (113 translations),
     SNR 0.05; particle diameter 180 A (coarse size 40): 16.7 M coarse poses per particle;
   * uneven work: box 192, 1.2 A pixels, a 100-blob map, coarse order 3 with the order-4 fine pass of the
-    oversampling, offsets 6 px at 2 px steps (29 translations), per-particle SNR log-uniform in [0.01, 0.3]: the
-    significant-pose counts of the fine pass spread over orders of magnitude;
+    oversampling, offsets 6 px at 2 px steps (29 translations), per-particle SNR log-uniform in [0.001, 0.3] (the
+    workload's range [0.01, 0.3] widened so that 6 particles already span the spread: the oracle evaluates 128 to
+    2,240 fine poses for them, where six draws from [0.01, 0.3] can all end with one significant coarse pose);
 both against the FP64 oracle with the parity suite's tolerances (tests/test_gpu_parity.py: per-pose diff2 within
 1e-5, identical best poses except near-ties, backprojection within 1e-4), on a few particles so that the oracle
 finishes in seconds;
@@ -34,9 +35,9 @@ def test_classification_box128_four_classes_order3():
 
 
 def test_uneven_box192_fine_order4():
-    print("[configs] box 192, log-uniform SNR 0.01-0.3, order 3 -> 4, 29 translations, 6 particles", flush=True)
+    print("[configs] box 192, log-uniform SNR 0.001-0.3, order 3 -> 4, 29 translations, 6 particles", flush=True)
     p = make_problem(box=192, px=1.2, order=3, offset_range_px=6, offset_step_px=2, diameter=180.0, n=6,
-                     snr=(0.01, 0.3), nr_blobs=100, seed=192)
+                     snr=(0.001, 0.3), nr_blobs=100, seed=193)
     nd, npsi, nt = ra.sampling_counts(p.cfg)
     assert (nd, npsi, nt) == (768, 48, 29)
     rg, ro = check_run(p, debug=(0,))

@@ -127,6 +127,11 @@ def check_run(p, debug=(0, 1), bp_tol=1e-4, bands=None):
     for k in range(p.cfg.nr_classes):
         re, im, w = bpg[k]
         ref_d = bpd[k]
+        if not ref_d.any() and not bpw[k].any():
+            # no particle's significant poses in this class: nothing may have been backprojected into it
+            print(f"[parity] class {k}: empty in the oracle", flush=True)
+            assert not re.any() and not im.any() and not w.any(), f"class {k}: backprojected into an empty class"
+            continue
         num = np.sqrt(((re - ref_d[..., 0]) ** 2 + (im - ref_d[..., 1]) ** 2).sum())
         den = np.sqrt((ref_d ** 2).sum())
         errw = np.abs(w - bpw[k]).sum() / np.abs(bpw[k]).sum()

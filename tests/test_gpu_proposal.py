@@ -228,3 +228,49 @@ def test_proposal_predict_in_two_halves():
     assert np.array_equal(out["res"]["best_ihidden_over"], alone["best_ihidden_over"])
     es.close()
     pp.close()
+
+
+def test_proposal_trains_on_ml_assignments():
+    """rln_proposal_add_images: particle images labelled with the E-step's own max-likelihood poses enter training
+    with their labels. From one warm start (generated projections), fine-tuning on the particles with their ML poses
+    must lower the cross entropy of those poses' bins, and by more than fine-tuning on the same images with the
+    poses shuffled among the particles (the control): a mix-up of images and labels in add_images would not."""
+    n = 256
+    p = make_problem(box=64, px=2.0, order=3, n=n, snr=0.1, seed=96, offset_range_px=3, offset_step_px=1)
+    print(f"[proposal] ML-assignment training: box 64, order 3 exhaustive E-step of {n} particles", flush=True)
+    dims = ra.volume_dims(64)
+    pr = ra.AccProjector(0)
+    pr.setMdlDim(*dims, 2.0)
+    pr.initMdl(p.volumes[0])
+    bp = ra.AccBackprojector(0)
+    bp.setMdlDim(*dims, 2.0)
+    bp.initMdl()
+    es = ra.ExpectationStep(p.cfg, 0)
+    es.set_model([pr], [bp], p.pdf_class, p.sigma2_noise, pdf_direction=p.pdf_direction)
+    res, _ = es.run(p.fimg, p.fctf, p.meta)
+    es.close()
+    ml = np.stack([res["rot"], res["tilt"], res["psi"]], 1)
+    pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=7)
+    for ep in range(2):
+        pp.clear_training()
+        pp.generate(pr, 50000, ori_size=64, pixel_size=2.0, sigma2_noise=p.sigma2_noise, defocus=(1.0e4, 3.0e4),
+                    max_astigmatism=300.0, offset_max=3.0, first=ep * 50000)
+        pp.train(epochs=1, batch=256, lr=1e-3)
+    w0 = pp.get_params()
+    imgs = pp.images(p.fimg, p.fctf, n, 64)
+    labels = np.array([pp.pose_bin(*ml[i]) for i in range(n)], np.int32)
+    loss0 = pp.forward_backward(imgs, labels)[0]
+    perm = np.random.default_rng(4).permutation(n)
+    losses = {}
+    for name, poses in (("ml", ml), ("shuffled", ml[perm])):
+        pp.set_params(w0)
+        pp.clear_training()
+        pp.add_images(p.fimg, p.fctf, 64, poses)
+        assert pp.training_size() == n
+        pp.train(epochs=6, batch=64, lr=3e-4)
+        losses[name] = pp.forward_backward(imgs, labels)[0]
+    print(f"[proposal] cross entropy of the ML bins: warm start {loss0:.3f}, fine-tuned on ML poses "
+          f"{losses['ml']:.3f}, on shuffled poses {losses['shuffled']:.3f}", flush=True)
+    assert losses["ml"] < loss0 - 0.1
+    assert losses["ml"] < losses["shuffled"] - 0.5
+    pp.close()
