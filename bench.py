@@ -71,9 +71,13 @@ def parse():
                          "(the E-step workload of the run that saved them; for counter-collection runs)")
     ap.add_argument("--fine-order", type=int, default=-1,
                     help="order of the fine pass's work units (0 particle order, 1 orientation order band-major, "
-                         "2 orientation order item-major; -1 tuned)")
+                         "2 orientation order item-major, 3 orientation order band-major in per-XCD queues; -1 tuned)")
     ap.add_argument("--wsum-item-major", type=int, default=0,
                     help="weighted sums' unit order: 0 band-major, 1 item-major")
+    ap.add_argument("--pool", type=int, default=0,
+                    help="particles per E-step chunk (the reference's --pool; 0: the engine's default)")
+    ap.add_argument("--wsum-bands", type=int, default=0,
+                    help="radial pixel bands per weighted-sum work unit (0: the fine pass's tuned count)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the fine stage directly instead of through captured graphs (same kernels)")
     ap.add_argument("--classify-particles", type=int, default=1000,
@@ -107,6 +111,9 @@ def engine(cfg, args):
     if args.no_graphs:
         es.use_graphs(False)
     es.set_fine_order(args.fine_order, args.wsum_item_major)
+    es.set_wsum_bands(args.wsum_bands)
+    if args.pool > 0:
+        es.set_pool(args.pool)
     return es
 
 
@@ -115,7 +122,8 @@ def tuning_summary(es):
     t = es.fine_tuning()
     if t is None:
         return None
-    names = ("particle order, band-major", "orientation order, band-major", "orientation order, item-major")
+    names = ("particle order, band-major", "orientation order, band-major", "orientation order, item-major",
+             "orientation order, band-major, per-XCD queues")
     best = {names[o]: (round(float(t[:, o][t[:, o] >= 0].min()), 3) if (t[:, o] >= 0).any() else None)
             for o in range(t.shape[1])}
     c = es.counters()
@@ -285,8 +293,9 @@ def classify_leg(args, n):
 
 
 def committed_traffic(name, kernel):
-    """HBM bytes per launch of `kernel` from a committed counter summary (two rocprofv3 --pmc runs of the same
-    workload, FETCH_SIZE and WRITE_SIZE scaled per kernel by the request-size calibrations), or (None, None)."""
+    """HBM bytes per launch of `kernel` (its name without spaces, e.g. k_fine_diff2<0,1,1>) from a committed counter
+    summary (two rocprofv3 --pmc runs of the same workload, FETCH_SIZE and WRITE_SIZE scaled per kernel by the
+    request-size calibrations), or (None, None)."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None
@@ -873,7 +882,8 @@ def main():
     if args.traffic_kib is not None:
         traffic, traffic_src = round(args.traffic_kib * 1024.0), "--traffic-kib"
     elif is_prop and box == 256 and n == 10000 and stage == "fine_diff2":
-        traffic, traffic_src = committed_traffic("estep_proposal_box256_order3_10k_traffic.md", "rlnk::k_fine_diff2<0>")
+        shape = f"k_fine_diff2<0,{counters['fine_small_tc']},{counters['fine_wide_cw']}>"
+        traffic, traffic_src = committed_traffic("estep_proposal_box256_order3_10k_traffic.md", shape)
     roofline = {"kernel": stage, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 1),
                 "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "algorithmic": alg, "stage_ms": round(timings[stage], 2), "launches": counters["chunks"],

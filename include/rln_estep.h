@@ -248,14 +248,20 @@ int rln_estep_use_graphs(rln_estep *e, int enable);
 int rln_estep_set_fine_bands(rln_estep *e, int n_bands);
 /* Order of the fine pass's work units: 0 jobs in particle order, units band-major (all jobs' band 0, then band 1,
  * ...); 1 jobs sorted by class and coarse orientation (direction-major), band-major; 2 sorted, item-major (the bands
- * of a job one after the other); -1 (the default) lets the tuning choose. Jobs of one coarse direction sample the
+ * of a job one after the other); 3 sorted, band-major within per-XCD queues (XCD x first takes the x-th eighth of the
+ * sorted jobs, so that each XCD's L2 holds the planes of fewer directions, then helps the others); -1 (the default)
+ * lets the tuning choose. Jobs of one coarse direction sample the
  * same four fine-direction planes of the reference, so in orientation order the units in flight share its corner
  * blocks in the caches. wsum_item_major sets the weighted sums' unit order likewise (0 band-major, 1 item-major; they
  * follow the fine pass's job order). Results do not depend on it, up to the order of FP64 and FP32 atomic sums.
  * No counterpart in the reference (one particle at a time, src/acc/acc_helper_functions_impl.h:26-102).
  * RLN_ERR_ARG for other values. */
 int rln_estep_set_fine_order(rln_estep *e, int order, int wsum_item_major);
-/* The fine pass's tuning: the time in ms of every candidate tried ([shape][order][band choice][grid], 4 x 3 x 6 x 5
+/* Radial pixel bands per work unit of the weighted sums and backprojection (1, 2, 4, 8, 16, 32), or 0 (the default)
+ * for the fine pass's tuned count. The weighted sums accumulate into the backprojectors, so they are not timed by the
+ * tuning. Results do not depend on it, up to the order of FP32 atomic sums. RLN_ERR_ARG for other values. */
+int rln_estep_set_wsum_bands(rln_estep *e, int n_bands);
+/* The fine pass's tuning: the time in ms of every candidate tried ([shape][order][band choice][grid], 4 x 4 x 6 x 5
  * in that order; shapes (TC, CW) = (2, 2), (1, 2), (2, 1), (1, 1); orders as rln_estep_set_fine_order; band counts
  * 1 .. 32; grids 512, 1024, 2048, 4096 and the resident-block count), -1 for those not tried; *n_out = 0 before any
  * tuning. Copies min(n, capacity). */
@@ -296,7 +302,7 @@ int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
  * blocks), the fine-image pixels that are sampled per fine orientation (non-zero Minvsigma2 inside the
  * projection radius), the number of radial pixel bands the fine pass splits each work item into, its work-item
  * shape: coarse translations per small item and per wave of a wide item (all three tuned), the order of its units
- * (rln_estep_set_fine_order) and the weighted sums' unit order (1 item-major). */
+ * (rln_estep_set_fine_order), the weighted sums' unit order (1 item-major) and their band count. */
 int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out);
 
 /* ---------------------------------------------------------------------------------------------

@@ -158,6 +158,7 @@ _SIGNATURES = {
     "rln_estep_set_pool": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_fine_bands": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_fine_order": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "rln_estep_set_wsum_bands": (C.c_int, [_vp, C.c_int]),
     "rln_estep_fine_tuning": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(C.c_int)]),
     "rln_estep_coarse_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _ip, _ip, _ip]),
     "rln_estep_get_coarse": (C.c_int, [_vp, C.c_int, _fp, _dp, C.POINTER(C.c_ubyte)]),
@@ -224,6 +225,14 @@ def _load():
 
 
 lib = _load()
+
+
+def _float_pairs(a):
+    """Complex images as interleaved float32 (re, im) pairs; float arrays as they are (already pairs)."""
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        return np.ascontiguousarray(a, np.complex64).view(np.float32)
+    return np.ascontiguousarray(a, np.float32)
 
 
 def _check(rc):
@@ -570,9 +579,13 @@ class ExpectationStep:
 
     def set_fine_order(self, order=-1, wsum_item_major=0):
         """Order of the fine pass's work units (0: particle order, band-major; 1: orientation order, band-major;
-        2: orientation order, item-major; -1: the engine's tuning) and of the weighted sums' units (0 band-major,
-        1 item-major)."""
+        2: orientation order, item-major; 3: orientation order, band-major in per-XCD queues; -1: the engine's
+        tuning) and of the weighted sums' units (0 band-major, 1 item-major)."""
         _check(lib.rln_estep_set_fine_order(self._h, int(order), int(wsum_item_major)))
+
+    def set_wsum_bands(self, n_bands):
+        """Radial pixel bands per weighted-sum work unit (1, 2, 4, 8, 16, 32), or 0 for the fine pass's count."""
+        _check(lib.rln_estep_set_wsum_bands(self._h, int(n_bands)))
 
     def fine_tuning(self):
         """Times (ms) of the fine pass's tuning candidates as an array [shape][order][band choice][grid] (-1: not
@@ -583,7 +596,7 @@ class ExpectationStep:
             return None
         ms = np.zeros(n.value)
         _check(lib.rln_estep_fine_tuning(self._h, _ptr(ms, C.c_double), n.value, C.byref(n)))
-        return ms.reshape(4, 3, 6, 5)
+        return ms.reshape(4, 4, 6, 5)
 
     def set_orientation_lists(self, offsets, orients):
         """set_orientation_sets in the flat form: offsets[n+1], orients[offsets[n]] (int32)."""
@@ -618,9 +631,13 @@ class ExpectationStep:
             an = a if fimg_nomask is None else _vp(int(fimg_nomask))
             keep = None
         else:
-            fa = np.ascontiguousarray(fimg, np.float32)
+            fa = _float_pairs(fimg)
             fc = np.ascontiguousarray(fctf, np.float32)
-            fn = fa if fimg_nomask is None else np.ascontiguousarray(fimg_nomask, np.float32)
+            fn = fa if fimg_nomask is None else _float_pairs(fimg_nomask)
+            P = self.cfg.current_size * (self.cfg.current_size // 2 + 1)
+            for name, arr, per in (("fimg", fa, 2 * P), ("fimg_nomask", fn, 2 * P), ("fctf", fc, P)):
+                if arr.size != n * per:
+                    raise ValueError(f"{name}: {arr.size} values for {n} particles of {per}")
             a, c, an = _vp(fa.ctypes.data), _vp(fc.ctypes.data), _vp(fn.ctypes.data)
             keep = (fa, fc, fn)
         if not isinstance(meta, C.Array):
@@ -667,7 +684,7 @@ class ExpectationStep:
                 "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks",
                 "fine_graph_launches", "fine_capacity_retries", "fine_grid", "wsum_grid",
                 "fine_pixels_sampled", "fine_bands", "fine_small_tc", "fine_wide_cw", "fine_order",
-                "wsum_item_major")
+                "wsum_item_major", "wsum_bands")
 
     def counters(self):
         out = np.zeros(len(self.COUNTERS), np.int64)

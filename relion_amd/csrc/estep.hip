@@ -124,12 +124,15 @@ constexpr int kBandChoices[] = {1, 2, 4, 8, 16, 32};
 constexpr int kNumBandChoices = (int)(sizeof(kBandChoices) / sizeof(kBandChoices[0]));
 
 // Orders of the fine pass's work units: jobs in particle order or sorted by class and coarse orientation
-// (k_fine_items), units band-major or item-major (k_fine_diff2).
+// (k_fine_items), units band-major or item-major, in one queue or in per-XCD queues (k_fine_diff2).
 struct FineOrder {
-    int sorted, item_major;
+    int sorted, item_major, xcd_queues;
 };
-constexpr FineOrder kFineOrders[] = {{0, 0}, {1, 0}, {1, 1}};
+constexpr FineOrder kFineOrders[] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {1, 0, 1}};
 constexpr int kNumFineOrders = (int)(sizeof(kFineOrders) / sizeof(kFineOrders[0]));
+
+struct Engine;
+static int wsum_band_choice(const Engine &e);
 
 static std::vector<int2> banded_lists(const std::vector<int> &pix, int s, int xs) {
     const int ns = (int)pix.size();
@@ -230,6 +233,7 @@ struct Engine {
     int fine_order = 0;                      // order of the fine units (kFineOrders; tuned)
     int order_fixed = -1;                    // fine order set by rln_estep_set_fine_order (-1: tuned)
     int wsum_item_major = 0;                 // unit order of the weighted sums: 0 band-major, 1 item-major
+    int wsum_bands = 0;                      // band count of the weighted sums (rln_estep_set_wsum_bands; 0: the fine pass's)
     DevBuf d_job_order;                      // jobs in the fine pass's order (k_fine_items)
     int resident_shape[rlnk::kNumFineShapes] = {};  // resident blocks of each shape's kernel on the device
     DevBuf d_band_pix;                       // sampled fine pixels, one banded list per band choice
@@ -260,12 +264,20 @@ struct Engine {
     hipEvent_t ev[kStCount + 1] = {};
     double stage_ms[kStCount] = {};
     double host_ms[4] = {};  // host phases: chunk preparation, of it the first chunk's lists, epilogue, whole run
-    long long counters[22] = {};
+    long long counters[23] = {};
     long long pf_sampled = 0;  // fine pixels gathered per fine orientation
     std::vector<double> tune_ms;  // fine-pass tuning: time of each (band choice, grid) candidate, -1 untried
     bool have_timing = false;
     std::mutex mtx;
 };
+
+// Band choice of the weighted sums: the one set by rln_estep_set_wsum_bands, else the fine pass's.
+static int wsum_band_choice(const Engine &e) {
+    if (e.wsum_bands > 0)
+        for (int c = 0; c < kNumBandChoices; ++c)
+            if (kBandChoices[c] == e.wsum_bands) return c;
+    return e.band_choice;
+}
 
 namespace {
 
@@ -1078,6 +1090,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
         rlnk::FineArgs fa = L.fa;
         fa.n_bands = kBandChoices[ci];
         fa.item_major = kFineOrders[oi].item_major;
+        fa.xcd_queues = kFineOrders[oi].xcd_queues;
         fa.pix = e.d_band_pix.as<const int2>() + (size_t)ci * e.pf_sampled;
         hipLaunchKernelGGL(rlnk::k_reset_items, dim3(1), dim3(64), 0, st, L.plan);
         HIP_TRY(hipEventRecord(a, st));
@@ -1174,9 +1187,10 @@ static int run_fine_stage(Engine &e, const FineLaunch &L0) {
         L.shape = e.fine_shape;
         L.sorted = kFineOrders[e.fine_order].sorted;
         L.fa.item_major = kFineOrders[e.fine_order].item_major;
+        L.fa.xcd_queues = kFineOrders[e.fine_order].xcd_queues;
         L.fa.n_bands = kBandChoices[e.band_choice];
-        L.ws.n_bands = kBandChoices[e.band_choice];
-        L.ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)e.band_choice * e.n_wsum_pix;
+        L.ws.n_bands = kBandChoices[wsum_band_choice(e)];
+        L.ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)wsum_band_choice(e) * e.n_wsum_pix;
         L.fa.pix = e.d_band_pix.as<const int2>() + (size_t)e.band_choice * e.pf_sampled;
     }
     if (!e.graph_off) {
@@ -1486,6 +1500,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         fa.n_pix = (int)e.pf_sampled;
         fa.n_bands = kBandChoices[e.band_choice];
         fa.item_major = kFineOrders[e.fine_order].item_major;
+        fa.xcd_queues = kFineOrders[e.fine_order].xcd_queues;
         L.fbase = e.d_fbase.as<const double>();
         L.shape = e.fine_shape;
         fa.NT = NT;
@@ -1528,9 +1543,9 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         ws.scale_w = d_scale_w;
         ws.minv_w = e.d_minv_w.as<const float>();
         ws.mresol = e.d_mresol.as<const int>();
-        ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)e.band_choice * e.n_wsum_pix;
+        ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)wsum_band_choice(e) * e.n_wsum_pix;
         ws.n_wpix = e.n_wsum_pix;
-        ws.n_bands = kBandChoices[e.band_choice];
+        ws.n_bands = kBandChoices[wsum_band_choice(e)];
         ws.item_major = e.wsum_item_major;
         ws.job_order = e.d_job_order.as<const int>();
         ws.xa_shell = (cfg.do_scale_correction && e.d_xa_shell.ptr) ? e.d_xa_shell.as<const unsigned char>() : nullptr;
@@ -1660,6 +1675,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[19] = rlnk::kFineShapes[e.fine_shape].cw;
     e.counters[20] = e.fine_order;
     e.counters[21] = e.wsum_item_major;
+    e.counters[22] = kBandChoices[wsum_band_choice(e)];
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
         if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
@@ -2051,6 +2067,16 @@ extern "C" int rln_estep_set_fine_order(rln_estep *e, int order, int wsum_item_m
     e->order_fixed = order;
     if (order >= 0) e->fine_order = order;
     e->wsum_item_major = wsum_item_major;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_set_wsum_bands(rln_estep *e, int n_bands) {
+    if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_wsum_bands: null engine");
+    bool ok = n_bands == 0;
+    for (int c = 0; c < rln::kNumBandChoices; ++c) ok = ok || rln::kBandChoices[c] == n_bands;
+    if (!ok) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_wsum_bands: %d bands (1, 2, 4, 8, 16, 32 or 0)", n_bands);
+    std::lock_guard<std::mutex> lk(e->mtx);
+    e->wsum_bands = n_bands;
     return RLN_OK;
 }
 

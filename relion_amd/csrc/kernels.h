@@ -668,6 +668,7 @@ struct FinePlan {
     int B;               // particles in the chunk
     int next_fine;       // next work item of k_fine_diff2 (dynamic fetch; zeroed by k_fine_plan)
     int next_wsum;       // next work item of k_wsum_bp
+    int next_fine_x[8];  // next work unit of each XCD's share (k_fine_diff2 with per-XCD queues)
     int pad_;
 };
 
@@ -1544,6 +1545,7 @@ __global__ void __launch_bounds__(kWBlock) k_fine_plan(const int *__restrict__ n
         plan->n_jobs = jbase;
         plan->n_small = plan->n_wide = 0;
         plan->next_fine = plan->next_wsum = 0;
+        for (int x = 0; x < 8; ++x) plan->next_fine_x[x] = 0;
         plan->B = B;
         plan->overflow = (pbase > cap_pairs || jbase > cap_jobs) ? 1 : 0;
     }
@@ -1697,6 +1699,7 @@ struct FineArgs {
     int n_pix;
     int n_bands;                   // band b: pix[b n_pix / n_bands .. (b + 1) n_pix / n_bands)
     int item_major;                // unit order: 0 band-major (all items of band 0, then band 1, ...), 1 item-major
+    int xcd_queues;                // 1: each XCD takes its own eighth of the items (band-major), then helps the others
     int NT;
     int do_ctf;
     double *diff2;                 // [pair_off[b]*32 + s*32 + r*4 + u]: k_fine_init's base, plus each band's sum
@@ -2007,6 +2010,51 @@ __global__ void __launch_bounds__(kBlock, 3) k_fine_diff2(FineArgs a) {
     const int n_wide = a.plan->n_wide;
     const int n_items = n_wide + a.plan->n_small;
     const int n_units = n_items * a.n_bands;
+    if (a.xcd_queues) {
+        // Workgroups are dispatched to the 8 XCDs in turn, so block b runs on XCD b % 8. XCD x owns the x-th eighth of
+        // the wide and of the small items (contiguous in orientation order: a subset of the directions for its L2)
+        // and takes their units band-major; a block whose XCD's share is exhausted takes units of the next XCDs'
+        // shares, so the pass ends when every share is. Each unit index comes from one atomic increment of its
+        // share's counter, so every unit runs exactly once whichever XCD the blocks actually run on.
+        __shared__ int s_queue;
+        const int n_small = a.plan->n_small;
+        const int home = blockIdx.x & 7;
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int got = -1, q = 0;
+                for (int k = 0; k < 8 && got < 0; ++k) {
+                    const int x = (home + k) & 7;
+                    const int nx = ((x + 1) * n_wide / 8 - x * n_wide / 8) + ((x + 1) * n_small / 8 - x * n_small / 8);
+                    const int lim = nx * a.n_bands;
+                    if (*(volatile int *)&a.plan->next_fine_x[x] >= lim) continue;  // exhausted (a stale read only
+                                                                                    // costs the increment below)
+                    const int u = atomicAdd(&a.plan->next_fine_x[x], 1);
+                    if (u < lim) {
+                        got = u;
+                        q = x;
+                    }
+                }
+                s_item = got;
+                s_queue = q;
+            }
+            __syncthreads();
+            const int unit = __builtin_amdgcn_readfirstlane(s_item);
+            if (unit < 0) break;
+            const int x = __builtin_amdgcn_readfirstlane(s_queue);
+            const int wlo = x * n_wide / 8, nxw = (x + 1) * n_wide / 8 - wlo;
+            const int slo = x * n_small / 8, nx = nxw + (x + 1) * n_small / 8 - slo;
+            const int band = unit / nx, idx = unit - band * nx;
+            if (idx < nxw) {
+                const int2 it = a.items_w[wlo + idx];
+                fine_wide_item<CW>(a, it.x, it.y, band, lds.w);
+            } else {
+                const int2 it = a.items_s[slo + idx - nxw];
+                fine_small_item<TC>(a, it.x, it.y, band, lds.s);
+            }
+        }
+        return;
+    }
     for (;;) {
         __syncthreads();  // previous unit's LDS and s_item consumed
         if (threadIdx.x == 0) s_item = atomicAdd(&a.plan->next_fine, 1);
@@ -2038,6 +2086,7 @@ __global__ void k_reset_items(FinePlan *plan) {
     if (threadIdx.x == 0) {
         plan->next_fine = 0;
         plan->next_wsum = 0;
+        for (int x = 0; x < 8; ++x) plan->next_fine_x[x] = 0;
     }
 }
 
@@ -2104,6 +2153,7 @@ __global__ void __launch_bounds__(kBlock, 5) k_wsum_bp(WsumArgs a) {
     __shared__ int s_ix[kMaxSigT], s_iy[kMaxSigT];
     __shared__ float s_w[kMaxSigT];
     __shared__ int s_tmp[24];
+    __shared__ int s_cnt[8];
     __shared__ float shell[512];
     __shared__ double s_red[16];
     __shared__ long long sx_base[kBlock];
@@ -2156,7 +2206,15 @@ __global__ void __launch_bounds__(kBlock, 5) k_wsum_bp(WsumArgs a) {
     const int n4 = 4 * job.s_count;  // fine translations of one oversampled orientation
     const int w0 = band * a.n_wpix / a.n_bands, w1 = (band + 1) * a.n_wpix / a.n_bands;
     const int n_chunks = (w1 - w0 + kBlock - 1) / kBlock;
+    // significant fine poses per oversampled orientation, in one coalesced pass over the job's 32 weights per
+    // significant coarse translation (entry sidx * 32 + r * 4 + u)
+    if (tid < 8) s_cnt[tid] = 0;
+    __syncthreads();
+    for (int e = tid; e < 32 * job.s_count; e += kBlock)
+        if (a.weight[base + e] >= thr) atomicAdd(&s_cnt[(e >> 2) & 7], 1);
+    __syncthreads();
     for (int r = 0; r < 8; ++r) {
+        if (s_cnt[r] == 0) continue;  // uniform
         // significant fine translations of orientation r, compacted in (coarse translation, sub-shift) order; their
         // normalised weights' sums (pdf_class, pdf_direction, sigma2_offset) once per job and orientation, with band 0
         int cnt = 0;

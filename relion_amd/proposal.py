@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import (ProposalConfig, ProposalGen, _check, _ptr, _vp, lib)
+from . import (ProposalConfig, ProposalGen, _check, _float_pairs, _ptr, _vp, lib)
 
 
 class PoseProposal:
@@ -78,7 +78,7 @@ class PoseProposal:
             return None, None
         if on_device:
             return _vp(int(fimg)), None
-        fa = np.ascontiguousarray(fimg, np.float32)
+        fa = _float_pairs(fimg)
         return _vp(fa.ctypes.data), fa
 
     def predict(self, fimg, fctf, n, current_size, topk=16, on_device=False):

@@ -60,3 +60,13 @@ def test_synthetic_generator_is_seeded_and_batch_independent():
     xd, yd, zd, iy, iz, _ = ra.volume_dims(32)
     plane = v1[:, :, 0, 0] + 1j * v1[:, :, 0, 1]
     assert np.allclose(plane[::-1, ::-1], np.conj(plane), atol=1e-12)
+
+
+def test_complex_images_become_float_pairs():
+    # complex images are passed as interleaved (re, im) float32, never truncated to their real parts
+    z = (np.arange(6) + 1j * (10 + np.arange(6))).astype(np.complex128).reshape(1, 2, 3)
+    f = ra._float_pairs(z)
+    assert f.dtype == np.float32 and f.size == 12
+    assert np.array_equal(f.reshape(1, 2, 3, 2)[..., 0], z.real) and np.array_equal(f.reshape(1, 2, 3, 2)[..., 1], z.imag)
+    r = np.ones((1, 2, 3, 2), np.float64)
+    assert ra._float_pairs(r).shape == (1, 2, 3, 2)

@@ -85,7 +85,7 @@ def test_fine_orders_match_particle_order():
     """The fine pass's unit orders (jobs in particle order or sorted by class and coarse orientation; units band-major
     or item-major) and the weighted sums' unit order give the same results up to the order of atomic sums. Two
     classes at order 2 (9,216 keys, so the job keys are shifted to fit the histogram) and 40 particles, so that many
-    jobs share a coarse direction; four bands, so that the unit orders differ."""
+    jobs share a coarse direction; four bands, so that the unit orders differ; the weighted sums also over 16 bands."""
     n = 40
     p = make_problem(box=64, px=2.0, order=2, n=n, snr=0.1, nr_classes=2, seed=47)
     print(f"[orders] box 64 order 2 global, 2 classes, {n} particles", flush=True)
@@ -103,18 +103,19 @@ def test_fine_orders_match_particle_order():
     es = ra.ExpectationStep(cfg, 0)
     es.set_fine_bands(4)
     out = {}
-    for order, wim in ((0, 0), (1, 0), (2, 0), (2, 1), (1, 1)):
+    for order, wim, wb in ((0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0), (2, 1, 0), (1, 1, 0), (1, 0, 16)):
         es.set_fine_order(order, wim)
+        es.set_wsum_bands(wb)
         for bp in bps:
             bp.initMdl()
         es.set_model(prs, bps, p.pdf_class, p.sigma2_noise, pdf_direction=p.pdf_direction)
         res, wsum = es.run(p.fimg, p.fctf, p.meta)
         c = es.counters()
-        assert (c["fine_order"], c["wsum_item_major"], c["fine_bands"]) == (order, wim, 4)
-        out[(order, wim)] = (res, wsum, [bp.getMdlData() for bp in bps])
-    r0, w0, v0 = out[(0, 0)]
+        assert (c["fine_order"], c["wsum_item_major"], c["fine_bands"], c["wsum_bands"]) == (order, wim, 4, wb or 4)
+        out[(order, wim, wb)] = (res, wsum, [bp.getMdlData() for bp in bps])
+    r0, w0, v0 = out[(0, 0, 0)]
     for key, (r1, w1, v1) in out.items():
-        print(f"[orders] order {key[0]}, weighted-sum item-major {key[1]}", flush=True)
+        print(f"[orders] order {key[0]}, weighted-sum item-major {key[1]}, weighted-sum bands {key[2]}", flush=True)
         for f in ("best_ihidden_over", "iclass", "nr_fine_evaluated", "coarse_min_diff2"):
             assert np.array_equal(r0[f], r1[f]), f
         for f in ("min_diff2", "sum_weight", "pmax", "dLL", "significant_weight", "signal_product",
@@ -127,5 +128,7 @@ def test_fine_orders_match_particle_order():
                 err = np.linalg.norm(a - b) / np.linalg.norm(a)
                 assert err <= 1e-5, f"class {k} {name}: relative norm difference {err:.3g}"
     with pytest.raises(ra.RelionError):
-        es.set_fine_order(3, 0)
+        es.set_fine_order(4, 0)
+    with pytest.raises(ra.RelionError):
+        es.set_wsum_bands(3)
     es.close()
