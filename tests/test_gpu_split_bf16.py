@@ -1,8 +1,8 @@
 """Split-BF16 MFMA coarse pass (RLN_PREC_SPLIT_BF16) against the FP64 oracle.
 
 Tolerances for this path (it trades the coarse cross term's precision for MFMA throughput):
-  * coarse diff2 within 1e-4 relative of the oracle (the plain-FP32 path holds 1e-5); the measured maximum is
-    printed;
+  * coarse diff2 within 1e-5 relative of the oracle, as the plain-FP32 path (measured 1.5e-6 to 3.7e-6 on these
+    cases; the box-256 exhaustive case of tests/test_gpu_fullsize.py holds the same 1e-5); the maximum is printed;
   * the fine pass, weights and backprojection run in the same FP32/FP64 kernels as the plain path, so the
     max-likelihood poses must agree with the oracle for >= 99% of particles (all but one in these small runs)
     and the backprojected data within 1e-4 norm-wise.
@@ -53,7 +53,7 @@ def test_split_bf16_estep(kind):
         assert np.array_equal(d2g >= 0, ok)
         worst = max(worst, rel(d2g[ok], d2o[ok]).max())
     print(f"split-BF16 coarse diff2 max rel err vs FP64 oracle ({kind}): {worst:.2e}")
-    assert worst < 1e-4
+    assert worst < 1e-5
     same = rg["best_ihidden_over"] == ro["best_ihidden_over"]
     assert len(same) - same.sum() <= max(1, int(0.01 * len(same)))
     assert rel(rg["min_diff2"], ro["min_diff2"]).max() < 1e-5  # fine pass is FP32 direct
