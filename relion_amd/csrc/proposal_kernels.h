@@ -327,29 +327,42 @@ __global__ void __launch_bounds__(kB) k_wt_bwd(const bf16 *__restrict__ W, int C
 // oy = (y + 1 - ky) / stride where that divides and lies inside (likewise ox), as bf16. An implicit GEMM: rows
 // (img, y, x), k = tap*Cout + co, columns c; each lane's A fragment (8 channels of one tap) is read straight from
 // the NHWC dZ, the B fragments from Wd (k_wt_bwd). Block = 4 waves x 16 rows, NT column tiles (Cin = 16 NT),
-// Cout % 8 == 0.
+// Cout % 8 == 0. For stride 2 (H % 16 == 0, Cout % 32 == 0) the rows run by parity class of (y, x): row
+// ((img * 4 + 2 py + px) * H/2 + y') * H/2 + x' is pixel (2 y' + py, 2 x' + px), so that a block's 64 rows share
+// one class, and only the taps that reach an output for that class are visited (ky = 1 for even y, 0 and 2 for odd
+// y; likewise kx): 1, 2 or 4 of the 9 taps.
 template <int NT>
 __global__ void __launch_bounds__(kB) k_conv_bwd_data(const bf16 *__restrict__ dZ, int n, int H, int Cout,
                                                       int stride, int Ho, const bf16 *__restrict__ Wd,
                                                       const bf16 *__restrict__ Y, bf16 *__restrict__ dX) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, gq = lane >> 4;
-    const long long M = (long long)n * H * H;
+    const long long HH = (long long)H * H;
+    const long long M = (long long)n * HH;
+    const bool par = stride == 2 && H % 16 == 0 && Cout % 32 == 0;
+    // pixel (img, y, x) of row mm
+    auto decode = [&](long long mm, int &im, int &yy, int &xx) {
+        im = (int)(mm / HH);
+        const int pix = (int)(mm - (long long)im * HH);
+        if (par) {
+            const int h = H >> 1, q = h * h;
+            const int c = pix / q, r = pix - c * q, ry = r / h;
+            yy = 2 * ry + (c >> 1);
+            xx = 2 * (r - ry * h) + (c & 1);
+        } else {
+            yy = pix / H;
+            xx = pix - yy * H;
+        }
+    };
     const long long m = ((long long)blockIdx.x * 4 + wave) * 16 + r16;  // this lane's A row: input pixel
     int img = 0, y = 0, x = 0;
     const bool mok = m < M;
-    if (mok) {
-        img = (int)(m / ((long long)H * H));
-        const int pix = (int)(m - (long long)img * H * H);
-        y = pix / H;
-        x = pix - y * H;
-    }
+    if (mok) decode(m, img, y, x);
     const int Kd = 9 * Cout;
     floatx4_t acc[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < Kd; k0 += 32) {
-        const int k = k0 + 8 * gq;
+    auto kstep = [&](int k) {  // this lane's 8 k values from k
         uint4 av = make_uint4(0u, 0u, 0u, 0u);
         if (mok && k < Kd) {
             const int tap = k / Cout, co = k - tap * Cout;
@@ -368,6 +381,16 @@ __global__ void __launch_bounds__(kB) k_conv_bwd_data(const bf16 *__restrict__ d
             if (k < Kd) bv = *reinterpret_cast<const uint4 *>(Wd + (size_t)col * Kd + k);
             acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8_t, bv), acc[j], 0, 0, 0);
         }
+    };
+    if (par) {
+        // the block's class (H^2 / 4 is a multiple of 64, so its rows share one)
+        const int cls = (int)((((long long)blockIdx.x * 64) % HH) / (HH / 4));
+        const int py = cls >> 1, px = cls & 1;
+        for (int ky = 1 - py; ky < 3; ky += 2)
+            for (int kx = 1 - px; kx < 3; kx += 2)
+                for (int co0 = 0; co0 < Cout; co0 += 32) kstep((3 * ky + kx) * Cout + co0 + 8 * gq);
+    } else {
+        for (int k0 = 0; k0 < Kd; k0 += 32) kstep(k0 + 8 * gq);
     }
     const int Cin = 16 * NT;
 #pragma unroll
@@ -377,7 +400,9 @@ __global__ void __launch_bounds__(kB) k_conv_bwd_data(const bf16 *__restrict__ d
         for (int r = 0; r < 4; ++r) {
             const long long mm = ((long long)blockIdx.x * 4 + wave) * 16 + 4 * gq + r;
             if (mm < M) {
-                const size_t idx = (size_t)mm * Cin + col;
+                int im, yy, xx;
+                decode(mm, im, yy, xx);
+                const size_t idx = (((size_t)im * H + yy) * H + xx) * Cin + col;
                 dX[idx] = to_bf16(from_bf16(Y[idx]) > 0.f ? acc[j][r] : 0.f);
             }
         }
