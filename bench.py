@@ -72,8 +72,10 @@ def parse():
     ap.add_argument("--fine-order", type=int, default=-1,
                     help="order of the fine pass's work units (0 particle order, 1 orientation order band-major, "
                          "2 orientation order item-major, 3 orientation order band-major in per-XCD queues; -1 tuned)")
-    ap.add_argument("--wsum-item-major", type=int, default=0,
-                    help="weighted sums' unit order: 0 band-major, 1 item-major")
+    ap.add_argument("--fine-shape", type=int, default=-1,
+                    help="work-item shape of the fine pass (0-3: (TC, CW) = (2, 2), (1, 2), (2, 1), (1, 1); -1 tuned)")
+    ap.add_argument("--wsum-order", type=int, default=0,
+                    help="weighted sums' unit order: 0 band-major, 1 item-major, 2 band-major in per-XCD queues")
     ap.add_argument("--pool", type=int, default=0,
                     help="particles per E-step chunk (the reference's --pool; 0: the engine's default)")
     ap.add_argument("--wsum-bands", type=int, default=0,
@@ -110,15 +112,21 @@ def engine(cfg, args):
     es = ra.ExpectationStep(cfg, 0)
     if args.no_graphs:
         es.use_graphs(False)
-    es.set_fine_order(args.fine_order, args.wsum_item_major)
+    es.set_fine_order(args.fine_order, args.wsum_order)
+    es.set_fine_shape(args.fine_shape)
     es.set_wsum_bands(args.wsum_bands)
     if args.pool > 0:
         es.set_pool(args.pool)
     return es
 
 
+# work-item shapes of the fine pass: (coarse translations per small item, per wave of a wide item)
+FINE_SHAPES = ((2, 2), (1, 2), (2, 1), (1, 1))
+
+
 def tuning_summary(es):
-    """The fine pass's tuning on the first chunk: the fastest time (ms) per unit order and the choice kept."""
+    """The fine pass's tuning on the first chunk: the fastest time (ms) per unit order and per work-item shape, and
+    the choice kept."""
     t = es.fine_tuning()
     if t is None:
         return None
@@ -126,8 +134,10 @@ def tuning_summary(es):
              "orientation order, band-major, per-XCD queues")
     best = {names[o]: (round(float(t[:, o][t[:, o] >= 0].min()), 3) if (t[:, o] >= 0).any() else None)
             for o in range(t.shape[1])}
+    by_shape = {f"tc{tc}_cw{cw}": (round(float(t[s][t[s] >= 0].min()), 3) if (t[s] >= 0).any() else None)
+                for s, (tc, cw) in enumerate(FINE_SHAPES)}
     c = es.counters()
-    return {"fastest_ms_by_order": best, "kept": {"order": names[c["fine_order"]], "bands": c["fine_bands"],
+    return {"fastest_ms_by_order": best, "fastest_ms_by_shape": by_shape, "kept": {"order": names[c["fine_order"]], "bands": c["fine_bands"],
                                                    "grid": c["fine_grid"], "small_tc": c["fine_small_tc"],
                                                    "wide_cw": c["fine_wide_cw"]}}
 

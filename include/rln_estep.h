@@ -252,15 +252,19 @@ int rln_estep_set_fine_bands(rln_estep *e, int n_bands);
  * sorted jobs, so that each XCD's L2 holds the planes of fewer directions, then helps the others); -1 (the default)
  * lets the tuning choose. Jobs of one coarse direction sample the
  * same four fine-direction planes of the reference, so in orientation order the units in flight share its corner
- * blocks in the caches. wsum_item_major sets the weighted sums' unit order likewise (0 band-major, 1 item-major; they
- * follow the fine pass's job order). Results do not depend on it, up to the order of FP64 and FP32 atomic sums.
+ * blocks in the caches. wsum_order sets the weighted sums' unit order likewise (0 band-major, 1 item-major,
+ * 2 band-major in per-XCD queues; they follow the fine pass's job order). Results do not depend on it, up to the order of FP64 and FP32 atomic sums.
  * No counterpart in the reference (one particle at a time, src/acc/acc_helper_functions_impl.h:26-102).
  * RLN_ERR_ARG for other values. */
-int rln_estep_set_fine_order(rln_estep *e, int order, int wsum_item_major);
+int rln_estep_set_fine_order(rln_estep *e, int order, int wsum_order);
 /* Radial pixel bands per work unit of the weighted sums and backprojection (1, 2, 4, 8, 16, 32), or 0 (the default)
  * for the fine pass's tuned count. The weighted sums accumulate into the backprojectors, so they are not timed by the
  * tuning. Results do not depend on it, up to the order of FP32 atomic sums. RLN_ERR_ARG for other values. */
 int rln_estep_set_wsum_bands(rln_estep *e, int n_bands);
+/* Work-item shape of the fine pass (index into the shapes below), or -1 (the default) for the tuning's choice; the
+ * tuning then times only the other parameters. Results do not depend on it, up to the order of FP32 partial sums.
+ * RLN_ERR_ARG for other values. */
+int rln_estep_set_fine_shape(rln_estep *e, int shape);
 /* The fine pass's tuning: the time in ms of every candidate tried ([shape][order][band choice][grid], 4 x 4 x 6 x 5
  * in that order; shapes (TC, CW) = (2, 2), (1, 2), (2, 1), (1, 1); orders as rln_estep_set_fine_order; band counts
  * 1 .. 32; grids 512, 1024, 2048, 4096 and the resident-block count), -1 for those not tried; *n_out = 0 before any
@@ -302,7 +306,7 @@ int rln_estep_last_timings(rln_estep *e, double *ms, int n_max, int *n_out);
  * blocks), the fine-image pixels that are sampled per fine orientation (non-zero Minvsigma2 inside the
  * projection radius), the number of radial pixel bands the fine pass splits each work item into, its work-item
  * shape: coarse translations per small item and per wave of a wide item (all three tuned), the order of its units
- * (rln_estep_set_fine_order), the weighted sums' unit order (1 item-major) and their band count. */
+ * (rln_estep_set_fine_order), the weighted sums' unit order and their band count. */
 int rln_estep_last_counters(rln_estep *e, long long *out, int n_max, int *n_out);
 
 /* ---------------------------------------------------------------------------------------------

@@ -159,6 +159,7 @@ _SIGNATURES = {
     "rln_estep_set_fine_bands": (C.c_int, [_vp, C.c_int]),
     "rln_estep_set_fine_order": (C.c_int, [_vp, C.c_int, C.c_int]),
     "rln_estep_set_wsum_bands": (C.c_int, [_vp, C.c_int]),
+    "rln_estep_set_fine_shape": (C.c_int, [_vp, C.c_int]),
     "rln_estep_fine_tuning": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(C.c_int)]),
     "rln_estep_coarse_size": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), _ip, _ip, _ip]),
     "rln_estep_get_coarse": (C.c_int, [_vp, C.c_int, _fp, _dp, C.POINTER(C.c_ubyte)]),
@@ -577,11 +578,16 @@ class ExpectationStep:
         """Radial pixel bands per fine work item (1, 2, 4, 8, 16, 32), or 0 for the engine's own tuning."""
         _check(lib.rln_estep_set_fine_bands(self._h, int(n_bands)))
 
-    def set_fine_order(self, order=-1, wsum_item_major=0):
+    def set_fine_order(self, order=-1, wsum_order=0):
         """Order of the fine pass's work units (0: particle order, band-major; 1: orientation order, band-major;
         2: orientation order, item-major; 3: orientation order, band-major in per-XCD queues; -1: the engine's
-        tuning) and of the weighted sums' units (0 band-major, 1 item-major)."""
-        _check(lib.rln_estep_set_fine_order(self._h, int(order), int(wsum_item_major)))
+        tuning) and of the weighted sums' units (0 band-major, 1 item-major, 2 band-major in per-XCD queues)."""
+        _check(lib.rln_estep_set_fine_order(self._h, int(order), int(wsum_order)))
+
+    def set_fine_shape(self, shape):
+        """Work-item shape of the fine pass (0 .. 3: (TC, CW) = (2, 2), (1, 2), (2, 1), (1, 1)), or -1 for the
+        tuning's choice."""
+        _check(lib.rln_estep_set_fine_shape(self._h, int(shape)))
 
     def set_wsum_bands(self, n_bands):
         """Radial pixel bands per weighted-sum work unit (1, 2, 4, 8, 16, 32), or 0 for the fine pass's count."""
@@ -684,7 +690,7 @@ class ExpectationStep:
                 "wsum_orientations", "wsum_pairs", "grid_projections", "coarse_tiles", "fine_jobs", "chunks",
                 "fine_graph_launches", "fine_capacity_retries", "fine_grid", "wsum_grid",
                 "fine_pixels_sampled", "fine_bands", "fine_small_tc", "fine_wide_cw", "fine_order",
-                "wsum_item_major", "wsum_bands")
+                "wsum_order", "wsum_bands")
 
     def counters(self):
         out = np.zeros(len(self.COUNTERS), np.int64)

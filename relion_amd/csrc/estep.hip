@@ -232,7 +232,8 @@ struct Engine {
     int fine_shape = 0;                      // index into rlnk::kFineShapes of the work-item shape (tuned)
     int fine_order = 0;                      // order of the fine units (kFineOrders; tuned)
     int order_fixed = -1;                    // fine order set by rln_estep_set_fine_order (-1: tuned)
-    int wsum_item_major = 0;                 // unit order of the weighted sums: 0 band-major, 1 item-major
+    int shape_fixed = -1;                    // work-item shape set by rln_estep_set_fine_shape (-1: tuned)
+    int wsum_order = 0;                 // unit order of the weighted sums: 0 band-major, 1 item-major, 2 per-XCD queues
     int wsum_bands = 0;                      // band count of the weighted sums (rln_estep_set_wsum_bands; 0: the fine pass's)
     DevBuf d_job_order;                      // jobs in the fine pass's order (k_fine_items)
     int resident_shape[rlnk::kNumFineShapes] = {};  // resident blocks of each shape's kernel on the device
@@ -1055,8 +1056,8 @@ static int launch_fine(Engine &e, const FineLaunch &L, int part) {
 // (coarse translations per small item and per wave of a wide item, which set the register tiles and the projections
 // shared per item), the order of the units (jobs in particle order or in orientation order, units band-major or
 // item-major; kFineOrders), the band count (pixels per unit: all sampled pixels, or the radial shells of
-// 1/2 .. 1/32 of them) and the grid (512 .. 4096 blocks or the shape's resident-block count). The plan's items are
-// rebuilt for each shape and order. The fastest combination is kept for the engine's life. The kernel only adds into
+// 1/2 .. 1/32 of them) and the grid (512 .. 4096 blocks or the shape's resident-block count); then the other shapes
+// at every grid, since the best grid depends on the shape. The plan's items are rebuilt for each shape and order. The fastest combination is kept for the engine's life. The kernel only adds into
 // fine diff2, which the production pass initialises again, so repeating it is harmless; its unit counter is reset
 // before every candidate. tune_ms keeps every candidate's time ([shape][order][band][grid], -1 untried) for
 // rln_estep_fine_tuning.
@@ -1113,10 +1114,11 @@ static int tune_fine(Engine &e, FineLaunch L) {
     int best_c = 0;
     for (int ci = 0; ci < kNumBandChoices; ++ci)
         if (e.pf_sampled / kBandChoices[ci] >= 256 && kBandChoices[ci] <= 8) best_c = ci;
-    int best_s = 0, best_g = n_grid - 1;
+    int best_s = e.shape_fixed >= 0 ? e.shape_fixed : 0, best_g = n_grid - 1;
     float best = 1e30f;
     for (int sh = 0; sh < rlnk::kNumFineShapes; ++sh) {
-        if (sh > 0) {
+        if (e.shape_fixed >= 0 && sh != e.shape_fixed) continue;
+        if (sh > 0 || e.shape_fixed >= 0) {
             RLN_TRY(plan_for(sh, best_o, ok));
             if (!ok) continue;
         }
@@ -1157,6 +1159,24 @@ static int tune_fine(Engine &e, FineLaunch L) {
         if (ms < best) {
             best = ms;
             best_g = gi;
+        }
+    }
+    // the other shapes again, at the order and band count kept and every grid: a shape's best grid can differ
+    if (e.shape_fixed < 0) {
+        const int s0 = best_s;
+        for (int sh = 0; sh < rlnk::kNumFineShapes; ++sh) {
+            if (sh == s0) continue;
+            RLN_TRY(plan_for(sh, best_o, ok));
+            if (!ok) continue;
+            for (int gi = 0; gi < n_grid; ++gi) {
+                float ms = 0.f;
+                RLN_TRY(time_one(sh, best_o, best_c, gi, ms));
+                if (ms < best) {
+                    best = ms;
+                    best_s = sh;
+                    best_g = gi;
+                }
+            }
         }
     }
     (void)hipEventDestroy(a);
@@ -1546,7 +1566,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
         ws.wpix = e.d_wsum_pix.as<const int2>() + (size_t)wsum_band_choice(e) * e.n_wsum_pix;
         ws.n_wpix = e.n_wsum_pix;
         ws.n_bands = kBandChoices[wsum_band_choice(e)];
-        ws.item_major = e.wsum_item_major;
+        ws.unit_order = e.wsum_order;
         ws.job_order = e.d_job_order.as<const int>();
         ws.xa_shell = (cfg.do_scale_correction && e.d_xa_shell.ptr) ? e.d_xa_shell.as<const unsigned char>() : nullptr;
         ws.ex = e.d_ex_f.as<const float2>();
@@ -1674,7 +1694,7 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[18] = rlnk::kFineShapes[e.fine_shape].tc;
     e.counters[19] = rlnk::kFineShapes[e.fine_shape].cw;
     e.counters[20] = e.fine_order;
-    e.counters[21] = e.wsum_item_major;
+    e.counters[21] = e.wsum_order;
     e.counters[22] = kBandChoices[wsum_band_choice(e)];
     for (int b = 0; b < B; ++b) {
         const rlnk::PassStats &sf = stf[b];
@@ -2058,15 +2078,25 @@ extern "C" int rln_estep_set_fine_bands(rln_estep *e, int n_bands) {
     RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_bands: %d bands (1, 2, 4, 8, 16, 32 or 0)", n_bands);
 }
 
-extern "C" int rln_estep_set_fine_order(rln_estep *e, int order, int wsum_item_major) {
+extern "C" int rln_estep_set_fine_order(rln_estep *e, int order, int wsum_order) {
     if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_order: null engine");
-    if (order < -1 || order >= rln::kNumFineOrders || wsum_item_major < 0 || wsum_item_major > 1)
-        RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_order: order %d (-1 .. %d), weighted-sum order %d (0 or 1)", order,
-                 rln::kNumFineOrders - 1, wsum_item_major);
+    if (order < -1 || order >= rln::kNumFineOrders || wsum_order < 0 || wsum_order > 2)
+        RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_order: order %d (-1 .. %d), weighted-sum order %d (0, 1 or 2)", order,
+                 rln::kNumFineOrders - 1, wsum_order);
     std::lock_guard<std::mutex> lk(e->mtx);
     e->order_fixed = order;
     if (order >= 0) e->fine_order = order;
-    e->wsum_item_major = wsum_item_major;
+    e->wsum_order = wsum_order;
+    return RLN_OK;
+}
+
+extern "C" int rln_estep_set_fine_shape(rln_estep *e, int shape) {
+    if (!e) RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_shape: null engine");
+    if (shape < -1 || shape >= rlnk::kNumFineShapes)
+        RLN_FAIL(RLN_ERR_ARG, "rln_estep_set_fine_shape: shape %d (-1 .. %d)", shape, rlnk::kNumFineShapes - 1);
+    std::lock_guard<std::mutex> lk(e->mtx);
+    e->shape_fixed = shape;
+    if (shape >= 0) e->fine_shape = shape;
     return RLN_OK;
 }
 

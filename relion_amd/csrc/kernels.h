@@ -669,6 +669,7 @@ struct FinePlan {
     int next_fine;       // next work item of k_fine_diff2 (dynamic fetch; zeroed by k_fine_plan)
     int next_wsum;       // next work item of k_wsum_bp
     int next_fine_x[8];  // next work unit of each XCD's share (k_fine_diff2 with per-XCD queues)
+    int next_wsum_x[8];  // the same for k_wsum_bp
     int pad_;
 };
 
@@ -1545,7 +1546,7 @@ __global__ void __launch_bounds__(kWBlock) k_fine_plan(const int *__restrict__ n
         plan->n_jobs = jbase;
         plan->n_small = plan->n_wide = 0;
         plan->next_fine = plan->next_wsum = 0;
-        for (int x = 0; x < 8; ++x) plan->next_fine_x[x] = 0;
+        for (int x = 0; x < 8; ++x) plan->next_fine_x[x] = plan->next_wsum_x[x] = 0;
         plan->B = B;
         plan->overflow = (pbase > cap_pairs || jbase > cap_jobs) ? 1 : 0;
     }
@@ -1997,8 +1998,8 @@ __device__ __forceinline__ void fine_wide_item(const FineArgs &a, int job_id, in
 // (the bands of an item one after the other), so that with items in orientation order (k_fine_items) the units in
 // flight sample the planes of few coarse directions. Each unit adds its partial
 // sums to the FP64 diff2 (k_fine_init wrote the pose-independent part). Three waves per SIMD, as the two item
-// paths have when compiled alone (at most 168 VGPRs). kTuning only separates the tuning launches from the
-// production ones in kernel traces.
+// paths have when compiled alone (at most 168 VGPRs). kTuning only separates the tuning launches from the production
+// ones in kernel traces.
 template <int kTuning, int TC, int CW>
 __global__ void __launch_bounds__(kBlock, 3) k_fine_diff2(FineArgs a) {
     __shared__ union {
@@ -2086,7 +2087,7 @@ __global__ void k_reset_items(FinePlan *plan) {
     if (threadIdx.x == 0) {
         plan->next_fine = 0;
         plan->next_wsum = 0;
-        for (int x = 0; x < 8; ++x) plan->next_fine_x[x] = 0;
+        for (int x = 0; x < 8; ++x) plan->next_fine_x[x] = plan->next_wsum_x[x] = 0;
     }
 }
 
@@ -2123,7 +2124,8 @@ struct WsumArgs {
                                     // n_bands bands of equal count by radius, pixel order within a band
     int n_wpix;
     int n_bands;
-    int item_major;                 // unit order: 0 band-major, 1 item-major (the bands of an item in a row)
+    int unit_order;                 // 0 band-major, 1 item-major (the bands of a job in a row), 2 band-major in per-XCD
+                                    // queues
     const unsigned char *xa_shell;  // [K][nshell] data_vs_prior > 3 (scale correction) or null
     const float2 *ex;
     const float2 *ey;
@@ -2166,16 +2168,42 @@ __global__ void __launch_bounds__(kBlock, 5) k_wsum_bp(WsumArgs a) {
     const int tid = threadIdx.x;
     // persistent: units fetched dynamically. A unit is one job (a coarse orientation of a particle with its
     // significant coarse translations) on one radial band of pixels; it visits the job's 8 oversampled orientations
-    // and skips those without a significant fine pose (most of them). Band-major or item-major, in the fine pass's
-    // job order.
+    // and skips those without a significant fine pose (most of them). Band-major or item-major in one queue, or
+    // band-major in per-XCD queues (XCD b % 8 first takes its eighth of the jobs, contiguous in the fine pass's
+    // orientation order, then helps the others), in the fine pass's job order.
+    __shared__ int s_queue;
     for (;;) {
     __syncthreads();  // previous unit's LDS and s_item consumed
-    if (tid == 0) s_item = atomicAdd(&a.plan->next_wsum, 1);
+    if (tid == 0) {
+        if (a.unit_order == 2) {
+            const int home = blockIdx.x & 7;
+            int got = -1, q = 0;
+            for (int k = 0; k < 8 && got < 0; ++k) {
+                const int x = (home + k) & 7;
+                const int lim = ((x + 1) * n_items / 8 - x * n_items / 8) * a.n_bands;
+                if (*(volatile int *)&a.plan->next_wsum_x[x] >= lim) continue;
+                const int u = atomicAdd(&a.plan->next_wsum_x[x], 1);
+                if (u < lim) {
+                    got = u;
+                    q = x;
+                }
+            }
+            s_item = got;
+            s_queue = q;
+        } else {
+            s_item = atomicAdd(&a.plan->next_wsum, 1);
+        }
+    }
     __syncthreads();
     const int unit = s_item;
-    if (unit >= n_units) break;
+    if (unit >= n_units || unit < 0) break;
     int band, item;
-    if (a.item_major) {
+    if (a.unit_order == 2) {
+        const int x = s_queue;
+        const int lo = x * n_items / 8, nx = (x + 1) * n_items / 8 - lo;
+        band = unit / nx;
+        item = lo + unit - band * nx;
+    } else if (a.unit_order == 1) {
         item = unit / a.n_bands;
         band = unit - item * a.n_bands;
     } else {

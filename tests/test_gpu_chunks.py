@@ -103,7 +103,7 @@ def test_fine_orders_match_particle_order():
     es = ra.ExpectationStep(cfg, 0)
     es.set_fine_bands(4)
     out = {}
-    for order, wim, wb in ((0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0), (2, 1, 0), (1, 1, 0), (1, 0, 16)):
+    for order, wim, wb in ((0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0), (2, 1, 0), (1, 1, 0), (3, 2, 0), (1, 0, 16)):
         es.set_fine_order(order, wim)
         es.set_wsum_bands(wb)
         for bp in bps:
@@ -111,7 +111,7 @@ def test_fine_orders_match_particle_order():
         es.set_model(prs, bps, p.pdf_class, p.sigma2_noise, pdf_direction=p.pdf_direction)
         res, wsum = es.run(p.fimg, p.fctf, p.meta)
         c = es.counters()
-        assert (c["fine_order"], c["wsum_item_major"], c["fine_bands"], c["wsum_bands"]) == (order, wim, 4, wb or 4)
+        assert (c["fine_order"], c["wsum_order"], c["fine_bands"], c["wsum_bands"]) == (order, wim, 4, wb or 4)
         out[(order, wim, wb)] = (res, wsum, [bp.getMdlData() for bp in bps])
     r0, w0, v0 = out[(0, 0, 0)]
     for key, (r1, w1, v1) in out.items():
@@ -129,6 +129,8 @@ def test_fine_orders_match_particle_order():
                 assert err <= 1e-5, f"class {k} {name}: relative norm difference {err:.3g}"
     with pytest.raises(ra.RelionError):
         es.set_fine_order(4, 0)
+    with pytest.raises(ra.RelionError):
+        es.set_fine_order(1, 3)
     with pytest.raises(ra.RelionError):
         es.set_wsum_bands(3)
     es.close()

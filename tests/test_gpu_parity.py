@@ -19,8 +19,11 @@ from problems import make_problem
 
 pytestmark = pytest.mark.gpu
 
+# the fine pass's work-item shapes (coarse translations per small item, per wave of a wide item)
+FINE_SHAPES = [(2, 2), (1, 2), (2, 1), (1, 1)]
 
-def run_gpu(p, keep=True, bands=None):
+
+def run_gpu(p, keep=True, bands=None, shape=None):
     cfg = p.cfg
     K = cfg.nr_classes
     dims = ra.volume_dims(cfg.current_size, cfg.padding_factor)
@@ -40,6 +43,8 @@ def run_gpu(p, keep=True, bands=None):
     es.keep_intermediates(keep)
     if bands is not None:
         es.set_fine_bands(bands)
+    if shape is not None:
+        es.set_fine_shape(shape)
     res, wsum = es.run(p.fimg, p.fctf, p.meta, fimg_nomask=p.fimg_nomask)
     bp_out = [b.getMdlData() for b in bps]
     return es, res, wsum, bp_out, (projs, bps)
@@ -85,10 +90,13 @@ def check_particle(es, ip, dbg, res_g, res_o):
         assert abs(w[bo] - w[bg]) <= 1e-4 * w[bo], f"best pose differs: gpu {bg} oracle {bo}"
 
 
-def check_run(p, debug=(0, 1), bp_tol=1e-4, bands=None):
-    es, rg, wg, bpg, _ = run_gpu(p, bands=bands)
+def check_run(p, debug=(0, 1), bp_tol=1e-4, bands=None, shape=None):
+    es, rg, wg, bpg, _ = run_gpu(p, bands=bands, shape=shape)
     if bands is not None:
         assert es.counters()["fine_bands"] == bands
+    if shape is not None:
+        c = es.counters()
+        assert (c["fine_small_tc"], c["fine_wide_cw"]) == FINE_SHAPES[shape]
     ro, wo, (bpd, bpw), dbg = run_oracle(p, debug)
     n = len(p.meta)
     for ip in debug:
@@ -158,12 +166,13 @@ def test_projector_roundtrip():
     assert not re.any() and not im.any() and not w.any()
 
 
-@pytest.mark.parametrize("bands", [None, 8])
-def test_global_search_box64(bands):
+@pytest.mark.parametrize("bands,shape", [(None, None), (8, None), (None, 0), (8, 1), (None, 2), (8, 3)])
+def test_global_search_box64(bands, shape):
     """The reference's CPU-runnable case in shape: box 64, HEALPix order 2, offsets 5 px step 1, global search,
-    noisy particles; the fine pass as tuned (one band at this size) and split into 8 radial pixel bands."""
+    noisy particles; the fine pass as tuned (one band at this size), split into 8 radial pixel bands, and in each of
+    its four work-item shapes."""
     p = make_problem(box=64, px=2.0, order=2, offset_range_px=5, offset_step_px=1, n=8, snr=0.1)
-    rg, ro = check_run(p, bands=bands)
+    rg, ro = check_run(p, bands=bands, shape=shape)
     assert (rg["nr_fine_evaluated"] > 0).all()
 
 
@@ -198,12 +207,13 @@ def test_four_classes_global():
     assert len(np.unique(ro["iclass"])) > 1
 
 
-def test_uneven_snr_fine_pass():
+@pytest.mark.parametrize("shape", [None, 0, 1, 2, 3])
+def test_uneven_snr_fine_pass(shape):
     """uneven work: per-particle SNR log-uniform in [0.01, 0.3] spreads the significant-pose counts of the fine
-    pass over orders of magnitude."""
+    pass over orders of magnitude, so that both the small and the wide work items run, in each work-item shape."""
     p = make_problem(box=48, px=3.0, order=2, offset_range_px=3, offset_step_px=1, n=8, snr=(0.01, 0.3),
                      nr_blobs=30, seed=47)
-    rg, ro = check_run(p, debug=(0, 1, 2))
+    rg, ro = check_run(p, debug=(0, 1, 2), shape=shape)
     fe = rg["nr_fine_evaluated"]
     print(f"[uneven] fine poses per particle: min {fe.min()}, max {fe.max()}")
     assert fe.max() >= 10 * fe.min()
