@@ -1061,6 +1061,10 @@ static int launch_fine(Engine &e, const FineLaunch &L, int part) {
 // fine diff2, which the production pass initialises again, so repeating it is harmless; its unit counter is reset
 // before every candidate. tune_ms keeps every candidate's time ([shape][order][band][grid], -1 untried) for
 // rln_estep_fine_tuning.
+// A candidate replaces the best one found so far only when it is at least 1% faster, so that candidates within the
+// timing noise of each other do not alternate between engines.
+constexpr float kTuneMargin = 0.99f;
+
 static int tune_fine(Engine &e, FineLaunch L) {
     hipStream_t st = e.stream;
     hipEvent_t a, b;
@@ -1124,7 +1128,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
         }
         float ms = 0.f;
         RLN_TRY(time_one(sh, best_o, best_c, best_g, ms));
-        if (ms < best) {
+        if (ms < kTuneMargin * best) {
             best = ms;
             best_s = sh;
         }
@@ -1137,7 +1141,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
             if (!ok) continue;
             float ms = 0.f;
             RLN_TRY(time_one(best_s, oi, best_c, best_g, ms));
-            if (ms < best) {
+            if (ms < kTuneMargin * best) {
                 best = ms;
                 best_o = oi;
             }
@@ -1148,7 +1152,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
         if (ci == best_c || (kBandChoices[ci] > 1 && e.pf_sampled / kBandChoices[ci] < 256)) continue;
         float ms = 0.f;
         RLN_TRY(time_one(best_s, best_o, ci, best_g, ms));
-        if (ms < best) {
+        if (ms < kTuneMargin * best) {
             best = ms;
             best_c = ci;
         }
@@ -1156,7 +1160,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
     for (int gi = 0; gi + 1 < n_grid; ++gi) {
         float ms = 0.f;
         RLN_TRY(time_one(best_s, best_o, best_c, gi, ms));
-        if (ms < best) {
+        if (ms < kTuneMargin * best) {
             best = ms;
             best_g = gi;
         }
@@ -1171,7 +1175,7 @@ static int tune_fine(Engine &e, FineLaunch L) {
             for (int gi = 0; gi < n_grid; ++gi) {
                 float ms = 0.f;
                 RLN_TRY(time_one(sh, best_o, best_c, gi, ms));
-                if (ms < best) {
+                if (ms < kTuneMargin * best) {
                     best = ms;
                     best_s = sh;
                     best_g = gi;
