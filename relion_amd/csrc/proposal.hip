@@ -135,9 +135,9 @@ int conv_fwd_lds_t(rln_proposal &p, const ConvLayer &L, int n) {
     const size_t lds = (size_t)16 * NT * (L.Kp + rlnp::kConvLdsPad) * sizeof(rlnp::bf16);
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_fwd_lds<NT>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const long long tiles = (M + 64 * rlnp::kConvMT - 1) / (64 * rlnp::kConvMT);
+    const long long tiles = (M + rlnp::kConvTile - 1) / rlnp::kConvTile;
     const unsigned grid = (unsigned)std::min<long long>(tiles, 1024);
-    hipLaunchKernelGGL((rlnp::k_conv_fwd_lds<NT>), dim3(grid), dim3(rlnp::kB), lds, p.stream,
+    hipLaunchKernelGGL((rlnp::k_conv_fwd_lds<NT>), dim3(grid), dim3(rlnp::kConvThreads), lds, p.stream,
                        p.x[&L - p.conv].as<const rlnp::bf16>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp,
                        p.wb.as<const rlnp::bf16>() + L.w_off, p.w.as<const float>() + L.b_off,
                        p.x[&L - p.conv + 1].as<rlnp::bf16>());

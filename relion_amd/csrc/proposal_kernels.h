@@ -581,13 +581,17 @@ __global__ void __launch_bounds__(kB) k_conv_fwd(const bf16 *__restrict__ X, int
 // --------------------------------------------------------------------------------------------------
 // Forward 3x3 convolution as an implicit GEMM with the weights resident in LDS (inference, Cin % 8 == 0).
 // The block loads the whole weight matrix [Cout][Kp] into LDS once (rows padded by 8 elements against bank
-// conflicts) and then walks output-pixel tiles of 256 (4 waves x 4 row tiles of 16) with a grid stride. Per
-// K step of 32 a wave reads 4 A fragments (8 channels of one tap per lane) from the NHWC input and NT B
-// fragments from LDS and issues 4 x NT MFMAs; the A fragments of the next step are loaded before the MFMAs of
-// the current one.
+// conflicts) and then walks output-pixel tiles of 256 (8 waves x 2 row tiles of 16) with a grid stride. Per
+// K step of 32 a wave reads 2 A fragments (8 channels of one tap per lane) from the NHWC input and NT B
+// fragments from LDS and issues 2 x NT MFMAs; the A fragments of the next step are loaded before the MFMAs of
+// the current one. Eight waves share one copy of the weights, so that the two blocks that fit a CU's LDS keep
+// 4 waves per SIMD in flight against the latency of the A loads.
 // --------------------------------------------------------------------------------------------------
-constexpr int kConvMT = 4;         // row tiles (16 output pixels each) per wave
-constexpr int kConvLdsPad = 8;     // bf16 elements of padding per weight row in LDS
+constexpr int kConvMT = 2;                      // row tiles (16 output pixels each) per wave
+constexpr int kConvWaves = 8;                   // waves per block
+constexpr int kConvThreads = 64 * kConvWaves;
+constexpr int kConvTile = 16 * kConvMT * kConvWaves;  // output pixels per block tile
+constexpr int kConvLdsPad = 8;                  // bf16 elements of padding per weight row in LDS
 
 __device__ __forceinline__ uint4 conv_a_frag(const bf16 *__restrict__ X, int H, int Cin, int stride, int Ho,
                                              long long M, long long m, int k) {
@@ -605,7 +609,7 @@ __device__ __forceinline__ uint4 conv_a_frag(const bf16 *__restrict__ X, int H, 
 }
 
 template <int NT>  // column tiles of 16 output channels
-__global__ void __launch_bounds__(kB) k_conv_fwd_lds(const bf16 *__restrict__ X, int n, int H, int Cin, int stride,
+__global__ void __launch_bounds__(kConvThreads) k_conv_fwd_lds(const bf16 *__restrict__ X, int n, int H, int Cin, int stride,
                                                      int Ho, int Kp, const bf16 *__restrict__ Wt,
                                                      const float *__restrict__ bias, bf16 *__restrict__ Y) {
     extern __shared__ uint4 s_w4[];  // [16 * NT][Kp + kConvLdsPad] bf16
@@ -613,7 +617,7 @@ __global__ void __launch_bounds__(kB) k_conv_fwd_lds(const bf16 *__restrict__ X,
     const int ldw = Kp + kConvLdsPad;
     const int Cout = 16 * NT;
     // weights into LDS, 16 bytes at a time (Kp % 8 == 0)
-    for (int q = threadIdx.x; q < Cout * (Kp / 8); q += kB) {
+    for (int q = threadIdx.x; q < Cout * (Kp / 8); q += kConvThreads) {
         const int row = q / (Kp / 8), c8 = q - row * (Kp / 8);
         *reinterpret_cast<uint4 *>(s_w + (size_t)row * ldw + 8 * c8) =
             *reinterpret_cast<const uint4 *>(Wt + (size_t)row * Kp + 8 * c8);
@@ -622,9 +626,9 @@ __global__ void __launch_bounds__(kB) k_conv_fwd_lds(const bf16 *__restrict__ X,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, gq = lane >> 4;
     const long long M = (long long)n * Ho * Ho;
-    const long long n_tiles = (M + 64 * kConvMT - 1) / (64 * kConvMT);
+    const long long n_tiles = (M + kConvTile - 1) / kConvTile;
     for (long long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        const long long mbase = (tile * 4 + wave) * (16 * kConvMT);  // this wave's first output pixel
+        const long long mbase = (tile * kConvWaves + wave) * (16 * kConvMT);  // this wave's first output pixel
         floatx4_t acc[kConvMT][NT];
 #pragma unroll
         for (int i = 0; i < kConvMT; ++i)
