@@ -225,9 +225,9 @@ int rln_estep_run(rln_estep *e, int n, const float *fimg, const float *fimg_noma
                   int inputs_on_device, const rln_particle_meta *meta,
                   rln_particle_result *results, rln_wsum *wsum);
 
-/* Particles per chunk at most (default 4,096; the first chunk of a run a quarter of it, so that the device starts
- * early; fewer when a chunk's coarse intermediates would exceed the device budget): the batch the E-step processes
- * at once, as the reference's pool of particles per task (--pool,
+/* Particles per chunk at most (default 16,384; the first chunk of a run a quarter of it and at most 1,024, so that the
+ * device starts early; fewer when a chunk's coarse intermediates would exceed the device budget): the batch the
+ * E-step processes at once, as the reference's pool of particles per task (--pool,
  * src/ml_optimiser.cpp:466, 2482-2483, 3754). Results do not depend on it, up to the order of floating-point sums
  * over particles. Ignored while intermediates are kept (one chunk). RLN_ERR_ARG below 1. */
 int rln_estep_set_pool(rln_estep *e, int max_particles);

@@ -249,7 +249,7 @@ struct Engine {
     bool graph_off = false;               // capture unsupported: launch directly
     // intermediates kept for tests
     bool keep = false;
-    int max_pool = 4096;  // particles per chunk at most (rln_estep_set_pool)
+    int max_pool = 16384;  // particles per chunk at most (rln_estep_set_pool)
     struct Kept {
         long long n_coarse = 0;
         int n_dir = 0, n_psi = 0;
@@ -2163,9 +2163,9 @@ extern "C" int rln_estep_run(rln_estep *e, int n, const float *fimg, const float
     const double budget = 24.0e9;
     std::vector<int> starts;
     {
-        // the first chunk holds at most a quarter of the pool: its lists are built before any device work of the
-        // run, the later chunks' while the device runs the chunk before
-        const int first_pool = std::max(1, e->max_pool / 4);
+        // the first chunk holds at most a quarter of the pool and at most 1,024 particles: its lists are built
+        // before any device work of the run, the later chunks' while the device runs the chunk before
+        const int first_pool = std::max(1, std::min(e->max_pool / 4, 1024));
         double used = 0.;
         int cnt = 0;
         for (int i = 0; i < n; ++i) {
