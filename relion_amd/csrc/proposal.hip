@@ -636,7 +636,7 @@ extern "C" int rln_proposal_predict_launch(rln_proposal *p, int n, const float *
     }
     // chunks run back to back on the stream (they share the activation buffers in stream order); the top-k
     // of all particles collect on the device and come back in one copy into pinned memory
-    const int chunk = 1024;
+    const int chunk = 2048;
     RLN_TRY(ensure_batch(*p, std::min(n, chunk)));
     RLN_TRY(p->top_bins.ensure(sizeof(int) * (size_t)std::max(n, 1) * topk));
     RLN_TRY(p->top_probs.ensure(sizeof(float) * (size_t)std::max(n, 1) * topk));
