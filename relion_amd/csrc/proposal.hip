@@ -35,6 +35,7 @@ unsigned grid1(long long n) { return (unsigned)((n + rlnp::kB - 1) / rlnp::kB); 
 
 struct rln_proposal {
     int device = 0;
+    int n_cu = 0;  // compute units of the device (persistent grids)
     rln_proposal_config cfg{};
     hipStream_t stream = nullptr;
     std::mutex mtx;
@@ -145,12 +146,59 @@ int conv_fwd_lds_t(rln_proposal &p, const ConvLayer &L, int n) {
     return RLN_OK;
 }
 
+// The 64-channel layers with staged input patches (k_conv_fwd_patch): tiles of 256 / stride output pixels that start
+// at row or image boundaries, and a patch that fits beside the weights in a CU's 160 KB of LDS.
+bool patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
+    if (L.cin != 64 || L.cout != 64 || L.Kp != rlnp::kPatchK || (L.stride != 1 && L.stride != 2)) return false;
+    g.H = L.H;
+    g.Ho = L.Ho;
+    g.TP = 256 / L.stride;
+    const int hoho = L.Ho * L.Ho;
+    int th;
+    if (g.TP >= hoho) {
+        if (g.TP % hoho) return false;
+        g.n_img_t = g.TP / hoho;
+        th = L.Ho;
+    } else {
+        if (hoho % g.TP || g.TP % L.Ho) return false;
+        g.n_img_t = 1;
+        th = g.TP / L.Ho;
+    }
+    g.PR = (th - 1) * L.stride + 3;
+    g.units = g.n_img_t * g.PR * L.H * 8;
+    if (g.units > rlnp::kPatchRegs * rlnp::kPatchThreads) return false;
+    lds = sizeof(rlnp::bf16) * ((size_t)64 * (rlnp::kPatchK + rlnp::kConvLdsPad) +
+                                (size_t)g.n_img_t * g.PR * L.H * rlnp::kPatchPix);
+    return lds <= 160 * 1024;
+}
+
+template <int S>
+int conv_fwd_patch_t(rln_proposal &p, const ConvLayer &L, int n, const rlnp::PatchGeom &g, size_t lds) {
+    const long long M = (long long)n * L.Ho * L.Ho;
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_fwd_patch<S>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long tiles = (M + g.TP - 1) / g.TP;
+    const unsigned grid = (unsigned)std::min<long long>(tiles, p.n_cu > 0 ? p.n_cu : 256);
+    hipLaunchKernelGGL((rlnp::k_conv_fwd_patch<S>), dim3(grid), dim3(rlnp::kPatchThreads), lds, p.stream,
+                       p.x[&L - p.conv].as<const rlnp::bf16>(), n, g, p.wb.as<const rlnp::bf16>() + L.w_off,
+                       p.w.as<const float>() + L.b_off, p.x[&L - p.conv + 1].as<rlnp::bf16>());
+    HIP_TRY(hipGetLastError());
+    return RLN_OK;
+}
+
 // logits of the n images in x[0]; with keep_col the im2col buffers are written (the backward pass needs them),
 // otherwise the convolutions run as implicit GEMMs straight from the activations
 int forward(rln_proposal &p, int n, bool keep_col = true) {
     for (int l = 0; l < kLayers; ++l) {
         const ConvLayer &L = p.conv[l];
         const long long M = (long long)n * L.Ho * L.Ho;
+        rlnp::PatchGeom pg{};
+        size_t plds = 0;
+        if (!keep_col && patch_geom(L, pg, plds)) {
+            if (L.stride == 1) RLN_TRY(conv_fwd_patch_t<1>(p, L, n, pg, plds));
+            else RLN_TRY(conv_fwd_patch_t<2>(p, L, n, pg, plds));
+            continue;
+        }
         if (!keep_col && L.cout % 16 == 0 && L.cout <= 64 && conv_lds_ok(L)) {
             switch (L.cout / 16) {
                 case 1: RLN_TRY(conv_fwd_lds_t<1>(p, L, n)); break;
@@ -345,6 +393,7 @@ extern "C" int rln_proposal_create(int device, const rln_proposal_config *cfg, r
     HIP_TRY(hipSetDevice(device));
     rln_proposal *p = new rln_proposal();
     p->device = device;
+    if (hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) p->n_cu = 0;
     p->cfg = *cfg;
     p->W = W;
     p->C = C;

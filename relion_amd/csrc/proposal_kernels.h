@@ -676,6 +676,151 @@ __global__ void __launch_bounds__(kConvThreads) k_conv_fwd_lds(const bf16 *__res
 }
 
 // --------------------------------------------------------------------------------------------------
+// Forward 3x3 convolution (inference) of the 64-channel layers (Cin = Cout = 64, K = 576) with both operands in
+// LDS. The block keeps the whole weight matrix resident and walks tiles of output pixels with a grid stride; per
+// tile it stages the input rows the tile reads (a patch: 64 channels per pixel, padded to 72 against bank
+// conflicts; rows outside the image as zeros), so each input element comes from HBM or L2 once per tile instead of
+// once per tap. A tile is 256 / S consecutive output pixels (S the stride): TH whole output rows of one image, or
+// whole images when an image has fewer pixels. While the MFMAs of a tile run, the next tile's patch is loaded into
+// registers; it goes to LDS after the barrier that ends the tile. Wave w owns two row tiles of 16 pixels and 4 / S
+// column tiles of 16 channels; per K step of 32 (half the channels of one tap) it reads its A fragments from the
+// patch and its B fragments from the weights and issues 8 MFMAs. The fragments and the k order are those of
+// k_conv_fwd_lds, so both kernels round alike.
+// --------------------------------------------------------------------------------------------------
+constexpr int kPatchWaves = 8;
+constexpr int kPatchThreads = 64 * kPatchWaves;
+constexpr int kPatchPix = 72;   // bf16 per staged input pixel
+constexpr int kPatchRegs = 9;   // 16-byte staging registers per thread: a patch holds at most 9 * 512 units
+constexpr int kPatchK = 576;    // 9 taps x 64 channels
+
+struct PatchGeom {
+    int H, Ho;
+    int TP;       // output pixels per tile
+    int n_img_t;  // images per tile (1 when a tile is part of an image)
+    int PR;       // staged input rows per image: (TH - 1) * S + 3
+    int units;    // 16-byte units of a patch: n_img_t * PR * H * 8
+};
+
+template <int S>  // stride 1 or 2
+__global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__restrict__ X, int n, PatchGeom g,
+                                                                  const bf16 *__restrict__ Wt,
+                                                                  const float *__restrict__ bias,
+                                                                  bf16 *__restrict__ Y) {
+    extern __shared__ uint4 s_dyn[];
+    constexpr int kLdw = kPatchK + kConvLdsPad;
+    bf16 *s_w = reinterpret_cast<bf16 *>(s_dyn);  // [64][kLdw]
+    bf16 *s_p = s_w + 64 * kLdw;                  // [n_img_t][PR][H][kPatchPix]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, gq = lane >> 4;
+    for (int q = tid; q < 64 * (kPatchK / 8); q += kPatchThreads) {
+        const int row = q / (kPatchK / 8), c8 = q - row * (kPatchK / 8);
+        *reinterpret_cast<uint4 *>(s_w + row * kLdw + 8 * c8) =
+            *reinterpret_cast<const uint4 *>(Wt + (size_t)row * kPatchK + 8 * c8);
+    }
+    const int HoHo = g.Ho * g.Ho;
+    const long long M = (long long)n * HoHo;
+    const long long n_tiles = (M + g.TP - 1) / g.TP;
+    // first image and first output row of tile t (tiles start at row boundaries, or at image boundaries)
+    auto tile_origin = [&](long long t, int &img0, int &oy0) {
+        const long long m0 = t * g.TP;
+        img0 = (int)(m0 / HoHo);
+        oy0 = (int)((m0 - (long long)img0 * HoHo) / g.Ho);
+    };
+    uint4 pre[kPatchRegs];
+    // unit u of the patch: 8 channels c8 of staged pixel (image it, row, col), col fastest; staged row r of an image
+    // is input row oy0 * S - 1 + r
+    auto load_patch = [&](long long t) {
+        int img0, oy0;
+        tile_origin(t, img0, oy0);
+#pragma unroll
+        for (int r = 0; r < kPatchRegs; ++r) {
+            const int u = tid + r * kPatchThreads;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (u < g.units) {
+                const int c8 = u & 7, pix = u >> 3;
+                const int rr = pix / g.H, col = pix - rr * g.H;
+                const int it = rr / g.PR, row = rr - it * g.PR;
+                const int img = img0 + it, iy = oy0 * S - 1 + row;
+                if (img < n && iy >= 0 && iy < g.H)
+                    v = *reinterpret_cast<const uint4 *>(X + (((size_t)img * g.H + iy) * g.H + col) * 64 + 8 * c8);
+            }
+            pre[r] = v;
+        }
+    };
+    constexpr int NTW = 4 / S;                       // column tiles per wave
+    const int rp = (S == 1) ? wave : (wave >> 1);    // the wave's row tiles: 2 rp, 2 rp + 1
+    const int ct0 = (S == 1) ? 0 : NTW * (wave & 1); // its first column tile
+    if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
+    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        __syncthreads();  // weights stored (first tile); the previous tile's patch reads done
+#pragma unroll
+        for (int r = 0; r < kPatchRegs; ++r) {
+            const int u = tid + r * kPatchThreads;
+            if (u < g.units) *reinterpret_cast<uint4 *>(s_p + (size_t)(u >> 3) * kPatchPix + 8 * (u & 7)) = pre[r];
+        }
+        __syncthreads();
+        if (t + gridDim.x < n_tiles) load_patch(t + gridDim.x);
+        int img0, oy0;
+        tile_origin(t, img0, oy0);
+        const long long m0 = t * g.TP;
+        // this lane's two A rows: staged pixel (image, row, column) of tap (0, 0), or none past the last pixel
+        int pb[2], px[2];
+        bool pok[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const long long m = m0 + (2 * rp + i) * 16 + r16;
+            pok[i] = m < M;
+            const int img = (int)(m / HoHo);
+            const int pl = (int)(m - (long long)img * HoHo);
+            const int oy = pl / g.Ho, ox = pl - oy * g.Ho;
+            pb[i] = ((img - img0) * g.PR + (oy - oy0) * S) * g.H;
+            px[i] = ox * S - 1;
+        }
+        floatx4_t acc[2][NTW];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < kPatchK; k0 += 32) {
+            const int tap = k0 >> 6, c = (k0 & 63) + 8 * gq;
+            const int ky = tap / 3, kx = tap - 3 * ky;
+            bf16x8_t a[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int ix = px[i] + kx;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (pok[i] && ix >= 0 && ix < g.H)
+                    v = *reinterpret_cast<const uint4 *>(s_p + (size_t)(pb[i] + ky * g.H + ix) * kPatchPix + c);
+                a[i] = __builtin_bit_cast(bf16x8_t, v);
+            }
+            bf16x8_t b[NTW];
+#pragma unroll
+            for (int j = 0; j < NTW; ++j)
+                b[j] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(
+                                                        s_w + (16 * (ct0 + j) + r16) * kLdw + k0 + 8 * gq));
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < NTW; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        // D rows 4 gq + r of each 16-pixel tile, column r16 of each channel tile: bias, ReLU, bf16 NHWC
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+            const int col = 16 * (ct0 + j) + r16;
+            const float bj = bias[col];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long long mm = m0 + (2 * rp + i) * 16 + 4 * gq + r;
+                    if (mm < M) Y[(size_t)mm * 64 + col] = to_bf16(fmaxf(acc[i][j][r] + bj, 0.f));
+                }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
 // Loss and prediction over NB bins: one block per image.
 // --------------------------------------------------------------------------------------------------
 __device__ __forceinline__ float block_max(float v, float *red) {

@@ -166,21 +166,23 @@ def test_proposal_seeds_order3_search_and_keeps_best_pose():
 
 
 def test_proposal_predict_matches_training_forward():
-    """Inference runs the convolutions as implicit GEMMs (no im2col buffer); training runs im2col + GEMM. Both
-    contract the same fragments in the same k order, so the top-k bins agree exactly."""
-    p = make_problem(box=64, px=2.0, order=2, n=8, snr=0.1, seed=94)
+    """Inference runs the convolutions as implicit GEMMs (no im2col buffer; the 64-channel layers with their input
+    rows staged in LDS); training runs im2col + GEMM. Both contract the same fragments in the same k order, so the
+    top-k bins agree exactly. 9 particles: the last stride-2 layer's tiles of two images end on a partial one."""
+    n = 9
+    p = make_problem(box=64, px=2.0, order=2, n=n, snr=0.1, seed=94)
     pp = PoseProposal(window=64, channels=64, healpix_order=2, n_psi_bins=24, seed=11)
     rng = np.random.default_rng(12)
     w = pp.get_params()
     w += rng.normal(0, 0.01, w.size).astype(np.float32)  # non-zero biases
     pp.set_params(w)
-    imgs = pp.images(p.fimg, p.fctf, 8, 64)
-    _, _, logits = pp.forward_backward(imgs, np.zeros(8, np.int32))
-    bins, probs = pp.predict(p.fimg, p.fctf, 8, 64, topk=16)
+    imgs = pp.images(p.fimg, p.fctf, n, 64)
+    _, _, logits = pp.forward_backward(imgs, np.zeros(n, np.int32))
+    bins, probs = pp.predict(p.fimg, p.fctf, n, 64, topk=16)
     z = logits.astype(np.float64)
     pr = np.exp(z - z.max(1, keepdims=True))
     pr /= pr.sum(1, keepdims=True)
-    for i in range(8):
+    for i in range(n):
         ref = np.argsort(-z[i], kind="stable")[:16]
         assert np.array_equal(bins[i], ref), f"particle {i}: top-k bins differ"
         assert np.allclose(probs[i], pr[i][ref], rtol=1e-5, atol=1e-7)
