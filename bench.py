@@ -634,20 +634,28 @@ def main():
         cfg_g.precision = ra.RLN_PREC_SPLIT_BF16
         es_p = engine(cfg_g, args)
         pred_ms = []
+        pred_split = []  # (network, orientation lists, hand-over to the engine) per step, ms
 
         def proposal_step(count=n, mt=meta):
             bp.initMdl()
             es_p.set_model([proj], [bp], [1.0], sigma2, pdf_direction=pdf_dir)
             t1 = time.time()
             bins, probs = pp.predict(d_img.address, d_ctf.address, count, box, topk=args.topk, on_device=True)
+            t2 = time.time()
             off, og = pp.orientation_lists(bins, probs, order, n_psi, min_top1=args.min_top1)
+            t3 = time.time()
             es_p.set_orientation_lists(off, og)
-            pred_ms.append(1000.0 * (time.time() - t1))
+            t4 = time.time()
+            pred_ms.append(1000.0 * (t4 - t1))
+            pred_split.append((1000.0 * (t2 - t1), 1000.0 * (t3 - t2), 1000.0 * (t4 - t3)))
             res, _ = es_p.run(d_img.address, d_ctf.address, mt, on_device=True)
             return res, off
 
         runs["proposal"] = timed("proposal", proposal_step, n, args.steps, args.warmup, es_p)
         extra["proposal_predict_ms"] = round(float(np.mean(pred_ms[-args.steps:])), 2)
+        sp = np.mean(np.array(pred_split[-args.steps:]), axis=0)
+        extra["proposal_predict_split_ms"] = {"network": round(float(sp[0]), 2), "orientation_lists": round(float(sp[1]), 2),
+                                              "set_orientation_lists": round(float(sp[2]), 2)}
         extra["fine_tuning"] = tuning_summary(es_p)
 
         def pipelined_steps(steps):

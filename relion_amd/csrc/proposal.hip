@@ -36,6 +36,10 @@ unsigned grid1(long long n) { return (unsigned)((n + rlnp::kB - 1) / rlnp::kB); 
 struct rln_proposal {
     int device = 0;
     int n_cu = 0;  // compute units of the device (persistent grids)
+    // grid orientations of every bin for the last (grid order, grid psi count) asked, built once
+    mutable std::mutex orient_mtx;
+    mutable std::vector<std::vector<int>> orient_cache;
+    mutable int orient_order = -1, orient_npsi = -1;
     rln_proposal_config cfg{};
     hipStream_t stream = nullptr;
     std::mutex mtx;
@@ -782,10 +786,16 @@ extern "C" int rln_proposal_orientation_sets(const rln_proposal *p, int n, const
         RLN_FAIL(RLN_ERR_ARG, "rln_proposal_orientation_sets: invalid grid");
     for (long long q = 0; q < (long long)n * topk; ++q)
         if (bins[q] < 0 || bins[q] >= p->NB) RLN_FAIL(RLN_ERR_ARG, "bin %d out of range", bins[q]);
-    // orientations of every bin (the grid is fixed), then the particles' sorted unions in parallel; offsets and
+    // orientations of every bin (built once per grid), then the particles' sorted unions in parallel; offsets and
     // the copy into the caller's arrays follow in particle order
-    std::vector<std::vector<int>> cache(p->NB);
-    for (int b = 0; b < p->NB; ++b) bin_orients(*p, grid_order, n_psi_grid, b, cache[b]);
+    std::lock_guard<std::mutex> lk(p->orient_mtx);
+    if (p->orient_order != grid_order || p->orient_npsi != n_psi_grid || (int)p->orient_cache.size() != p->NB) {
+        p->orient_cache.assign(p->NB, std::vector<int>());
+        for (int b = 0; b < p->NB; ++b) bin_orients(*p, grid_order, n_psi_grid, b, p->orient_cache[b]);
+        p->orient_order = grid_order;
+        p->orient_npsi = n_psi_grid;
+    }
+    const std::vector<std::vector<int>> &cache = p->orient_cache;
     std::vector<std::vector<int>> lists(n);
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     auto build = [&](int t0) {

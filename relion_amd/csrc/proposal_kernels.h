@@ -564,17 +564,24 @@ __global__ void __launch_bounds__(kB) k_conv_fwd(const bf16 *__restrict__ X, int
             acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8_t, bv), acc[j], 0, 0, 0);
         }
     }
-    // D rows 4*gq + r of the wave's 16 pixels, column r16 of each tile
-    const int Cout = 16 * NT;
+    // D rows 4*gq + r of the wave's 16 pixels, column r16 of each tile: bias and ReLU into LDS, then the wave's 16
+    // consecutive NHWC pixels (16 x Cout contiguous bf16) leave in 16-byte stores
+    constexpr int Cout = 16 * NT;
+    __shared__ __align__(16) bf16 s_out[4][16][Cout + 8];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
         const int col = 16 * j + r16;
         const float bj = bias[col];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const long long mm = ((long long)blockIdx.x * 4 + wave) * 16 + 4 * gq + r;
-            if (mm < M) Y[(size_t)mm * Cout + col] = to_bf16(fmaxf(acc[j][r] + bj, 0.f));
-        }
+        for (int r = 0; r < 4; ++r) s_out[wave][4 * gq + r][col] = to_bf16(fmaxf(acc[j][r] + bj, 0.f));
+    }
+    __syncthreads();
+    const long long mb = ((long long)blockIdx.x * 4 + wave) * 16;
+    for (int u = lane; u < 2 * Cout; u += 64) {
+        const int px = u / (Cout / 8), c8 = u - px * (Cout / 8);
+        if (mb + px < M)
+            *reinterpret_cast<uint4 *>(Y + (size_t)(mb + px) * Cout + 8 * c8) =
+                *reinterpret_cast<const uint4 *>(&s_out[wave][px][8 * c8]);
     }
 }
 

@@ -132,7 +132,7 @@ class PoseProposal:
         n, topk = bins.shape
         cap = n * topk * len(self.bin_orientations(grid_order, n_psi_grid, 0))
         off = np.zeros(n + 1, np.int32)
-        og = np.zeros(max(cap, 1), np.int32)
+        og = np.empty(max(cap, 1), np.int32)  # the library writes the first off[n] entries, the rest is not returned
         _check(lib.rln_proposal_orientation_sets(self._h, n, _ptr(bins, C.c_int), _ptr(probs, C.c_float), topk,
                                                  grid_order, n_psi_grid, min_top1, _ptr(off, C.c_int),
                                                  _ptr(og, C.c_int), cap))

@@ -108,6 +108,17 @@ def test_proposal_bins_cover_the_grid():
         d, j = divmod(int(og), n_psi)
         b = pp.pose_bin(rot[d], tilt[d], psi[j])
         assert og in pp.bin_orientations(3, n_psi, b)
+    # orientation lists: the sorted union of the top bins' orientations (empty below the confidence floor), also
+    # when the grid changes between calls
+    bins = rng.integers(0, pp.n_bins, size=(5, 4)).astype(np.int32)
+    probs = np.array([[0.5, 0.2, 0.1, 0.1], [0.1, 0.05, 0.05, 0.05], [0.9, 0.0, 0.0, 0.0], [0.3, 0.3, 0.2, 0.1],
+                      [0.25, 0.25, 0.25, 0.25]], np.float32)
+    for order, npsi in ((3, n_psi), (4, 96), (3, n_psi)):
+        sets = pp.orientation_sets(bins, probs, order, npsi, min_top1=0.2)
+        for i in range(5):
+            want = (np.unique(np.concatenate([pp.bin_orientations(order, npsi, int(b)) for b in bins[i]]))
+                    if probs[i, 0] >= 0.2 else np.zeros(0))
+            assert np.array_equal(sets[i], want), (order, i)
     pp.close()
 
 
