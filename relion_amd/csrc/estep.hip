@@ -216,6 +216,7 @@ struct Engine {
     DevBuf d_fdiff2, d_fweight, d_fine_n, d_nent, d_foff;
     DevBuf d_fbase;  // pose-independent part of each particle's fine diff2 (FP64)
     DevBuf d_shell_acc, d_part_acc, d_pdfc_acc, d_pdfd_acc;
+    DevBuf d_shell_tot;  // [nshell] the chunk's sigma2_noise sums over its particles
     // fine pass planned on the device: capacities (elements) the buffers were sized for, the chunk capacity B,
     // the plan header, per-chunk image pointers, work items, the tuned persistent grids and the captured graph
     int cap_B = 0;
@@ -967,6 +968,7 @@ static int ensure_fine_capacity(Engine &e) {
     RLN_TRY(e.d_foff.ensure(sizeof(long long) * cB));
     RLN_TRY(e.d_stats_f.ensure(sizeof(rlnk::PassStats) * cB));
     RLN_TRY(e.d_shell_acc.ensure(sizeof(double) * e.nshell * cB));
+    RLN_TRY(e.d_shell_tot.ensure(sizeof(double) * e.nshell));
     RLN_TRY(e.d_part_acc.ensure(sizeof(double) * 8 * cB));
     RLN_TRY(e.d_pdfc_acc.ensure(sizeof(double) * e.K * cB));
     RLN_TRY(e.d_plan.ensure(sizeof(rlnk::FinePlan)));
@@ -1628,9 +1630,12 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     HIP_TRY(hipEventRecord(e.ev[kStCount], st));
     // results
     std::vector<rlnk::PassStats> stf(B);
-    std::vector<double> shells((size_t)e.nshell * B), pacc(8 * (size_t)B), pdfc((size_t)K * B);
+    std::vector<double> shells(e.nshell), pacc(8 * (size_t)B), pdfc((size_t)K * B);
+    hipLaunchKernelGGL(rlnk::k_shell_total, dim3(e.nshell), dim3(rlnk::kBlock), 0, st, e.d_shell_acc.as<const double>(),
+                       B, e.nshell, e.d_shell_tot.as<double>());
+    RLN_TRY(launch_check("shell sums"));
     HIP_TRY(hipMemcpyAsync(stf.data(), e.d_stats_f.ptr, sizeof(rlnk::PassStats) * B, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(shells.data(), e.d_shell_acc.ptr, sizeof(double) * shells.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(shells.data(), e.d_shell_tot.ptr, sizeof(double) * shells.size(), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(pacc.data(), e.d_part_acc.ptr, sizeof(double) * pacc.size(), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(pdfc.data(), e.d_pdfc_acc.ptr, sizeof(double) * pdfc.size(), hipMemcpyDeviceToHost, st));
     // best fine pose per particle: the pair's coarse index and the job's orientation entry
@@ -1701,42 +1706,60 @@ static int run_chunk(Engine &e, int B, const float *fimg, const float *fimgn, co
     e.counters[21] = e.wsum_order;
     e.counters[22] = kBandChoices[wsum_band_choice(e)];
     for (int b = 0; b < B; ++b) {
-        const rlnk::PassStats &sf = stf[b];
-        if (sf.status != 0) RLN_FAIL(sf.status, "fine pass of particle %d: zero or NaN sum of weights", b);
-        rln_particle_result &r = results[b];
-        std::memset(&r, 0, sizeof(r));
-        r.coarse_min_diff2 = stc[b].min_diff2;
-        r.coarse_sum_weight = stc[b].sum_weight;
-        r.coarse_significant_weight = stc[b].threshold;
-        r.nr_significant = (double)stc[b].nr_sign;
-        r.min_diff2 = sf.min_diff2;
-        r.sum_weight = sf.sum_weight;
-        r.significant_weight = sf.threshold;
-        r.nr_fine_evaluated = 32 * counts[b];
-        r.pmax = sf.max_weight / sf.sum_weight;
-        // decode the best pose
-        if (sf.best_index < 0) RLN_FAIL(RLN_ERR_FILTERED_ZERO, "no positive fine weight for particle %d", b);
-        const int c = all_pairs[pair_off[b] + (sf.best_index >> 5)];
-        const int o = c / NT, t = c % NT;
-        const int rr = (int)((sf.best_index >> 2) & 7), u = (int)(sf.best_index & 3);
-        const int og = h.or_frame[h.or_off[b] + o] / 8, k = h.or_class[h.or_off[b] + o];
-        e.grid.oversampled_orientation(og / e.grid.n_psi, og % e.grid.n_psi, rr, r.rot, r.tilt, r.psi);
-        r.xoff = meta[b].old_offset_x + e.over_x[t * 4 + u];
-        r.yoff = meta[b].old_offset_y + e.over_y[t * 4 + u];
-        r.iclass = k + 1;
-        r.best_ihidden_over = ((long long)o * NT + t) * 32 + rr * 4 + u;
-        r.wsum_norm_correction = pacc[8 * b + 2];
-        double xa = pacc[8 * b + 3], aa = pacc[8 * b + 4];
-        if (cfg.do_scale_correction) {
-            xa /= meta[b].scale;
-            aa /= meta[b].scale * meta[b].scale;
+        if (stf[b].status != 0) RLN_FAIL(stf[b].status, "fine pass of particle %d: zero or NaN sum of weights", b);
+        if (stf[b].best_index < 0) RLN_FAIL(RLN_ERR_FILTERED_ZERO, "no positive fine weight for particle %d", b);
+    }
+    // per-particle results (the best pose decoded to angles and offsets) on several host threads; the sums over
+    // particles below in particle order
+    auto decode = [&](int b0, int b1) {
+        for (int b = b0; b < b1; ++b) {
+            const rlnk::PassStats &sf = stf[b];
+            rln_particle_result &r = results[b];
+            std::memset(&r, 0, sizeof(r));
+            r.coarse_min_diff2 = stc[b].min_diff2;
+            r.coarse_sum_weight = stc[b].sum_weight;
+            r.coarse_significant_weight = stc[b].threshold;
+            r.nr_significant = (double)stc[b].nr_sign;
+            r.min_diff2 = sf.min_diff2;
+            r.sum_weight = sf.sum_weight;
+            r.significant_weight = sf.threshold;
+            r.nr_fine_evaluated = 32 * counts[b];
+            r.pmax = sf.max_weight / sf.sum_weight;
+            // decode the best pose
+            const int c = all_pairs[pair_off[b] + (sf.best_index >> 5)];
+            const int o = c / NT, t = c % NT;
+            const int rr = (int)((sf.best_index >> 2) & 7), u = (int)(sf.best_index & 3);
+            const int og = h.or_frame[h.or_off[b] + o] / 8, k = h.or_class[h.or_off[b] + o];
+            e.grid.oversampled_orientation(og / e.grid.n_psi, og % e.grid.n_psi, rr, r.rot, r.tilt, r.psi);
+            r.xoff = meta[b].old_offset_x + e.over_x[t * 4 + u];
+            r.yoff = meta[b].old_offset_y + e.over_y[t * 4 + u];
+            r.iclass = k + 1;
+            r.best_ihidden_over = ((long long)o * NT + t) * 32 + rr * 4 + u;
+            r.wsum_norm_correction = pacc[8 * b + 2];
+            double xa = pacc[8 * b + 3], aa = pacc[8 * b + 4];
+            if (cfg.do_scale_correction) {
+                xa /= meta[b].scale;
+                aa /= meta[b].scale * meta[b].scale;
+            }
+            r.signal_product = xa;
+            r.reference_power = aa;
+            r.dLL = log(sf.sum_weight) - sf.min_diff2 - e.logsigma2;
         }
-        r.signal_product = xa;
-        r.reference_power = aa;
-        r.dLL = log(sf.sum_weight) - sf.min_diff2 - e.logsigma2;
+    };
+    {
+        const int nt = B >= 1024 ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t)
+            pool.emplace_back(decode, (int)((long long)B * t / nt), (int)((long long)B * (t + 1) / nt));
+        decode(0, (int)((long long)B / nt));
+        for (auto &th : pool) th.join();
+    }
+    if (wsum && wsum->sigma2_noise)
+        for (int s = 0; s < e.nshell; ++s) wsum->sigma2_noise[s] += shells[s];
+    for (int b = 0; b < B; ++b) {
+        const rln_particle_result &r = results[b];
+        const double xa = r.signal_product, aa = r.reference_power;
         if (wsum) {
-            if (wsum->sigma2_noise)
-                for (int s = 0; s < e.nshell; ++s) wsum->sigma2_noise[s] += shells[(size_t)b * e.nshell + s];
             if (wsum->pdf_class)
                 for (int kk = 0; kk < K; ++kk) wsum->pdf_class[kk] += pdfc[(size_t)b * K + kk];
             wsum->sumw_group += pacc[8 * b + 0];
@@ -1970,7 +1993,7 @@ extern "C" int rln_estep_destroy(rln_estep *e) {
                            &e->d_njobs, &e->d_nchunks, &e->d_pair_off, &e->d_job_off, &e->d_chunk_off, &e->d_pairs,
                            &e->d_jobs, &e->d_job_order, &e->d_chunk_job, &e->d_chunk_first, &e->d_fdiff2, &e->d_fbase, &e->d_fweight,
                            &e->d_fine_n, &e->d_nent, &e->d_foff, &e->d_shell_acc, &e->d_part_acc, &e->d_pdfc_acc, &e->d_pdfd_acc,
-                           &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w, &e->d_wsegs, &e->d_seg_first,
+                           &e->d_shell_tot, &e->d_plan, &e->d_cp, &e->d_items_s, &e->d_items_w, &e->d_wsegs, &e->d_seg_first,
                            &e->d_wpart, &e->d_hcnt, &e->d_hsum, &e->d_wcand, &e->d_segbest, &e->d_ocnt};
     rln::drop_fine_graphs(*e);
     for (rln::DevBuf *b : bufs) b->release();

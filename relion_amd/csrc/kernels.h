@@ -2092,6 +2092,23 @@ __global__ void k_reset_items(FinePlan *plan) {
 }
 
 // Zero fill used inside captured graphs instead of memset nodes.
+// Sum over the B particles of each shell's weighted sigma2_noise contribution: block s sums column s of the
+// [B][nshell] per-particle accumulators in a fixed order (strided partial sums, then a tree). grid (nshell).
+__global__ void __launch_bounds__(kBlock) k_shell_total(const double *__restrict__ acc, int B, int nshell,
+                                                        double *__restrict__ out) {
+    __shared__ double red[kBlock];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    double v = 0.;
+    for (int b = tid; b < B; b += kBlock) v += acc[(size_t)b * nshell + s];
+    red[tid] = v;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) out[s] = red[0];
+}
+
 __global__ void __launch_bounds__(kBlock) k_zero_d(double *__restrict__ p, long long n) {
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) p[i] = 0.;
 }
