@@ -134,3 +134,29 @@ def test_fine_orders_match_particle_order():
     with pytest.raises(ra.RelionError):
         es.set_wsum_bands(3)
     es.close()
+
+
+def test_large_chunks_decode_on_host_threads():
+    """Chunks of at least 1,024 particles decode their per-particle results on several host threads and sum the
+    sigma2_noise shells on the device: 1,100 particles at the default pool (a first chunk of 1,024, then 76) give
+    the results of a pool of 1,000 (chunks of 250 and 850, each decoded on one thread), up to the order of
+    floating-point sums."""
+    n = 1100
+    p = make_problem(box=32, px=4.0, order=1, n=n, snr=0.2, seed=61, offset_range_px=3, offset_step_px=1)
+    print(f"[chunks] box 32 order 1 global, {n} particles, default pool vs pool 1,000", flush=True)
+    es, pr, bp = engine(p, ra.RLN_PREC_PLAIN_FP32)
+    r1, w1, v1, c1 = run(es, pr, bp, p, None)
+    es.set_pool(1000)
+    r2, w2, v2, c2 = run(es, pr, bp, p, None)
+    print(f"[chunks] {c1} chunks vs {c2} chunks", flush=True)
+    assert c1 == 2 and c2 == 2
+    for f in ("best_ihidden_over", "iclass", "nr_fine_evaluated", "rot", "tilt", "psi", "xoff", "yoff"):
+        assert np.array_equal(r1[f], r2[f]), f
+    for f in ("min_diff2", "sum_weight", "pmax", "dLL", "coarse_min_diff2"):
+        assert np.allclose(r1[f], r2[f], rtol=1e-9, atol=0), f
+    for f, rtol in (("sigma2_noise", 1e-5), ("pdf_class", 1e-9), ("LL", 1e-9)):
+        assert np.allclose(np.asarray(w1[f]), np.asarray(w2[f]), rtol=rtol, atol=1e-300), f
+    for a, b, name in zip(v1, v2, ("data re", "data im", "weight")):
+        err = np.linalg.norm(a - b) / np.linalg.norm(a)
+        assert err <= 1e-5, f"{name}: relative norm difference {err:.3g}"
+    es.close()
