@@ -52,9 +52,10 @@ def parse():
     ap.add_argument("--box", type=int, default=256)
     ap.add_argument("--train-samples", type=int, default=200000)
     ap.add_argument("--train-epochs", type=int, default=5)
-    ap.add_argument("--finetune-samples", type=int, default=100000,
+    ap.add_argument("--finetune-samples", type=int, default=25000,
                     help="fresh projections of the warm-started training of the next iteration, which also sees "
-                         "the particles labelled with this iteration's max-likelihood poses (0: skip)")
+                         "the particles labelled with this iteration's max-likelihood poses (0: skip); 25,000, "
+                         "50,000 and 100,000 kept the exhaustive best pose alike on the 2,000 particles checked")
     ap.add_argument("--topk", type=int, default=16)
     ap.add_argument("--min-top1", type=float, default=0.2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
