@@ -190,15 +190,23 @@ int conv_fwd_patch_t(rln_proposal &p, const ConvLayer &L, int n, const rlnp::Pat
     return RLN_OK;
 }
 
-// logits of the n images in x[0]; with keep_col the im2col buffers are written (the backward pass needs them),
-// otherwise the convolutions run as implicit GEMMs straight from the activations
+// logits of the n images in x[0]; with keep_col the im2col buffers are written (the weight gradients of the
+// backward pass read them), otherwise the convolutions run as implicit GEMMs straight from the activations. The
+// 64-channel layers compute their outputs with k_conv_fwd_patch either way, so training and prediction round alike.
 int forward(rln_proposal &p, int n, bool keep_col = true) {
     for (int l = 0; l < kLayers; ++l) {
         const ConvLayer &L = p.conv[l];
         const long long M = (long long)n * L.Ho * L.Ho;
         rlnp::PatchGeom pg{};
         size_t plds = 0;
-        if (!keep_col && patch_geom(L, pg, plds)) {
+        if (patch_geom(L, pg, plds)) {
+            if (keep_col) {
+                const long long threads = M * 9 * (L.cin / 8);
+                hipLaunchKernelGGL(rlnp::k_im2col, dim3(grid1(threads)), dim3(rlnp::kB), 0, p.stream,
+                                   p.x[l].as<const rlnp::bf16>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp,
+                                   p.col[l].as<rlnp::bf16>());
+                HIP_TRY(hipGetLastError());
+            }
             if (L.stride == 1) RLN_TRY(conv_fwd_patch_t<1>(p, L, n, pg, plds));
             else RLN_TRY(conv_fwd_patch_t<2>(p, L, n, pg, plds));
             continue;
