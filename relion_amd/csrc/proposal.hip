@@ -53,6 +53,7 @@ struct rln_proposal {
     int cap = 0;
     rln::DevBuf x[rln::kLayers + 1], col[rln::kLayers], logits, dlog, dtmp, dz_a, dz_b, loss, labels, idx;
     rln::DevBuf wt;  // one layer's weights rearranged for its data gradient (k_wt_bwd)
+    rln::DevBuf dzt;  // a 64-channel layer's output gradient transposed to [64][M] for its weight gradient
     rln::DevBuf src, src_ctf;  // particle images and CTFs staged from the host
     rln::DevBuf top_bins, top_probs;  // predictions of one predict call
     // a prediction launched and not yet finished: its size, the pinned host copies of its top-k, its end event
@@ -99,6 +100,7 @@ int ensure_batch(rln_proposal &p, int n) {
     const size_t zmax = (size_t)cap * W * W * C;
     RLN_TRY(p.dz_a.ensure(sizeof(rlnp::bf16) * zmax));
     RLN_TRY(p.dz_b.ensure(sizeof(rlnp::bf16) * zmax));
+    RLN_TRY(p.dzt.ensure(sizeof(rlnp::bf16) * zmax));
     RLN_TRY(p.loss.ensure(sizeof(double) * cap));
     RLN_TRY(p.labels.ensure(sizeof(int) * cap));
     RLN_TRY(p.idx.ensure(sizeof(int) * cap));
@@ -200,10 +202,9 @@ int forward(rln_proposal &p, int n, bool keep_col = true) {
         rlnp::PatchGeom pg{};
         size_t plds = 0;
         if (patch_geom(L, pg, plds)) {
-            if (keep_col) {
-                const long long threads = M * 9 * (L.cin / 8);
-                hipLaunchKernelGGL(rlnp::k_im2col, dim3(grid1(threads)), dim3(rlnp::kB), 0, p.stream,
-                                   p.x[l].as<const rlnp::bf16>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp,
+            if (keep_col) {  // k-major columns (colT[k][m]) for the weight gradient
+                hipLaunchKernelGGL(rlnp::k_im2col_t64, dim3((unsigned)((M + 63) / 64), 9), dim3(rlnp::kB), 0,
+                                   p.stream, p.x[l].as<const rlnp::bf16>(), n, L.H, L.stride, L.Ho,
                                    p.col[l].as<rlnp::bf16>());
                 HIP_TRY(hipGetLastError());
             }
@@ -267,11 +268,23 @@ int backward(rln_proposal &p, int n) {
     for (int l = kLayers - 1; l >= 0; --l) {
         const ConvLayer &L = p.conv[l];
         const int M = n * L.Ho * L.Ho;
-        // dW = dZ^T col, split over the rows
+        // dW = dZ^T col, split over the rows (the pixels)
         const int kchunk = 32 * std::max(1, std::min((M + 31) / 32, 64));
-        RLN_TRY((gemm<false, false, rlnp::kEpiAtomicF32>(p, dz, 1, L.cout, p.col[l].as<const rlnp::bf16>(), L.Kp, 1,
-                                                          L.cout, L.Kp, M, nullptr, gw + L.w_off, nullptr, L.Kp,
-                                                          kchunk)));
+        rlnp::PatchGeom pg{};
+        size_t plds = 0;
+        if (patch_geom(L, pg, plds)) {
+            // k-major columns (forward) and dZ transposed: both operands with the pixel index contiguous
+            hipLaunchKernelGGL(rlnp::k_transpose64, dim3((unsigned)((M + 63) / 64)), dim3(rlnp::kB), 0, p.stream, dz,
+                               (long long)M, p.dzt.as<rlnp::bf16>());
+            HIP_TRY(hipGetLastError());
+            RLN_TRY((gemm<true, true, rlnp::kEpiAtomicF32>(p, p.dzt.as<const rlnp::bf16>(), M, 1,
+                                                            p.col[l].as<const rlnp::bf16>(), 1, M, L.cout, L.Kp, M,
+                                                            nullptr, gw + L.w_off, nullptr, L.Kp, kchunk)));
+        } else {
+            RLN_TRY((gemm<false, false, rlnp::kEpiAtomicF32>(p, dz, 1, L.cout, p.col[l].as<const rlnp::bf16>(), L.Kp,
+                                                              1, L.cout, L.Kp, M, nullptr, gw + L.w_off, nullptr,
+                                                              L.Kp, kchunk)));
+        }
         col_sum(p, dz, M, L.cout, gw + L.b_off);
         HIP_TRY(hipGetLastError());
         if (l == 0) break;

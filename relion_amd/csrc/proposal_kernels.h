@@ -201,6 +201,65 @@ __global__ void __launch_bounds__(kB) k_window_image(const float2 *__restrict__ 
 // im2col for a 3x3 convolution, pad 1: X [n][H][H][C] bf16 -> col [n*Ho*Ho][Kp], k = tap*C + c (k >= 9C: 0).
 // C % 8 == 0: one thread per (m, tap, 8 channels); C == 1: one thread per (m, k).
 // --------------------------------------------------------------------------------------------------
+// Tiles of 64 pixels x 64 channels transposed through LDS: k_im2col_t64 writes the im2col columns of a 64-channel
+// layer k-major (colT[k][m], k = tap * 64 + c, m the output pixel) and k_transpose64 a [M][64] gradient as [64][M],
+// so that the weight gradient dW = dZ^T col reads both operands with the pixel index contiguous. Block: 64
+// consecutive pixels (and one tap, blockIdx.y, for the columns); M % 64 == 0 is not required.
+__device__ __forceinline__ void store_tile_t64(const bf16 (*t)[72], long long m0, long long M, bf16 *__restrict__ out,
+                                               long long row0) {
+    // thread q: channel q >> 3, pixels 8 (q & 7) .. + 7 of the tile -> out[(row0 + c) * M + m0 + ...]
+    for (int q = threadIdx.x; q < 512; q += kB) {
+        const int c = q >> 3, p8 = (q & 7) * 8;
+        bf16 e[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] = t[p8 + j][c];
+        bf16 *dst = out + (size_t)(row0 + c) * M + m0 + p8;
+        if (m0 + p8 + 8 <= M) {
+            *reinterpret_cast<uint4 *>(dst) =
+                make_uint4((unsigned)e[0] | ((unsigned)e[1] << 16), (unsigned)e[2] | ((unsigned)e[3] << 16),
+                           (unsigned)e[4] | ((unsigned)e[5] << 16), (unsigned)e[6] | ((unsigned)e[7] << 16));
+        } else {
+            for (int j = 0; j < 8 && m0 + p8 + j < M; ++j) dst[j] = e[j];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kB) k_im2col_t64(const bf16 *__restrict__ X, int n, int H, int stride, int Ho,
+                                                   bf16 *__restrict__ colT) {
+    __shared__ __attribute__((aligned(16))) bf16 t[64][72];
+    const long long M = (long long)n * Ho * Ho;
+    const long long m0 = (long long)blockIdx.x * 64;
+    const int tap = blockIdx.y, ky = tap / 3, kx = tap - 3 * (tap / 3);
+    for (int q = threadIdx.x; q < 512; q += kB) {
+        const int pm = q >> 3, c8 = q & 7;
+        const long long m = m0 + pm;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (m < M) {
+            const int img = (int)(m / (Ho * Ho)), pix = (int)(m - (long long)img * Ho * Ho);
+            const int oy = pix / Ho, ox = pix - oy * Ho;
+            const int iy = oy * stride + ky - 1, ix = ox * stride + kx - 1;
+            if (iy >= 0 && iy < H && ix >= 0 && ix < H)
+                v = *reinterpret_cast<const uint4 *>(X + (((size_t)img * H + iy) * H + ix) * 64 + 8 * c8);
+        }
+        *reinterpret_cast<uint4 *>(&t[pm][8 * c8]) = v;
+    }
+    __syncthreads();
+    store_tile_t64(t, m0, M, colT, (long long)tap * 64);
+}
+
+__global__ void __launch_bounds__(kB) k_transpose64(const bf16 *__restrict__ A, long long M, bf16 *__restrict__ AT) {
+    __shared__ __attribute__((aligned(16))) bf16 t[64][72];
+    const long long m0 = (long long)blockIdx.x * 64;
+    for (int q = threadIdx.x; q < 512; q += kB) {
+        const int pm = q >> 3, c8 = q & 7;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (m0 + pm < M) v = *reinterpret_cast<const uint4 *>(A + (size_t)(m0 + pm) * 64 + 8 * c8);
+        *reinterpret_cast<uint4 *>(&t[pm][8 * c8]) = v;
+    }
+    __syncthreads();
+    store_tile_t64(t, m0, M, AT, 0);
+}
+
 __global__ void __launch_bounds__(kB) k_im2col(const bf16 *__restrict__ X, int n, int H, int C, int stride, int Ho,
                                                int Kp, bf16 *__restrict__ col) {
     const long long idx = (long long)blockIdx.x * kB + threadIdx.x;
