@@ -178,18 +178,26 @@ bool patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
     return lds <= 160 * 1024;
 }
 
-template <int S>
-int conv_fwd_patch_t(rln_proposal &p, const ConvLayer &L, int n, const rlnp::PatchGeom &g, size_t lds) {
-    const long long M = (long long)n * L.Ho * L.Ho;
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_fwd_patch<S>),
+// k_conv_fwd_patch over n images of geometry g: input X, weights Wt, bias (EPI 0) or ReLU mask Ymask (EPI 1), output Y
+template <int S, int EPI>
+int conv_patch_launch(rln_proposal &p, int n, const rlnp::PatchGeom &g, size_t lds, const rlnp::bf16 *X,
+                      const rlnp::bf16 *Wt, const float *bias, rlnp::bf16 *Y, const rlnp::bf16 *Ymask) {
+    const long long M = (long long)n * g.Ho * g.Ho;
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_fwd_patch<S, EPI>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long tiles = (M + g.TP - 1) / g.TP;
     const unsigned grid = (unsigned)std::min<long long>(tiles, p.n_cu > 0 ? p.n_cu : 256);
-    hipLaunchKernelGGL((rlnp::k_conv_fwd_patch<S>), dim3(grid), dim3(rlnp::kPatchThreads), lds, p.stream,
-                       p.x[&L - p.conv].as<const rlnp::bf16>(), n, g, p.wb.as<const rlnp::bf16>() + L.w_off,
-                       p.w.as<const float>() + L.b_off, p.x[&L - p.conv + 1].as<rlnp::bf16>());
+    hipLaunchKernelGGL((rlnp::k_conv_fwd_patch<S, EPI>), dim3(grid), dim3(rlnp::kPatchThreads), lds, p.stream, X, n,
+                       g, Wt, bias, Y, Ymask);
     HIP_TRY(hipGetLastError());
     return RLN_OK;
+}
+
+template <int S>
+int conv_fwd_patch_t(rln_proposal &p, const ConvLayer &L, int n, const rlnp::PatchGeom &g, size_t lds) {
+    return conv_patch_launch<S, 0>(p, n, g, lds, p.x[&L - p.conv].as<const rlnp::bf16>(),
+                                   p.wb.as<const rlnp::bf16>() + L.w_off, p.w.as<const float>() + L.b_off,
+                                   p.x[&L - p.conv + 1].as<rlnp::bf16>(), nullptr);
 }
 
 // logits of the n images in x[0]; with keep_col the im2col buffers are written (the weight gradients of the
@@ -288,7 +296,15 @@ int backward(rln_proposal &p, int n) {
         col_sum(p, dz, M, L.cout, gw + L.b_off);
         HIP_TRY(hipGetLastError());
         if (l == 0) break;
-        if (bwd_data_direct(L)) {
+        if (L.stride == 1 && patch_geom(L, pg, plds)) {
+            // a stride-1 64-channel layer: its data gradient is a stride-1 convolution of dZ with the flipped,
+            // transposed weights, masked by the ReLU of its input (same geometry as its forward pass)
+            hipLaunchKernelGGL(rlnp::k_wt_flip, dim3(grid1(64 * rlnp::kPatchK)), dim3(rlnp::kB), 0, p.stream,
+                               wbf(p, L.w_off), p.wt.as<rlnp::bf16>());
+            HIP_TRY(hipGetLastError());
+            RLN_TRY((conv_patch_launch<1, 1>(p, n, pg, plds, dz, p.wt.as<const rlnp::bf16>(), nullptr, dz_next,
+                                             p.x[l].as<const rlnp::bf16>())));
+        } else if (bwd_data_direct(L)) {
             const int nwt = 9 * L.cin * L.cout;
             hipLaunchKernelGGL(rlnp::k_wt_bwd, dim3(grid1(nwt)), dim3(rlnp::kB), 0, p.stream, wbf(p, L.w_off), L.cin,
                                L.cout, L.Kp, p.wt.as<rlnp::bf16>());

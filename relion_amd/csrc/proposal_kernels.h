@@ -767,11 +767,15 @@ struct PatchGeom {
     int units;    // 16-byte units of a patch: n_img_t * PR * H * 8
 };
 
-template <int S>  // stride 1 or 2
+// EPI 0: bias and ReLU (a forward layer). EPI 1 (stride 1): the data gradient of a stride-1 layer, X its output
+// gradient dZ, Wt its weights transposed with the taps flipped (k_wt_flip), the result masked by the ReLU of the
+// layer's input Ymask (dX = (Ymask > 0) * sum), no bias.
+template <int S, int EPI = 0>
 __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__restrict__ X, int n, PatchGeom g,
                                                                   const bf16 *__restrict__ Wt,
                                                                   const float *__restrict__ bias,
-                                                                  bf16 *__restrict__ Y) {
+                                                                  bf16 *__restrict__ Y,
+                                                                  const bf16 *__restrict__ Ymask = nullptr) {
     extern __shared__ uint4 s_dyn[];
     constexpr int kLdw = kPatchK + kConvLdsPad;
     bf16 *s_w = reinterpret_cast<bf16 *>(s_dyn);  // [64][kLdw]
@@ -870,20 +874,35 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__
                 for (int j = 0; j < NTW; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-        // D rows 4 gq + r of each 16-pixel tile, column r16 of each channel tile: bias, ReLU, bf16 NHWC
+        // D rows 4 gq + r of each 16-pixel tile, column r16 of each channel tile: bias and ReLU (or the ReLU mask of
+        // Ymask), bf16 NHWC
 #pragma unroll
         for (int j = 0; j < NTW; ++j) {
             const int col = 16 * (ct0 + j) + r16;
-            const float bj = bias[col];
+            const float bj = EPI == 0 ? bias[col] : 0.f;
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const long long mm = m0 + (2 * rp + i) * 16 + 4 * gq + r;
-                    if (mm < M) Y[(size_t)mm * 64 + col] = to_bf16(fmaxf(acc[i][j][r] + bj, 0.f));
+                    if (mm >= M) continue;
+                    if (EPI == 0)
+                        Y[(size_t)mm * 64 + col] = to_bf16(fmaxf(acc[i][j][r] + bj, 0.f));
+                    else
+                        Y[(size_t)mm * 64 + col] =
+                            to_bf16(from_bf16(Ymask[(size_t)mm * 64 + col]) > 0.f ? acc[i][j][r] : 0.f);
                 }
         }
     }
+}
+
+// Weights of a 64-channel layer for its data gradient as a stride-1 convolution of dZ: transposed with the taps
+// flipped, Wf[c][t*64 + co] = W[co][(8 - t)*64 + c] (output pixel y takes dZ at y + 1 - ky = y + ky' - 1, ky' = 2 - ky).
+__global__ void __launch_bounds__(kB) k_wt_flip(const bf16 *__restrict__ W, bf16 *__restrict__ Wf) {
+    const int idx = blockIdx.x * kB + threadIdx.x;
+    if (idx >= 64 * kPatchK) return;
+    const int c = idx / kPatchK, k = idx - c * kPatchK, t = k >> 6, co = k & 63;
+    Wf[idx] = W[(size_t)co * kPatchK + (8 - t) * 64 + c];
 }
 
 // --------------------------------------------------------------------------------------------------
