@@ -178,6 +178,32 @@ bool patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
     return lds <= 160 * 1024;
 }
 
+// The stride-2 64-channel layers' data gradient (k_conv_bwd_s2_patch): tiles of 256 positions of the dZ grid that
+// start at row or image boundaries, each staged with the dZ row after it.
+bool bwd_s2_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
+    if (L.cin != 64 || L.cout != 64 || L.Kp != rlnp::kPatchK || L.stride != 2 || L.H != 2 * L.Ho) return false;
+    g.H = L.H;
+    g.Ho = L.Ho;
+    g.TP = 256;
+    const int hoho = L.Ho * L.Ho;
+    int th;
+    if (g.TP >= hoho) {
+        if (g.TP % hoho) return false;
+        g.n_img_t = g.TP / hoho;
+        th = L.Ho;
+    } else {
+        if (hoho % g.TP || g.TP % L.Ho) return false;
+        g.n_img_t = 1;
+        th = g.TP / L.Ho;
+    }
+    g.PR = th + 1;
+    g.units = g.n_img_t * g.PR * L.Ho * 8;
+    if (g.units > rlnp::kPatchRegs * rlnp::kPatchThreads) return false;
+    lds = sizeof(rlnp::bf16) * ((size_t)64 * (rlnp::kPatchK + rlnp::kConvLdsPad) +
+                                (size_t)g.n_img_t * g.PR * L.Ho * rlnp::kPatchPix);
+    return lds <= 160 * 1024;
+}
+
 // k_conv_fwd_patch over n images of geometry g: input X, weights Wt, bias (EPI 0) or ReLU mask Ymask (EPI 1), output Y
 template <int S, int EPI>
 int conv_patch_launch(rln_proposal &p, int n, const rlnp::PatchGeom &g, size_t lds, const rlnp::bf16 *X,
@@ -304,6 +330,18 @@ int backward(rln_proposal &p, int n) {
             HIP_TRY(hipGetLastError());
             RLN_TRY((conv_patch_launch<1, 1>(p, n, pg, plds, dz, p.wt.as<const rlnp::bf16>(), nullptr, dz_next,
                                              p.x[l].as<const rlnp::bf16>())));
+        } else if (bwd_s2_geom(L, pg, plds)) {
+            // a stride-2 64-channel layer: the four parity classes of each 2 x 2 input block from a staged dZ patch
+            hipLaunchKernelGGL(rlnp::k_wt_bwd, dim3(grid1(64 * rlnp::kPatchK)), dim3(rlnp::kB), 0, p.stream,
+                               wbf(p, L.w_off), L.cin, L.cout, L.Kp, p.wt.as<rlnp::bf16>());
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_bwd_s2_patch),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
+            const long long tiles = ((long long)n * L.Ho * L.Ho + pg.TP - 1) / pg.TP;
+            const unsigned grid = (unsigned)std::min<long long>(tiles, p.n_cu > 0 ? p.n_cu : 256);
+            hipLaunchKernelGGL(rlnp::k_conv_bwd_s2_patch, dim3(grid), dim3(rlnp::kPatchThreads), plds, p.stream, dz, n,
+                               pg, p.wt.as<const rlnp::bf16>(), p.x[l].as<const rlnp::bf16>(), dz_next);
+            HIP_TRY(hipGetLastError());
         } else if (bwd_data_direct(L)) {
             const int nwt = 9 * L.cin * L.cout;
             hipLaunchKernelGGL(rlnp::k_wt_bwd, dim3(grid1(nwt)), dim3(rlnp::kB), 0, p.stream, wbf(p, L.w_off), L.cin,

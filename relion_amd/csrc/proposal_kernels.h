@@ -905,6 +905,144 @@ __global__ void __launch_bounds__(kB) k_wt_flip(const bf16 *__restrict__ W, bf16
     Wf[idx] = W[(size_t)co * kPatchK + (8 - t) * 64 + c];
 }
 
+// Data gradient of a stride-2 64-channel layer (input H x H, output Ho x Ho, H = 2 Ho), masked by the ReLU of its
+// input Ymask. Input pixel (2a + py, 2b + px) takes dZ at (a + dy, b + dx) through the taps ky with
+// 2 oy + ky - 1 = 2a + py, i.e. ky = 1 (dy = 0) for py = 0 and ky = 0 (dy = 1), 2 (dy = 0) for py = 1; likewise kx.
+// So every position (a, b) of the dZ grid yields the four pixels of its 2 x 2 block, one per parity class (py, px),
+// from the dZ pixels at (a..a+1, b..b+1), through 1, 2, 2 and 4 taps: 9 taps of 64 channels per position. Tiles of
+// 256 positions start at row or image boundaries (PatchGeom over the dZ grid: H the input size, Ho the dZ size, PR
+// = the tile's dZ rows plus the next one, zero past the image); the weights Wd[c][tap*64 + co] = W[co][tap*64 + c]
+// (k_wt_bwd) stay in LDS. Each wave: two 16-position row tiles x 64 channels, class by class.
+__global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 *__restrict__ dZ, int n, PatchGeom g,
+                                                                     const bf16 *__restrict__ Wd,
+                                                                     const bf16 *__restrict__ Ymask,
+                                                                     bf16 *__restrict__ dX) {
+    extern __shared__ uint4 s_dyn[];
+    constexpr int kLdw = kPatchK + kConvLdsPad;
+    bf16 *s_w = reinterpret_cast<bf16 *>(s_dyn);  // [64][kLdw]
+    bf16 *s_p = s_w + 64 * kLdw;                  // [n_img_t][PR][Ho][kPatchPix]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, gq = lane >> 4;
+    for (int q = tid; q < 64 * (kPatchK / 8); q += kPatchThreads) {
+        const int row = q / (kPatchK / 8), c8 = q - row * (kPatchK / 8);
+        *reinterpret_cast<uint4 *>(s_w + row * kLdw + 8 * c8) =
+            *reinterpret_cast<const uint4 *>(Wd + (size_t)row * kPatchK + 8 * c8);
+    }
+    const int Ho = g.Ho, H = g.H, HoHo = Ho * Ho;
+    const long long M = (long long)n * HoHo;
+    const long long n_tiles = (M + g.TP - 1) / g.TP;
+    auto tile_origin = [&](long long t, int &img0, int &a0) {
+        const long long m0 = t * g.TP;
+        img0 = (int)(m0 / HoHo);
+        a0 = (int)((m0 - (long long)img0 * HoHo) / Ho);
+    };
+    uint4 pre[kPatchRegs];
+    // unit u of the patch: 8 channels c8 of staged pixel (image it, row, col), col fastest; staged row r of an image
+    // is dZ row a0 + r
+    auto load_patch = [&](long long t) {
+        int img0, a0;
+        tile_origin(t, img0, a0);
+#pragma unroll
+        for (int r = 0; r < kPatchRegs; ++r) {
+            const int u = tid + r * kPatchThreads;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (u < g.units) {
+                const int c8 = u & 7, pix = u >> 3;
+                const int rr = pix / Ho, col = pix - rr * Ho;
+                const int it = rr / g.PR, row = rr - it * g.PR;
+                const int img = img0 + it, iy = a0 + row;
+                if (img < n && iy < Ho)
+                    v = *reinterpret_cast<const uint4 *>(dZ + (((size_t)img * Ho + iy) * Ho + col) * 64 + 8 * c8);
+            }
+            pre[r] = v;
+        }
+    };
+    if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
+    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        __syncthreads();  // weights stored (first tile); the previous tile's patch reads done
+#pragma unroll
+        for (int r = 0; r < kPatchRegs; ++r) {
+            const int u = tid + r * kPatchThreads;
+            if (u < g.units) *reinterpret_cast<uint4 *>(s_p + (size_t)(u >> 3) * kPatchPix + 8 * (u & 7)) = pre[r];
+        }
+        __syncthreads();
+        if (t + gridDim.x < n_tiles) load_patch(t + gridDim.x);
+        int img0, a0;
+        tile_origin(t, img0, a0);
+        const long long m0 = t * g.TP;
+        // this lane's two A rows: the staged pixel of position (a, b) and its column b, or none past the last position
+        int pb[2], pcol[2];
+        bool pok[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const long long m = m0 + (2 * wave + i) * 16 + r16;
+            pok[i] = m < M;
+            const int img = (int)(m / HoHo);
+            const int pl = (int)(m - (long long)img * HoHo);
+            const int a = pl / Ho;
+            pcol[i] = pl - a * Ho;
+            pb[i] = ((img - img0) * g.PR + (a - a0)) * Ho;
+        }
+#pragma unroll
+        for (int cls = 0; cls < 4; ++cls) {
+            const int py = cls >> 1, px = cls & 1;
+            floatx4_t acc[2][4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ky = 1 - py; ky < 3; ky += 2)
+#pragma unroll
+                for (int kx = 1 - px; kx < 3; kx += 2) {
+                    const int dy = (py + 1 - ky) >> 1, dx = (px + 1 - kx) >> 1;
+                    const int tap = 3 * ky + kx;
+#pragma unroll
+                    for (int c0 = 0; c0 < 64; c0 += 32) {
+                        const int c = c0 + 8 * gq;
+                        bf16x8_t a[2];
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) {
+                            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                            if (pok[i] && pcol[i] + dx < Ho)
+                                v = *reinterpret_cast<const uint4 *>(
+                                    s_p + (size_t)(pb[i] + dy * Ho + pcol[i] + dx) * kPatchPix + c);
+                            a[i] = __builtin_bit_cast(bf16x8_t, v);
+                        }
+                        bf16x8_t b[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            b[j] = __builtin_bit_cast(
+                                bf16x8_t, *reinterpret_cast<const uint4 *>(s_w + (16 * j + r16) * kLdw + tap * 64 + c));
+#pragma unroll
+                        for (int i = 0; i < 2; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    }
+                }
+            // D rows 4 gq + r of each 16-position tile, column r16 of each channel tile: the class's pixel of the
+            // position, masked, bf16 NHWC
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long long mm = m0 + (2 * wave + i) * 16 + 4 * gq + r;
+                    if (mm >= M) continue;
+                    const int img = (int)(mm / HoHo);
+                    const int pl = (int)(mm - (long long)img * HoHo);
+                    const int a = pl / Ho, b = pl - a * Ho;
+                    const size_t pix = ((size_t)img * H + 2 * a + py) * H + 2 * b + px;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const size_t idx = pix * 64 + 16 * j + r16;
+                        dX[idx] = to_bf16(from_bf16(Ymask[idx]) > 0.f ? acc[i][j][r] : 0.f);
+                    }
+                }
+        }
+    }
+}
+
 // --------------------------------------------------------------------------------------------------
 // Loss and prediction over NB bins: one block per image.
 // --------------------------------------------------------------------------------------------------

@@ -50,10 +50,12 @@ def rel_norm(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_proposal_gradients_match_oracle():
+@pytest.mark.parametrize("n", [4, 3])
+def test_proposal_gradients_match_oracle(n):
+    """4 images fill the last stride-2 layer's tiles of 4 images (8 x 8 outputs each); 3 end on a partial one."""
     W, C, order, npsi = 64, 64, 2, 24
     NB = 12 * 4 ** order * npsi
-    print("[proposal] forward/backward of the full network, 4 images", flush=True)
+    print(f"[proposal] forward/backward of the full network, {n} images", flush=True)
     pp = PoseProposal(window=W, channels=C, healpix_order=order, n_psi_bins=npsi, seed=2)
     assert pp.n_bins == NB and pp.n_params == pc.n_params(W, C, NB)
     rng = np.random.default_rng(8)
@@ -63,9 +65,9 @@ def test_proposal_gradients_match_oracle():
     w = pc.pack(conv, Wfc, rng.normal(0, 0.05, NB)).astype(np.float32)
     pp.set_params(w)
     assert np.array_equal(pp.get_params(), w)
-    p = make_problem(box=64, px=2.0, order=2, n=4, snr=0.1, seed=92)
-    imgs = np.stack([pc.input_image(p.fimg[i], 64, W) for i in range(4)]).astype(np.float32)
-    labels = rng.integers(0, NB, 4)
+    p = make_problem(box=64, px=2.0, order=2, n=n, snr=0.1, seed=92)
+    imgs = np.stack([pc.input_image(p.fimg[i], 64, W) for i in range(n)]).astype(np.float32)
+    labels = rng.integers(0, NB, n)
     loss, g, logits = pp.forward_backward(imgs, labels)
     w64, i64 = w.astype(np.float64), imgs.astype(np.float64)
     lo, go, logo = pc.loss_and_grads(w64, i64, labels, C, NB, bf=True)
