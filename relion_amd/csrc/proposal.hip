@@ -54,6 +54,8 @@ struct rln_proposal {
     rln::DevBuf x[rln::kLayers + 1], col[rln::kLayers], logits, dlog, dtmp, dz_a, dz_b, loss, labels, idx;
     rln::DevBuf wt;  // one layer's weights rearranged for its data gradient (k_wt_bwd)
     rln::DevBuf dzt;  // a 64-channel layer's output gradient transposed to [64][M] for its weight gradient
+    rln::DevBuf part;  // partial sums of a k-split GEMM
+    rln::DevBuf dwpart;  // per-block weight and bias gradient sums of k_conv_dw_patch
     rln::DevBuf src, src_ctf;  // particle images and CTFs staged from the host
     rln::DevBuf top_bins, top_probs;  // predictions of one predict call
     // a prediction launched and not yet finished: its size, the pinned host copies of its top-k, its end event
@@ -75,6 +77,7 @@ bool bwd_data_direct(const ConvLayer &L) { return L.cin % 16 == 0 && L.cin <= 64
 
 // Bias gradients: out[j] += column sums of Z [M][N] (bf16)
 void col_sum(rln_proposal &p, const rlnp::bf16 *Z, int M, int N, float *out);
+bool dw_patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds);
 
 int ensure_batch(rln_proposal &p, int n) {
     if (n <= p.cap) return RLN_OK;
@@ -84,7 +87,9 @@ int ensure_batch(rln_proposal &p, int n) {
     for (int l = 0; l < kLayers; ++l) {
         const ConvLayer &L = p.conv[l];
         const size_t M = (size_t)cap * L.Ho * L.Ho;
-        RLN_TRY(p.col[l].ensure(sizeof(rlnp::bf16) * M * L.Kp));
+        rlnp::PatchGeom dg{};
+        size_t dlds = 0;
+        if (!dw_patch_geom(L, dg, dlds)) RLN_TRY(p.col[l].ensure(sizeof(rlnp::bf16) * M * L.Kp));
         RLN_TRY(p.x[l + 1].ensure(sizeof(rlnp::bf16) * M * C));
     }
     size_t dmax = (size_t)cap * std::max(p.NB, p.F);
@@ -101,6 +106,7 @@ int ensure_batch(rln_proposal &p, int n) {
     RLN_TRY(p.dz_a.ensure(sizeof(rlnp::bf16) * zmax));
     RLN_TRY(p.dz_b.ensure(sizeof(rlnp::bf16) * zmax));
     RLN_TRY(p.dzt.ensure(sizeof(rlnp::bf16) * zmax));
+    RLN_TRY(p.dwpart.ensure(sizeof(float) * rlnp::kDwPart * (size_t)(p.n_cu > 0 ? p.n_cu : 256)));
     RLN_TRY(p.loss.ensure(sizeof(double) * cap));
     RLN_TRY(p.labels.ensure(sizeof(int) * cap));
     RLN_TRY(p.idx.ensure(sizeof(int) * cap));
@@ -112,10 +118,34 @@ template <bool AK, bool BK, int EPI>
 int gemm(rln_proposal &p, const rlnp::bf16 *A, long long sam, long long sak, const rlnp::bf16 *B, long long sbk,
          long long sbn, int M, int N, int K, const float *bias, float *Cf, rlnp::bf16 *Cb, long long ldc,
          int kchunk = 0) {
+    const long long tiles = (long long)((M + 63) / 64) * ((N + 63) / 64);
+    if (EPI == rlnp::kEpiF32 && kchunk <= 0) {
+        // few output tiles (the fully connected layer of a small batch): k split into nz chunks of at least 512 so
+        // that the launch holds about four blocks per CU, partial sums per chunk added in chunk order
+        const long long cu = p.n_cu > 0 ? p.n_cu : 256;
+        int nz = (int)std::min<long long>({8, (4 * cu + tiles - 1) / tiles, K / 512});
+        if (nz > 1) {
+            const int kc = 32 * ((K + 32 * nz - 1) / (32 * nz));
+            nz = (K + kc - 1) / kc;
+            RLN_TRY(p.part.ensure(sizeof(float) * (size_t)nz * M * ldc));
+            rlnp::GemmArgs a{A, B, sam, sak, sbk, sbn, M, N, K, kc, bias, p.part.as<float>(), nullptr, ldc,
+                             (long long)M * ldc};
+            hipLaunchKernelGGL((rlnp::k_gemm<AK, BK, rlnp::kEpiF32>), dim3((M + 63) / 64, (N + 63) / 64, nz),
+                               dim3(rlnp::kB), 0, p.stream, a);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(rlnp::k_sum_parts, dim3(grid1((long long)M * N)), dim3(rlnp::kB), 0, p.stream,
+                               p.part.as<const float>(), nz, M, N, ldc, Cf);
+            HIP_TRY(hipGetLastError());
+            return RLN_OK;
+        }
+    }
     if (kchunk <= 0) kchunk = 32 * ((K + 31) / 32);
-    rlnp::GemmArgs a{A, B, sam, sak, sbk, sbn, M, N, K, kchunk, bias, Cf, Cb, ldc};
+    rlnp::GemmArgs a{A, B, sam, sak, sbk, sbn, M, N, K, kchunk, bias, Cf, Cb, ldc, 0};
     dim3 grid((M + 63) / 64, (N + 63) / 64, (K + kchunk - 1) / kchunk);
-    hipLaunchKernelGGL((rlnp::k_gemm<AK, BK, EPI>), grid, dim3(rlnp::kB), 0, p.stream, a);
+    if (EPI == rlnp::kEpiAtomicF32 && grid.z == 1)  // one writer per element: a plain read-add-write
+        hipLaunchKernelGGL((rlnp::k_gemm<AK, BK, rlnp::kEpiAddF32>), grid, dim3(rlnp::kB), 0, p.stream, a);
+    else
+        hipLaunchKernelGGL((rlnp::k_gemm<AK, BK, EPI>), grid, dim3(rlnp::kB), 0, p.stream, a);
     HIP_TRY(hipGetLastError());
     return RLN_OK;
 }
@@ -204,6 +234,16 @@ bool bwd_s2_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
     return lds <= 160 * 1024;
 }
 
+// The 64-channel layers whose weight and bias gradients k_conv_dw_patch computes from the staged input rows (the
+// forward's tiles; output rows of whole 8-position fragments, H = stride * Ho); its LDS bytes in lds.
+bool dw_patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
+    size_t flds = 0;
+    if (!patch_geom(L, g, flds) || L.Ho % 8 || L.H != L.stride * L.Ho || g.TP % 32) return false;
+    if ((g.n_img_t * g.PR * L.H + 63) / 64 > rlnp::kPatchRegs) return false;  // staged pixels: 64 per register
+    lds = sizeof(rlnp::bf16) * 3 * 64 * (size_t)rlnp::dw_plane(g);
+    return lds <= 160 * 1024;
+}
+
 // k_conv_fwd_patch over n images of geometry g: input X, weights Wt, bias (EPI 0) or ReLU mask Ymask (EPI 1), output Y
 template <int S, int EPI>
 int conv_patch_launch(rln_proposal &p, int n, const rlnp::PatchGeom &g, size_t lds, const rlnp::bf16 *X,
@@ -236,7 +276,9 @@ int forward(rln_proposal &p, int n, bool keep_col = true) {
         rlnp::PatchGeom pg{};
         size_t plds = 0;
         if (patch_geom(L, pg, plds)) {
-            if (keep_col) {  // k-major columns (colT[k][m]) for the weight gradient
+            rlnp::PatchGeom dg{};
+            size_t dlds = 0;
+            if (keep_col && !dw_patch_geom(L, dg, dlds)) {  // k-major columns (colT[k][m]) for the weight gradient
                 hipLaunchKernelGGL(rlnp::k_im2col_t64, dim3((unsigned)((M + 63) / 64), 9), dim3(rlnp::kB), 0,
                                    p.stream, p.x[l].as<const rlnp::bf16>(), n, L.H, L.stride, L.Ho,
                                    p.col[l].as<rlnp::bf16>());
@@ -306,7 +348,37 @@ int backward(rln_proposal &p, int n) {
         const int kchunk = 32 * std::max(1, std::min((M + 31) / 32, 64));
         rlnp::PatchGeom pg{};
         size_t plds = 0;
-        if (patch_geom(L, pg, plds)) {
+        bool bias_done = false;
+        if (dw_patch_geom(L, pg, plds)) {
+            // weight and bias gradients from the staged input rows and dZ transposed, per-block sums reduced
+            hipLaunchKernelGGL(rlnp::k_transpose64, dim3((unsigned)((M + 63) / 64)), dim3(rlnp::kB), 0, p.stream, dz,
+                               (long long)M, p.dzt.as<rlnp::bf16>());
+            HIP_TRY(hipGetLastError());
+            const long long tiles = ((long long)M + pg.TP - 1) / pg.TP;
+            const int nb = (int)std::min<long long>(tiles, p.n_cu > 0 ? p.n_cu : 256);
+            if ((size_t)nb * rlnp::kDwPart * sizeof(float) > p.dwpart.bytes)
+                RLN_FAIL(RLN_ERR_STATE, "weight-gradient partial sums: %d blocks exceed the buffer", nb);
+            if (L.stride == 1) {
+                HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_dw_patch<1>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
+                hipLaunchKernelGGL(rlnp::k_conv_dw_patch<1>, dim3(nb), dim3(rlnp::kPatchThreads), plds, p.stream,
+                                   p.x[l].as<const rlnp::bf16>(), n, pg, p.dzt.as<const rlnp::bf16>(),
+                                   p.dwpart.as<float>());
+            } else {
+                HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_dw_patch<2>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
+                hipLaunchKernelGGL(rlnp::k_conv_dw_patch<2>, dim3(nb), dim3(rlnp::kPatchThreads), plds, p.stream,
+                                   p.x[l].as<const rlnp::bf16>(), n, pg, p.dzt.as<const rlnp::bf16>(),
+                                   p.dwpart.as<float>());
+            }
+            HIP_TRY(hipGetLastError());
+            // the bias follows the weights in the parameter vector: one reduction covers both
+            hipLaunchKernelGGL(rlnp::k_dw_reduce, dim3((unsigned)((rlnp::kDwPart / 4 + rlnp::kB - 1) / rlnp::kB),
+                                                        (unsigned)std::max(1, nb / 32)),
+                               dim3(rlnp::kB), 0, p.stream, p.dwpart.as<const float>(), nb, gw + L.w_off);
+            HIP_TRY(hipGetLastError());
+            bias_done = true;
+        } else if (patch_geom(L, pg, plds)) {
             // k-major columns (forward) and dZ transposed: both operands with the pixel index contiguous
             hipLaunchKernelGGL(rlnp::k_transpose64, dim3((unsigned)((M + 63) / 64)), dim3(rlnp::kB), 0, p.stream, dz,
                                (long long)M, p.dzt.as<rlnp::bf16>());
@@ -319,7 +391,7 @@ int backward(rln_proposal &p, int n) {
                                                               1, L.cout, L.Kp, M, nullptr, gw + L.w_off, nullptr,
                                                               L.Kp, kchunk)));
         }
-        col_sum(p, dz, M, L.cout, gw + L.b_off);
+        if (!bias_done) col_sum(p, dz, M, L.cout, gw + L.b_off);
         HIP_TRY(hipGetLastError());
         if (l == 0) break;
         if (L.stride == 1 && patch_geom(L, pg, plds)) {
@@ -376,8 +448,9 @@ void col_sum(rln_proposal &p, const rlnp::bf16 *Z, int M, int N, float *out) {
         hipLaunchKernelGGL(rlnp::k_col_sum8, dim3(1, (unsigned)((M + rows - 1) / rows)), dim3(rlnp::kB), 0, p.stream,
                            Z, M, N, rows, out);
     } else {
-        hipLaunchKernelGGL(rlnp::k_col_sum, dim3(grid1(N), (M + 255) / 256), dim3(rlnp::kB), 0, p.stream, Z, M, N,
-                           256, out);
+        const int rows = std::max(16, (M + 63) / 64);  // at most 64 row blocks
+        hipLaunchKernelGGL(rlnp::k_col_sum, dim3(grid1(N), (M + rows - 1) / rows), dim3(rlnp::kB), 0, p.stream, Z,
+                           M, N, rows, out);
     }
 }
 

@@ -471,24 +471,29 @@ __global__ void __launch_bounds__(kB) k_conv_bwd_data(const bf16 *__restrict__ d
 // --------------------------------------------------------------------------------------------------
 // GEMM on MFMA: C(m, n) = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn], bf16.
 // AK: A is k-contiguous (sak == 1, K % 8 == 0) else m-contiguous (sam == 1, M % 8 == 0); BK likewise with n.
-// Tile 64 x 64 x 32 per block of 4 waves (each 32 x 32: 2 x 2 MFMA tiles), LDS-staged; blockIdx.z splits k
-// into chunks of kchunk (a multiple of 32).
+// Tile 64 x 64 per block of 4 waves (each 32 x 32: 2 x 2 MFMA tiles), k in LDS stages of 64, the next stage's
+// operands loaded into registers while the MFMAs consume the current one; blockIdx.z splits k into chunks of
+// kchunk (a multiple of 32). Epilogues: kEpiF32 stores (split k: chunk z's partial sums at Cf + z * zstride, bias
+// from chunk 0 only), kEpiAtomicF32 adds atomically (split k), kEpiAddF32 adds without atomics (one k chunk: every
+// element has one writer), kEpiBiasReluBf16 stores bias + ReLU as bf16.
 // --------------------------------------------------------------------------------------------------
-enum { kEpiF32 = 0, kEpiAtomicF32 = 1, kEpiBiasReluBf16 = 2 };
+enum { kEpiF32 = 0, kEpiAtomicF32 = 1, kEpiBiasReluBf16 = 2, kEpiAddF32 = 3 };
 
 struct GemmArgs {
     const bf16 *A, *B;
     long long sam, sak, sbk, sbn;
     int M, N, K, kchunk;
-    const float *bias;  // [N] or null
-    float *Cf;          // kEpiF32 / kEpiAtomicF32
-    bf16 *Cb;           // kEpiBiasReluBf16
+    const float *bias;   // [N] or null
+    float *Cf;           // kEpiF32 / kEpiAtomicF32 / kEpiAddF32
+    bf16 *Cb;            // kEpiBiasReluBf16
     long long ldc;
+    long long zstride;   // kEpiF32: offset of chunk z's partial sums (0 when k is not split)
 };
 
 template <bool AK, bool BK, int EPI>
 __global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
-    constexpr int kPitch = 40;  // bf16 per LDS row (32 + 8): 80-byte rows keep 16-byte alignment
+    constexpr int kKs = 64;          // k per LDS stage
+    constexpr int kPitch = kKs + 8;  // bf16 per LDS row: 144-byte rows keep 16-byte alignment
     __shared__ __attribute__((aligned(16))) bf16 As[64 * kPitch];
     __shared__ __attribute__((aligned(16))) bf16 Bs[64 * kPitch];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -501,51 +506,81 @@ __global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < ke; k0 += 32) {
-        if (AK) {
-            const int m = tid >> 2, kk = (tid & 3) * 8;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (m0 + m < g.M && k0 + kk < ke)
-                v = *reinterpret_cast<const uint4 *>(g.A + (size_t)(m0 + m) * g.sam + (k0 + kk));
-            *reinterpret_cast<uint4 *>(&As[m * kPitch + kk]) = v;
-        } else {
-            const int k = tid >> 3, mm = (tid & 7) * 8;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (k0 + k < ke && m0 + mm < g.M)
-                v = *reinterpret_cast<const uint4 *>(g.A + (size_t)(k0 + k) * g.sak + (m0 + mm));
-            const bf16 *e = reinterpret_cast<const bf16 *>(&v);
+    // this thread's 16-byte pieces of a stage: k-contiguous operands as row tid / 4, k 8 (tid % 4) + 32 h;
+    // m- or n-contiguous ones as k row tid / 8 + 32 h, columns 8 (tid % 8) .. + 7
+    uint4 ra[2], rb[2];
+    auto gload = [&](int k0) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) As[(mm + j) * kPitch + k] = e[j];
+        for (int h = 0; h < 2; ++h) {
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (AK) {
+                const int m = tid >> 2, kk = (tid & 3) * 8 + 32 * h;
+                if (m0 + m < g.M && k0 + kk < ke)
+                    v = *reinterpret_cast<const uint4 *>(g.A + (size_t)(m0 + m) * g.sam + (k0 + kk));
+            } else {
+                const int k = (tid >> 3) + 32 * h, mm = (tid & 7) * 8;
+                if (k0 + k < ke && m0 + mm < g.M)
+                    v = *reinterpret_cast<const uint4 *>(g.A + (size_t)(k0 + k) * g.sak + (m0 + mm));
+            }
+            ra[h] = v;
+            v = make_uint4(0u, 0u, 0u, 0u);
+            if (BK) {
+                const int n = tid >> 2, kk = (tid & 3) * 8 + 32 * h;
+                if (n0 + n < g.N && k0 + kk < ke)
+                    v = *reinterpret_cast<const uint4 *>(g.B + (size_t)(n0 + n) * g.sbn + (k0 + kk));
+            } else {
+                const int k = (tid >> 3) + 32 * h, nn = (tid & 7) * 8;
+                if (k0 + k < ke && n0 + nn < g.N)
+                    v = *reinterpret_cast<const uint4 *>(g.B + (size_t)(k0 + k) * g.sbk + (n0 + nn));
+            }
+            rb[h] = v;
         }
-        if (BK) {
-            const int n = tid >> 2, kk = (tid & 3) * 8;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (n0 + n < g.N && k0 + kk < ke)
-                v = *reinterpret_cast<const uint4 *>(g.B + (size_t)(n0 + n) * g.sbn + (k0 + kk));
-            *reinterpret_cast<uint4 *>(&Bs[n * kPitch + kk]) = v;
-        } else {
-            const int k = tid >> 3, nn = (tid & 7) * 8;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (k0 + k < ke && n0 + nn < g.N)
-                v = *reinterpret_cast<const uint4 *>(g.B + (size_t)(k0 + k) * g.sbk + (n0 + nn));
-            const bf16 *e = reinterpret_cast<const bf16 *>(&v);
+    };
+    auto lstore = [&]() {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) Bs[(nn + j) * kPitch + k] = e[j];
+        for (int h = 0; h < 2; ++h) {
+            if (AK) {
+                *reinterpret_cast<uint4 *>(&As[(tid >> 2) * kPitch + (tid & 3) * 8 + 32 * h]) = ra[h];
+            } else {
+                const int k = (tid >> 3) + 32 * h, mm = (tid & 7) * 8;
+                const bf16 *e = reinterpret_cast<const bf16 *>(&ra[h]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) As[(mm + j) * kPitch + k] = e[j];
+            }
+            if (BK) {
+                *reinterpret_cast<uint4 *>(&Bs[(tid >> 2) * kPitch + (tid & 3) * 8 + 32 * h]) = rb[h];
+            } else {
+                const int k = (tid >> 3) + 32 * h, nn = (tid & 7) * 8;
+                const bf16 *e = reinterpret_cast<const bf16 *>(&rb[h]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) Bs[(nn + j) * kPitch + k] = e[j];
+            }
         }
+    };
+    if (kb < ke) gload(kb);
+    for (int k0 = kb; k0 < ke; k0 += kKs) {
+        lstore();
         __syncthreads();
-        bf16x8_t a[2], b[2];
+        if (k0 + kKs < ke) gload(k0 + kKs);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-            a[i] = *reinterpret_cast<const bf16x8_t *>(&As[(32 * wm + 16 * i + r16) * kPitch + 8 * gq]);
+        for (int ks = 0; ks < kKs; ks += 32) {
+            bf16x8_t a[2], b[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-            b[j] = *reinterpret_cast<const bf16x8_t *>(&Bs[(32 * wn + 16 * j + r16) * kPitch + 8 * gq]);
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8_t *>(&As[(32 * wm + 16 * i + r16) * kPitch + ks + 8 * gq]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8_t *>(&Bs[(32 * wn + 16 * j + r16) * kPitch + ks + 8 * gq]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
+    float *Cz = g.Cf;
+    if (EPI == kEpiF32) Cz += (size_t)blockIdx.z * g.zstride;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -558,11 +593,24 @@ __global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
                 const int m = m0 + 32 * wm + 16 * i + 4 * gq + r;
                 if (m >= g.M) continue;
                 const float v = acc[i][j][r] + bias;
-                if (EPI == kEpiF32) g.Cf[(size_t)m * g.ldc + n] = v;
+                if (EPI == kEpiF32) Cz[(size_t)m * g.ldc + n] = v;
                 else if (EPI == kEpiAtomicF32) atomicAdd(&g.Cf[(size_t)m * g.ldc + n], v);
+                else if (EPI == kEpiAddF32) g.Cf[(size_t)m * g.ldc + n] += v;
                 else g.Cb[(size_t)m * g.ldc + n] = to_bf16(fmaxf(v, 0.f));
             }
         }
+}
+
+// C[m][n] = sum over z < nz of P[z][m][n] (partial sums of a k-split kEpiF32 GEMM, z stride M * ldc), in z order.
+__global__ void __launch_bounds__(kB) k_sum_parts(const float *__restrict__ P, int nz, int M, int N, long long ldc,
+                                                  float *__restrict__ C) {
+    const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+    if (i >= (long long)M * N) return;
+    const int m = (int)(i / N), n = (int)(i - (long long)m * N);
+    const size_t o = (size_t)m * ldc + n, zs = (size_t)M * ldc;
+    float s = P[o];
+    for (int z = 1; z < nz; ++z) s += P[o + z * zs];
+    C[o] = s;
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -1041,6 +1089,159 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
                 }
         }
     }
+}
+
+// Weight and bias gradients of a 64-channel layer (3 x 3, pad 1, stride S) without im2col columns:
+// gW[co][tap*64 + ci] = sum_m dZ[m][co] X[pixel(m, tap)][ci], gb[co] = sum_m dZ[m][co], m over the n Ho x Ho output
+// positions. Tiles of the forward geometry (PatchGeom: 256 / S positions from a row or image boundary, PR input rows
+// per image). The tile's input rows are staged in LDS transposed, once per column tap kx: copy kx holds
+// X[row][S ox + kx - 1][ci] at [kx][ci][(image, row)][ox] (zero outside the image), so the 8 consecutive positions of
+// a B fragment (one output row, Ho % 8 == 0) are 16 contiguous bytes; rows of a plane are CP = n_img_t PR Ho + 8
+// bf16 apart. A fragments (dZ transposed) are read from dZt [64][M] (k_transpose64). Wave w holds the sums of
+// channels 16 (w % 4) .. +15 over columns 288 (w / 4) .. +287 (18 MFMA tiles) across its block's tiles, and the
+// waves 0-3 the bias sums; the block writes them to part[block][64 * 576 + 64] (weights, then bias), which
+// k_dw_reduce adds to the gradient.
+constexpr int kDwPart = 64 * kPatchK + 64;
+
+__host__ __device__ inline int dw_plane(const PatchGeom &g) { return g.n_img_t * g.PR * g.Ho + 8; }
+
+template <int S>
+__global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__restrict__ X, int n, PatchGeom g,
+                                                                 const bf16 *__restrict__ dZt,
+                                                                 float *__restrict__ part) {
+    extern __shared__ uint4 s_dyn[];
+    bf16 *s_c = reinterpret_cast<bf16 *>(s_dyn);  // [3][64][CP]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, gq = lane >> 4;
+    const int ct = wave & 3, kh = wave >> 2;
+    const int H = g.H, Ho = g.Ho, HoHo = Ho * Ho, CP = dw_plane(g);
+    const int rows = g.n_img_t * g.PR;  // staged rows of a plane
+    const long long M = (long long)n * HoHo;
+    const long long n_tiles = (M + g.TP - 1) / g.TP;
+    // entries no staged pixel reaches: column ox = 0 of copy 0 (input column -1), and for S = 1 column Ho - 1 of
+    // copy 2 (input column H)
+    for (int q = tid; q < 64 * rows; q += kPatchThreads) {
+        const int ci = q / rows, rr = q - ci * rows;
+        s_c[(size_t)ci * CP + rr * Ho] = 0;
+        if (S == 1) s_c[(size_t)(128 + ci) * CP + rr * Ho + Ho - 1] = 0;
+    }
+    auto tile_origin = [&](long long t, int &img0, int &oy0) {
+        const long long m0 = t * g.TP;
+        img0 = (int)(m0 / HoHo);
+        oy0 = (int)((m0 - (long long)img0 * HoHo) / Ho);
+    };
+    uint4 pre[kPatchRegs];
+    // unit u: channels 8 c8 .. + 7, c8 = (u / 64) % 8, of staged pixel (u / 512) * 64 + u % 64 (image it, staged row,
+    // column ix; column fastest), so that the 64 lanes of a wave store one plane at consecutive columns; staged row r
+    // of an image is input row oy0 * S - 1 + r
+    const int npix = rows * H;
+    auto load_patch = [&](long long t) {
+        int img0, oy0;
+        tile_origin(t, img0, oy0);
+#pragma unroll
+        for (int r = 0; r < kPatchRegs; ++r) {
+            const int u = tid + r * kPatchThreads;
+            const int pix = (u >> 9) * 64 + (u & 63), c8 = (u >> 6) & 7;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (pix < npix) {
+                const int rr = pix / H, ix = pix - rr * H;
+                const int it = rr / g.PR, row = rr - it * g.PR;
+                const int img = img0 + it, iy = oy0 * S - 1 + row;
+                if (img < n && iy >= 0 && iy < H)
+                    v = *reinterpret_cast<const uint4 *>(X + (((size_t)img * H + iy) * H + ix) * 64 + 8 * c8);
+            }
+            pre[r] = v;
+        }
+    };
+    floatx4_t acc[18];
+#pragma unroll
+    for (int j = 0; j < 18; ++j) acc[j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
+    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        __syncthreads();  // zero columns stored (first tile); the previous tile's reads done
+#pragma unroll
+        for (int r = 0; r < kPatchRegs; ++r) {
+            const int u = tid + r * kPatchThreads;
+            const int pix = (u >> 9) * 64 + (u & 63), c8 = (u >> 6) & 7;
+            if (pix < npix) {
+                const int rr = pix / H, ix = pix - rr * H;
+                const bf16 *e = reinterpret_cast<const bf16 *>(&pre[r]);
+#pragma unroll
+                for (int kx = 0; kx < 3; ++kx) {
+                    const int tt = ix + 1 - kx;
+                    if (tt < 0 || (S == 2 && (tt & 1))) continue;
+                    const int ox = tt / S;
+                    if (ox >= Ho) continue;
+                    bf16 *d = s_c + (size_t)(kx * 64 + 8 * c8) * CP + rr * Ho + ox;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) d[(size_t)c * CP] = e[c];
+                }
+            }
+        }
+        __syncthreads();
+        if (t + gridDim.x < n_tiles) load_patch(t + gridDim.x);
+        int img0, oy0;
+        tile_origin(t, img0, oy0);
+        const long long m0 = t * g.TP;
+        const long long row0 = m0 / Ho;  // global output row of the tile's first position
+        for (int s = 0; s < g.TP / 32; ++s) {
+            const int q0 = 32 * s + 8 * gq;  // this lane's 8 positions q0 .. q0 + 7 of the tile
+            const long long mg = m0 + q0;
+            uint4 av = make_uint4(0u, 0u, 0u, 0u);
+            if (mg < M) av = *reinterpret_cast<const uint4 *>(dZt + (size_t)(16 * ct + r16) * M + mg);
+            if (kh == 0) {
+                const bf16 *e = reinterpret_cast<const bf16 *>(&av);
+#pragma unroll
+                for (int c = 0; c < 8; ++c) bsum += from_bf16(e[c]);
+            }
+            const bf16x8_t a = __builtin_bit_cast(bf16x8_t, av);
+            const long long R = row0 + q0 / Ho;
+            const int img = (int)(R / Ho), oy = (int)(R - (long long)img * Ho);
+            const int pbase = ((img - img0) * g.PR + (oy - oy0) * S) * Ho + (q0 % Ho);
+#pragma unroll
+            for (int j = 0; j < 18; ++j) {
+                const int kt = 18 * kh + j, tap = kt >> 2, ci0 = 16 * (kt & 3);
+                const int ky = tap / 3, kx = tap - 3 * ky;
+                const bf16x8_t b = __builtin_bit_cast(
+                    bf16x8_t, *reinterpret_cast<const uint4 *>(s_c + (size_t)(kx * 64 + ci0 + r16) * CP + pbase + ky * Ho));
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+            }
+        }
+    }
+    // D row 4 gq + r of the channel tile, column r16 of MFMA tile kt
+    float *o = part + (size_t)blockIdx.x * kDwPart;
+#pragma unroll
+    for (int j = 0; j < 18; ++j) {
+        const int kt = 18 * kh + j;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) o[(size_t)(16 * ct + 4 * gq + r) * kPatchK + 16 * kt + r16] = acc[j][r];
+    }
+    if (kh == 0) {
+        bsum += __shfl_xor(bsum, 16);
+        bsum += __shfl_xor(bsum, 32);
+        if (gq == 0) o[64 * kPatchK + 16 * ct + r16] = bsum;
+    }
+}
+
+// out[e] += sum over blocks b of part[b][e], e < kDwPart (four consecutive entries per thread); blockIdx.y sums
+// the blocks [y nb / gy, (y + 1) nb / gy) and adds its sums atomically.
+__global__ void __launch_bounds__(kB) k_dw_reduce(const float *__restrict__ part, int nb, float *__restrict__ out) {
+    const int e4 = blockIdx.x * kB + threadIdx.x;
+    if (4 * e4 >= kDwPart) return;
+    const int b0 = (int)((long long)blockIdx.y * nb / gridDim.y), b1 = (int)((long long)(blockIdx.y + 1) * nb / gridDim.y);
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int b = b0; b < b1; ++b) {
+        const float4 v = reinterpret_cast<const float4 *>(part + (size_t)b * kDwPart)[e4];
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    atomicAdd(&out[4 * e4], s.x);
+    atomicAdd(&out[4 * e4 + 1], s.y);
+    atomicAdd(&out[4 * e4 + 2], s.z);
+    atomicAdd(&out[4 * e4 + 3], s.w);
 }
 
 // --------------------------------------------------------------------------------------------------
