@@ -229,8 +229,9 @@ bool bwd_s2_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
     g.PR = th + 1;
     g.units = g.n_img_t * g.PR * L.Ho * 8;
     if (g.units > rlnp::kPatchRegs * rlnp::kPatchThreads) return false;
+    // weights, the dZ patch and the 8 waves' output staging (32 positions each)
     lds = sizeof(rlnp::bf16) * ((size_t)64 * (rlnp::kPatchK + rlnp::kConvLdsPad) +
-                                (size_t)g.n_img_t * g.PR * L.Ho * rlnp::kPatchPix);
+                                ((size_t)g.n_img_t * g.PR * L.Ho + 8 * 32) * rlnp::kPatchPix);
     return lds <= 160 * 1024;
 }
 
@@ -341,6 +342,7 @@ int backward(rln_proposal &p, int n) {
     hipLaunchKernelGGL(rlnp::k_mask_relu, dim3(grid1((long long)n * p.F)), dim3(rlnp::kB), 0, p.stream,
                        p.dtmp.as<const float>(), x6, (long long)n * p.F, dz);
     HIP_TRY(hipGetLastError());
+    bool bias_below_done = false;  // this layer's bias gradient came with the layer above's data gradient
     for (int l = kLayers - 1; l >= 0; --l) {
         const ConvLayer &L = p.conv[l];
         const int M = n * L.Ho * L.Ho;
@@ -374,7 +376,7 @@ int backward(rln_proposal &p, int n) {
             HIP_TRY(hipGetLastError());
             // the bias follows the weights in the parameter vector: one reduction covers both
             hipLaunchKernelGGL(rlnp::k_dw_reduce, dim3((unsigned)((rlnp::kDwPart / 4 + rlnp::kB - 1) / rlnp::kB),
-                                                        (unsigned)std::max(1, nb / 32)),
+                                                        (unsigned)std::max(1, nb / 16)),
                                dim3(rlnp::kB), 0, p.stream, p.dwpart.as<const float>(), nb, gw + L.w_off);
             HIP_TRY(hipGetLastError());
             bias_done = true;
@@ -391,7 +393,8 @@ int backward(rln_proposal &p, int n) {
                                                               1, L.cout, L.Kp, M, nullptr, gw + L.w_off, nullptr,
                                                               L.Kp, kchunk)));
         }
-        if (!bias_done) col_sum(p, dz, M, L.cout, gw + L.b_off);
+        if (!bias_done && !bias_below_done) col_sum(p, dz, M, L.cout, gw + L.b_off);
+        bias_below_done = false;
         HIP_TRY(hipGetLastError());
         if (l == 0) break;
         if (L.stride == 1 && patch_geom(L, pg, plds)) {
@@ -411,9 +414,21 @@ int backward(rln_proposal &p, int n) {
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
             const long long tiles = ((long long)n * L.Ho * L.Ho + pg.TP - 1) / pg.TP;
             const unsigned grid = (unsigned)std::min<long long>(tiles, p.n_cu > 0 ? p.n_cu : 256);
+            // the layer below's bias gradient from the masked dX, unless its own weight-gradient kernel sums it
+            rlnp::PatchGeom dg{};
+            size_t dlds = 0;
+            const bool colsum = !dw_patch_geom(p.conv[l - 1], dg, dlds) &&
+                                (size_t)grid * 64 * sizeof(float) <= p.dwpart.bytes;
             hipLaunchKernelGGL(rlnp::k_conv_bwd_s2_patch, dim3(grid), dim3(rlnp::kPatchThreads), plds, p.stream, dz, n,
-                               pg, p.wt.as<const rlnp::bf16>(), p.x[l].as<const rlnp::bf16>(), dz_next);
+                               pg, p.wt.as<const rlnp::bf16>(), p.x[l].as<const rlnp::bf16>(), dz_next,
+                               colsum ? p.dwpart.as<float>() : nullptr);
             HIP_TRY(hipGetLastError());
+            if (colsum) {
+                hipLaunchKernelGGL(rlnp::k_colsum_parts, dim3(1), dim3(1024), 0, p.stream, p.dwpart.as<const float>(),
+                                   (int)grid, gw + p.conv[l - 1].b_off);
+                HIP_TRY(hipGetLastError());
+                bias_below_done = true;
+            }
         } else if (bwd_data_direct(L)) {
             const int nwt = 9 * L.cin * L.cout;
             hipLaunchKernelGGL(rlnp::k_wt_bwd, dim3(grid1(nwt)), dim3(rlnp::kB), 0, p.stream, wbf(p, L.w_off), L.cin,

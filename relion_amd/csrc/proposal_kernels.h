@@ -960,11 +960,14 @@ __global__ void __launch_bounds__(kB) k_wt_flip(const bf16 *__restrict__ W, bf16
 // from the dZ pixels at (a..a+1, b..b+1), through 1, 2, 2 and 4 taps: 9 taps of 64 channels per position. Tiles of
 // 256 positions start at row or image boundaries (PatchGeom over the dZ grid: H the input size, Ho the dZ size, PR
 // = the tile's dZ rows plus the next one, zero past the image); the weights Wd[c][tap*64 + co] = W[co][tap*64 + c]
-// (k_wt_bwd) stay in LDS. Each wave: two 16-position row tiles x 64 channels, class by class.
+// (k_wt_bwd) stay in LDS. Each wave: two 16-position row tiles x 64 channels, class by class, stored through LDS as
+// 16-byte rows. With colsum_part, block b writes the column sums of its part of the masked dX to colsum_part[64 b ..]
+// (k_colsum_parts adds them up: the bias gradient of the layer whose output gradient dX is).
 __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 *__restrict__ dZ, int n, PatchGeom g,
                                                                      const bf16 *__restrict__ Wd,
                                                                      const bf16 *__restrict__ Ymask,
-                                                                     bf16 *__restrict__ dX) {
+                                                                     bf16 *__restrict__ dX,
+                                                                     float *__restrict__ colsum_part) {
     extern __shared__ uint4 s_dyn[];
     constexpr int kLdw = kPatchK + kConvLdsPad;
     bf16 *s_w = reinterpret_cast<bf16 *>(s_dyn);  // [64][kLdw]
@@ -1005,6 +1008,9 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
             pre[r] = v;
         }
     };
+    // this wave's output staging: 32 positions x 64 channels, kPatchPix apart
+    bf16 *s_o = s_p + (size_t)g.n_img_t * g.PR * Ho * kPatchPix + (size_t)wave * 32 * kPatchPix;
+    float csum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // channels 8 (lane % 8) .. + 7 of dX
     if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
     for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         __syncthreads();  // weights stored (first tile); the previous tile's patch reads done
@@ -1031,9 +1037,27 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
             pcol[i] = pl - a * Ho;
             pb[i] = ((img - img0) * g.PR + (a - a0)) * Ho;
         }
+        const int c8 = lane & 7;
 #pragma unroll
         for (int cls = 0; cls < 4; ++cls) {
             const int py = cls >> 1, px = cls & 1;
+            // this lane's epilogue pixels (8 channels of positions m0 + 32 wave + lane / 8 + 8 q): their ReLU mask
+            // is loaded before the class's products
+            size_t oidx[4];
+            uint4 mk[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const long long mm = m0 + 32 * wave + (lane >> 3) + 8 * q;
+                oidx[q] = ~(size_t)0;
+                mk[q] = make_uint4(0u, 0u, 0u, 0u);
+                if (mm < M) {
+                    const int img = (int)(mm / HoHo);
+                    const int pl = (int)(mm - (long long)img * HoHo);
+                    const int a = pl / Ho, b = pl - a * Ho;
+                    oidx[q] = (((size_t)img * H + 2 * a + py) * H + 2 * b + px) * 64 + 8 * c8;
+                    mk[q] = *reinterpret_cast<const uint4 *>(Ymask + oidx[q]);
+                }
+            }
             floatx4_t acc[2][4];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
@@ -1069,25 +1093,76 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
                                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
                     }
                 }
-            // D rows 4 gq + r of each 16-position tile, column r16 of each channel tile: the class's pixel of the
-            // position, masked, bf16 NHWC
+            // D rows 4 gq + r of each 16-position tile, column r16 of each channel tile, as bf16 into the wave's own
+            // staging rows 16 i + 4 gq + r (position m0 + 32 wave + row); then each lane moves 8 channels of 4
+            // positions: the class's pixel of the position, masked, 16-byte NHWC stores. The rows are the wave's
+            // alone: wave-scope ordering of its LDS accesses suffices.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const long long mm = m0 + (2 * wave + i) * 16 + 4 * gq + r;
-                    if (mm >= M) continue;
-                    const int img = (int)(mm / HoHo);
-                    const int pl = (int)(mm - (long long)img * HoHo);
-                    const int a = pl / Ho, b = pl - a * Ho;
-                    const size_t pix = ((size_t)img * H + 2 * a + py) * H + 2 * b + px;
+                for (int j = 0; j < 4; ++j)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const size_t idx = pix * 64 + 16 * j + r16;
-                        dX[idx] = to_bf16(from_bf16(Ymask[idx]) > 0.f ? acc[i][j][r] : 0.f);
-                    }
+                    for (int r = 0; r < 4; ++r)
+                        s_o[(16 * i + 4 * gq + r) * kPatchPix + 16 * j + r16] = to_bf16(acc[i][j][r]);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (oidx[q] == ~(size_t)0) continue;
+                const int row = (lane >> 3) + 8 * q;
+                const uint4 v = *reinterpret_cast<const uint4 *>(s_o + row * kPatchPix + 8 * c8);
+                const bf16 *ve = reinterpret_cast<const bf16 *>(&v);
+                const bf16 *me = reinterpret_cast<const bf16 *>(&mk[q]);
+                uint4 o;
+                bf16 *oe = reinterpret_cast<bf16 *>(&o);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    oe[e] = from_bf16(me[e]) > 0.f ? ve[e] : (bf16)0;
+                    csum[e] += from_bf16(oe[e]);
                 }
+                *reinterpret_cast<uint4 *>(dX + oidx[q]) = o;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
+    }
+    if (colsum_part) {  // the block's column sums of dX: lanes of one channel group, then the 8 waves in LDS
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            csum[e] += __shfl_xor(csum[e], 8);
+            csum[e] += __shfl_xor(csum[e], 16);
+            csum[e] += __shfl_xor(csum[e], 32);
+        }
+        float *s_red = reinterpret_cast<float *>(s_w);  // [8][64]: the weights are no longer read
+        __syncthreads();
+        if (lane < 8)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s_red[wave * 64 + 8 * lane + e] = csum[e];
+        __syncthreads();
+        if (tid < 64) {
+            float t = 0.f;
+            for (int w = 0; w < kPatchWaves; ++w) t += s_red[w * 64 + tid];
+            colsum_part[(size_t)blockIdx.x * 64 + tid] = t;
+        }
+    }
+}
+
+// out[c] += sum over blocks b < nb of part[64 b + c]; one block of 1024 threads: thread (g, c) sums the blocks
+// b = g (mod 16), then the 16 groups are added in order.
+__global__ void __launch_bounds__(1024) k_colsum_parts(const float *__restrict__ part, int nb,
+                                                       float *__restrict__ out) {
+    __shared__ float s[16][64];
+    const int c = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    float t = 0.f;
+    for (int b = grp; b < nb; b += 16) t += part[(size_t)b * 64 + c];
+    s[grp][c] = t;
+    __syncthreads();
+    if (grp == 0) {
+        float u = 0.f;
+        for (int k = 0; k < 16; ++k) u += s[k][c];
+        out[c] += u;
     }
 }
 
@@ -1231,6 +1306,7 @@ __global__ void __launch_bounds__(kB) k_dw_reduce(const float *__restrict__ part
     if (4 * e4 >= kDwPart) return;
     const int b0 = (int)((long long)blockIdx.y * nb / gridDim.y), b1 = (int)((long long)(blockIdx.y + 1) * nb / gridDim.y);
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 8
     for (int b = b0; b < b1; ++b) {
         const float4 v = reinterpret_cast<const float4 *>(part + (size_t)b * kDwPart)[e4];
         s.x += v.x;
