@@ -916,30 +916,40 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__
             for (int j = 0; j < NTW; ++j)
                 b[j] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4 *>(
                                                         s_w + (16 * (ct0 + j) + r16) * kLdw + k0 + 8 * gq));
+            // weights as the A operand: D tiles are channels x pixels, so that a lane holds 4 consecutive channels
+            // of one pixel
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < NTW; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
         }
-        // D rows 4 gq + r of each 16-pixel tile, column r16 of each channel tile: bias and ReLU (or the ReLU mask of
-        // Ymask), bf16 NHWC
+        // D rows 4 gq + r of each channel tile, column r16 of each 16-pixel tile: bias and ReLU (or the ReLU mask of
+        // Ymask), bf16 NHWC, 8 bytes per lane
 #pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-            const int col = 16 * (ct0 + j) + r16;
-            const float bj = EPI == 0 ? bias[col] : 0.f;
+        for (int i = 0; i < 2; ++i) {
+            const long long mm = m0 + (2 * rp + i) * 16 + r16;
+            if (mm >= M) continue;
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < NTW; ++j) {
+                const int c0 = 16 * (ct0 + j) + 4 * gq;
+                const size_t idx = (size_t)mm * 64 + c0;
+                uint2 o;
+                bf16 *oe = reinterpret_cast<bf16 *>(&o);
+                if (EPI == 0) {
+                    const float4 bj = *reinterpret_cast<const float4 *>(bias + c0);
+                    oe[0] = to_bf16(fmaxf(acc[i][j][0] + bj.x, 0.f));
+                    oe[1] = to_bf16(fmaxf(acc[i][j][1] + bj.y, 0.f));
+                    oe[2] = to_bf16(fmaxf(acc[i][j][2] + bj.z, 0.f));
+                    oe[3] = to_bf16(fmaxf(acc[i][j][3] + bj.w, 0.f));
+                } else {
+                    const uint2 mk = *reinterpret_cast<const uint2 *>(Ymask + idx);
+                    const bf16 *me = reinterpret_cast<const bf16 *>(&mk);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const long long mm = m0 + (2 * rp + i) * 16 + 4 * gq + r;
-                    if (mm >= M) continue;
-                    if (EPI == 0)
-                        Y[(size_t)mm * 64 + col] = to_bf16(fmaxf(acc[i][j][r] + bj, 0.f));
-                    else
-                        Y[(size_t)mm * 64 + col] =
-                            to_bf16(from_bf16(Ymask[(size_t)mm * 64 + col]) > 0.f ? acc[i][j][r] : 0.f);
+                    for (int r = 0; r < 4; ++r) oe[r] = to_bf16(from_bf16(me[r]) > 0.f ? acc[i][j][r] : 0.f);
                 }
+                *reinterpret_cast<uint2 *>(Y + idx) = o;
+            }
         }
     }
 }
