@@ -507,7 +507,10 @@ __global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
     // this thread's 16-byte pieces of a stage: k-contiguous operands as row tid / 4, k 8 (tid % 4) + 32 h;
-    // m- or n-contiguous ones as k row tid / 8 + 32 h, columns 8 (tid % 8) .. + 7
+    // m- or n-contiguous ones as k row tid / 8 + 32 h, columns 8 (tid % 8) .. + 7. In LDS, the 8-element chunk c of
+    // row r sits at chunk c ^ ((r / 16) % 4): the transposing stores of rows 8 apart then fall into distinct banks,
+    // and the 16 rows of an MFMA fragment read share one swizzle.
+    auto sw = [](int row, int k) { return row * kPitch + ((((k >> 3) ^ (row >> 4)) & 7) << 3) + (k & 7); };
     uint4 ra[2], rb[2];
     auto gload = [&](int k0) {
 #pragma unroll
@@ -540,20 +543,20 @@ __global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (AK) {
-                *reinterpret_cast<uint4 *>(&As[(tid >> 2) * kPitch + (tid & 3) * 8 + 32 * h]) = ra[h];
+                *reinterpret_cast<uint4 *>(&As[sw(tid >> 2, (tid & 3) * 8 + 32 * h)]) = ra[h];
             } else {
                 const int k = (tid >> 3) + 32 * h, mm = (tid & 7) * 8;
                 const bf16 *e = reinterpret_cast<const bf16 *>(&ra[h]);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) As[(mm + j) * kPitch + k] = e[j];
+                for (int j = 0; j < 8; ++j) As[sw(mm + j, k)] = e[j];
             }
             if (BK) {
-                *reinterpret_cast<uint4 *>(&Bs[(tid >> 2) * kPitch + (tid & 3) * 8 + 32 * h]) = rb[h];
+                *reinterpret_cast<uint4 *>(&Bs[sw(tid >> 2, (tid & 3) * 8 + 32 * h)]) = rb[h];
             } else {
                 const int k = (tid >> 3) + 32 * h, nn = (tid & 7) * 8;
                 const bf16 *e = reinterpret_cast<const bf16 *>(&rb[h]);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) Bs[(nn + j) * kPitch + k] = e[j];
+                for (int j = 0; j < 8; ++j) Bs[sw(nn + j, k)] = e[j];
             }
         }
     };
@@ -567,10 +570,10 @@ __global__ void __launch_bounds__(kB) k_gemm(GemmArgs g) {
             bf16x8_t a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                a[i] = *reinterpret_cast<const bf16x8_t *>(&As[(32 * wm + 16 * i + r16) * kPitch + ks + 8 * gq]);
+                a[i] = *reinterpret_cast<const bf16x8_t *>(&As[sw(32 * wm + 16 * i + r16, ks + 8 * gq)]);
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                b[j] = *reinterpret_cast<const bf16x8_t *>(&Bs[(32 * wn + 16 * j + r16) * kPitch + ks + 8 * gq]);
+                b[j] = *reinterpret_cast<const bf16x8_t *>(&Bs[sw(32 * wn + 16 * j + r16, ks + 8 * gq)]);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
