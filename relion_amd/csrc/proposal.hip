@@ -239,9 +239,9 @@ bool bwd_s2_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
 // forward's tiles; output rows of whole 8-position fragments, H = stride * Ho); its LDS bytes in lds.
 bool dw_patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
     size_t flds = 0;
-    if (!patch_geom(L, g, flds) || L.Ho % 8 || L.H != L.stride * L.Ho || g.TP % 32) return false;
-    if ((g.n_img_t * g.PR * L.H + 63) / 64 > rlnp::kPatchRegs) return false;  // staged pixels: 64 per register
-    lds = sizeof(rlnp::bf16) * 3 * 64 * (size_t)rlnp::dw_plane(g);
+    if (!patch_geom(L, g, flds) || L.Ho % 8 || L.H != L.stride * L.Ho || g.TP != 256 / L.stride) return false;
+    if ((g.n_img_t * g.PR * L.H + 63) / 64 > rlnp::dw_xregs(L.stride)) return false;  // staged pixels: 64 per register
+    lds = sizeof(rlnp::bf16) * rlnp::dw_lds_elems(g, L.stride);
     return lds <= 160 * 1024;
 }
 

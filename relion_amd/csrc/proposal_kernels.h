@@ -1185,25 +1185,36 @@ __global__ void __launch_bounds__(1024) k_colsum_parts(const float *__restrict__
 // per image). The tile's input rows are staged in LDS transposed, once per column tap kx: copy kx holds
 // X[row][S ox + kx - 1][ci] at [kx][ci][(image, row)][ox] (zero outside the image), so the 8 consecutive positions of
 // a B fragment (one output row, Ho % 8 == 0) are 16 contiguous bytes; rows of a plane are CP = n_img_t PR Ho + 8
-// bf16 apart. A fragments (dZ transposed) are read from dZt [64][M] (k_transpose64). Wave w holds the sums of
-// channels 16 (w % 4) .. +15 over columns 288 (w / 4) .. +287 (18 MFMA tiles) across its block's tiles, and the
-// waves 0-3 the bias sums; the block writes them to part[block][64 * 576 + 64] (weights, then bias), which
-// k_dw_reduce adds to the gradient.
+// bf16 apart. A fragments: the tile's slice of dZt [64][M] (dZ transposed by k_transpose64), staged in LDS with the
+// input rows (both loaded into registers during the previous tile's products). Wave w holds the sums of
+// channels 32 (w % 2) .. +31 over columns 144 (w / 2) .. +143 (2 x 9 MFMA tiles: each B fragment read from LDS
+// serves two channel tiles) across its block's tiles, and waves 0-1 the bias sums; the block writes them to
+// part[block][64 * 576 + 64] (weights, then bias), which k_dw_reduce adds to the gradient.
 constexpr int kDwPart = 64 * kPatchK + 64;
 
 __host__ __device__ inline int dw_plane(const PatchGeom &g) { return g.n_img_t * g.PR * g.Ho + 8; }
+// registers of 16 bytes per thread staging the input rows (64 pixels each) and the dZ slice, by stride
+__host__ __device__ constexpr int dw_xregs(int S) { return S == 1 ? 5 : 9; }
+__host__ __device__ constexpr int dw_zregs(int S) { return 4 / S; }  // 64 x (256 / S) bf16 over 512 threads
+// LDS bf16 of the kernel: the three transposed copies, then the dZ slice [64][256 / S + 8]
+__host__ __device__ inline size_t dw_lds_elems(const PatchGeom &g, int S) {
+    return (size_t)3 * 64 * dw_plane(g) + (size_t)64 * (256 / S + 8);
+}
 
 template <int S>
 __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__restrict__ X, int n, PatchGeom g,
                                                                  const bf16 *__restrict__ dZt,
                                                                  float *__restrict__ part) {
     extern __shared__ uint4 s_dyn[];
+    constexpr int kTP = 256 / S, kZP = kTP + 8;   // positions per tile; dZ slice pitch
+    constexpr int kXR = dw_xregs(S), kZR = dw_zregs(S);
     bf16 *s_c = reinterpret_cast<bf16 *>(s_dyn);  // [3][64][CP]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r16 = lane & 15, gq = lane >> 4;
-    const int ct = wave & 3, kh = wave >> 2;
+    const int cp = wave & 1, kq = wave >> 1;  // channels 32 cp .. +31, columns 144 kq .. +143 (9 MFMA tiles)
     const int H = g.H, Ho = g.Ho, HoHo = Ho * Ho, CP = dw_plane(g);
     const int rows = g.n_img_t * g.PR;  // staged rows of a plane
+    bf16 *s_z = s_c + (size_t)3 * 64 * CP;  // [64][kZP]: dZt[c][m0 + q], q < kTP
     const long long M = (long long)n * HoHo;
     const long long n_tiles = (M + g.TP - 1) / g.TP;
     // entries no staged pixel reaches: column ox = 0 of copy 0 (input column -1), and for S = 1 column Ho - 1 of
@@ -1218,16 +1229,23 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
         img0 = (int)(m0 / HoHo);
         oy0 = (int)((m0 - (long long)img0 * HoHo) / Ho);
     };
-    uint4 pre[kPatchRegs];
+    uint4 pre[kXR], zpre[kZR];
     // unit u: channels 8 c8 .. + 7, c8 = (u / 64) % 8, of staged pixel (u / 512) * 64 + u % 64 (image it, staged row,
     // column ix; column fastest), so that the 64 lanes of a wave store one plane at consecutive columns; staged row r
-    // of an image is input row oy0 * S - 1 + r
+    // of an image is input row oy0 * S - 1 + r. dZ unit u: channel u / (kTP / 8), positions 8 (u % (kTP / 8)) .. + 7.
     const int npix = rows * H;
     auto load_patch = [&](long long t) {
         int img0, oy0;
         tile_origin(t, img0, oy0);
 #pragma unroll
-        for (int r = 0; r < kPatchRegs; ++r) {
+        for (int r = 0; r < kZR; ++r) {
+            const int u = tid + r * kPatchThreads;
+            const int c = u / (kTP / 8), q8 = u - c * (kTP / 8);
+            const long long m = t * g.TP + 8 * q8;
+            zpre[r] = m < M ? *reinterpret_cast<const uint4 *>(dZt + (size_t)c * M + m) : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int r = 0; r < kXR; ++r) {
             const int u = tid + r * kPatchThreads;
             const int pix = (u >> 9) * 64 + (u & 63), c8 = (u >> 6) & 7;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
@@ -1241,15 +1259,23 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
             pre[r] = v;
         }
     };
-    floatx4_t acc[18];
+    floatx4_t acc[2][9];
 #pragma unroll
-    for (int j = 0; j < 18; ++j) acc[j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-    float bsum = 0.f;
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    float bsum[2] = {0.f, 0.f};
     if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
     for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         __syncthreads();  // zero columns stored (first tile); the previous tile's reads done
 #pragma unroll
-        for (int r = 0; r < kPatchRegs; ++r) {
+        for (int r = 0; r < kZR; ++r) {
+            const int u = tid + r * kPatchThreads;
+            const int c = u / (kTP / 8), q8 = u - c * (kTP / 8);
+            *reinterpret_cast<uint4 *>(s_z + c * kZP + 8 * q8) = zpre[r];
+        }
+#pragma unroll
+        for (int r = 0; r < kXR; ++r) {
             const int u = tid + r * kPatchThreads;
             const int pix = (u >> 9) * 64 + (u & 63), c8 = (u >> 6) & 7;
             if (pix < npix) {
@@ -1275,40 +1301,50 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
         const long long row0 = m0 / Ho;  // global output row of the tile's first position
         for (int s = 0; s < g.TP / 32; ++s) {
             const int q0 = 32 * s + 8 * gq;  // this lane's 8 positions q0 .. q0 + 7 of the tile
-            const long long mg = m0 + q0;
-            uint4 av = make_uint4(0u, 0u, 0u, 0u);
-            if (mg < M) av = *reinterpret_cast<const uint4 *>(dZt + (size_t)(16 * ct + r16) * M + mg);
-            if (kh == 0) {
-                const bf16 *e = reinterpret_cast<const bf16 *>(&av);
+            bf16x8_t a[2];
 #pragma unroll
-                for (int c = 0; c < 8; ++c) bsum += from_bf16(e[c]);
+            for (int i = 0; i < 2; ++i) {
+                const uint4 av = *reinterpret_cast<const uint4 *>(s_z + (32 * cp + 16 * i + r16) * kZP + q0);
+                if (kq == 0) {
+                    const bf16 *e = reinterpret_cast<const bf16 *>(&av);
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) bsum[i] += from_bf16(e[c]);
+                }
+                a[i] = __builtin_bit_cast(bf16x8_t, av);
             }
-            const bf16x8_t a = __builtin_bit_cast(bf16x8_t, av);
             const long long R = row0 + q0 / Ho;
             const int img = (int)(R / Ho), oy = (int)(R - (long long)img * Ho);
             const int pbase = ((img - img0) * g.PR + (oy - oy0) * S) * Ho + (q0 % Ho);
 #pragma unroll
-            for (int j = 0; j < 18; ++j) {
-                const int kt = 18 * kh + j, tap = kt >> 2, ci0 = 16 * (kt & 3);
+            for (int j = 0; j < 9; ++j) {
+                const int kt = 9 * kq + j, tap = kt >> 2, ci0 = 16 * (kt & 3);
                 const int ky = tap / 3, kx = tap - 3 * ky;
                 const bf16x8_t b = __builtin_bit_cast(
                     bf16x8_t, *reinterpret_cast<const uint4 *>(s_c + (size_t)(kx * 64 + ci0 + r16) * CP + pbase + ky * Ho));
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b, acc[i][j], 0, 0, 0);
             }
         }
     }
-    // D row 4 gq + r of the channel tile, column r16 of MFMA tile kt
+    // D row 4 gq + r of channel tile 2 cp + i, column r16 of MFMA tile kt
     float *o = part + (size_t)blockIdx.x * kDwPart;
 #pragma unroll
-    for (int j = 0; j < 18; ++j) {
-        const int kt = 18 * kh + j;
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) o[(size_t)(16 * ct + 4 * gq + r) * kPatchK + 16 * kt + r16] = acc[j][r];
-    }
-    if (kh == 0) {
-        bsum += __shfl_xor(bsum, 16);
-        bsum += __shfl_xor(bsum, 32);
-        if (gq == 0) o[64 * kPatchK + 16 * ct + r16] = bsum;
+        for (int j = 0; j < 9; ++j) {
+            const int kt = 9 * kq + j;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                o[(size_t)(32 * cp + 16 * i + 4 * gq + r) * kPatchK + 16 * kt + r16] = acc[i][j][r];
+        }
+    if (kq == 0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            float b = bsum[i];
+            b += __shfl_xor(b, 16);
+            b += __shfl_xor(b, 32);
+            if (gq == 0) o[64 * kPatchK + 32 * cp + 16 * i + r16] = b;
+        }
     }
 }
 
