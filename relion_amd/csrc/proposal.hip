@@ -81,6 +81,9 @@ bool dw_patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds);
 
 int ensure_batch(rln_proposal &p, int n) {
     if (n <= p.cap) return RLN_OK;
+    // the convolution kernels index a layer's n x H x H positions in 32 bits
+    if ((long long)n * p.W * p.W >= (1LL << 31))
+        RLN_FAIL(RLN_ERR_ARG, "batch of %d images of %d x %d pixels: at most 2^31 pixels per layer", n, p.W, p.W);
     const int cap = std::max(n, 64);
     const int W = p.W, C = p.C;
     RLN_TRY(p.x[0].ensure(sizeof(rlnp::bf16) * (size_t)cap * W * W));

@@ -839,18 +839,18 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__
             *reinterpret_cast<const uint4 *>(Wt + (size_t)row * kPatchK + 8 * c8);
     }
     const int HoHo = g.Ho * g.Ho;
-    const long long M = (long long)n * HoHo;
-    const long long n_tiles = (M + g.TP - 1) / g.TP;
+    const int M = n * HoHo;  // below 2^31: the host caps the batch
+    const int n_tiles = (M + g.TP - 1) / g.TP;
     // first image and first output row of tile t (tiles start at row boundaries, or at image boundaries)
-    auto tile_origin = [&](long long t, int &img0, int &oy0) {
-        const long long m0 = t * g.TP;
+    auto tile_origin = [&](int t, int &img0, int &oy0) {
+        const int m0 = t * g.TP;
         img0 = (int)(m0 / HoHo);
-        oy0 = (int)((m0 - (long long)img0 * HoHo) / g.Ho);
+        oy0 = (int)((m0 - img0 * HoHo) / g.Ho);
     };
     uint4 pre[kPatchRegs];
     // unit u of the patch: 8 channels c8 of staged pixel (image it, row, col), col fastest; staged row r of an image
     // is input row oy0 * S - 1 + r
-    auto load_patch = [&](long long t) {
+    auto load_patch = [&](int t) {
         int img0, oy0;
         tile_origin(t, img0, oy0);
 #pragma unroll
@@ -871,8 +871,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__
     constexpr int NTW = 4 / S;                       // column tiles per wave
     const int rp = (S == 1) ? wave : (wave >> 1);    // the wave's row tiles: 2 rp, 2 rp + 1
     const int ct0 = (S == 1) ? 0 : NTW * (wave & 1); // its first column tile
-    if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
-    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    if ((int)blockIdx.x < n_tiles) load_patch(blockIdx.x);
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         __syncthreads();  // weights stored (first tile); the previous tile's patch reads done
 #pragma unroll
         for (int r = 0; r < kPatchRegs; ++r) {
@@ -883,16 +883,16 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__
         if (t + gridDim.x < n_tiles) load_patch(t + gridDim.x);
         int img0, oy0;
         tile_origin(t, img0, oy0);
-        const long long m0 = t * g.TP;
+        const int m0 = t * g.TP;
         // this lane's two A rows: staged pixel (image, row, column) of tap (0, 0), or none past the last pixel
         int pb[2], px[2];
         bool pok[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const long long m = m0 + (2 * rp + i) * 16 + r16;
+            const int m = m0 + (2 * rp + i) * 16 + r16;
             pok[i] = m < M;
             const int img = (int)(m / HoHo);
-            const int pl = (int)(m - (long long)img * HoHo);
+            const int pl = (int)(m - img * HoHo);
             const int oy = pl / g.Ho, ox = pl - oy * g.Ho;
             pb[i] = ((img - img0) * g.PR + (oy - oy0) * S) * g.H;
             px[i] = ox * S - 1;
@@ -931,7 +931,7 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__
         // Ymask), bf16 NHWC, 8 bytes per lane
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const long long mm = m0 + (2 * rp + i) * 16 + r16;
+            const int mm = m0 + (2 * rp + i) * 16 + r16;
             if (mm >= M) continue;
 #pragma unroll
             for (int j = 0; j < NTW; ++j) {
@@ -993,17 +993,17 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
             *reinterpret_cast<const uint4 *>(Wd + (size_t)row * kPatchK + 8 * c8);
     }
     const int Ho = g.Ho, H = g.H, HoHo = Ho * Ho;
-    const long long M = (long long)n * HoHo;
-    const long long n_tiles = (M + g.TP - 1) / g.TP;
-    auto tile_origin = [&](long long t, int &img0, int &a0) {
-        const long long m0 = t * g.TP;
+    const int M = n * HoHo;  // below 2^31: the host caps the batch
+    const int n_tiles = (M + g.TP - 1) / g.TP;
+    auto tile_origin = [&](int t, int &img0, int &a0) {
+        const int m0 = t * g.TP;
         img0 = (int)(m0 / HoHo);
-        a0 = (int)((m0 - (long long)img0 * HoHo) / Ho);
+        a0 = (int)((m0 - img0 * HoHo) / Ho);
     };
     uint4 pre[kPatchRegs];
     // unit u of the patch: 8 channels c8 of staged pixel (image it, row, col), col fastest; staged row r of an image
     // is dZ row a0 + r
-    auto load_patch = [&](long long t) {
+    auto load_patch = [&](int t) {
         int img0, a0;
         tile_origin(t, img0, a0);
 #pragma unroll
@@ -1024,8 +1024,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
     // this wave's output staging: 32 positions x 64 channels, kPatchPix apart
     bf16 *s_o = s_p + (size_t)g.n_img_t * g.PR * Ho * kPatchPix + (size_t)wave * 32 * kPatchPix;
     float csum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // channels 8 (lane % 8) .. + 7 of dX
-    if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
-    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    if ((int)blockIdx.x < n_tiles) load_patch(blockIdx.x);
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         __syncthreads();  // weights stored (first tile); the previous tile's patch reads done
 #pragma unroll
         for (int r = 0; r < kPatchRegs; ++r) {
@@ -1036,16 +1036,16 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
         if (t + gridDim.x < n_tiles) load_patch(t + gridDim.x);
         int img0, a0;
         tile_origin(t, img0, a0);
-        const long long m0 = t * g.TP;
+        const int m0 = t * g.TP;
         // this lane's two A rows: the staged pixel of position (a, b) and its column b, or none past the last position
         int pb[2], pcol[2];
         bool pok[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const long long m = m0 + (2 * wave + i) * 16 + r16;
+            const int m = m0 + (2 * wave + i) * 16 + r16;
             pok[i] = m < M;
             const int img = (int)(m / HoHo);
-            const int pl = (int)(m - (long long)img * HoHo);
+            const int pl = (int)(m - img * HoHo);
             const int a = pl / Ho;
             pcol[i] = pl - a * Ho;
             pb[i] = ((img - img0) * g.PR + (a - a0)) * Ho;
@@ -1060,12 +1060,12 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_bwd_s2_patch(const bf16 
             uint4 mk[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const long long mm = m0 + 32 * wave + (lane >> 3) + 8 * q;
+                const int mm = m0 + 32 * wave + (lane >> 3) + 8 * q;
                 oidx[q] = ~(size_t)0;
                 mk[q] = make_uint4(0u, 0u, 0u, 0u);
                 if (mm < M) {
                     const int img = (int)(mm / HoHo);
-                    const int pl = (int)(mm - (long long)img * HoHo);
+                    const int pl = (int)(mm - img * HoHo);
                     const int a = pl / Ho, b = pl - a * Ho;
                     oidx[q] = (((size_t)img * H + 2 * a + py) * H + 2 * b + px) * 64 + 8 * c8;
                     mk[q] = *reinterpret_cast<const uint4 *>(Ymask + oidx[q]);
@@ -1215,8 +1215,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
     const int H = g.H, Ho = g.Ho, HoHo = Ho * Ho, CP = dw_plane(g);
     const int rows = g.n_img_t * g.PR;  // staged rows of a plane
     bf16 *s_z = s_c + (size_t)3 * 64 * CP;  // [64][kZP]: dZt[c][m0 + q], q < kTP
-    const long long M = (long long)n * HoHo;
-    const long long n_tiles = (M + g.TP - 1) / g.TP;
+    const int M = n * HoHo;  // below 2^31: the host caps the batch
+    const int n_tiles = (M + g.TP - 1) / g.TP;
     // entries no staged pixel reaches: column ox = 0 of copy 0 (input column -1), and for S = 1 column Ho - 1 of
     // copy 2 (input column H)
     for (int q = tid; q < 64 * rows; q += kPatchThreads) {
@@ -1224,24 +1224,24 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
         s_c[(size_t)ci * CP + rr * Ho] = 0;
         if (S == 1) s_c[(size_t)(128 + ci) * CP + rr * Ho + Ho - 1] = 0;
     }
-    auto tile_origin = [&](long long t, int &img0, int &oy0) {
-        const long long m0 = t * g.TP;
+    auto tile_origin = [&](int t, int &img0, int &oy0) {
+        const int m0 = t * g.TP;
         img0 = (int)(m0 / HoHo);
-        oy0 = (int)((m0 - (long long)img0 * HoHo) / Ho);
+        oy0 = (int)((m0 - img0 * HoHo) / Ho);
     };
     uint4 pre[kXR], zpre[kZR];
     // unit u: channels 8 c8 .. + 7, c8 = (u / 64) % 8, of staged pixel (u / 512) * 64 + u % 64 (image it, staged row,
     // column ix; column fastest), so that the 64 lanes of a wave store one plane at consecutive columns; staged row r
     // of an image is input row oy0 * S - 1 + r. dZ unit u: channel u / (kTP / 8), positions 8 (u % (kTP / 8)) .. + 7.
     const int npix = rows * H;
-    auto load_patch = [&](long long t) {
+    auto load_patch = [&](int t) {
         int img0, oy0;
         tile_origin(t, img0, oy0);
 #pragma unroll
         for (int r = 0; r < kZR; ++r) {
             const int u = tid + r * kPatchThreads;
             const int c = u / (kTP / 8), q8 = u - c * (kTP / 8);
-            const long long m = t * g.TP + 8 * q8;
+            const int m = t * g.TP + 8 * q8;
             zpre[r] = m < M ? *reinterpret_cast<const uint4 *>(dZt + (size_t)c * M + m) : make_uint4(0u, 0u, 0u, 0u);
         }
 #pragma unroll
@@ -1265,8 +1265,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
 #pragma unroll
         for (int j = 0; j < 9; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
     float bsum[2] = {0.f, 0.f};
-    if ((long long)blockIdx.x < n_tiles) load_patch(blockIdx.x);
-    for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    if ((int)blockIdx.x < n_tiles) load_patch(blockIdx.x);
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         __syncthreads();  // zero columns stored (first tile); the previous tile's reads done
 #pragma unroll
         for (int r = 0; r < kZR; ++r) {
@@ -1297,8 +1297,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
         if (t + gridDim.x < n_tiles) load_patch(t + gridDim.x);
         int img0, oy0;
         tile_origin(t, img0, oy0);
-        const long long m0 = t * g.TP;
-        const long long row0 = m0 / Ho;  // global output row of the tile's first position
+        const int m0 = t * g.TP;
+        const int row0 = m0 / Ho;  // global output row of the tile's first position
         for (int s = 0; s < g.TP / 32; ++s) {
             const int q0 = 32 * s + 8 * gq;  // this lane's 8 positions q0 .. q0 + 7 of the tile
             bf16x8_t a[2];
@@ -1312,8 +1312,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_dw_patch(const bf16 *__r
                 }
                 a[i] = __builtin_bit_cast(bf16x8_t, av);
             }
-            const long long R = row0 + q0 / Ho;
-            const int img = (int)(R / Ho), oy = (int)(R - (long long)img * Ho);
+            const int R = row0 + q0 / Ho;
+            const int img = (int)(R / Ho), oy = (int)(R - img * Ho);
             const int pbase = ((img - img0) * g.PR + (oy - oy0) * S) * Ho + (q0 % Ho);
 #pragma unroll
             for (int j = 0; j < 9; ++j) {
