@@ -156,12 +156,12 @@ int gemm(rln_proposal &p, const rlnp::bf16 *A, long long sam, long long sak, con
 const rlnp::bf16 *wbf(const rln_proposal &p, size_t off) { return p.wb.as<const rlnp::bf16>() + off; }
 
 template <int NT>
-void conv_fwd_t(rln_proposal &p, const ConvLayer &L, int n) {
+void conv_fwd_t(rln_proposal &p, const ConvLayer &L, int n, rlnp::bf16 *col = nullptr) {
     const long long M = (long long)n * L.Ho * L.Ho;
     hipLaunchKernelGGL((rlnp::k_conv_fwd<NT>), dim3((unsigned)((M + 63) / 64)), dim3(rlnp::kB), 0, p.stream,
                        p.x[&L - p.conv].as<const rlnp::bf16>(), n, L.H, L.cin, L.stride, L.Ho, L.Kp,
                        p.wb.as<const rlnp::bf16>() + L.w_off, p.w.as<const float>() + L.b_off,
-                       p.x[&L - p.conv + 1].as<rlnp::bf16>());
+                       p.x[&L - p.conv + 1].as<rlnp::bf16>(), col);
 }
 
 // LDS-resident weights (Cin % 8 == 0, Kp % 32 == 0, weights within 80 KB so that two blocks share a CU)
@@ -238,6 +238,10 @@ bool bwd_s2_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
     return lds <= 160 * 1024;
 }
 
+// A first layer (one input channel, 9 taps padded to 16) runs its forward pass as k_conv_fwd in training too, which
+// stores the im2col columns of its weight gradient on the way.
+bool direct_first(const ConvLayer &L) { return L.cin == 1 && L.Kp == 16; }
+
 // The 64-channel layers whose weight and bias gradients k_conv_dw_patch computes from the staged input rows (the
 // forward's tiles; output rows of whole 8-position fragments, H = stride * Ho); its LDS bytes in lds.
 bool dw_patch_geom(const ConvLayer &L, rlnp::PatchGeom &g, size_t &lds) {
@@ -301,12 +305,13 @@ int forward(rln_proposal &p, int n, bool keep_col = true) {
             }
             continue;
         }
-        if (!keep_col && L.cout % 16 == 0 && L.cout <= 64) {
+        if ((!keep_col || direct_first(L)) && L.cout % 16 == 0 && L.cout <= 64) {
+            rlnp::bf16 *col = keep_col ? p.col[l].as<rlnp::bf16>() : nullptr;
             switch (L.cout / 16) {
-                case 1: conv_fwd_t<1>(p, L, n); break;
-                case 2: conv_fwd_t<2>(p, L, n); break;
-                case 3: conv_fwd_t<3>(p, L, n); break;
-                default: conv_fwd_t<4>(p, L, n); break;
+                case 1: conv_fwd_t<1>(p, L, n, col); break;
+                case 2: conv_fwd_t<2>(p, L, n, col); break;
+                case 3: conv_fwd_t<3>(p, L, n, col); break;
+                default: conv_fwd_t<4>(p, L, n, col); break;
             }
             HIP_TRY(hipGetLastError());
             continue;

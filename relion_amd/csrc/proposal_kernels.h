@@ -620,12 +620,13 @@ __global__ void __launch_bounds__(kB) k_sum_parts(const float *__restrict__ P, i
 // Forward 3x3 convolution as an implicit GEMM (inference): the A operand of each MFMA (8 consecutive k =
 // 8 channels of one tap of one output pixel) is read straight from the NHWC input, so no im2col buffer is
 // written or read. Block = 4 waves x 16 output pixels x all Cout (Cout / 16 MFMA column tiles, <= 4).
-// Cin % 8 == 0, or Cin == 1 (then k = tap, 9 taps padded to Kp = 16 in the weights).
+// Cin % 8 == 0, or Cin == 1 (then k = tap, 9 taps padded to Kp = 16 in the weights; with col, the lanes also store
+// each pixel's Kp taps there: the im2col columns of the layer's weight gradient).
 // --------------------------------------------------------------------------------------------------
 template <int NT>  // column tiles of 16 output channels
 __global__ void __launch_bounds__(kB) k_conv_fwd(const bf16 *__restrict__ X, int n, int H, int Cin, int stride, int Ho,
                                                  int Kp, const bf16 *__restrict__ Wt, const float *__restrict__ bias,
-                                                 bf16 *__restrict__ Y) {
+                                                 bf16 *__restrict__ Y, bf16 *__restrict__ col = nullptr) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, gq = lane >> 4;
     const long long M = (long long)n * Ho * Ho;
@@ -659,6 +660,7 @@ __global__ void __launch_bounds__(kB) k_conv_fwd(const bf16 *__restrict__ X, int
             }
             av = make_uint4((unsigned)e[0] | ((unsigned)e[1] << 16), (unsigned)e[2] | ((unsigned)e[3] << 16),
                             (unsigned)e[4] | ((unsigned)e[5] << 16), (unsigned)e[6] | ((unsigned)e[7] << 16));
+            if (col && mok && k < Kp) *reinterpret_cast<uint4 *>(col + (size_t)m * Kp + k) = av;  // im2col row
         } else if (mok) {
             const int tap = k / Cin, c = k - tap * Cin;
             const int iy = oy * stride + tap / 3 - 1, ix = ox * stride + tap % 3 - 1;
