@@ -50,12 +50,14 @@ def rel_norm(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("n", [4, 3])
-def test_proposal_gradients_match_oracle(n):
-    """4 images fill the last stride-2 layer's tiles of 4 images (8 x 8 outputs each); 3 end on a partial one."""
-    W, C, order, npsi = 64, 64, 2, 24
+@pytest.mark.parametrize("W,C,n", [(64, 64, 4), (64, 64, 3), (32, 16, 3)])
+def test_proposal_gradients_match_oracle(W, C, n):
+    """64 x 64 input, 64 channels: the LDS-staged convolution kernels; 4 images fill the last stride-2 layer's tiles
+    of 4 images (8 x 8 outputs each), 3 end on a partial one. 32 x 32 input, 16 channels: the general kernels
+    (im2col GEMMs, direct data gradients) that other network sizes take."""
+    order, npsi = 2, 24
     NB = 12 * 4 ** order * npsi
-    print(f"[proposal] forward/backward of the full network, {n} images", flush=True)
+    print(f"[proposal] forward/backward, window {W}, {C} channels, {n} images", flush=True)
     pp = PoseProposal(window=W, channels=C, healpix_order=order, n_psi_bins=npsi, seed=2)
     assert pp.n_bins == NB and pp.n_params == pc.n_params(W, C, NB)
     rng = np.random.default_rng(8)
