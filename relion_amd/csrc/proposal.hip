@@ -42,6 +42,12 @@ struct rln_proposal {
     mutable int orient_order = -1, orient_npsi = -1;
     rln_proposal_config cfg{};
     hipStream_t stream = nullptr;
+    // backward pass: the weight gradients' partial-sum reductions run on `side` beside the data gradients;
+    // ev_dw orders each after its weight-gradient kernel, ev_red the next weight-gradient kernel (and the end of the
+    // pass) after the last reduction
+    hipStream_t side = nullptr;
+    hipEvent_t ev_dw = nullptr, ev_red = nullptr;
+    bool red_pending = false;
     std::mutex mtx;
     int W = 0, C = 0, NB = 0, F = 0, n_dir_bins = 0;
     rln::ConvLayer conv[rln::kLayers];
@@ -56,6 +62,7 @@ struct rln_proposal {
     rln::DevBuf dzt;  // a 64-channel layer's output gradient transposed to [64][M] for its weight gradient
     rln::DevBuf part;  // partial sums of a k-split GEMM
     rln::DevBuf dwpart;  // per-block weight and bias gradient sums of k_conv_dw_patch
+    rln::DevBuf colpart;  // per-block column sums of k_conv_bwd_s2_patch
     rln::DevBuf src, src_ctf;  // particle images and CTFs staged from the host
     rln::DevBuf top_bins, top_probs;  // predictions of one predict call
     // a prediction launched and not yet finished: its size, the pinned host copies of its top-k, its end event
@@ -110,6 +117,7 @@ int ensure_batch(rln_proposal &p, int n) {
     RLN_TRY(p.dz_b.ensure(sizeof(rlnp::bf16) * zmax));
     RLN_TRY(p.dzt.ensure(sizeof(rlnp::bf16) * zmax));
     RLN_TRY(p.dwpart.ensure(sizeof(float) * rlnp::kDwPart * (size_t)(p.n_cu > 0 ? p.n_cu : 256)));
+    RLN_TRY(p.colpart.ensure(sizeof(float) * 64 * (size_t)(p.n_cu > 0 ? p.n_cu : 256)));
     RLN_TRY(p.loss.ensure(sizeof(double) * cap));
     RLN_TRY(p.labels.ensure(sizeof(int) * cap));
     RLN_TRY(p.idx.ensure(sizeof(int) * cap));
@@ -368,6 +376,7 @@ int backward(rln_proposal &p, int n) {
             const int nb = (int)std::min<long long>(tiles, p.n_cu > 0 ? p.n_cu : 256);
             if ((size_t)nb * rlnp::kDwPart * sizeof(float) > p.dwpart.bytes)
                 RLN_FAIL(RLN_ERR_STATE, "weight-gradient partial sums: %d blocks exceed the buffer", nb);
+            if (p.red_pending) HIP_TRY(hipStreamWaitEvent(p.stream, p.ev_red, 0));  // the partials were read
             if (L.stride == 1) {
                 HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_dw_patch<1>),
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
@@ -382,11 +391,16 @@ int backward(rln_proposal &p, int n) {
                                    p.dwpart.as<float>());
             }
             HIP_TRY(hipGetLastError());
-            // the bias follows the weights in the parameter vector: one reduction covers both
+            // the bias follows the weights in the parameter vector: one reduction covers both; it runs on the side
+            // stream while this layer's data gradient runs
+            HIP_TRY(hipEventRecord(p.ev_dw, p.stream));
+            HIP_TRY(hipStreamWaitEvent(p.side, p.ev_dw, 0));
             hipLaunchKernelGGL(rlnp::k_dw_reduce, dim3((unsigned)((rlnp::kDwPart / 4 + rlnp::kB - 1) / rlnp::kB),
                                                         (unsigned)std::max(1, nb / 16)),
-                               dim3(rlnp::kB), 0, p.stream, p.dwpart.as<const float>(), nb, gw + L.w_off);
+                               dim3(rlnp::kB), 0, p.side, p.dwpart.as<const float>(), nb, gw + L.w_off);
             HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(p.ev_red, p.side));
+            p.red_pending = true;
             bias_done = true;
         } else if (patch_geom(L, pg, plds)) {
             // k-major columns (forward) and dZ transposed: both operands with the pixel index contiguous
@@ -426,13 +440,13 @@ int backward(rln_proposal &p, int n) {
             rlnp::PatchGeom dg{};
             size_t dlds = 0;
             const bool colsum = !dw_patch_geom(p.conv[l - 1], dg, dlds) &&
-                                (size_t)grid * 64 * sizeof(float) <= p.dwpart.bytes;
+                                (size_t)grid * 64 * sizeof(float) <= p.colpart.bytes;
             hipLaunchKernelGGL(rlnp::k_conv_bwd_s2_patch, dim3(grid), dim3(rlnp::kPatchThreads), plds, p.stream, dz, n,
                                pg, p.wt.as<const rlnp::bf16>(), p.x[l].as<const rlnp::bf16>(), dz_next,
-                               colsum ? p.dwpart.as<float>() : nullptr);
+                               colsum ? p.colpart.as<float>() : nullptr);
             HIP_TRY(hipGetLastError());
             if (colsum) {
-                hipLaunchKernelGGL(rlnp::k_colsum_parts, dim3(1), dim3(1024), 0, p.stream, p.dwpart.as<const float>(),
+                hipLaunchKernelGGL(rlnp::k_colsum_parts, dim3(1), dim3(1024), 0, p.stream, p.colpart.as<const float>(),
                                    (int)grid, gw + p.conv[l - 1].b_off);
                 HIP_TRY(hipGetLastError());
                 bias_below_done = true;
@@ -461,6 +475,10 @@ int backward(rln_proposal &p, int n) {
             HIP_TRY(hipGetLastError());
         }
         std::swap(dz, dz_next);
+    }
+    if (p.red_pending) {  // the gradient is whole once the last reduction is done
+        HIP_TRY(hipStreamWaitEvent(p.stream, p.ev_red, 0));
+        p.red_pending = false;
     }
     return RLN_OK;
 }
@@ -600,11 +618,18 @@ extern "C" int rln_proposal_create(int device, const rln_proposal_config *cfg, r
     p->rng.seed(cfg->seed);
     int rc = RLN_OK;
     auto fail = [&](int r) {
+        if (p->stream) (void)hipStreamDestroy(p->stream);
+        if (p->side) (void)hipStreamDestroy(p->side);
+        if (p->ev_dw) (void)hipEventDestroy(p->ev_dw);
+        if (p->ev_red) (void)hipEventDestroy(p->ev_red);
         delete p;
         return r;
     };
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail((set_error("hipStreamCreate failed"), RLN_ERR_HIP));
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_dw, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_red, hipEventDisableTiming) != hipSuccess)
+        return fail((set_error("hipStreamCreate / hipEventCreate failed"), RLN_ERR_HIP));
     const size_t nb = sizeof(float) * p->n_params;
     if ((rc = p->w.ensure(nb)) || (rc = p->g.ensure(nb)) || (rc = p->m1.ensure(nb)) || (rc = p->m2.ensure(nb)) ||
         (rc = p->wb.ensure(sizeof(rlnp::bf16) * p->n_params)))
@@ -637,6 +662,12 @@ extern "C" int rln_proposal_destroy(rln_proposal *p) {
         (void)hipStreamSynchronize(p->stream);
         (void)hipStreamDestroy(p->stream);
     }
+    if (p->side) {
+        (void)hipStreamSynchronize(p->side);
+        (void)hipStreamDestroy(p->side);
+    }
+    if (p->ev_dw) (void)hipEventDestroy(p->ev_dw);
+    if (p->ev_red) (void)hipEventDestroy(p->ev_red);
     if (p->pend_ev) (void)hipEventDestroy(p->pend_ev);
     if (p->h_top) (void)hipHostFree(p->h_top);
     delete p;
