@@ -58,7 +58,8 @@ struct rln_proposal {
     // activations for a batch of up to cap images
     int cap = 0;
     rln::DevBuf x[rln::kLayers + 1], col[rln::kLayers], logits, dlog, dtmp, dz_a, dz_b, loss, labels, idx;
-    rln::DevBuf wt;  // one layer's weights rearranged for its data gradient (k_wt_bwd)
+    rln::DevBuf wt;  // the layers' weights rearranged for their data gradients, layer l at l * wt_slot
+    size_t wt_slot = 0;
     rln::DevBuf dzt;  // a 64-channel layer's output gradient transposed to [64][M] for its weight gradient
     rln::DevBuf part;  // partial sums of a k-split GEMM
     rln::DevBuf dwpart;  // per-block weight and bias gradient sums of k_conv_dw_patch
@@ -108,7 +109,8 @@ int ensure_batch(rln_proposal &p, int n) {
             dmax = std::max(dmax, (size_t)cap * p.conv[l].Ho * p.conv[l].Ho * p.conv[l].Kp);
     size_t wtmax = 1;
     for (int l = 1; l < kLayers; ++l) wtmax = std::max(wtmax, (size_t)9 * p.conv[l].cin * p.conv[l].cout);
-    RLN_TRY(p.wt.ensure(sizeof(rlnp::bf16) * wtmax));
+    p.wt_slot = wtmax;
+    RLN_TRY(p.wt.ensure(sizeof(rlnp::bf16) * wtmax * kLayers));  // one slot per layer
     RLN_TRY(p.logits.ensure(sizeof(float) * (size_t)cap * p.NB));
     RLN_TRY(p.dlog.ensure(sizeof(rlnp::bf16) * (size_t)cap * p.NB));
     RLN_TRY(p.dtmp.ensure(sizeof(float) * dmax));
@@ -359,6 +361,26 @@ int backward(rln_proposal &p, int n) {
                        p.dtmp.as<const float>(), x6, (long long)n * p.F, dz);
     HIP_TRY(hipGetLastError());
     bool bias_below_done = false;  // this layer's bias gradient came with the layer above's data gradient
+    {  // the 64-channel layers' rearranged weights for their data gradients, in one launch
+        rlnp::WtPlan plan{};
+        int ne = 0;
+        for (int l = 1; l < kLayers; ++l) {
+            rlnp::PatchGeom g{};
+            size_t lds = 0;
+            const ConvLayer &L = p.conv[l];
+            const int mode = (L.stride == 1 && patch_geom(L, g, lds)) ? 1 : bwd_s2_geom(L, g, lds) ? 0 : -1;
+            if (mode < 0) continue;
+            plan.w_off[ne] = (long long)L.w_off;
+            plan.off_out[ne] = (long long)(l * p.wt_slot);
+            plan.mode[ne] = mode;
+            ++ne;
+        }
+        if (ne > 0) {
+            hipLaunchKernelGGL(rlnp::k_wt_layers, dim3(grid1(64 * rlnp::kPatchK), ne), dim3(rlnp::kB), 0, p.stream,
+                               p.wb.as<const rlnp::bf16>(), plan, p.wt.as<rlnp::bf16>());
+            HIP_TRY(hipGetLastError());
+        }
+    }
     for (int l = kLayers - 1; l >= 0; --l) {
         const ConvLayer &L = p.conv[l];
         const int M = n * L.Ho * L.Ho;
@@ -422,16 +444,10 @@ int backward(rln_proposal &p, int n) {
         if (L.stride == 1 && patch_geom(L, pg, plds)) {
             // a stride-1 64-channel layer: its data gradient is a stride-1 convolution of dZ with the flipped,
             // transposed weights, masked by the ReLU of its input (same geometry as its forward pass)
-            hipLaunchKernelGGL(rlnp::k_wt_flip, dim3(grid1(64 * rlnp::kPatchK)), dim3(rlnp::kB), 0, p.stream,
-                               wbf(p, L.w_off), p.wt.as<rlnp::bf16>());
-            HIP_TRY(hipGetLastError());
-            RLN_TRY((conv_patch_launch<1, 1>(p, n, pg, plds, dz, p.wt.as<const rlnp::bf16>(), nullptr, dz_next,
-                                             p.x[l].as<const rlnp::bf16>())));
+            RLN_TRY((conv_patch_launch<1, 1>(p, n, pg, plds, dz, p.wt.as<const rlnp::bf16>() + l * p.wt_slot, nullptr,
+                                             dz_next, p.x[l].as<const rlnp::bf16>())));
         } else if (bwd_s2_geom(L, pg, plds)) {
             // a stride-2 64-channel layer: the four parity classes of each 2 x 2 input block from a staged dZ patch
-            hipLaunchKernelGGL(rlnp::k_wt_bwd, dim3(grid1(64 * rlnp::kPatchK)), dim3(rlnp::kB), 0, p.stream,
-                               wbf(p, L.w_off), L.cin, L.cout, L.Kp, p.wt.as<rlnp::bf16>());
-            HIP_TRY(hipGetLastError());
             HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&rlnp::k_conv_bwd_s2_patch),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
             const long long tiles = ((long long)n * L.Ho * L.Ho + pg.TP - 1) / pg.TP;
@@ -442,8 +458,8 @@ int backward(rln_proposal &p, int n) {
             const bool colsum = !dw_patch_geom(p.conv[l - 1], dg, dlds) &&
                                 (size_t)grid * 64 * sizeof(float) <= p.colpart.bytes;
             hipLaunchKernelGGL(rlnp::k_conv_bwd_s2_patch, dim3(grid), dim3(rlnp::kPatchThreads), plds, p.stream, dz, n,
-                               pg, p.wt.as<const rlnp::bf16>(), p.x[l].as<const rlnp::bf16>(), dz_next,
-                               colsum ? p.colpart.as<float>() : nullptr);
+                               pg, p.wt.as<const rlnp::bf16>() + l * p.wt_slot, p.x[l].as<const rlnp::bf16>(),
+                               dz_next, colsum ? p.colpart.as<float>() : nullptr);
             HIP_TRY(hipGetLastError());
             if (colsum) {
                 hipLaunchKernelGGL(rlnp::k_colsum_parts, dim3(1), dim3(1024), 0, p.stream, p.colpart.as<const float>(),

@@ -821,7 +821,7 @@ struct PatchGeom {
 };
 
 // EPI 0: bias and ReLU (a forward layer). EPI 1 (stride 1): the data gradient of a stride-1 layer, X its output
-// gradient dZ, Wt its weights transposed with the taps flipped (k_wt_flip), the result masked by the ReLU of the
+// gradient dZ, Wt its weights transposed with the taps flipped (k_wt_layers), the result masked by the ReLU of the
 // layer's input Ymask (dX = (Ymask > 0) * sum), no bias.
 template <int S, int EPI = 0>
 __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__restrict__ X, int n, PatchGeom g,
@@ -959,13 +959,22 @@ __global__ void __launch_bounds__(kPatchThreads) k_conv_fwd_patch(const bf16 *__
     }
 }
 
-// Weights of a 64-channel layer for its data gradient as a stride-1 convolution of dZ: transposed with the taps
-// flipped, Wf[c][t*64 + co] = W[co][(8 - t)*64 + c] (output pixel y takes dZ at y + 1 - ky = y + ky' - 1, ky' = 2 - ky).
-__global__ void __launch_bounds__(kB) k_wt_flip(const bf16 *__restrict__ W, bf16 *__restrict__ Wf) {
+// The weights of up to 8 64-channel layers rearranged for their data gradients in one launch (blockIdx.y = layer
+// entry; entry e's output at wt + off_out[e]): mode 0 for k_conv_bwd_s2_patch, transposed,
+// Wd[c][t*64 + co] = W[co][t*64 + c] (as k_wt_bwd); mode 1 for the stride-1 layers' data gradient as a stride-1
+// convolution of dZ, transposed with the taps flipped, Wf[c][t*64 + co] = W[co][(8 - t)*64 + c] (output pixel y takes
+// dZ at y + 1 - ky = y + ky' - 1, ky' = 2 - ky).
+struct WtPlan {
+    long long w_off[8], off_out[8];
+    int mode[8];
+};
+__global__ void __launch_bounds__(kB) k_wt_layers(const bf16 *__restrict__ W, WtPlan plan, bf16 *__restrict__ wt) {
     const int idx = blockIdx.x * kB + threadIdx.x;
     if (idx >= 64 * kPatchK) return;
+    const int e = blockIdx.y;
     const int c = idx / kPatchK, k = idx - c * kPatchK, t = k >> 6, co = k & 63;
-    Wf[idx] = W[(size_t)co * kPatchK + (8 - t) * 64 + c];
+    const int src_tap = plan.mode[e] == 0 ? t : 8 - t;
+    wt[plan.off_out[e] + idx] = W[plan.w_off[e] + (size_t)co * kPatchK + src_tap * 64 + c];
 }
 
 // Data gradient of a stride-2 64-channel layer (input H x H, output Ho x Ho, H = 2 Ho), masked by the ReLU of its
